@@ -1,0 +1,254 @@
+"""Closed forms and invariants that pin the oracle independently of the golden vectors.
+
+Each check is chosen so that a plausible restatement mistake (a dropped term, a flipped
+sign or slot, a wrong neighbour index, a transposed operand) fails it.
+"""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from fvm_solver_amd.mesh_gen.channel import channel_mesh_2d, channel_mesh_3d
+from oracle import fvm_oracle as O
+from tests.golden_cases import load, mesh_tuple, phys_of
+
+
+def _phys_rest(d):
+    """Channel physics, but with the inlet/exit states at rest (V = 0)."""
+    p = phys_of(d)
+    p.inlet = O.BCSpec(v_n_inf=0.0, v_t_inf=0.0, rho_inf=1.0, T_inf=100.0, v_inf=(0.0, 0.0, 0.0))
+    p.exit = O.BCSpec(v_n_inf=0.0, v_t_inf=0.0, rho_inf=1.0, T_inf=100.0, v_inf=(0.0, 0.0, 0.0))
+    return p
+
+
+@pytest.fixture(scope="module", params=["ref_2d.npz", "ref_3d.npz"])
+def fp64(request):
+    d = load(request.param)
+    mesh, fvm = O.from_mesh_tuple(*mesh_tuple(d), phys_of(d), dtype=torch.float64)
+    return d, mesh, fvm
+
+
+def test_lsq_gradient_exact_on_linear_fields(fp64):
+    d, mesh, fvm = fp64
+    dim, N = mesh.dim, mesh.n_cells
+    rng = np.random.default_rng(0)
+    a, b = rng.standard_normal(), torch.from_numpy(rng.standard_normal(dim))
+    pts = torch.cat([mesh.centroids, mesh.midpoints[mesh.bc_facet_mask]])
+    u = (a + pts @ b).unsqueeze(1)
+    g = fvm.G.spMM(u).view(N, dim)
+    assert torch.max(torch.abs(g - b)) < 1e-9
+
+
+def test_gradient_rows_sum_to_zero(fp64):
+    d, mesh, fvm = fp64
+    crow, col, val = fvm.G.csr()
+    rows = torch.repeat_interleave(torch.arange(crow.numel() - 1), crow[1:] - crow[:-1])
+    s = torch.zeros(crow.numel() - 1, dtype=torch.float64).index_add_(0, rows, val)
+    assert torch.max(torch.abs(s)) < 1e-9 * torch.max(torch.abs(val))
+
+
+def test_closed_cells_and_flux_matrix(fp64):
+    d, mesh, fvm = fp64
+    n = mesh.normals[mesh.cell_to_facet] * mesh.cell_facet_signs.unsqueeze(-1).double()
+    assert torch.max(torch.abs(n.sum(dim=1))) < 1e-12
+    v = torch.tensor([0.3, -1.1, 0.7][:mesh.dim], dtype=torch.float64)
+    flux = (mesh.normals @ v).unsqueeze(1)
+    div = fvm.flux_mat.spMM(flux)
+    assert torch.max(torch.abs(div * mesh.volumes.unsqueeze(1))) < 1e-12
+
+
+def test_signs_point_outward(fp64):
+    d, mesh, fvm = fp64
+    nh = fvm.normals_hat.squeeze(-1)[mesh.cell_to_facet]
+    dot = (nh * mesh.cent_to_facet_disp).sum(-1) * mesh.cell_facet_signs.double()
+    assert torch.all(dot > 0)
+    # slot 0 of every interior facet is the cell the normal points away from
+    left = mesh.facet_to_cell_main[:, 0]
+    nm = mesh.normals[~mesh.bc_facet_mask]
+    dl = mesh.midpoints[~mesh.bc_facet_mask] - mesh.centroids[left]
+    assert torch.all((nm * dl).sum(-1) > 0)
+
+
+def test_global_conservation(fp64):
+    """sum_c vol_c dU_c/dt = sum over boundary facets of (2*slot-1) F_f: interior fluxes cancel."""
+    d, mesh, fvm = fp64
+    prim = torch.from_numpy(d["rhs0_prim"]).double()
+    fvm.cell_grads = None
+    dudt = fvm.forward(prim)
+    lhs = (mesh.volumes.unsqueeze(1) * dudt).sum(0)
+    bslot = 1 - fvm.bc_facet_side.long()
+    rhs = ((2 * bslot - 1).double().unsqueeze(1) * fvm.fluxes[fvm.bc_locations]).sum(0)
+    scale = (mesh.volumes.unsqueeze(1) * dudt.abs()).sum(0)
+    assert torch.all(torch.abs(lhs - rhs) <= 1e-10 * scale)
+
+
+@pytest.mark.parametrize("fname", ["ref_2d.npz", "ref_3d.npz"])
+def test_free_stream_at_rest_is_steady(fname):
+    d = load(fname)
+    mesh, fvm = O.from_mesh_tuple(*mesh_tuple(d), _phys_rest(d), dtype=torch.float64)
+    dim = mesh.dim
+    prim = torch.zeros(mesh.n_cells, dim + 2, dtype=torch.float64)
+    prim[:, dim] = 1.0
+    prim[:, dim + 1] = 100.0
+    for _ in range(2):      # second call exercises the carried-over gradients
+        dudt = fvm.forward(prim)
+        P = fvm.p.R * 1.0 * 100.0
+        scale = P * float(torch.max(fvm.facet_size / mesh.volumes.min()))
+        assert torch.max(torch.abs(dudt)) < 1e-10 * scale
+
+
+def test_characteristic_bc_fixed_point(fp64):
+    d, mesh, fvm = fp64
+    dim = mesh.dim
+    g = fvm.groups[0]
+    prim = torch.from_numpy(d["rhs0_prim"]).double().clone()
+    cells = g["cells"]
+    n = g["normals"]
+    # put every boundary cell at its own facet's farfield state (v_t = v_t_inf, v_n = v_n_inf)
+    prim[cells, :dim] = g["v_n_inf"].unsqueeze(-1) * n + g["v_t_inf"]
+    prim[cells, dim] = g["rho_inf"]
+    prim[cells, dim + 1] = g["T_inf"]
+    ub = fvm.set_face_values(prim, None)[g["bc_idx"]]
+    assert torch.allclose(ub, prim[cells], rtol=1e-12, atol=1e-12)
+
+
+def test_limiter_closed_forms():
+    class L:
+        pass
+    for lp in (1, 2, 3, 4, 5):
+        f = O.OracleFVM.__new__(O.OracleFVM)
+        f.p = O.Phys(lim_p=lp)
+        f.eps_p = torch.tensor(1e-6, dtype=torch.float64)
+        rng = np.random.default_rng(lp)
+        dU = torch.from_numpy(rng.standard_normal((50, 1, 3)))
+        delta = torch.from_numpy(np.abs(rng.standard_normal((50, 1, 3)))) * torch.sign(dU)
+        phi = f._limit(delta, dU)
+        a, b = delta.abs(), dU.abs()
+        e = 1e-6
+        if lp == 1:
+            exp = torch.clamp(delta / (torch.sign(dU) * (b + 1e-8)), 0, 1)
+        elif lp == 2:
+            exp = (delta ** 2 + e + 2 * delta * dU) / (delta ** 2 + 2 * dU ** 2 + delta * dU + e)
+        elif lp == 3:
+            exp = torch.where(a < 2 * b, (a ** 3 + e + 4 * a * b ** 2) / (a ** 3 + e + b * (delta ** 2 + 4 * b ** 2)), 1.0)
+        elif lp == 4:
+            S = 2 * b * (a ** 2 - 2 * b * (a - 2 * b))
+            exp = torch.where(a < 2 * b, (a ** 4 + e + a * S) / (a ** 4 + e + b * (delta ** 3 + S)), 1.0)
+        else:
+            S = 8 * b ** 2 * (a ** 2 - 2 * b * (a - b))
+            exp = torch.where(a < 2 * b, (a ** 5 + e + a * S) / (a ** 5 + e + b * (delta ** 4 + S)), 1.0)
+        assert torch.allclose(phi, exp, rtol=1e-12, atol=1e-12)
+    # phi = 1 where the bound is at least twice the extrapolation (p >= 3)
+    f.p = O.Phys(lim_p=4)
+    assert torch.all(f._limit(torch.tensor([[[3.0]]], dtype=torch.float64), torch.tensor([[[1.0]]], dtype=torch.float64)) == 1)
+
+
+class _Linear:
+    """y' = lam * y, used to pin the integrators' coefficients."""
+    dtype = torch.float64
+
+    def __init__(self, lam):
+        self.lam = lam
+
+    def forward(self, prim):
+        return self.lam * prim
+
+    def state_to_primative(self, U):
+        return U
+
+
+def _stepper(name, extra, lam, dt):
+    return O.OracleStepper(_Linear(lam), name, extra, dt=dt)
+
+
+def test_rk4_stability_polynomial():
+    lam, dt = -3.0, 0.1
+    st = _stepper("Butcher", "RK4", lam, dt)
+    y1 = st.step(torch.ones(1, 1, dtype=torch.float64))
+    z = lam * float(torch.tensor(dt, dtype=torch.float32))
+    assert abs(float(y1) - (1 + z + z ** 2 / 2 + z ** 3 / 6 + z ** 4 / 24)) < 1e-7
+
+
+@pytest.mark.parametrize("name,extra,order", [("Butcher", "RK4", 4), ("Butcher", "RK3_SSP4", 3),
+                                              ("Butcher", "RK3_SSP5", 3), ("Butcher", "RK4_SSP5", 4),
+                                              ("Butcher", "RK4_SSP10", 4), ("AB3", "", 3)])
+def test_integrator_order(name, extra, order):
+    # lam = -4 keeps the truncation error far above the fp32 tableau rounding (~1e-8).
+    lam, T = -4.0, 1.0
+    errs = []
+    for n in (10, 20):
+        dt = T / n
+        st = _stepper(name, extra, lam, dt)
+        st.dt = torch.tensor(dt, dtype=torch.float64)
+        if name == "AB3":   # seed the history with the exact past slopes
+            st.hist = (torch.full((1, 1), lam * math.exp(-lam * dt), dtype=torch.float64),
+                       torch.full((1, 1), lam * math.exp(-2 * lam * dt), dtype=torch.float64))
+        y = torch.ones(1, 1, dtype=torch.float64)
+        for _ in range(n):
+            y = st.step(y)
+        errs.append(abs(float(y) - math.exp(lam * T)))
+    rate = math.log2(errs[0] / errs[1])
+    assert abs(rate - order) < 0.4
+
+
+def test_adams3_pc_consistency():
+    class Const(_Linear):
+        def forward(self, prim):
+            return torch.full_like(prim, 2.0)
+    st = O.OracleStepper(Const(0.0), "Adams3_PC", dt=0.01)
+    y = st.step(torch.zeros(3, 1, dtype=torch.float64))
+    assert torch.allclose(y, torch.full_like(y, 2.0 * float(torch.tensor(0.01, dtype=torch.float32))), rtol=1e-12)
+
+
+def test_cfl_and_conservation_closed_forms(fp64):
+    d, mesh, fvm = fp64
+    dim = mesh.dim
+    prim = torch.zeros(mesh.n_cells, dim + 2, dtype=torch.float64)
+    prim[:, dim] = 1.0
+    prim[:, dim + 1] = 100.0
+    c = math.sqrt(fvm.p.gamma * fvm.p.R * 100.0)
+    per = np.zeros(mesh.n_cells)
+    fs = fvm.facet_size.squeeze(-1).numpy()
+    for k in range(dim + 1):
+        per += fs[mesh.cell_to_facet[:, k].numpy()]
+    h = dim * mesh.volumes.numpy() / per
+    assert abs(fvm.cfl_dt(prim, 0.5) - 0.5 * h.min() / c) < 1e-14
+    U = torch.ones(mesh.n_cells, dim + 2, dtype=torch.float64) * torch.arange(1, dim + 3)
+    tot = fvm.conservation_totals(U)
+    assert torch.allclose(tot, float(mesh.volumes.sum()) * torch.arange(1, dim + 3).double(), rtol=1e-14)
+
+
+def test_csr_to_ell_and_spmm_brute_force():
+    rng = np.random.default_rng(3)
+    for M, N, kmax in ((1, 1, 1), (9, 5, 3), (40, 70, 6)):
+        dense = np.zeros((M, N))
+        for r in range(M):
+            k = rng.integers(0, kmax + 1)
+            dense[r, rng.choice(N, size=min(k, N), replace=False)] = rng.standard_normal(min(k, N))
+        A = torch.from_numpy(dense).to_sparse_csr()
+        v, c = O.csr_to_ell(A.crow_indices(), A.col_indices(), A.values())
+        nnz = (dense != 0).sum(1)
+        K = v.shape[1]
+        for r in range(M):
+            cols = np.flatnonzero(dense[r])
+            assert np.array_equal(c[r, :nnz[r]].numpy(), cols)
+            assert np.all(c[r, nnz[r]:].numpy() == 0) and np.all(v[r, nnz[r]:].numpy() == 0)
+            assert np.allclose(v[r, :nnz[r]].numpy(), dense[r, cols])
+        x = torch.from_numpy(rng.standard_normal((N, 4)))
+        b = torch.from_numpy(rng.standard_normal(M))
+        assert torch.allclose(O.ell_spmm(v, c, x, b), torch.from_numpy(dense) @ x + b.unsqueeze(1))
+        assert K == (nnz.max() if M else 0)
+    with pytest.raises(ValueError):
+        A = torch.eye(3, dtype=torch.float64).to_sparse_csr()
+        O.csr_to_ell(A.crow_indices(), A.col_indices(), A.values(), K=0)
+
+
+def test_generated_meshes_are_valid():
+    for gen, n in ((channel_mesh_2d, 2000), (channel_mesh_3d, 2000)):
+        pts, cells, (fi, fb), tags = gen(n, seed=5)
+        mesh, fvm = O.from_mesh_tuple(pts, cells, fi, fb, tags, O.Phys(), dtype=torch.float64)
+        assert torch.all(mesh.volumes > 0)
+        assert set(tags) == {"Left", "Right", "NavierWall"}
+        # interior facets are lexicographically sorted unique vertex tuples
+        assert np.all(np.diff(fi[:, 0]) >= 0)

@@ -1,0 +1,223 @@
+"""Benchmark: fused HIP RK4 step of the 2-D compressible NS channel at ~1M triangle cells.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--cells N] [--no-cpu-baseline]
+
+Workload: seeded Poisson-disk/Delaunay mesh of [0,2]x[0,1] minus a disk r=0.1 at
+(0.5,0.5), ~1M cells, 4 fp32 conserved variables, freestream Mach 0.3 + Gaussian
+density bump, Re 200, classical RK4 (Butcher "RK4") at CFL 0.5.  One step = 4 fused RHS
+stages.  Prints one JSON line: metric = cell-updates/s (cells x steps / s), the dominant
+kernel's HBM roofline (algorithmic bytes / measured duration), and the CPU oracle timed on
+a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+import torch
+
+HBM_PEAK_GBS = 8000.0
+
+
+def _parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--cells", type=int, default=1_000_000)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--cpu-steps", type=int, default=4)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--kernel-timing-steps", type=int, default=20)
+    return ap.parse_args()
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def flux_kernel_bytes(N, F, Fb, D, stage, stages):
+    """Algorithmic (compulsory) bytes of one k_flux launch: every array read or written
+    once, fp32 values, int32 indices.  Per cell: prim, the gradient record
+    ((D+1)C floats), the facet/neighbour lists, the packed info word, centroid->facet
+    displacements, 1/vol, U0, the RK accumulator (read after stage 0; written, or the
+    new state written on the last stage) and the next-stage primitives; per facet the
+    geometry record (n, n_hat, |n|, 1/(n.d), X_orthog); per boundary facet the boundary
+    values and the two face-gradient coefficients."""
+    C, NL = D + 2, D + 1
+    cell = 4 * (C + (D + 1) * C + NL + NL + 1 + NL * D + 1 + C + (C if stage > 0 else 0) + C + C)
+    facet = 4 * (3 * D + 2)
+    bc = 4 * (3 * C)
+    return N * cell + F * facet + Fb * bc
+
+
+def grad_kernel_bytes(N, Fb, D):
+    """k_grad: prim, neighbour list, info, LSQ values ((NL+1) D), displacements, eps;
+    writes the gradient record; boundary values read once."""
+    C, NL = D + 2, D + 1
+    return N * 4 * (C + NL + 1 + (NL + 1) * D + NL * D + 1 + (D + 1) * C) + Fb * 4 * C
+
+
+def main():
+    a = _parse()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {a.gpus}); the multi-GPU design is described in "
+              "DESIGN.md and no multi-GPU figure is measured", file=sys.stderr)
+        sys.exit(2)
+    from fvm_solver_amd import _native as nat
+    from fvm_solver_amd.config_fvm import ConfigChannel
+    from fvm_solver_amd.fvm_equation import FluidConstitution2D, FVMEquation
+    from fvm_solver_amd.mesh_gen.channel import channel_mesh_2d
+    from fvm_solver_amd.mesh_utils.fvm_mesh import mesh_from_tuple
+    from fvm_solver_amd.mesh_utils.mesh_store import channel_bc_tags
+
+    if not torch.cuda.is_available():
+        raise RuntimeError("bench.py needs a GPU")
+    dev = torch.device("cuda:0")
+    t0 = time.time()
+    pts, cells, (fi, fb), tags = channel_mesh_2d(a.cells, seed=a.seed)
+    log(f"mesh: {len(cells)} cells, {len(fi) + len(fb)} facets ({len(fb)} boundary) in {time.time() - t0:.1f}s")
+    cfg = ConfigChannel(device="cuda:0", n_cells=a.cells, solver_name="Butcher", solver_extra="RK4")
+    t0 = time.time()
+    mesh = mesh_from_tuple(pts, cells, fi, fb, device=dev)
+    phy = FluidConstitution2D(cfg)
+    N, F, Fb, D = mesh.n_cells, mesh.n_facets, mesh.n_bc_facet, 2
+    c = mesh.geo64["centroids"]
+    prim = np.zeros((N, 4))
+    prim[:, 0] = cfg.inlet_cfg.v_n_inf
+    r2 = ((c - np.asarray(cfg.bump_centre)) ** 2).sum(1)
+    prim[:, 2] = 1.0 + cfg.bump_amp * np.exp(-r2 / (2 * cfg.bump_sigma ** 2))
+    prim[:, 3] = cfg.inlet_cfg.T_inf
+    prim = torch.from_numpy(prim).float()
+    V, rho, T = prim[:, :2], prim[:, 2:3], prim[:, 3:]
+    U0 = torch.cat(phy.primatives_to_state(V, rho, T), -1)
+    eq = FVMEquation(cfg, phy, mesh, 4, channel_bc_tags(tags, 2), us_init=U0)
+    dt = float(eq.cfl_dt(prim.to(dev), cfg.cfl))
+    eq.t_solver.set_dt(dt)
+    log(f"set-up {time.time() - t0:.1f}s, dt={dt:.4e}")
+    ts = eq.t_solver
+
+    for _ in range(a.warmup):
+        ts._solve_step(None)
+    torch.cuda.synchronize()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    wall0 = time.perf_counter()
+    ev0.record()
+    for _ in range(a.steps):
+        ts._solve_step(None)
+    ev1.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - wall0
+    gpu_ms = ev0.elapsed_time(ev1)
+    ms_per_step = gpu_ms / a.steps
+    value = N * a.steps / (gpu_ms / 1e3)
+    state = eq.cells.state
+    finite = bool(torch.isfinite(state).all())
+    log(f"{a.steps} steps: {gpu_ms:.2f} ms GPU ({wall * 1e3:.2f} ms wall), {ms_per_step * 1e3:.1f} us/step, finite={finite}")
+
+    # ---- per-kernel durations: the same stages, launched part by part between events ----
+    E = eq.E_props
+    parts = {1: "k_bc_sum", 2: "k_bc", 4: "k_grad", 8: "k_flux"}
+    dur = {k: [] for k in parts.values()}
+    flux_bytes = []
+    U = eq.cells.state
+    st_prim = ts._stage_prim0(U)
+    s = ts.stages
+    for _ in range(a.kernel_timing_steps):
+        cur = st_prim
+        out = ts._out[1] if U.data_ptr() == ts._out[0].data_ptr() else ts._out[0]
+        for i in range(s):
+            nxt = ts._prim[1 - ts._flip]
+            st = nat.FvmStage(prim=cur.data_ptr(), U0=U.data_ptr(), acc=ts._acc.data_ptr(), acc2=None, kbuf=None,
+                              prim_next=nxt.data_ptr(), U_out=out.data_ptr(), dt=ts.dt.data_ptr(), stage=i,
+                              last=int(i == s - 1), store_k=0, has_prev=1, b=float(ts.b[i]), b2=float(ts.b2[i]))
+            if i < s - 1:
+                terms = [(j, float(ts.A[i + 1][j])) for j in range(i + 1) if ts.A[i + 1][j] != 0]
+                st.n_inc = len(terms)
+                for q, (j, cc) in enumerate(terms):
+                    st.inc_idx[q], st.inc_coef[q] = j, cc
+            for mask, name in parts.items():
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record()
+                nat.check(nat.lib().fvm_rk_stage_part_f32(E.mesh.struct, E.phys, E.work, st, mask,
+                                                          nat.stream_ptr(dev)), "fvm_rk_stage_part_f32")
+                e1.record()
+                dur[name].append((e0, e1))
+            flux_bytes.append(flux_kernel_bytes(N, F, Fb, D, i, s))
+            ts._flip = 1 - ts._flip
+            cur = nxt
+        ts._last_out = out
+        eq.cells.update_cells(out)
+        U = out
+        st_prim = cur
+    torch.cuda.synchronize()
+    avg_us = {k: 1e3 * float(np.mean([e0.elapsed_time(e1) for e0, e1 in v])) for k, v in dur.items()}
+    log("kernel averages (us): " + ", ".join(f"{k}={v:.1f}" for k, v in avg_us.items()))
+    dom = max(avg_us, key=avg_us.get)
+    if dom == "k_flux":
+        dom_bytes = float(np.mean(flux_bytes))
+    elif dom == "k_grad":
+        dom_bytes = float(grad_kernel_bytes(N, Fb, D))
+    else:
+        dom_bytes = float(Fb * 4 * 12)
+    achieved = dom_bytes / (avg_us[dom] * 1e-6) / 1e9
+    stage_bytes = float(np.mean(flux_bytes)) + grad_kernel_bytes(N, Fb, D)
+    log(f"dominant {dom}: {dom_bytes / 1e6:.1f} MB/launch -> {achieved:.0f} GB/s; "
+        f"grad+flux {stage_bytes / N:.0f} B/cell/stage")
+
+    out = {
+        "metric": "cell-updates/s (fused HIP RK4 step, 2-D NS channel)",
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded Poisson-disk Delaunay mesh, freestream + Gaussian bump)",
+        "config": {"workload": "2-D compressible NS channel with circular obstacle, RK4 at CFL 0.5",
+                   "cells": N, "facets": F, "boundary_facets": Fb, "integrator": "Butcher RK4",
+                   "dt": dt, "finite": finite, "kernel_avg_us": avg_us},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+    }
+
+    if not a.no_cpu_baseline:
+        sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+        from oracle import fvm_oracle as O
+        ins, ex = cfg.inlet_cfg, cfg.exit_cfg
+        ph = O.Phys(T_0=cfg.T_0, viscosity=cfg.viscosity, visc_bulk=cfg.visc_bulk, thermal_cond=cfg.thermal_cond,
+                    S_const=cfg.S_const, gamma=cfg.gamma, C_v=cfg.C_v, v_factor=cfg.v_factor, lim_p=cfg.lim_p,
+                    lim_K=cfg.lim_K, inlet=O.BCSpec(ins.v_n_inf, ins.v_t_inf, ins.rho_inf, ins.T_inf, ins.v_inf),
+                    exit=O.BCSpec(ex.v_n_inf, ex.v_t_inf, ex.rho_inf, ex.T_inf, ex.v_inf))
+        threads = os.cpu_count() or 1
+        try:
+            threads = min(threads, len(os.sched_getaffinity(0)))
+        except AttributeError:
+            pass
+        torch.set_num_threads(threads)
+        _, ofvm = O.from_mesh_tuple(pts, cells, fi, fb, tags, ph)
+        stepper = O.OracleStepper(ofvm, "Butcher", "RK4", dt=dt)
+        Uc = U0.clone()
+        Uc = stepper.step(Uc)            # warm-up
+        t0 = time.perf_counter()
+        for _ in range(a.cpu_steps):
+            Uc = stepper.step(Uc)
+        cpu_s = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": N * a.cpu_steps / cpu_s, "unit": "cell-updates/s", "cores": threads,
+                               "kind": "port",
+                               "sample": f"{a.cpu_steps} RK4 steps of the same {N}-cell mesh (after 1 warm-up step), "
+                                         f"fp32 torch-CPU restatement of the reference, {cpu_s:.1f}s"}
+        log(f"cpu baseline: {cpu_s:.1f}s for {a.cpu_steps} steps on {threads} threads")
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,747 @@
+// Fused right-hand side of the compressible Navier-Stokes FVM step for gfx950.
+//
+// One RHS evaluation (FVMEquation.forward, fvm_equation.py:406-435) is four launches:
+//   k_bc_sum   per-group partial sums of the boundary speed of sound (deterministic)
+//   k_bc       boundary face values: A_bc.x + b_bc, the Neumann non-orthogonal
+//              correction with the previous call's limited gradients, then the
+//              characteristic BC (facet_boundary.py:124-168, 418-512)
+//   k_grad     per cell: LSQ gradient G.[U; U_bc] and the slope limiter
+//              (facet_process.py:387-418, limiter.py); stores the unlimited gradient and
+//              the limiter factor phi, from which every consumer rebuilds the limited
+//              face values and gradients with the reference's own operation order
+//   k_flux     per cell: for each of its facets (ascending facet index) the complete
+//              facet flux -- face reconstruction, normal-gradient correction, stress,
+//              advection, pressure, viscosity, heating, KT dissipation
+//              (facet_process.py:332-385, fvm_equation.py:62-368) -- summed into the
+//              divergence (flux_mat row), then an epilogue: write dU/dt, or apply an
+//              explicit Runge-Kutta stage update and emit the next stage's primitives.
+// Interior facet fluxes are computed by both neighbours with identical inputs and
+// identical instruction sequences, so the two contributions cancel exactly and no
+// atomics or facet-flux round trip through HBM are needed.
+//
+// This file is compiled with -ffp-contract=off: every product and sum is rounded
+// separately, in the order the reference's ATen op sequence performs them, so that the
+// branchy limiter sees the same inputs as the CPU restatement.
+#include "fvm_common.hpp"
+
+namespace fvm {
+
+constexpr int kThreads = 256;
+constexpr int kBcPart = 64;  // max blocks of k_bc_sum; bc_part holds [group][sum|count][64]
+
+template <int D>
+struct Dims {
+    static constexpr int C = D + 2;
+    static constexpr int NL = D + 1;
+    static constexpr int REC = (D + 1) * (D + 2);  // gp record: g[D][C] then phi[C]
+    static constexpr int GEO = D == 2 ? 8 : 12;
+    static constexpr int CH = 2 * D + 3;
+};
+
+__device__ __forceinline__ float lim_phi(int lp, float delta, float dU, float eps) {
+    if (lp == 4) {
+        const float a = fabsf(delta), b = fabsf(dU);
+        const float b2 = 2.0f * b;
+        const float a2 = a * a;
+        const float a_eps = a2 * a2 + eps;
+        const float S = b2 * (a2 + b2 * (b2 - a));
+        const float d3 = (delta * delta) * delta;
+        const float phi = (a_eps + a * S) / (a_eps + b * (d3 + S));
+        return a < b2 ? phi : 1.0f;
+    }
+    if (lp == 1) {
+        const float s = 2.0f * ((dU > 0.0f ? 1.0f : 0.0f) - 0.5f);
+        const float r = delta / (s * (fabsf(dU) + 1e-8f));
+        return fminf(fmaxf(r, 0.0f), 1.0f);
+    }
+    if (lp == 2) {
+        const float d2 = delta * delta;
+        return ((d2 + eps) + (2.0f * delta) * dU) / (((d2 + 2.0f * (dU * dU)) + delta * dU) + eps);
+    }
+    if (lp == 3) {
+        const float a = fabsf(delta), b = fabsf(dU);
+        const float a_eps = (a * a) * a + eps;
+        const float S = 4.0f * (b * b);
+        const float phi = (a_eps + a * S) / (a_eps + b * (delta * delta + S));
+        return a < 2.0f * b ? phi : 1.0f;
+    }
+    const float a = fabsf(delta), b = fabsf(dU);
+    const float a_eps = powf(a, 5.0f) + eps;
+    const float S = (8.0f * (b * b)) * (a * a - (2.0f * b) * (a - b));
+    const float phi = (a_eps + a * S) / (a_eps + b * (powf(delta, 4.0f) + S));
+    return a < 2.0f * b ? phi : 1.0f;
+}
+
+template <int D>
+__device__ __forceinline__ void load_prim(const float *__restrict__ src, float (&u)[D + 2]) {
+#pragma unroll
+    for (int j = 0; j < D + 2; ++j) u[j] = src[j];
+}
+
+template <int D>
+__device__ __forceinline__ void state_to_prim(const float (&U)[D + 2], float inv_cv, float (&P)[D + 2]) {
+    const float rho = U[D];
+    float sq = 0.0f;
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+        P[i] = U[i] / rho;
+        sq = (i == 0) ? P[i] * P[i] : sq + P[i] * P[i];
+    }
+    P[D] = rho;
+    P[D + 1] = inv_cv * (U[D + 1] / rho - 0.5f * sq);
+}
+
+// ------------------------------------------------------------------------------------
+// Boundary face values
+// ------------------------------------------------------------------------------------
+template <int D>
+__global__ void __launch_bounds__(kThreads) k_bc_sum(FvmMeshDev m, FvmPhys p, const float *__restrict__ prim,
+                                                     float *__restrict__ part) {
+    constexpr int C = D + 2;
+    __shared__ float ssum[kThreads];
+    __shared__ float scnt[kThreads];
+    for (int g = 0; g < m.n_groups; ++g) {
+        float s = 0.0f, n = 0.0f;
+        for (int64_t b = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x; b < m.n_bc;
+             b += static_cast<int64_t>(gridDim.x) * kThreads) {
+            if (m.bc_group[b] == g) {
+                const float T = prim[static_cast<int64_t>(m.bc_cell[b]) * C + D + 1];
+                s += sqrtf(p.gR * T);
+                n += 1.0f;
+            }
+        }
+        ssum[threadIdx.x] = s;
+        scnt[threadIdx.x] = n;
+        __syncthreads();
+        for (int w = kThreads / 2; w > 0; w >>= 1) {
+            if (threadIdx.x < w) {
+                ssum[threadIdx.x] += ssum[threadIdx.x + w];
+                scnt[threadIdx.x] += scnt[threadIdx.x + w];
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            part[g * 2 * kBcPart + blockIdx.x] = ssum[0];
+            part[g * 2 * kBcPart + kBcPart + blockIdx.x] = scnt[0];
+        }
+        __syncthreads();
+    }
+}
+
+template <int D>
+__global__ void __launch_bounds__(kThreads)
+k_bc(FvmMeshDev m, FvmPhys p, const float *__restrict__ prim, const float *__restrict__ gp, float *__restrict__ ubc,
+     const float *__restrict__ part, int nbp, int has_prev) {
+    using Dm = Dims<D>;
+    constexpr int C = Dm::C;
+    const int64_t b = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
+    if (b >= m.n_bc) return;
+    const int64_t cell = m.bc_cell[b];
+    float P[C];
+    load_prim<D>(prim + cell * C, P);
+    const int flags = m.bc_flags[b];
+    float u[C];
+#pragma unroll
+    for (int j = 0; j < C; ++j) {
+        const float v = m.bc_val[b * C + j];
+        u[j] = ((flags >> j) & 1) ? P[j] + v : v;
+    }
+    if (has_prev && flags) {
+        const float *g = gp + cell * Dm::REC;
+#pragma unroll
+        for (int j = 0; j < C; ++j) {
+            if ((flags >> j) & 1) {
+                const float ph = g[D * C + j];
+                float s = m.bc_l[b * D] * (g[j] * ph);
+#pragma unroll
+                for (int i = 1; i < D; ++i) s = s + m.bc_l[b * D + i] * (g[i * C + j] * ph);
+                u[j] = u[j] + s;
+            }
+        }
+    }
+    const int grp = m.bc_group[b];
+    if (grp >= 0) {
+        float sum = 0.0f, cnt = 0.0f;
+        for (int i = 0; i < nbp; ++i) {
+            sum += part[grp * 2 * kBcPart + i];
+            cnt += part[grp * 2 * kBcPart + kBcPart + i];
+        }
+        const float cmean = sum / cnt;
+        const float *ch = m.bc_char + b * Dm::CH;
+        float n[D], vt[D];
+        float vn = 0.0f;
+#pragma unroll
+        for (int i = 0; i < D; ++i) {
+            n[i] = ch[i];
+            vn = (i == 0) ? P[i] * n[i] : vn + P[i] * n[i];
+        }
+#pragma unroll
+        for (int i = 0; i < D; ++i) vt[i] = P[i] - vn * n[i];
+        const float rho = P[D], T = P[D + 1];
+        const float pr = (p.R * rho) * T;
+        const float c = sqrtf(p.gR * T);
+        const float drho = ch[D] - rho;
+        const float dvn = ch[D + 1] - vn;
+        const float dp = ch[D + 2] - pr;
+        const float inv_c = 1.0f / c;
+        const float inv_c2 = inv_c * inv_c;
+        const float h_dp = (0.5f * inv_c2) * dp;
+        const float h_dvn = ((0.5f * rho) * inv_c) * dvn;
+        float x0 = -h_dvn + h_dp;
+        float x1 = drho - inv_c2 * dp;
+        float x2 = h_dvn + h_dp;
+        const float scale = 10.0f / cmean;
+        const float g0 = 0.5f * (1.0f - tanhf(scale * (vn - c)));
+        const float g1 = 0.5f * (1.0f - tanhf(scale * vn));
+        const float g2 = 0.5f * (1.0f - tanhf(scale * (vn + c)));
+        x0 = g0 * x0;
+        x1 = g1 * x1;
+        x2 = g2 * x2;
+        const float c2 = c * c;
+        const float cor = c / rho;
+        const float rho_b = rho + ((x0 + x1) + x2);
+        const float vn_b = vn + ((-cor) * x0 + cor * x2);
+        const float p_b = pr + c2 * (x0 + x2);
+#pragma unroll
+        for (int i = 0; i < D; ++i) {
+            const float vtb = g1 * vt[i] + (1.0f - g1) * ch[D + 3 + i];
+            u[i] = vn_b * n[i] + vtb;
+        }
+        u[D] = rho_b;
+        u[D + 1] = p_b / (p.R * rho_b);
+    }
+#pragma unroll
+    for (int j = 0; j < C; ++j) ubc[b * C + j] = u[j];
+}
+
+// ------------------------------------------------------------------------------------
+// LSQ gradient + slope limiter
+// ------------------------------------------------------------------------------------
+template <int D>
+__global__ void __launch_bounds__(kThreads)
+k_grad(FvmMeshDev m, FvmPhys p, const float *__restrict__ prim, const float *__restrict__ ubc, float *__restrict__ gp) {
+    using Dm = Dims<D>;
+    constexpr int C = Dm::C, NL = Dm::NL;
+    const int64_t N = m.n_cells;
+    const int64_t c = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
+    if (c >= N) return;
+    float P[C];
+    load_prim<D>(prim + c * C, P);
+    float Un[NL][C];
+#pragma unroll
+    for (int k = 0; k < NL; ++k) {
+        const int64_t nb = m.cell_nbr[k * N + c];
+        const float *src = nb < N ? prim + nb * C : ubc + (nb - N) * C;
+        load_prim<D>(src, Un[k]);
+    }
+    const int info = m.cell_info[c];
+    float g[D][C];
+#pragma unroll
+    for (int jj = 0; jj <= NL; ++jj) {
+        const int sel = (info >> (16 + 3 * jj)) & 7;
+        float U[C];
+#pragma unroll
+        for (int j = 0; j < C; ++j) {
+            float v = P[j];
+#pragma unroll
+            for (int k = 0; k < NL; ++k) v = (sel == k) ? Un[k][j] : v;
+            U[j] = v;
+        }
+#pragma unroll
+        for (int i = 0; i < D; ++i) {
+            const float gv = m.cell_gval[(jj * D + i) * N + c];
+#pragma unroll
+            for (int j = 0; j < C; ++j) g[i][j] = (jj == 0) ? gv * U[j] : g[i][j] + gv * U[j];
+        }
+    }
+    float disp[NL][D];
+#pragma unroll
+    for (int k = 0; k < NL; ++k)
+#pragma unroll
+        for (int i = 0; i < D; ++i) disp[k][i] = m.cell_disp[(k * D + i) * N + c];
+    const float eps = m.cell_eps[c];
+    float phi[C];
+#pragma unroll
+    for (int j = 0; j < C; ++j) {
+        float mx = P[j], mn = P[j];
+#pragma unroll
+        for (int k = 0; k < NL; ++k) {
+            mx = fmaxf(mx, Un[k][j]);
+            mn = fminf(mn, Un[k][j]);
+        }
+        const float upper = mx - P[j], lower = mn - P[j];
+        float ph = 0.0f;
+#pragma unroll
+        for (int k = 0; k < NL; ++k) {
+            float dU = g[0][j] * disp[k][0];
+#pragma unroll
+            for (int i = 1; i < D; ++i) dU = dU + g[i][j] * disp[k][i];
+            const float lk = lim_phi(p.lim_p, dU > 0.0f ? upper : lower, dU, eps);
+            ph = (k == 0) ? lk : fminf(ph, lk);
+        }
+        phi[j] = ph;
+    }
+    float *o = gp + c * Dm::REC;
+#pragma unroll
+    for (int i = 0; i < D; ++i)
+#pragma unroll
+        for (int j = 0; j < C; ++j) o[i * C + j] = g[i][j];
+#pragma unroll
+    for (int j = 0; j < C; ++j) o[D * C + j] = phi[j];
+}
+
+// ------------------------------------------------------------------------------------
+// Facet flux (canonical slot order) -- identical for both neighbours of a facet
+// ------------------------------------------------------------------------------------
+template <int D>
+__device__ __forceinline__ void facet_flux(const FvmPhys &p, const float (&U0)[D + 2], const float (&U1)[D + 2],
+                                           const float (&gl0)[D][D + 1], const float (&gl1)[D][D + 1],
+                                           const float (&gdn)[D + 1], const float *__restrict__ geo,
+                                           float (&F)[D + 2]) {
+    float n[D], nh[D], X[D];
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+        n[i] = geo[i];
+        nh[i] = geo[D + i];
+        X[i] = geo[2 * D + 2 + i];
+    }
+    const float fs = geo[2 * D];
+    // Face gradient: mean of both sides' limited gradients, normal part replaced by the
+    // two-point difference with non-orthogonal correction (facet_process.py:362-385).
+    float gls[D][D + 1];
+#pragma unroll
+    for (int i = 0; i < D; ++i)
+#pragma unroll
+        for (int j = 0; j <= D; ++j) gls[i][j] = (gl0[i][j] + gl1[i][j]) * 0.5f;
+    float dFdn[D + 1], gdotn[D + 1];
+#pragma unroll
+    for (int j = 0; j <= D; ++j) {
+        float sx = gls[0][j] * X[0], sn = gls[0][j] * nh[0];
+#pragma unroll
+        for (int i = 1; i < D; ++i) {
+            sx = sx + gls[i][j] * X[i];
+            sn = sn + gls[i][j] * nh[i];
+        }
+        dFdn[j] = gdn[j] - sx;
+        gdotn[j] = sn;
+    }
+    float gV[D][D];
+#pragma unroll
+    for (int i = 0; i < D; ++i)
+#pragma unroll
+        for (int j = 0; j < D; ++j) gV[i][j] = gls[i][j] + (dFdn[j] - gdotn[j]) * nh[i];
+    const float gTn = dFdn[D];
+
+    // Conserved face values and normal velocities (facet_process.py:353-359)
+    float mom0[D], mom1[D], sq0 = 0.0f, sq1 = 0.0f, phi0 = 0.0f, phi1 = 0.0f;
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+        mom0[i] = U0[i] * U0[D];
+        mom1[i] = U1[i] * U1[D];
+        sq0 = (i == 0) ? U0[i] * U0[i] : sq0 + U0[i] * U0[i];
+        sq1 = (i == 0) ? U1[i] * U1[i] : sq1 + U1[i] * U1[i];
+        phi0 = (i == 0) ? U0[i] * n[i] : phi0 + U0[i] * n[i];
+        phi1 = (i == 0) ? U1[i] * n[i] : phi1 + U1[i] * n[i];
+    }
+    const float r0 = U0[D], r1 = U1[D], T0 = U0[D + 1], T1 = U1[D + 1];
+    const float Q0 = r0 * (p.cv * T0 + 0.5f * sq0);
+    const float Q1 = r1 * (p.cv * T1 + 0.5f * sq1);
+
+    // Stress tensor (FluidConstitution{2D,3D}._tau; 3-D divergence uses D00 + D11 only)
+    const float Tm = (T0 + T1) * 0.5f;
+    float Dm_[D][D];
+#pragma unroll
+    for (int i = 0; i < D; ++i)
+#pragma unroll
+        for (int j = 0; j < D; ++j) Dm_[i][j] = 0.5f * (gV[i][j] + gV[j][i]);
+    const float I1 = Dm_[0][0] + Dm_[1][1];
+    const float mu = ((p.mu0 * powf(Tm / p.T0, 1.5f)) * p.T0pS) / (Tm + p.S);
+    const float mub = (p.mub0 * (Tm * Tm)) / p.T0sq;
+    const float a0 = (-mub) * I1;
+    const float a1 = -2.0f * mu;
+    float tau[D][D];
+#pragma unroll
+    for (int i = 0; i < D; ++i)
+#pragma unroll
+        for (int j = 0; j < D; ++j) tau[i][j] = (i == j ? a0 : 0.0f) + a1 * Dm_[i][j];
+
+    const float P0 = (p.R * r0) * T0, P1 = (p.R * r1) * T1;
+    const float c0 = sqrtf(p.gR * T0), c1 = sqrtf(p.gR * T1);
+
+    // Advection (fvm_equation.py:210-231) and pressure (:309-321)
+#pragma unroll
+    for (int i = 0; i < D; ++i) F[i] = (mom0[i] * phi0 + mom1[i] * phi1) * 0.5f;
+    F[D] = (r0 * phi0 + r1 * phi1) * 0.5f;
+    F[D + 1] = ((Q0 + P0) * phi0 + (Q1 + P1) * phi1) * 0.5f;
+    const float Pm = (P0 + P1) * 0.5f;
+#pragma unroll
+    for (int i = 0; i < D; ++i) F[i] = F[i] + Pm * n[i];
+    // Viscosity (:246-257)
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+        float v = tau[0][j] * n[0];
+#pragma unroll
+        for (int i = 1; i < D; ++i) v = v + tau[i][j] * n[i];
+        F[j] = F[j] + v;
+    }
+    // Viscous heating and conduction (:272-294)
+    float Vb[D];
+#pragma unroll
+    for (int j = 0; j < D; ++j) Vb[j] = (U0[j] + U1[j]) * 0.5f;
+    float heat = 0.0f;
+#pragma unroll
+    for (int i = 0; i < D; ++i)
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+            const float t = (tau[i][j] * Vb[j]) * n[i];
+            heat = (i == 0 && j == 0) ? t : heat + t;
+        }
+    heat = heat + ((-p.kappa) * gTn) * fs;
+    F[D + 1] = F[D + 1] + heat;
+    // Kurganov-Tadmor dissipation (:337-368)
+    const float Vn0 = sqrtf(sq0), Vn1 = sqrtf(sq1);
+    const float Vmax = fmaxf(Vn0, Vn1);
+    const float cm = fmaxf(c0, c1);
+    const float M = fabsf(Vmax / (cm + 1e-8f));
+    const float vf = fminf(fmaxf(M, p.v_factor), 1.0f);
+    const float a_vel = vf * cm + Vmax;
+    const float a_oth = cm + Vmax;
+#pragma unroll
+    for (int i = 0; i < D; ++i) F[i] = F[i] + (0.5f * (a_vel * (mom0[i] - mom1[i]))) * fs;
+    F[D] = F[D] + (0.5f * (a_oth * (r0 - r1))) * fs;
+    F[D + 1] = F[D + 1] + (0.5f * (a_oth * (Q0 - Q1))) * fs;
+}
+
+// Limited face value and limited gradient (grad components V..., T) of one cell side.
+template <int D>
+__device__ __forceinline__ void side_values(const float (&P)[D + 2], const float *__restrict__ g,
+                                            const float (&disp)[D], float (&U)[D + 2], float (&gl)[D][D + 1]) {
+    constexpr int C = D + 2;
+#pragma unroll
+    for (int j = 0; j < C; ++j) {
+        const float ph = g[D * C + j];
+        float dU = g[j] * disp[0];
+#pragma unroll
+        for (int i = 1; i < D; ++i) dU = dU + g[i * C + j] * disp[i];
+        U[j] = P[j] + dU * ph;
+    }
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+#pragma unroll
+        for (int j = 0; j < D; ++j) gl[i][j] = g[i * C + j] * g[D * C + j];
+        gl[i][D] = g[i * C + D + 1] * g[D * C + D + 1];
+    }
+}
+
+enum { EPI_DUDT = 0, EPI_STAGE = 1 };
+
+template <int D, int EPI>
+__global__ void __launch_bounds__(kThreads)
+k_flux(FvmMeshDev m, FvmPhys p, const float *__restrict__ prim, const float *__restrict__ gp,
+       const float *__restrict__ ubc, float *__restrict__ dudt, FvmStage st, double *__restrict__ red) {
+    using Dm = Dims<D>;
+    constexpr int C = Dm::C, NL = Dm::NL;
+    const int64_t N = m.n_cells;
+    const int64_t c = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
+    const bool live = c < N;
+    float acc[C];
+#pragma unroll
+    for (int j = 0; j < C; ++j) acc[j] = 0.0f;
+    if (live) {
+        float Pc[C];
+        load_prim<D>(prim + c * C, Pc);
+        const float *gc = gp + c * Dm::REC;
+        const int info = m.cell_info[c];
+        const float inv_vol = m.cell_inv_vol[c];
+#pragma unroll
+        for (int k = 0; k < NL; ++k) {
+            const int64_t f = m.cell_facet[k * N + c];
+            const int64_t nb = m.cell_nbr[k * N + c];
+            const int own_slot = (info >> k) & 1;
+            float dc[D];
+#pragma unroll
+            for (int i = 0; i < D; ++i) dc[i] = m.cell_disp[(k * D + i) * N + c];
+            float Uo[C], glo[D][D + 1];
+            side_values<D>(Pc, gc, dc, Uo, glo);
+            float Ux[C], glx[D][D + 1], gdn[D + 1];
+            const float *geo = m.fac_geo + f * Dm::GEO;
+            if (nb < N) {
+                float Pn[C];
+                load_prim<D>(prim + nb * C, Pn);
+                const int kn = (info >> (8 + 2 * k)) & 3;
+                float dn[D];
+#pragma unroll
+                for (int i = 0; i < D; ++i) dn[i] = m.cell_disp[(kn * D + i) * N + nb];
+                side_values<D>(Pn, gp + nb * Dm::REC, dn, Ux, glx);
+                // A_face_grad row: (-1/d, +1/d) on (slot-0, slot-1) cells, ascending columns
+                const float inv_d = geo[2 * D + 1];
+                const bool own_left = own_slot == 0;
+                const bool left_first = own_left ? (c < nb) : (nb < c);
+#pragma unroll
+                for (int jg = 0; jg <= D; ++jg) {
+                    const int j = jg < D ? jg : D + 1;
+                    const float pl = own_left ? Pc[j] : Pn[j];
+                    const float pr = own_left ? Pn[j] : Pc[j];
+                    const float tl = (-inv_d) * pl, tr = inv_d * pr;
+                    gdn[jg] = (left_first ? tl + tr : tr + tl) + 0.0f;
+                }
+            } else {
+                const int64_t b = nb - N;
+                load_prim<D>(ubc + b * C, Ux);
+#pragma unroll
+                for (int i = 0; i < D; ++i)
+#pragma unroll
+                    for (int j = 0; j <= D; ++j) glx[i][j] = glo[i][j];
+#pragma unroll
+                for (int jg = 0; jg <= D; ++jg) {
+                    const int j = jg < D ? jg : D + 1;
+                    gdn[jg] = m.bc_ga[b * C + j] * Pc[j] + m.bc_gb[b * C + j];
+                }
+            }
+            float F[C];
+            if (own_slot == 0)
+                facet_flux<D>(p, Uo, Ux, glo, glx, gdn, geo, F);
+            else
+                facet_flux<D>(p, Ux, Uo, glx, glo, gdn, geo, F);
+            const float v = own_slot ? inv_vol : -inv_vol;
+#pragma unroll
+            for (int j = 0; j < C; ++j) acc[j] = acc[j] + v * F[j];
+        }
+    }
+    if (EPI == EPI_DUDT) {
+        if (live)
+#pragma unroll
+            for (int j = 0; j < C; ++j) dudt[c * C + j] = acc[j];
+        return;
+    }
+    // ---------------- explicit RK stage epilogue ----------------
+    double sq[3 * C];
+#pragma unroll
+    for (int j = 0; j < 3 * C; ++j) sq[j] = 0.0;
+    if (live) {
+        const float dt = *st.dt;
+        const int64_t o = c * C;
+        float kk[C], a[C], U0[C];
+#pragma unroll
+        for (int j = 0; j < C; ++j) {
+            kk[j] = dt * acc[j];
+            U0[j] = st.U0[o + j];
+            a[j] = st.stage == 0 ? st.b * kk[j] : st.acc[o + j] + st.b * kk[j];
+        }
+        if (st.store_k)
+#pragma unroll
+            for (int j = 0; j < C; ++j) st.kbuf[static_cast<int64_t>(st.stage) * N * C + o + j] = kk[j];
+        float a2[C];
+        if (st.acc2) {
+#pragma unroll
+            for (int j = 0; j < C; ++j) a2[j] = st.stage == 0 ? st.b2 * kk[j] : st.acc2[o + j] + st.b2 * kk[j];
+        }
+        float Un[C], Pn[C];
+        if (!st.last) {
+#pragma unroll
+            for (int j = 0; j < C; ++j) {
+                float inc = 0.0f;
+                for (int t = 0; t < st.n_inc; ++t) {
+                    const int sidx = st.inc_idx[t];
+                    const float kv = sidx == st.stage ? kk[j] : st.kbuf[static_cast<int64_t>(sidx) * N * C + o + j];
+                    const float term = st.inc_coef[t] * kv;
+                    inc = t == 0 ? term : inc + term;
+                }
+                Un[j] = U0[j] + inc;
+                st.acc[o + j] = a[j];
+                if (st.acc2) st.acc2[o + j] = a2[j];
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < C; ++j) {
+                Un[j] = U0[j] + a[j];
+                st.U_out[o + j] = Un[j];
+                if (st.acc2) {
+                    const double dd = static_cast<double>(a[j] - a2[j]);
+                    sq[j] = dd * dd;
+                    sq[C + j] = static_cast<double>(a[j]) * a[j];
+                    sq[2 * C + j] = static_cast<double>(U0[j]) * U0[j];
+                }
+            }
+        }
+        state_to_prim<D>(Un, p.inv_cv, Pn);
+#pragma unroll
+        for (int j = 0; j < C; ++j) st.prim_next[o + j] = Pn[j];
+    }
+    if (st.last && st.acc2) {
+        // fp64 block partials of the three column norms (Adaptive.update_stepsize)
+        __shared__ double sred[kThreads];
+        for (int q = 0; q < 3 * C; ++q) {
+            sred[threadIdx.x] = sq[q];
+            __syncthreads();
+            for (int w = kThreads / 2; w > 0; w >>= 1) {
+                if (threadIdx.x < w) sred[threadIdx.x] += sred[threadIdx.x + w];
+                __syncthreads();
+            }
+            if (threadIdx.x == 0) red[static_cast<int64_t>(blockIdx.x) * 3 * C + q] = sred[0];
+            __syncthreads();
+        }
+    }
+}
+
+__global__ void k_update_stepsize(const double *__restrict__ red, int C, int nblk, float atol, float rtol, float mtol,
+                                  float order, float a_gt, float a_le, float *dt) {
+    // one wave: lane q < 3C sums its column over the blocks in a fixed order
+    __shared__ double tot[64];
+    const int q = threadIdx.x;
+    double s = 0.0;
+    if (q < 3 * C)
+        for (int b = 0; b < nblk; ++b) s += red[static_cast<int64_t>(b) * 3 * C + q];
+    tot[q] = s;
+    __syncthreads();
+    if (q == 0) {
+        float E = 0.0f;
+        for (int j = 0; j < C; ++j) {
+            const float nd = static_cast<float>(sqrt(tot[j]));
+            const float nh = static_cast<float>(sqrt(tot[C + j]));
+            const float nu = static_cast<float>(sqrt(tot[2 * C + j]));
+            const float e = nd / ((atol + rtol * nh) + mtol * nu);
+            E = j == 0 ? e : E + e;
+        }
+        E = E / static_cast<float>(C);
+        const float factor = 0.9f * powf(E, -1.0f / order);
+        const float alpha = E > 1.0f ? a_gt : a_le;
+        *dt = *dt * (alpha + (1.0f - alpha) * factor);
+    }
+}
+
+__global__ void __launch_bounds__(kThreads) k_state_to_prim2(const float *__restrict__ s, float *__restrict__ p,
+                                                             int64_t N, float inv_cv) {
+    const int64_t c = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
+    if (c >= N) return;
+    float U[4], P[4];
+    load_prim<2>(s + c * 4, U);
+    state_to_prim<2>(U, inv_cv, P);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) p[c * 4 + j] = P[j];
+}
+
+__global__ void __launch_bounds__(kThreads) k_state_to_prim3(const float *__restrict__ s, float *__restrict__ p,
+                                                             int64_t N, float inv_cv) {
+    const int64_t c = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
+    if (c >= N) return;
+    float U[5], P[5];
+    load_prim<3>(s + c * 5, U);
+    state_to_prim<3>(U, inv_cv, P);
+#pragma unroll
+    for (int j = 0; j < 5; ++j) p[c * 5 + j] = P[j];
+}
+
+static int check_mesh(const FvmMeshDev *m, const FvmPhys *p, const FvmWork *w, const char *who) {
+    if (!m || !p || !w || (m->dim != 2 && m->dim != 3) || m->n_cells <= 0 || m->n_groups < 0 || m->n_groups > 4 ||
+        m->n_bc < 0 || !w->gp || (m->n_bc > 0 && !w->ubc) || (m->n_groups > 0 && !w->bc_part) ||
+        p->lim_p < 1 || p->lim_p > 5 || m->n_cells > (int64_t(1) << 31) - 1) {
+        set_error("%s: invalid mesh/work/phys arguments", who);
+        return FVM_ERR_ARG;
+    }
+    return FVM_OK;
+}
+
+template <int D>
+static int launch_rhs(const FvmMeshDev &m, const FvmPhys &p, const FvmWork &w, const float *prim, float *dudt,
+                      const FvmStage *st, hipStream_t s, int has_prev, int parts = 15) {
+    const int64_t nbp_full = (m.n_bc + kThreads - 1) / kThreads;
+    const int nbp = static_cast<int>(nbp_full < kBcPart ? nbp_full : kBcPart);
+    if ((parts & 1) && m.n_groups > 0 && nbp > 0)
+        hipLaunchKernelGGL(k_bc_sum<D>, dim3(nbp), dim3(kThreads), 0, s, m, p, prim, w.bc_part);
+    if ((parts & 2) && m.n_bc > 0)
+        hipLaunchKernelGGL(k_bc<D>, dim3(blocks_for(m.n_bc, kThreads)), dim3(kThreads), 0, s, m, p, prim, w.gp, w.ubc,
+                           w.bc_part, nbp, has_prev);
+    const dim3 grid(blocks_for(m.n_cells, kThreads));
+    if (parts & 4) hipLaunchKernelGGL(k_grad<D>, grid, dim3(kThreads), 0, s, m, p, prim, w.ubc, w.gp);
+    if (!(parts & 8)) return check_launch("fvm_rk_stage_part_f32");
+    if (st) {
+        hipLaunchKernelGGL((k_flux<D, EPI_STAGE>), grid, dim3(kThreads), 0, s, m, p, prim, w.gp, w.ubc, nullptr, *st,
+                           w.red);
+    } else {
+        FvmStage none{};
+        hipLaunchKernelGGL((k_flux<D, EPI_DUDT>), grid, dim3(kThreads), 0, s, m, p, prim, w.gp, w.ubc, dudt, none,
+                           nullptr);
+    }
+    return check_launch(st ? "fvm_rk_stage_f32" : "fvm_rhs_f32");
+}
+
+}  // namespace fvm
+
+using namespace fvm;
+
+extern "C" {
+
+int fvm_rhs_f32(const FvmMeshDev *m, const FvmPhys *p, const FvmWork *w, const float *prim, float *dudt,
+                int32_t has_prev, void *stream) {
+    int e = check_mesh(m, p, w, "fvm_rhs_f32");
+    if (e) return e;
+    if (!prim || !dudt) {
+        set_error("fvm_rhs_f32: null prim/dudt");
+        return FVM_ERR_ARG;
+    }
+    return m->dim == 2 ? launch_rhs<2>(*m, *p, *w, prim, dudt, nullptr, as_stream(stream), has_prev)
+                       : launch_rhs<3>(*m, *p, *w, prim, dudt, nullptr, as_stream(stream), has_prev);
+}
+
+int fvm_rk_stage_f32(const FvmMeshDev *m, const FvmPhys *p, const FvmWork *w, const FvmStage *st, void *stream) {
+    int e = check_mesh(m, p, w, "fvm_rk_stage_f32");
+    if (e) return e;
+    if (!st || !st->prim || !st->U0 || !st->acc || !st->prim_next || !st->dt || (st->last && !st->U_out) ||
+        st->n_inc < 0 || st->n_inc > 10 || (st->store_k && !st->kbuf) || (st->last && st->acc2 && !w->red)) {
+        set_error("fvm_rk_stage_f32: invalid stage arguments");
+        return FVM_ERR_ARG;
+    }
+    for (int t = 0; t < st->n_inc; ++t)
+        if (st->inc_idx[t] < 0 || st->inc_idx[t] > st->stage || (st->inc_idx[t] != st->stage && !st->kbuf)) {
+            set_error("fvm_rk_stage_f32: increment refers to a stage that is not stored");
+            return FVM_ERR_ARG;
+        }
+    return m->dim == 2 ? launch_rhs<2>(*m, *p, *w, st->prim, nullptr, st, as_stream(stream), st->has_prev)
+                       : launch_rhs<3>(*m, *p, *w, st->prim, nullptr, st, as_stream(stream), st->has_prev);
+}
+
+int fvm_rk_stage_part_f32(const FvmMeshDev *m, const FvmPhys *p, const FvmWork *w, const FvmStage *st,
+                          int32_t parts, void *stream) {
+    if (parts < 1 || parts > 15) {
+        set_error("fvm_rk_stage_part_f32: parts must be a nonzero mask of 1|2|4|8");
+        return FVM_ERR_ARG;
+    }
+    int e = check_mesh(m, p, w, "fvm_rk_stage_part_f32");
+    if (e) return e;
+    if (!st || !st->prim || !st->U0 || !st->acc || !st->prim_next || !st->dt || (st->last && !st->U_out) ||
+        st->n_inc < 0 || st->n_inc > 10 || (st->store_k && !st->kbuf) || (st->last && st->acc2 && !w->red)) {
+        set_error("fvm_rk_stage_part_f32: invalid stage arguments");
+        return FVM_ERR_ARG;
+    }
+    return m->dim == 2 ? launch_rhs<2>(*m, *p, *w, st->prim, nullptr, st, as_stream(stream), st->has_prev, parts)
+                       : launch_rhs<3>(*m, *p, *w, st->prim, nullptr, st, as_stream(stream), st->has_prev, parts);
+}
+
+int fvm_update_stepsize(const FvmWork *w, int32_t C, int32_t nblk, float atol, float rtol, float mtol, float order,
+                        float a_gt, float a_le, float *dt, void *stream) {
+    if (!w || !w->red || !dt || C < 1 || 3 * C > 64 || nblk < 1) {
+        set_error("fvm_update_stepsize: invalid arguments");
+        return FVM_ERR_ARG;
+    }
+    hipLaunchKernelGGL(k_update_stepsize, dim3(1), dim3(64), 0, as_stream(stream), w->red, C, nblk, atol, rtol, mtol,
+                       order, a_gt, a_le, dt);
+    return check_launch("fvm_update_stepsize");
+}
+
+int fvm_state_to_prim_f32(const float *state, float *prim, int64_t N, int32_t dim, float inv_cv, void *stream) {
+    if (!state || !prim || N < 0 || (dim != 2 && dim != 3)) {
+        set_error("fvm_state_to_prim_f32: invalid arguments");
+        return FVM_ERR_ARG;
+    }
+    if (N == 0) return FVM_OK;
+    if (dim == 2)
+        hipLaunchKernelGGL(k_state_to_prim2, dim3(blocks_for(N, kThreads)), dim3(kThreads), 0, as_stream(stream),
+                           state, prim, N, inv_cv);
+    else
+        hipLaunchKernelGGL(k_state_to_prim3, dim3(blocks_for(N, kThreads)), dim3(kThreads), 0, as_stream(stream),
+                           state, prim, N, inv_cv);
+    return check_launch("fvm_state_to_prim_f32");
+}
+
+}  // extern "C"

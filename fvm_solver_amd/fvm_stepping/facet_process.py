@@ -1,0 +1,306 @@
+"""Device mesh + fused RHS driver -- replaces ``time_fvm/fvm_stepping/facet_process.py``.
+
+``FacetCalc`` keeps the reference's constructor (facet_process.py:261) and role: it owns
+the device-resident mesh operators and evaluates the face/flux pipeline.  Instead of the
+reference's SPM objects and ~150 ATen launches per RHS, the operators are packed into the
+per-cell / per-facet arrays of ``FvmMeshDev`` (include/fvm_hip.h) and one RHS is four
+HIP launches (see csrc/rhs.hip).  What the reference assembles as
+
+    flux_mat    (facet_process.py:200-226)   -> cell_inv_vol + slot bits, facets ascending
+    G_mats      (fvm_mesh.py:5-42, 263-344)  -> cell_gval in ascending column order
+    A_face_grad (facet_process.py:114-198)   -> fac_geo[1/(n.d)] + bc_ga/bc_gb
+    A_bc, b_bc  (facet_boundary.py:238-280)  -> bc_flags + bc_val
+    A_corr      (facet_boundary.py:170-236)  -> bc_l
+    BC groups   (facet_boundary.py:32-80, 289-385) -> bc_group + bc_char
+
+holds the same numbers (float32 values rounded from the float64 geometry).  State kept
+between calls: the unlimited gradient and limiter factor of the last call (``work.gp``),
+from which ``cell_grads`` (= the reference's limited gradients) is derived on request.
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from fvm_solver_amd import _native as nat
+from fvm_solver_amd.config_fvm import BCMode
+
+_K_BC_PART = 64
+
+
+def _f32(x):
+    return np.asarray(x, dtype=np.float32)
+
+
+def _bc_tables(bc_tags, C):
+    """Flags and values per boundary facet and component from the Facet descriptors
+    (FacetCalc._init_bc, facet_process.py:297-330)."""
+    nb = len(bc_tags)
+    dm = np.zeros((nb, C), bool)
+    nm = np.zeros((nb, C), bool)
+    dv = np.full((nb, C), np.nan, np.float32)
+    nv = np.full((nb, C), np.nan, np.float32)
+    ff = np.zeros(nb, bool)
+    il = np.zeros(nb, bool)
+    for i, (k, e) in enumerate(bc_tags.items()):
+        if k != i:
+            raise ValueError("bc_tags must be keyed 0..F_b-1 in boundary-facet order")
+        dm[i] = e.dirichlet()
+        nm[i] = e.neumann()
+        dv[i] = np.asarray(e.U, np.float32)
+        nv[i] = np.asarray(e.dUdn, np.float32)
+        ff[i] = e.farfield()
+        il[i] = e.inlet()
+    return dm, nm, dv, nv, ff, il
+
+
+class DeviceMesh:
+    """Builds the FvmMeshDev arrays on ``device`` from an FVMMesh and the BC tags."""
+
+    def __init__(self, mesh, n_comp: int, bc_tags, cfg, device):
+        g = mesh.geo64
+        D = mesh.dim
+        C = n_comp
+        if C != D + 2:
+            raise ValueError(f"n_comp must be dim + 2 (got {C} for dim {D})")
+        NL = D + 1
+        N, F, Fb = mesh.n_cells, mesh.n_facets, mesh.n_bc_facet
+        if N >= 2 ** 31 - 1 or F >= 2 ** 31 - 1:
+            raise ValueError("mesh too large for int32 indices")
+        self.dim, self.C, self.N, self.F, self.Fb = D, C, N, F, Fb
+        bcm = mesh.bc_facet_mask.numpy()
+        c2f, signs, comb = g["c2f"], g["signs"], g["combined"]
+        slots = (signs == -1).astype(np.int64)           # int(-s + 1/2), facet_process.py:71
+
+        # ---- per-cell lists in ascending facet order (flux_mat ELL order) ----
+        perm = np.argsort(c2f, axis=1, kind="stable")
+        invp = np.argsort(perm, axis=1, kind="stable")
+        c2f_s = np.take_along_axis(c2f, perm, 1)
+        nbr_s = np.take_along_axis(comb, perm, 1)
+        slot_s = np.take_along_axis(slots, perm, 1)
+        disp_s = np.take_along_axis(g["cent_to_facet_disp"], perm[..., None], 1)
+        flat = c2f_s.reshape(-1)
+        o = np.argsort(flat, kind="stable")
+        counts = np.bincount(flat, minlength=F)
+        start = np.concatenate([[0], np.cumsum(counts)[:-1]])
+        two = counts == 2
+        pa = o[start[two]]
+        pb = o[start[two] + 1]
+        nbk = np.zeros(N * NL, np.int64)
+        nbk[pa] = pb % NL
+        nbk[pb] = pa % NL
+        nbk = nbk.reshape(N, NL)
+
+        # ---- LSQ rows in ascending column order (G_mats ELL order) ----
+        G32 = g["G"].astype(np.float32)                   # [N, D, NL], reference local order
+        diag = G32[:, :, 0].copy()
+        for k in range(1, NL):
+            diag = (diag + G32[:, :, k]).astype(np.float32)
+        cols = np.concatenate([comb, np.arange(N)[:, None]], 1)
+        vals = np.concatenate([G32, -diag[:, :, None]], 2)    # [N, D, NL+1]
+        eo = np.argsort(cols, axis=1, kind="stable")
+        gval = np.take_along_axis(vals, eo[:, None, :], 2)    # [N, D, NL+1]
+        src = np.where(eo == NL, 7, np.take_along_axis(invp, np.minimum(eo, NL - 1), 1))
+
+        info = np.zeros(N, np.int64)
+        for k in range(NL):
+            info |= slot_s[:, k] << k
+            info |= nbk[:, k] << (8 + 2 * k)
+        for j in range(NL + 1):
+            info |= src[:, j].astype(np.int64) << (16 + 3 * j)
+
+        vol32 = g["volumes"].astype(np.float32)
+        inv_vol = (np.float32(1.0) / vol32).astype(np.float32)
+        eps = np.power(np.float32(cfg.lim_K) * np.sqrt(vol32), np.float32(cfg.lim_p + 1)).astype(np.float32)
+
+        # ---- per-facet geometry ----
+        n64, nh64 = g["normals"], g["normals_hat"]
+        fs64 = np.linalg.norm(n64, axis=1)
+        cen = g["centroids"]
+        GEO = 8 if D == 2 else 12
+        geo = np.zeros((F, GEO), np.float32)
+        geo[:, :D] = n64
+        geo[:, D:2 * D] = nh64
+        geo[:, 2 * D] = fs64
+        inter = ~bcm
+        cd = cen[g["right"][inter]] - cen[g["left"][inter]]
+        dproj = np.einsum("nd,nd->n", nh64[inter], cd)
+        geo[inter, 2 * D + 1] = 1.0 / dproj
+        geo[inter, 2 * D + 2:3 * D + 2] = cd / dproj[:, None] - nh64[inter]
+
+        # ---- boundary facets ----
+        dm, nm, dv, nv, ffm, ilm = _bc_tables(bc_tags, C)
+        if len(dm) != Fb:
+            raise ValueError(f"{len(dm)} BC tags for {Fb} boundary facets")
+        fidx = np.flatnonzero(bcm)
+        bcell = g["left"][bcm]
+        kloc = np.argmax(c2f[bcell] == fidx[:, None], axis=1)
+        cell_slot = slots[bcell, kloc]
+        side = 1 - cell_slot                                   # the free slot (facet_process.py:98-101)
+        mid = g["midpoints"][bcm]
+        dist64 = np.einsum("nd,nd->n", mid - cen[bcell], nh64[bcm])
+        dist32 = dist64.astype(np.float32)
+        nv_masked = nv[nm]                                      # Neumann values, row-major entry order
+        comp = np.broadcast_to(np.arange(C), (Fb, C))
+        bc_val = np.zeros((Fb, C), np.float32)
+        bc_val[dm] = dv[dm]
+        bc_ga = np.zeros((Fb, C), np.float32)
+        bc_gb = np.zeros((Fb, C), np.float32)
+        if dm.any():
+            dd = np.broadcast_to(dist32[:, None], (Fb, C))[dm]
+            bc_ga[dm] = np.float32(-1.0) / dd
+            bc_gb[dm] = dv[dm] / dd
+        if nm.any():
+            # b_bc and b_face_grad index the masked Neumann values by component
+            # (facet_boundary.py:278, facet_process.py:192); kept as the reference does it.
+            if comp[nm].max() >= len(nv_masked):
+                raise IndexError("Neumann values indexed by component out of range")
+            bc_val[nm] = nv_masked[comp[nm]] * np.broadcast_to(dist32[:, None], (Fb, C))[nm]
+            bc_gb[nm] = nv_masked[comp[nm]]
+        flags = (nm * (1 << np.arange(C))).sum(1).astype(np.int32)
+        r = g["cent_to_facet_disp"][bcell, kloc]
+        nhb = nh64[bcm]
+        l_vec = r - nhb * np.einsum("nd,nd->n", r, nhb)[:, None]
+        l_vec[~nm.any(1)] = 0.0
+
+        # ---- characteristic BC groups (facet_boundary.py:32-80, 289-385) ----
+        nh32 = nhb.astype(np.float32)
+        sgn = (np.float32(2.0) * (side.astype(np.float32) - np.float32(0.5))).astype(np.float32)
+        n_out = (nh32 * sgn[:, None]).astype(np.float32)
+        group = np.full(Fb, -1, np.int32)
+        CH = 2 * D + 3
+        bc_char = np.zeros((Fb, CH), np.float32)
+        modes = []
+        R32 = np.float32(cfg.C_v * (cfg.gamma - 1))
+        for mask, bcfg in ((ffm, cfg.exit_cfg), (ilm, cfg.inlet_cfg)):
+            if not mask.any():
+                continue
+            if bcfg.mode != BCMode.Characteristic:
+                raise NotImplementedError(f"Unknown BCMode {bcfg.mode}: only the characteristic BC is supported "
+                                          "(the reference's Farfield mode is broken and Isentropic is unimplemented)")
+            if bcfg.mode not in modes:
+                modes.append(bcfg.mode)
+            group[mask] = modes.index(bcfg.mode)
+            nn = n_out[mask]
+            rho = np.float32(bcfg.rho_inf)
+            T = np.float32(bcfg.T_inf)
+            bc_char[mask, :D] = nn
+            bc_char[mask, D] = rho
+            bc_char[mask, D + 2] = (R32 * rho) * T
+            if D == 2:
+                bc_char[mask, D + 1] = np.float32(-bcfg.v_n_inf)
+                tang = np.stack([-nn[:, 1], nn[:, 0]], 1)
+                bc_char[mask, D + 3:] = tang * np.float32(bcfg.v_t_inf)
+            else:
+                vinf = _f32(bcfg.v_inf)
+                vn = (nn[:, 0] * vinf[0] + nn[:, 1] * vinf[1]) + nn[:, 2] * vinf[2]
+                bc_char[mask, D + 1] = vn
+                bc_char[mask, D + 3:] = vinf[None, :] - nn * vn[:, None]
+        self.n_groups = len(modes)
+
+        # ---- upload ----
+        dev = torch.device(device)
+
+        def up(a, dt):
+            return torch.from_numpy(np.ascontiguousarray(a)).to(dtype=dt, device=dev)
+
+        i32, f32 = torch.int32, torch.float32
+        self.t = dict(
+            cell_facet=up(c2f_s.T, i32), cell_nbr=up(nbr_s.T, i32), cell_info=up(info, i32),
+            cell_disp=up(disp_s.transpose(1, 2, 0), f32), cell_gval=up(gval.transpose(2, 1, 0), f32),
+            cell_inv_vol=up(inv_vol, f32), cell_eps=up(eps, f32), fac_geo=up(geo, f32),
+            bc_cell=up(bcell, i32), bc_group=up(group, i32), bc_flags=up(flags, i32), bc_val=up(bc_val, f32),
+            bc_l=up(l_vec, f32), bc_ga=up(bc_ga, f32), bc_gb=up(bc_gb, f32), bc_char=up(bc_char, f32))
+        self.host = dict(c2f_s=c2f_s, nbr_s=nbr_s, slot_s=slot_s, nbk=nbk, gval=gval, src=src, info=info,
+                         inv_vol=inv_vol, eps=eps, geo=geo, bcell=bcell, group=group, flags=flags, bc_val=bc_val,
+                         bc_l=l_vec.astype(np.float32), bc_ga=bc_ga, bc_gb=bc_gb, bc_char=bc_char, side=side)
+        if dev.type != "cuda":      # host-only layout (inspection and tests)
+            self.t, self.struct = None, None
+            return
+        self.volumes64 = up(g["volumes"], torch.float64)
+        per = fs64[c2f].sum(1)
+        self.h64 = up(D * g["volumes"] / per, torch.float64)     # CFL length d*vol/sum|n|
+        self.struct = nat.FvmMeshDev(
+            dim=D, n_groups=self.n_groups, n_cells=N, n_facets=F, n_bc=Fb,
+            **{k: v.data_ptr() for k, v in self.t.items()})
+        # views used by tests and the Saver
+        self.slots_sorted = slot_s
+        self.bc_facet_side = side
+        self.bc_cell_np = bcell
+
+
+def phys_struct(cfg) -> "nat.FvmPhys":
+    """Scalars of FluidConstitution / SlopeLimiter / KTDiffusion, rounded to float32 from
+    the double-precision Python expressions the reference evaluates them with."""
+    R = cfg.C_v * (cfg.gamma - 1)
+    return nat.FvmPhys(inv_cv=1 / cfg.C_v, cv=cfg.C_v, R=R, gR=cfg.gamma * R, mu0=cfg.viscosity, T0=cfg.T_0,
+                       T0pS=cfg.T_0 + cfg.S_const, S=cfg.S_const, mub0=cfg.visc_bulk, T0sq=cfg.T_0 ** 2,
+                       kappa=cfg.thermal_cond, v_factor=cfg.v_factor, lim_p=int(cfg.lim_p))
+
+
+class FacetCalc:
+    """Face/flux pipeline owner (facet_process.py:229-450) on the GPU."""
+
+    def __init__(self, phy_setup, cfg, mesh_setup, n_comp: int, bc_tags, device="cuda"):
+        self.device = torch.device(device)
+        if self.device.type != "cuda":
+            raise RuntimeError("FacetCalc runs on a GPU device only")
+        nat.load_library()
+        self.cfg = cfg
+        self.phy_setup = phy_setup
+        self.n_comp = n_comp
+        self.dim = mesh_setup.dim
+        self.n_local = self.dim + 1
+        self.n_cells = mesh_setup.n_cells
+        self.n_facets = mesh_setup.n_facets
+        self.n_facets_bc = mesh_setup.n_bc_facet
+        self.mesh = DeviceMesh(mesh_setup, n_comp, bc_tags, cfg, self.device)
+        self.fvm_mesh = mesh_setup
+        self.bc_type_str = [e.tag for e in bc_tags.values()]
+        self.phys = phys_struct(cfg)
+        D, C, N, Fb = self.dim, n_comp, self.n_cells, self.n_facets_bc
+        dev = self.device
+        self.nblk = (N + 255) // 256
+        self.ubc = torch.zeros((max(Fb, 1), C), dtype=torch.float32, device=dev)
+        self.gp = torch.zeros((N, (D + 1) * C), dtype=torch.float32, device=dev)
+        self.bc_part = torch.zeros(4 * 2 * _K_BC_PART, dtype=torch.float32, device=dev)
+        self.red = torch.zeros(self.nblk * 3 * C, dtype=torch.float64, device=dev)
+        self.work = nat.FvmWork(ubc=self.ubc.data_ptr(), gp=self.gp.data_ptr(), bc_part=self.bc_part.data_ptr(),
+                                red=self.red.data_ptr())
+        self.has_prev = False
+
+    # The reference keeps the previous call's limited gradients in ``cell_grads``.
+    @property
+    def cell_grads(self):
+        if not self.has_prev:
+            return None
+        D, C = self.dim, self.n_comp
+        g = self.gp.view(self.n_cells, D + 1, C)
+        return g[:, :D, :] * g[:, D:D + 1, :]
+
+    def reset_state(self):
+        """Forget the previous call's gradients (the reference's ``cell_grads = None``)."""
+        self.has_prev = False
+
+    def rhs(self, prim: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
+        nat.require_device(prim, "primatives")
+        if prim.shape != (self.n_cells, self.n_comp) or prim.dtype != torch.float32:
+            raise ValueError(f"primatives must be float32 [{self.n_cells}, {self.n_comp}]")
+        prim = prim.contiguous()
+        dudt = out if out is not None else torch.empty_like(prim)
+        rc = nat.lib().fvm_rhs_f32(self.mesh.struct, self.phys, self.work, nat.ptr(prim), nat.ptr(dudt),
+                                   int(self.has_prev), nat.stream_ptr(self.device))
+        nat.check(rc, "fvm_rhs_f32")
+        self.has_prev = True
+        return dudt
+
+    def stage(self, st: "nat.FvmStage"):
+        st.has_prev = int(self.has_prev)
+        rc = nat.lib().fvm_rk_stage_f32(self.mesh.struct, self.phys, self.work, st, nat.stream_ptr(self.device))
+        nat.check(rc, "fvm_rk_stage_f32")
+        self.has_prev = True
+
+    def clear_temp(self):
+        """The reference frees init-only tensors here (facet_process.py:290-295); the
+        device layout holds none."""
+        return None

@@ -1,0 +1,185 @@
+/*
+ * fvm_hip.h -- C ABI of the MI355X (gfx950) FVM step-loop library (libfvm_hip.so).
+ *
+ * Every pointer is a caller-owned device buffer unless stated otherwise; nothing on the
+ * hot path allocates.  Every call takes the HIP stream to launch on as `void *stream`
+ * (a hipStream_t; NULL is the default stream), launches asynchronously and returns 0 on
+ * success or a nonzero FVM_ERR_* code; fvm_last_error() then describes the failure
+ * (thread-local).  The struct arguments (FvmMeshDev, FvmPhys, FvmWork, FvmStage) live in
+ * host memory and carry device pointers; the library keeps no reference to them after
+ * the call returns.
+ *
+ * Layout conventions (N cells, F facets, Fb boundary facets, d = 2|3, C = d + 2,
+ * NL = d + 1 facets per cell; "[a][b]" = a-major, b contiguous):
+ *   state, prim, dudt : [N][C] fp32, component order as the reference
+ *                       (momentum/velocity..., rho, E/T) -- fvm_equation.py:35-55
+ *   per-cell slot arrays are slot-major, [slot][N] (one coalesced load per slot).
+ */
+#ifndef FVM_HIP_H
+#define FVM_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FVM_OK 0
+#define FVM_ERR_ARG 1          /* invalid argument (shape, size, null pointer)            */
+#define FVM_ERR_ROW_OVERFLOW 2 /* csr_to_ell: a row holds more nonzeros than K            */
+#define FVM_ERR_HIP 3          /* a HIP runtime call or kernel launch failed              */
+
+/* Description of the last failure on the calling thread ("" if none). */
+const char *fvm_last_error(void);
+/* Library ABI version (bumped on any struct or signature change). */
+int fvm_abi_version(void);
+
+/* ---------------------------------------------------------------------------------
+ * ELL sparse products.  Replace the Triton kernels of utils/ell_kernel.py:24-210
+ * (ell_spmm_kernel, _ell_spmv_kernel, ell_spmv, ell_spmm).
+ *   y[r, j] = sum_{k<K} vals[r, k] * x[cols[r, k], j]  (+ bias[r])
+ * vals/cols are addressed through explicit strides (element units), so both the
+ * reference's row-major [M, K] layout and this library's slot-major layout
+ * (stride_m = 1, stride_k = M) are accepted; cols are int32 (cols_is64 = 0) or int64.
+ * Accumulation is fp32 in slot order k = 0..K-1; bias may be NULL.
+ * ------------------------------------------------------------------------------- */
+int fvm_ell_spmm_f32(const float *vals, int64_t vals_stride_m, int64_t vals_stride_k,
+                     const void *cols, int32_t cols_is64, int64_t cols_stride_m, int64_t cols_stride_k,
+                     int64_t M, int32_t K,
+                     const float *x, int64_t x_stride_n, int64_t x_stride_b, int32_t B,
+                     const float *bias,
+                     float *y, int64_t y_stride_m, int64_t y_stride_b, void *stream);
+
+/* ---------------------------------------------------------------------------------
+ * CSR -> ELL packing.  Replaces csr_to_ell (utils/ell_kernel.py:225-256): slot k of
+ * row r holds the k-th stored entry of the row, padding is col 0 / val 0.
+ * fvm_csr_max_row_nnz writes max_r(crow[r+1]-crow[r]) to *out_max (device int32; the
+ * caller zeroes it first).  fvm_ell_pack_f32 writes slot-major vals/cols
+ * (index k * slot_stride + r) and sets *overflow (device int32, caller-zeroed) to 1 if
+ * a row holds more than K entries -- the reference raises ValueError there.
+ * ------------------------------------------------------------------------------- */
+int fvm_csr_max_row_nnz(const int64_t *crow, int64_t M, int32_t *out_max, void *stream);
+int fvm_ell_pack_f32(const int64_t *crow, const int64_t *col, const float *val, int64_t M, int32_t K,
+                     float *ell_vals, int32_t *ell_cols, int64_t slot_stride, int32_t *overflow,
+                     void *stream);
+
+/* ---------------------------------------------------------------------------------
+ * Fused RHS.  Device-resident mesh produced once at set-up from the reference's
+ * MeshCache / BoundarySetter / BC arrays (facet_process.py:58-226,
+ * facet_boundary.py:32-385).  Per-cell slot arrays list a cell's facets in ascending
+ * facet index (the column order of flux_mat's ELL rows).
+ * ------------------------------------------------------------------------------- */
+typedef struct FvmMeshDev {
+    int32_t dim;          /* 2 or 3                                                   */
+    int32_t n_groups;     /* characteristic BC groups (<= 4)                          */
+    int64_t n_cells;      /* N                                                        */
+    int64_t n_facets;     /* F                                                        */
+    int64_t n_bc;         /* Fb                                                       */
+    /* per cell -------------------------------------------------------------------- */
+    const int32_t *cell_facet;  /* [NL][N] facet ids, ascending                       */
+    const int32_t *cell_nbr;    /* [NL][N] neighbour cell, or N + boundary index      */
+    const int32_t *cell_info;   /* [N] bit k: this cell's slot in facet k;
+                                   bits 8+2k..9+2k: facet k's position in the
+                                   neighbour's list; bits 16+3j..18+3j: source of
+                                   LSQ column j (k = neighbour k, 7 = the cell)       */
+    const float *cell_disp;     /* [NL][d][N] centroid -> facet midpoint              */
+    const float *cell_gval;     /* [NL+1][d][N] LSQ gradient row values, ascending
+                                   column order (G_mats rows d*c+i)                   */
+    const float *cell_inv_vol;  /* [N] fp32(1/vol): |flux_mat| entries                */
+    const float *cell_eps;      /* [N] limiter epsilon (K sqrt(vol))^(p+1)            */
+    /* per facet ------------------------------------------------------------------- */
+    const float *fac_geo;       /* [F][G] G = 8 (2-D) / 12 (3-D): n[d], n_hat[d],
+                                   |n|, 1/(n_hat.d), X_orthog[d] (0 on the boundary)  */
+    /* per boundary facet (boundary-local index b) ---------------------------------- */
+    const int32_t *bc_cell;     /* [Fb] interior cell                                 */
+    const int32_t *bc_group;    /* [Fb] characteristic group, or -1                   */
+    const int32_t *bc_flags;    /* [Fb] bit j: Neumann comp j (A_bc entry + A_corr)   */
+    const float *bc_val;        /* [Fb][C] b_bc (Dirichlet value / dUdn * dist)       */
+    const float *bc_l;          /* [Fb][d] tangential centroid->facet vector l        */
+    const float *bc_ga;         /* [Fb][C] A_face_grad coefficient (-1/dist or 0)     */
+    const float *bc_gb;         /* [Fb][C] b_face_grad                                */
+    const float *bc_char;       /* [Fb][2d+3]: n_out[d], rho_inf, v_n_inf, p_inf,
+                                   v_t_inf[d]                                         */
+} FvmMeshDev;
+
+typedef struct FvmPhys {      /* scalars of FluidConstitution / SlopeLimiter / KT      */
+    float inv_cv;     /* fp32(1 / C_v)                                                */
+    float cv;         /* C_v                                                          */
+    float R;          /* C_v (gamma - 1)                                              */
+    float gR;         /* fp32(gamma * R)                                              */
+    float mu0, T0, T0pS, S, mub0, T0sq; /* Sutherland / bulk viscosity constants       */
+    float kappa;      /* thermal conductivity                                         */
+    float v_factor;   /* KT velocity clamp                                            */
+    int32_t lim_p;    /* limiter order 1..5                                           */
+} FvmPhys;
+
+typedef struct FvmWork {      /* device scratch owned by the caller                    */
+    float *ubc;       /* [Fb][C] boundary face values                                 */
+    float *gp;        /* [N][(d+1)C] unlimited LSQ gradient [d][C] then limiter phi[C];
+                         persists between calls (the next call's Neumann correction)  */
+    float *bc_part;   /* [4][2][64] per-group partial sums of c and facet counts     */
+    double *red;      /* [ceil(N/256)][3C] fp64 partial column norms (adaptive dt)    */
+} FvmWork;
+
+/* One RHS evaluation dU/dt = FVMEquation.forward(prim) (fvm_equation.py:406-435).
+ * has_prev != 0 applies the Neumann non-orthogonal correction with the gradients the
+ * previous call left in work->gp (facet_boundary.py:135-136). */
+int fvm_rhs_f32(const FvmMeshDev *m, const FvmPhys *p, const FvmWork *w, const float *prim,
+                float *dudt, int32_t has_prev, void *stream);
+
+/* One explicit Runge-Kutta stage with the RHS fused into the stage update
+ * (Butcher / ButcherAdapt, integrators.py:305-393).  With k_i = dt * f(prim_i):
+ *   acc  (+)= b_i k_i          acc2 (+)= b2_i k_i  (if acc2)      k store if store_k
+ *   not last: prim_next = prim(U0 + sum_j inc_coef[j] k_{inc_idx[j]})
+ *   last:     U_out = U0 + acc, prim_next = prim(U_out); with acc2 also fp64 partial
+ *             column sums of squares of (acc-acc2), acc, U0 into work->red.           */
+typedef struct FvmStage {
+    const float *prim;     /* [N][C] stage input primitives                           */
+    const float *U0;       /* [N][C] state at the start of the step                   */
+    float *acc;            /* [N][C] sum b_j k_j                                      */
+    float *acc2;           /* [N][C] sum b2_j k_j or NULL                             */
+    float *kbuf;           /* [s][N][C] stored stages or NULL                         */
+    float *prim_next;      /* [N][C]                                                  */
+    float *U_out;          /* [N][C] (last stage)                                     */
+    const float *dt;       /* device scalar                                           */
+    int32_t stage, last, store_k, has_prev;
+    float b, b2;
+    int32_t n_inc;
+    int32_t inc_idx[10];
+    float inc_coef[10];
+} FvmStage;
+
+int fvm_rk_stage_f32(const FvmMeshDev *m, const FvmPhys *p, const FvmWork *w, const FvmStage *st,
+                     void *stream);
+
+/* The same stage split into its launches, for per-kernel timing with stream events:
+ * parts is a mask of 1 = boundary c-sum, 2 = boundary values, 4 = gradient + limiter,
+ * 8 = flux + divergence + stage update.  Launching all four masks in order equals
+ * fvm_rk_stage_f32.  No reference counterpart (measurement hook). */
+int fvm_rk_stage_part_f32(const FvmMeshDev *m, const FvmPhys *p, const FvmWork *w, const FvmStage *st,
+                          int32_t parts, void *stream);
+
+/* Adaptive.update_stepsize (integrators.py:42-62) from the partial norms the last
+ * adaptive stage left in work->red: dt *= alpha + (1 - alpha) 0.9 E^(-1/order). */
+int fvm_update_stepsize(const FvmWork *w, int32_t C, int32_t n_blocks, float atol, float rtol,
+                        float mtol, float order, float alpha_gt1, float alpha_le1, float *dt,
+                        void *stream);
+
+/* state -> primitives (FluidConstitution.state_to_primative, fvm_equation.py:35-44). */
+int fvm_state_to_prim_f32(const float *state, float *prim, int64_t N, int32_t dim, float inv_cv,
+                          void *stream);
+
+/* sum_c vol_c U_c per conserved variable, accumulated in fp64 (no reference
+ * counterpart).  out[C] (device, fp64) is overwritten; deterministic. */
+int fvm_conservation_f64(const float *state, const double *vol, int64_t N, int32_t C, double *out,
+                         double *partial /* [256*C] */, void *stream);
+
+/* CFL time step (no reference counterpart; the reference takes a fixed cfg.dt):
+ * dt = cfl * min_c h_c / (|V_c| + c_c), h_c = d vol_c / sum_f |n_f|.  out (device fp64). */
+int fvm_cfl_dt(const float *prim, const double *h, int64_t N, int32_t dim, double gR, double cfl,
+               double *out, double *partial /* [256] */, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FVM_HIP_H */

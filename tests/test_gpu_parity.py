@@ -1,0 +1,186 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle and the golden
+vectors written from the reference.
+
+Tolerances (stated per test):
+  * ELL SpMV/SpMM: relative L2 error <= 1e-6 per output column against an fp64 product.
+  * ELL packing: column indices and values bit-exact.
+  * RHS: the product builds its operators from float64 geometry, the reference from
+    float32, so operator values differ by float32 round-off; per conserved variable the
+    relative L2 difference of dU/dt to the fp32 oracle is bounded by 1e-4, and the
+    product's error against the fp64 oracle may not exceed the fp32 oracle's own error
+    against it by more than a factor 10.
+  * Integrators: 10 steps against the golden trajectories within 1e-4 relative L2 per
+    variable; 200 RK4 steps on a 2k-cell channel within 1e-4 per variable.
+"""
+import numpy as np
+import pytest
+import torch
+
+from fvm_solver_amd.utils.ell_kernel import csr_to_ell, ell_spmm, ell_spmv
+from fvm_solver_amd.utils.sparse import to_sparse
+from oracle import fvm_oracle as O
+from tests.golden_cases import load, mesh_tuple, phys_of
+from tests.gpu_helpers import DEV, make_cfg, product_equation, rel_l2_cols, initial_state, oracle_phys
+
+pytestmark = pytest.mark.gpu
+
+
+def _announce(msg):
+    print(f"[gpu-test] {msg}", flush=True)
+
+
+# ------------------------------------------------------------------------------- ELL
+@pytest.mark.parametrize("M,N,K,B", [(1, 1, 1, 1), (1000, 700, 3, 4), (4097, 3000, 5, 5), (333, 999, 7, 1),
+                                     (2048, 2048, 2, 9), (5000, 10, 0, 3)])
+@pytest.mark.parametrize("bias", [False, True])
+def test_ell_spmm_random(M, N, K, B, bias):
+    _announce(f"ell_spmm M={M} N={N} K={K} B={B} bias={bias}")
+    rng = np.random.default_rng(M + K + B)
+    cols = rng.integers(0, N, size=(M, K))
+    vals = rng.standard_normal((M, K)).astype(np.float32)
+    x = rng.standard_normal((N, B)).astype(np.float32)
+    b = rng.standard_normal(M).astype(np.float32) if bias else None
+    ref = (vals.astype(np.float64)[:, :, None] * x.astype(np.float64)[cols]).sum(1)
+    if bias:
+        ref = ref + b[:, None]
+    for ct in (torch.int32, torch.int64):
+        for slot_major in (False, True):
+            v = torch.from_numpy(vals).to(DEV)
+            c = torch.from_numpy(cols).to(DEV, ct)
+            if slot_major:
+                v, c = v.t().contiguous().t(), c.t().contiguous().t()
+            xb = torch.from_numpy(x).to(DEV)
+            bb = torch.from_numpy(b).to(DEV) if bias else None
+            y = (ell_spmm(v, c, xb, bb) if B > 1 else ell_spmv(v, c, xb, bb)).cpu().numpy()
+            assert y.shape == (M, B)
+            err = rel_l2_cols(y, ref) if np.any(ref) else np.abs(y).max(0)
+            assert np.all(err <= 1e-6), err
+    y1 = ell_spmv(torch.from_numpy(vals).to(DEV), torch.from_numpy(cols).to(DEV),
+                  torch.from_numpy(x[:, 0]).to(DEV), torch.from_numpy(b).to(DEV) if bias else None)
+    assert y1.shape == (M,)
+
+
+def test_csr_to_ell_bit_exact_and_overflow():
+    d = load("ref_csr_to_ell.npz")
+    for i in range(3):
+        _announce(f"csr_to_ell case {i}")
+        A = torch.sparse_csr_tensor(torch.from_numpy(d[f"csr{i}_crow"]), torch.from_numpy(d[f"csr{i}_col"]),
+                                    torch.from_numpy(d[f"csr{i}_val"]), size=tuple(d[f"csr{i}_shape"])).to(DEV)
+        v, c = csr_to_ell(A)
+        assert np.array_equal(c.cpu().numpy(), d[f"csr{i}_ell_col"])
+        assert np.array_equal(v.cpu().numpy(), d[f"csr{i}_ell_val"])
+        K = d[f"csr{i}_ell_col"].shape[1]
+        if K > 0:
+            with pytest.raises(ValueError):
+                csr_to_ell(A, K=K - 1)
+
+
+@pytest.mark.parametrize("fname", ["ref_2d.npz", "ref_3d.npz"])
+@pytest.mark.parametrize("name", ["flux_mat", "G_mats", "A_face_grad", "A_bc", "A_corr"])
+def test_reference_operators(fname, name):
+    """The reference's own assembled operators through to_sparse -> spMV/spMM."""
+    _announce(f"operator {name} {fname}")
+    d = load(fname)
+    crow, col, val = (torch.from_numpy(d[f"{name}_{k}"]) for k in ("crow", "col", "val"))
+    shape = tuple(int(s) for s in d[f"{name}_shape"])
+    A = torch.sparse_csr_tensor(crow, col, val, size=shape)
+    op = to_sparse(A, DEV)
+    assert np.array_equal(op.cols.cpu().numpy(), d[f"{name}_ell_col"])
+    assert np.array_equal(op.vals.cpu().numpy(), d[f"{name}_ell_val"])
+    rng = np.random.default_rng(0)
+    dense = torch.sparse_csr_tensor(crow, col, val.double(), size=shape).to_dense().numpy()
+    for B in (1, 5):
+        x = rng.standard_normal((shape[1], B)).astype(np.float32)
+        b = rng.standard_normal(shape[0]).astype(np.float32)
+        ref = dense @ x.astype(np.float64) + b[:, None]
+        xt = torch.from_numpy(x).to(DEV)
+        y = (op.spMM(xt, torch.from_numpy(b).to(DEV)) if B > 1 else
+             op.spMV(xt[:, 0], torch.from_numpy(b).to(DEV))[:, None]).cpu().numpy()
+        assert np.all(rel_l2_cols(y, ref) <= 1e-6)
+
+
+# ------------------------------------------------------------------------------- RHS
+@pytest.mark.parametrize("fname", ["ref_2d.npz", "ref_3d.npz"])
+def test_rhs_two_calls_against_oracle(fname):
+    d = load(fname)
+    pts, cells, fi, fb, tags = mesh_tuple(d)
+    dim = cells.shape[1] - 1
+    _announce(f"rhs parity {fname} N={len(cells)}")
+    cfg = make_cfg(dim)
+    mesh, eq = product_equation(pts, cells, fi, fb, tags, cfg)
+    _, o32 = O.from_mesh_tuple(pts, cells, fi, fb, tags, phys_of(d))
+    _, o64 = O.from_mesh_tuple(pts, cells, fi, fb, tags, phys_of(d), dtype=torch.float64)
+    for call in range(2):
+        prim = torch.from_numpy(d[f"rhs{call}_prim"])
+        mine = eq.forward(prim.to(DEV)).cpu().numpy()
+        r32 = o32.forward(prim).numpy()
+        r64 = o64.forward(prim.double()).numpy()
+        assert np.allclose(r32, d[f"rhs{call}_dudt"], rtol=0, atol=0)      # oracle == reference
+        e_mine = rel_l2_cols(mine, r32)
+        e_mine64 = rel_l2_cols(mine, r64)
+        e_o64 = rel_l2_cols(r32, r64)
+        _announce(f"  call {call}: vs fp32 oracle {e_mine}, vs fp64 {e_mine64} (fp32 oracle vs fp64 {e_o64})")
+        assert np.all(e_mine <= 1e-4), e_mine
+        assert np.all(e_mine64 <= 10 * np.maximum(e_o64, 1e-7)), (e_mine64, e_o64)
+        # the carried gradients: the reference's cell_grads
+        g = eq.E_props.cell_grads.cpu().numpy()
+        assert np.all(rel_l2_cols(g.reshape(len(g), -1), o32.cell_grads.numpy().reshape(len(g), -1)) <= 1e-4)
+
+
+@pytest.mark.parametrize("fname", ["ref_2d.npz", "ref_3d.npz"])
+@pytest.mark.parametrize("name,extra", [("Butcher", "RK4"), ("Butcher_adapt", "RK3_SSP4"), ("RK3_SSP4", "")])
+def test_integrator_steps_against_golden(fname, name, extra):
+    d = load(fname)
+    pts, cells, fi, fb, tags = mesh_tuple(d)
+    dim = cells.shape[1] - 1
+    _announce(f"steps {name}{extra} {fname}")
+    dt = float(d["step_dt"])
+    cfg = make_cfg(dim, solver_name=name, solver_extra=extra, dt=dt)
+    U0 = torch.from_numpy(d["step_U0"])
+    mesh, eq = product_equation(pts, cells, fi, fb, tags, cfg, us_init=U0)
+    ts = eq.t_solver
+    for i in range(10):
+        ts._solve_step(None)
+        U = eq.cells.state.cpu().numpy()
+        ref = d[f"step_{name}{extra}_U"][i]
+        err = rel_l2_cols(U, ref)
+        assert np.all(err <= 1e-4), (i, err)
+        assert abs(float(ts.dt) - float(d[f"step_{name}{extra}_dt"][i])) <= 1e-5 * float(ts.dt)
+    _announce(f"  final rel L2 {err}")
+
+
+def test_rk4_200_steps_channel():
+    from fvm_solver_amd.mesh_gen.channel import channel_mesh_2d
+    pts, cells, (fi, fb), tags = channel_mesh_2d(2000, seed=3)
+    cfg = make_cfg(2, solver_name="Butcher", solver_extra="RK4")
+    mesh, eq = product_equation(pts, cells, fi, fb, tags, cfg)
+    prim0, U0 = initial_state(mesh.centroids.numpy(), cfg, 2)
+    eq.cells.state = U0.to(DEV)
+    dt = float(eq.cfl_dt(prim0.to(DEV), 0.5))
+    eq.t_solver.set_dt(dt)
+    _announce(f"rk4 200 steps N={mesh.n_cells} dt={dt:.3e}")
+    _, o32 = O.from_mesh_tuple(pts, cells, fi, fb, tags, oracle_phys(cfg))
+    st = O.OracleStepper(o32, "Butcher", "RK4", dt=dt)
+    U = U0.clone()
+    for i in range(200):
+        eq.t_solver._solve_step(None)
+        U = st.step(U)
+    err = rel_l2_cols(eq.cells.state.cpu().numpy(), U.numpy())
+    _announce(f"  rel L2 after 200 steps {err}")
+    assert np.all(err <= 1e-4), err
+    # fp64 conservation totals agree with a closed-form sum over the oracle's volumes
+    tot = eq.conservation_totals().cpu().numpy()
+    ref = (mesh.geo64["volumes"][:, None] * eq.cells.state.cpu().double().numpy()).sum(0)
+    assert np.allclose(tot, ref, rtol=1e-12)
+
+
+def test_cfl_dt_matches_closed_form():
+    d = load("ref_2d.npz")
+    pts, cells, fi, fb, tags = mesh_tuple(d)
+    cfg = make_cfg(2)
+    mesh, eq = product_equation(pts, cells, fi, fb, tags, cfg)
+    prim = torch.from_numpy(d["rhs0_prim"]).abs() + 1.0
+    _, o64 = O.from_mesh_tuple(pts, cells, fi, fb, tags, phys_of(d), dtype=torch.float64)
+    ref = o64.cfl_dt(prim.double(), 0.5)
+    got = float(eq.cfl_dt(prim.to(DEV), 0.5))
+    assert abs(got - ref) <= 1e-6 * ref     # fp32 vs fp64 vertices

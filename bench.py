@@ -33,6 +33,8 @@ def _parse():
     ap.add_argument("--cpu-steps", type=int, default=4)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--kernel-timing-steps", type=int, default=20)
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="also run one 256 MiB device-to-device copy (calibrates HBM byte counters)")
     return ap.parse_args()
 
 
@@ -118,6 +120,14 @@ def main():
     state = eq.cells.state
     finite = bool(torch.isfinite(state).all())
     log(f"{a.steps} steps: {gpu_ms:.2f} ms GPU ({wall * 1e3:.2f} ms wall), {ms_per_step * 1e3:.1f} us/step, finite={finite}")
+
+    if a.calib_copy:
+        src = torch.empty(64 * 2 ** 20, dtype=torch.float32, device=dev).fill_(1.0)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("calibration copy: 256 MiB read + 256 MiB written")
 
     # ---- per-kernel durations: the same stages, launched part by part between events ----
     E = eq.E_props

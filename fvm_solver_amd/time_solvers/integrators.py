@@ -124,6 +124,29 @@ class _FusedButcher(TSolver):
 class Butcher(_FusedButcher):
     """Fixed-dt explicit Runge-Kutta (integrators.py:349-393)."""
 
+    def capture_graph(self, n_steps: int = 2):
+        """Capture ``n_steps`` fused steps as one HIP graph (hipGraph via torch.cuda.CUDAGraph).
+
+        The state ping-pongs between two fixed buffers, so ``n_steps * stages`` and
+        ``n_steps`` must be even: a replay then starts and ends in the same buffers.  At
+        least one eager step must have run (the carried-gradient flag is then set and the
+        kernels are loaded).  Replaying advances ``cells.state`` in place by n_steps.
+        """
+        if n_steps % 2 or (n_steps * self.stages) % 2:
+            raise ValueError("n_steps and n_steps * stages must be even for a replayable graph")
+        if not self.eq.E_props.has_prev or self._last_out is None:
+            raise RuntimeError("run at least one eager step before capturing")
+        start = self.cells.state
+        flip0 = self._flip
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            for _ in range(n_steps):
+                self._solve_step(None)
+        if self.cells.state.data_ptr() != start.data_ptr() or self._flip != flip0:
+            raise RuntimeError("captured steps did not return to the starting buffers")
+        # capture recorded but did not execute the kernels; the state is unchanged
+        return g
+
 
 class ButcherAdapt(_FusedButcher):
     """Explicit RK with the embedded b2 error estimate and in-place dt update

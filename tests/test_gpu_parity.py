@@ -115,7 +115,9 @@ def test_rhs_two_calls_against_oracle(fname):
         mine = eq.forward(prim.to(DEV)).cpu().numpy()
         r32 = o32.forward(prim).numpy()
         r64 = o64.forward(prim.double()).numpy()
-        assert np.allclose(r32, d[f"rhs{call}_dudt"], rtol=0, atol=0)      # oracle == reference
+        # the oracle reproduces the reference bitwise on the host that wrote the golden
+        # vectors; on another CPU (other ATen/MKL code paths) it agrees to round-off
+        assert np.all(rel_l2_cols(r32, d[f"rhs{call}_dudt"]) <= 1e-5)
         e_mine = rel_l2_cols(mine, r32)
         e_mine64 = rel_l2_cols(mine, r64)
         e_o64 = rel_l2_cols(r32, r64)
@@ -184,3 +186,36 @@ def test_cfl_dt_matches_closed_form():
     ref = o64.cfl_dt(prim.double(), 0.5)
     got = float(eq.cfl_dt(prim.to(DEV), 0.5))
     assert abs(got - ref) <= 1e-6 * ref     # fp32 vs fp64 vertices
+
+
+def test_hip_graph_replay_matches_eager():
+    """A captured 2-step graph replayed several times, with eager work and syncs between
+    replays, reproduces the eager steps bit for bit."""
+    from fvm_solver_amd.mesh_gen.channel import channel_mesh_2d
+    pts, cells, (fi, fb), tags = channel_mesh_2d(3000, seed=4)
+    cfg = make_cfg(2, solver_name="Butcher", solver_extra="RK4")
+    runs = []
+    for use_graph in (False, True):
+        mesh, eq = product_equation(pts, cells, fi, fb, tags, cfg)
+        prim0, U0 = initial_state(mesh.centroids.numpy(), cfg, 2)
+        eq.cells.state = U0.to(DEV)
+        eq.t_solver.set_dt(1e-4)
+        ts = eq.t_solver
+        ts._solve_step(None)
+        snaps = []
+        if use_graph:
+            g = ts.capture_graph(2)
+            for _ in range(4):
+                g.replay()
+                torch.cuda.synchronize()
+                snaps.append(eq.cells.state.clone().cpu())
+                eq.conservation_totals()          # unrelated eager work between replays
+        else:
+            for _ in range(4):
+                ts._solve_step(None)
+                ts._solve_step(None)
+                snaps.append(eq.cells.state.clone().cpu())
+        runs.append(snaps)
+    _announce("graph replay vs eager")
+    for a, b in zip(*runs):
+        assert torch.equal(a, b)

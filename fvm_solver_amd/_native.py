@@ -12,7 +12,7 @@ import torch
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libfvm_hip.so")
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 FVM_ERR_ROW_OVERFLOW = 2
 
@@ -27,7 +27,9 @@ class FvmMeshDev(ctypes.Structure):
                 ("cell_disp", c_void_p), ("cell_gval", c_void_p), ("cell_inv_vol", c_void_p),
                 ("cell_eps", c_void_p), ("fac_geo", c_void_p),
                 ("bc_cell", c_void_p), ("bc_group", c_void_p), ("bc_flags", c_void_p), ("bc_val", c_void_p),
-                ("bc_l", c_void_p), ("bc_ga", c_void_p), ("bc_gb", c_void_p), ("bc_char", c_void_p)]
+                ("bc_l", c_void_p), ("bc_ga", c_void_p), ("bc_gb", c_void_p), ("bc_char", c_void_p),
+                ("n_slots", i32), ("pad0", i32), ("blk_slot", c_void_p), ("slot_blk", c_void_p),
+                ("group_count", f32 * 4)]
 
 
 class FvmPhys(ctypes.Structure):
@@ -36,13 +38,14 @@ class FvmPhys(ctypes.Structure):
 
 
 class FvmWork(ctypes.Structure):
-    _fields_ = [("ubc", c_void_p), ("gp", c_void_p), ("bc_part", c_void_p), ("red", c_void_p)]
+    _fields_ = [("ubc", c_void_p), ("gp", c_void_p), ("bc_part", c_void_p), ("bc_mean", c_void_p),
+                ("bc_counter", c_void_p), ("red", c_void_p)]
 
 
 class FvmStage(ctypes.Structure):
     _fields_ = [("prim", c_void_p), ("U0", c_void_p), ("acc", c_void_p), ("acc2", c_void_p), ("kbuf", c_void_p),
                 ("prim_next", c_void_p), ("U_out", c_void_p), ("dt", c_void_p),
-                ("stage", i32), ("last", i32), ("store_k", i32), ("has_prev", i32),
+                ("stage", i32), ("last", i32), ("store_k", i32), ("has_prev", i32), ("bc_ready", i32),
                 ("b", f32), ("b2", f32), ("n_inc", i32), ("inc_idx", i32 * 10), ("inc_coef", f32 * 10)]
 
 

@@ -1,20 +1,26 @@
 // Fused right-hand side of the compressible Navier-Stokes FVM step for gfx950.
 //
-// One RHS evaluation (FVMEquation.forward, fvm_equation.py:406-435) is four launches:
-//   k_bc_sum   per-group partial sums of the boundary speed of sound (deterministic)
-//   k_bc       boundary face values: A_bc.x + b_bc, the Neumann non-orthogonal
-//              correction with the previous call's limited gradients, then the
-//              characteristic BC (facet_boundary.py:124-168, 418-512)
-//   k_grad     per cell: LSQ gradient G.[U; U_bc] and the slope limiter
-//              (facet_process.py:387-418, limiter.py); stores the unlimited gradient and
-//              the limiter factor phi, from which every consumer rebuilds the limited
-//              face values and gradients with the reference's own operation order
+// One RHS evaluation (FVMEquation.forward, fvm_equation.py:406-435) is two launches over
+// the cells (plus, only when the boundary means are not already known, a small pre-pass):
+//   k_bc_mean  (pre-pass) per characteristic-BC group, the mean speed of sound of the
+//              boundary cells (the group-wide mean in facet_boundary.py:427), reduced
+//              deterministically: per-block partials in fixed slots, summed in slot
+//              order by the last block to finish
+//   k_grad     per cell: the values of its boundary facets -- A_bc.x + b_bc, the Neumann
+//              non-orthogonal correction with the previous call's limited gradients,
+//              the characteristic BC (facet_boundary.py:124-168, 418-512) -- then the
+//              LSQ gradient G.[U; U_bc] and the slope limiter (facet_process.py:387-418,
+//              limiter.py); stores the unlimited gradient and the limiter factor phi,
+//              from which every consumer rebuilds the limited face values and gradients
+//              with the reference's own operation order
 //   k_flux     per cell: for each of its facets (ascending facet index) the complete
 //              facet flux -- face reconstruction, normal-gradient correction, stress,
 //              advection, pressure, viscosity, heating, KT dissipation
 //              (facet_process.py:332-385, fvm_equation.py:62-368) -- summed into the
 //              divergence (flux_mat row), then an epilogue: write dU/dt, or apply an
-//              explicit Runge-Kutta stage update and emit the next stage's primitives.
+//              explicit Runge-Kutta stage update, emit the next stage's primitives and
+//              the boundary means of those primitives (so the next stage needs no
+//              pre-pass).
 // Interior facet fluxes are computed by both neighbours with identical inputs and
 // identical instruction sequences, so the two contributions cancel exactly and no
 // atomics or facet-flux round trip through HBM are needed.
@@ -27,7 +33,7 @@
 namespace fvm {
 
 constexpr int kThreads = 256;
-constexpr int kBcPart = 64;  // max blocks of k_bc_sum; bc_part holds [group][sum|count][64]
+constexpr int kMaxGroups = 4;
 
 template <int D>
 struct Dims {
@@ -94,124 +100,140 @@ __device__ __forceinline__ void state_to_prim(const float (&U)[D + 2], float inv
 // ------------------------------------------------------------------------------------
 // Boundary face values
 // ------------------------------------------------------------------------------------
+// Values of boundary facet b from its interior cell's primitives P and gradient record
+// gprev of the previous call (facet_boundary.py:124-168, characteristic BC :436-512).
 template <int D>
-__global__ void __launch_bounds__(kThreads) k_bc_sum(FvmMeshDev m, FvmPhys p, const float *__restrict__ prim,
-                                                     float *__restrict__ part) {
-    constexpr int C = D + 2;
-    __shared__ float ssum[kThreads];
-    __shared__ float scnt[kThreads];
-    for (int g = 0; g < m.n_groups; ++g) {
-        float s = 0.0f, n = 0.0f;
-        for (int64_t b = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x; b < m.n_bc;
-             b += static_cast<int64_t>(gridDim.x) * kThreads) {
-            if (m.bc_group[b] == g) {
-                const float T = prim[static_cast<int64_t>(m.bc_cell[b]) * C + D + 1];
-                s += sqrtf(p.gR * T);
-                n += 1.0f;
-            }
-        }
-        ssum[threadIdx.x] = s;
-        scnt[threadIdx.x] = n;
-        __syncthreads();
-        for (int w = kThreads / 2; w > 0; w >>= 1) {
-            if (threadIdx.x < w) {
-                ssum[threadIdx.x] += ssum[threadIdx.x + w];
-                scnt[threadIdx.x] += scnt[threadIdx.x + w];
-            }
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) {
-            part[g * 2 * kBcPart + blockIdx.x] = ssum[0];
-            part[g * 2 * kBcPart + kBcPart + blockIdx.x] = scnt[0];
-        }
-        __syncthreads();
-    }
-}
-
-template <int D>
-__global__ void __launch_bounds__(kThreads)
-k_bc(FvmMeshDev m, FvmPhys p, const float *__restrict__ prim, const float *__restrict__ gp, float *__restrict__ ubc,
-     const float *__restrict__ part, int nbp, int has_prev) {
+__device__ __forceinline__ void bc_value(const FvmMeshDev &m, const FvmPhys &p, int64_t b, const float (&P)[D + 2],
+                                         const float *__restrict__ gprev, int has_prev,
+                                         const float *__restrict__ bc_mean, float (&u)[D + 2]) {
     using Dm = Dims<D>;
     constexpr int C = Dm::C;
-    const int64_t b = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
-    if (b >= m.n_bc) return;
-    const int64_t cell = m.bc_cell[b];
-    float P[C];
-    load_prim<D>(prim + cell * C, P);
     const int flags = m.bc_flags[b];
-    float u[C];
 #pragma unroll
     for (int j = 0; j < C; ++j) {
         const float v = m.bc_val[b * C + j];
         u[j] = ((flags >> j) & 1) ? P[j] + v : v;
     }
     if (has_prev && flags) {
-        const float *g = gp + cell * Dm::REC;
 #pragma unroll
         for (int j = 0; j < C; ++j) {
             if ((flags >> j) & 1) {
-                const float ph = g[D * C + j];
-                float s = m.bc_l[b * D] * (g[j] * ph);
+                const float ph = gprev[D * C + j];
+                float s = m.bc_l[b * D] * (gprev[j] * ph);
 #pragma unroll
-                for (int i = 1; i < D; ++i) s = s + m.bc_l[b * D + i] * (g[i * C + j] * ph);
+                for (int i = 1; i < D; ++i) s = s + m.bc_l[b * D + i] * (gprev[i * C + j] * ph);
                 u[j] = u[j] + s;
             }
         }
     }
     const int grp = m.bc_group[b];
-    if (grp >= 0) {
-        float sum = 0.0f, cnt = 0.0f;
-        for (int i = 0; i < nbp; ++i) {
-            sum += part[grp * 2 * kBcPart + i];
-            cnt += part[grp * 2 * kBcPart + kBcPart + i];
-        }
-        const float cmean = sum / cnt;
-        const float *ch = m.bc_char + b * Dm::CH;
-        float n[D], vt[D];
-        float vn = 0.0f;
+    if (grp < 0) return;
+    const float cmean = bc_mean[grp];
+    const float *ch = m.bc_char + b * Dm::CH;
+    float n[D], vt[D];
+    float vn = 0.0f;
 #pragma unroll
-        for (int i = 0; i < D; ++i) {
-            n[i] = ch[i];
-            vn = (i == 0) ? P[i] * n[i] : vn + P[i] * n[i];
-        }
-#pragma unroll
-        for (int i = 0; i < D; ++i) vt[i] = P[i] - vn * n[i];
-        const float rho = P[D], T = P[D + 1];
-        const float pr = (p.R * rho) * T;
-        const float c = sqrtf(p.gR * T);
-        const float drho = ch[D] - rho;
-        const float dvn = ch[D + 1] - vn;
-        const float dp = ch[D + 2] - pr;
-        const float inv_c = 1.0f / c;
-        const float inv_c2 = inv_c * inv_c;
-        const float h_dp = (0.5f * inv_c2) * dp;
-        const float h_dvn = ((0.5f * rho) * inv_c) * dvn;
-        float x0 = -h_dvn + h_dp;
-        float x1 = drho - inv_c2 * dp;
-        float x2 = h_dvn + h_dp;
-        const float scale = 10.0f / cmean;
-        const float g0 = 0.5f * (1.0f - tanhf(scale * (vn - c)));
-        const float g1 = 0.5f * (1.0f - tanhf(scale * vn));
-        const float g2 = 0.5f * (1.0f - tanhf(scale * (vn + c)));
-        x0 = g0 * x0;
-        x1 = g1 * x1;
-        x2 = g2 * x2;
-        const float c2 = c * c;
-        const float cor = c / rho;
-        const float rho_b = rho + ((x0 + x1) + x2);
-        const float vn_b = vn + ((-cor) * x0 + cor * x2);
-        const float p_b = pr + c2 * (x0 + x2);
-#pragma unroll
-        for (int i = 0; i < D; ++i) {
-            const float vtb = g1 * vt[i] + (1.0f - g1) * ch[D + 3 + i];
-            u[i] = vn_b * n[i] + vtb;
-        }
-        u[D] = rho_b;
-        u[D + 1] = p_b / (p.R * rho_b);
+    for (int i = 0; i < D; ++i) {
+        n[i] = ch[i];
+        vn = (i == 0) ? P[i] * n[i] : vn + P[i] * n[i];
     }
 #pragma unroll
-    for (int j = 0; j < C; ++j) ubc[b * C + j] = u[j];
+    for (int i = 0; i < D; ++i) vt[i] = P[i] - vn * n[i];
+    const float rho = P[D], T = P[D + 1];
+    const float pr = (p.R * rho) * T;
+    const float c = sqrtf(p.gR * T);
+    const float drho = ch[D] - rho;
+    const float dvn = ch[D + 1] - vn;
+    const float dp = ch[D + 2] - pr;
+    const float inv_c = 1.0f / c;
+    const float inv_c2 = inv_c * inv_c;
+    const float h_dp = (0.5f * inv_c2) * dp;
+    const float h_dvn = ((0.5f * rho) * inv_c) * dvn;
+    float x0 = -h_dvn + h_dp;
+    float x1 = drho - inv_c2 * dp;
+    float x2 = h_dvn + h_dp;
+    const float scale = 10.0f / cmean;
+    const float g0 = 0.5f * (1.0f - tanhf(scale * (vn - c)));
+    const float g1 = 0.5f * (1.0f - tanhf(scale * vn));
+    const float g2 = 0.5f * (1.0f - tanhf(scale * (vn + c)));
+    x0 = g0 * x0;
+    x1 = g1 * x1;
+    x2 = g2 * x2;
+    const float c2 = c * c;
+    const float cor = c / rho;
+    const float rho_b = rho + ((x0 + x1) + x2);
+    const float vn_b = vn + ((-cor) * x0 + cor * x2);
+    const float p_b = pr + c2 * (x0 + x2);
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+        const float vtb = g1 * vt[i] + (1.0f - g1) * ch[D + 3 + i];
+        u[i] = vn_b * n[i] + vtb;
+    }
+    u[D] = rho_b;
+    u[D + 1] = p_b / (p.R * rho_b);
+}
+
+// Block-level part of the group means of c over the characteristic-BC facets: every
+// thread of the block calls it (uniform control flow).  Thread contributions go through
+// a fixed tree into partial slot `slot`; the last slot block to finish sums the slots in
+// slot order and writes the means.  Identical partials whichever kernel computes them.
+template <int D>
+__device__ void bc_mean_block(const FvmMeshDev &m, const FvmPhys &p, const FvmWork &w, int64_t c, bool live,
+                              float T, int slot) {
+    constexpr int NL = D + 1;
+    const int64_t N = m.n_cells;
+    float cs[kMaxGroups] = {0.0f, 0.0f, 0.0f, 0.0f};
+    if (live) {
+        const float cc = sqrtf(p.gR * T);
+        for (int k = 0; k < NL; ++k) {
+            const int64_t nb = m.cell_nbr[k * N + c];
+            if (nb >= N) {
+                const int g = m.bc_group[nb - N];
+#pragma unroll
+                for (int q = 0; q < kMaxGroups; ++q)
+                    if (g == q) cs[q] += cc;
+            }
+        }
+    }
+    __shared__ float red[kThreads];
+    __shared__ int is_last;
+    for (int g = 0; g < m.n_groups; ++g) {
+        float v = cs[0];
+#pragma unroll
+        for (int q = 1; q < kMaxGroups; ++q) v = (g == q) ? cs[q] : v;
+        red[threadIdx.x] = v;
+        __syncthreads();
+        for (int h = kThreads / 2; h > 0; h >>= 1) {
+            if (threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) w.bc_part[g * m.n_slots + slot] = red[0];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        __threadfence();
+        is_last = atomicAdd(w.bc_counter, 1) == m.n_slots - 1;
+    }
+    __syncthreads();
+    if (is_last && threadIdx.x < m.n_groups) {
+        __threadfence();
+        const int g = threadIdx.x;
+        float sum = 0.0f;
+        for (int i = 0; i < m.n_slots; ++i)
+            sum += __hip_atomic_load(w.bc_part + g * m.n_slots + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        w.bc_mean[g] = sum / m.group_count[g];
+    }
+    if (is_last && threadIdx.x == 0) atomicExch(w.bc_counter, 0);
+}
+
+// Pre-pass: boundary means of `prim` when no previous stage produced them.
+template <int D>
+__global__ void __launch_bounds__(kThreads) k_bc_mean(FvmMeshDev m, FvmPhys p, FvmWork w,
+                                                      const float *__restrict__ prim) {
+    const int64_t c = static_cast<int64_t>(m.slot_blk[blockIdx.x]) * kThreads + threadIdx.x;
+    const bool live = c < m.n_cells;
+    const float T = live ? prim[c * (D + 2) + D + 1] : 0.0f;
+    bc_mean_block<D>(m, p, w, c, live, T, blockIdx.x);
 }
 
 // ------------------------------------------------------------------------------------
@@ -219,20 +241,29 @@ k_bc(FvmMeshDev m, FvmPhys p, const float *__restrict__ prim, const float *__res
 // ------------------------------------------------------------------------------------
 template <int D>
 __global__ void __launch_bounds__(kThreads)
-k_grad(FvmMeshDev m, FvmPhys p, const float *__restrict__ prim, const float *__restrict__ ubc, float *__restrict__ gp) {
+k_grad(FvmMeshDev m, FvmPhys p, FvmWork w, const float *__restrict__ prim, int has_prev) {
     using Dm = Dims<D>;
     constexpr int C = Dm::C, NL = Dm::NL;
     const int64_t N = m.n_cells;
     const int64_t c = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
     if (c >= N) return;
+    float *__restrict__ gp = w.gp;
     float P[C];
     load_prim<D>(prim + c * C, P);
     float Un[NL][C];
 #pragma unroll
     for (int k = 0; k < NL; ++k) {
         const int64_t nb = m.cell_nbr[k * N + c];
-        const float *src = nb < N ? prim + nb * C : ubc + (nb - N) * C;
-        load_prim<D>(src, Un[k]);
+        if (nb < N) {
+            load_prim<D>(prim + nb * C, Un[k]);
+        } else {
+            // this cell is the boundary facet's only cell: it computes the facet's values
+            // (reading its own previous gradient record before overwriting it below)
+            const int64_t b = nb - N;
+            bc_value<D>(m, p, b, P, gp + c * Dm::REC, has_prev, w.bc_mean, Un[k]);
+#pragma unroll
+            for (int j = 0; j < C; ++j) w.ubc[b * C + j] = Un[k][j];
+        }
     }
     const int info = m.cell_info[c];
     float g[D][C];
@@ -437,8 +468,10 @@ enum { EPI_DUDT = 0, EPI_STAGE = 1 };
 
 template <int D, int EPI>
 __global__ void __launch_bounds__(kThreads)
-k_flux(FvmMeshDev m, FvmPhys p, const float *__restrict__ prim, const float *__restrict__ gp,
-       const float *__restrict__ ubc, float *__restrict__ dudt, FvmStage st, double *__restrict__ red) {
+k_flux(FvmMeshDev m, FvmPhys p, FvmWork w, const float *__restrict__ prim, float *__restrict__ dudt, FvmStage st) {
+    const float *__restrict__ gp = w.gp;
+    const float *__restrict__ ubc = w.ubc;
+    double *__restrict__ red = w.red;
     using Dm = Dims<D>;
     constexpr int C = Dm::C, NL = Dm::NL;
     const int64_t N = m.n_cells;
@@ -453,7 +486,9 @@ k_flux(FvmMeshDev m, FvmPhys p, const float *__restrict__ prim, const float *__r
         const float *gc = gp + c * Dm::REC;
         const int info = m.cell_info[c];
         const float inv_vol = m.cell_inv_vol[c];
-#pragma unroll
+        // not unrolled: one facet's flux at a time keeps the register footprint (and so
+        // the waves in flight per SIMD) at what a single facet needs
+#pragma unroll 1
         for (int k = 0; k < NL; ++k) {
             const int64_t f = m.cell_facet[k * N + c];
             const int64_t nb = m.cell_nbr[k * N + c];
@@ -498,11 +533,24 @@ k_flux(FvmMeshDev m, FvmPhys p, const float *__restrict__ prim, const float *__r
                     gdn[jg] = m.bc_ga[b * C + j] * Pc[j] + m.bc_gb[b * C + j];
                 }
             }
+            // canonical slot order by selects, not branches: lanes of a wavefront
+            // disagree on the slot, and a branch would execute the flux twice
+            const bool sw = own_slot != 0;
+            float U0s[C], U1s[C], g0s[D][D + 1], g1s[D][D + 1];
+#pragma unroll
+            for (int j = 0; j < C; ++j) {
+                U0s[j] = sw ? Ux[j] : Uo[j];
+                U1s[j] = sw ? Uo[j] : Ux[j];
+            }
+#pragma unroll
+            for (int i = 0; i < D; ++i)
+#pragma unroll
+                for (int j = 0; j <= D; ++j) {
+                    g0s[i][j] = sw ? glx[i][j] : glo[i][j];
+                    g1s[i][j] = sw ? glo[i][j] : glx[i][j];
+                }
             float F[C];
-            if (own_slot == 0)
-                facet_flux<D>(p, Uo, Ux, glo, glx, gdn, geo, F);
-            else
-                facet_flux<D>(p, Ux, Uo, glx, glo, gdn, geo, F);
+            facet_flux<D>(p, U0s, U1s, g0s, g1s, gdn, geo, F);
             const float v = own_slot ? inv_vol : -inv_vol;
 #pragma unroll
             for (int j = 0; j < C; ++j) acc[j] = acc[j] + v * F[j];
@@ -518,6 +566,9 @@ k_flux(FvmMeshDev m, FvmPhys p, const float *__restrict__ prim, const float *__r
     double sq[3 * C];
 #pragma unroll
     for (int j = 0; j < 3 * C; ++j) sq[j] = 0.0;
+    float Pn[C];
+#pragma unroll
+    for (int j = 0; j < C; ++j) Pn[j] = 0.0f;
     if (live) {
         const float dt = *st.dt;
         const int64_t o = c * C;
@@ -536,7 +587,7 @@ k_flux(FvmMeshDev m, FvmPhys p, const float *__restrict__ prim, const float *__r
 #pragma unroll
             for (int j = 0; j < C; ++j) a2[j] = st.stage == 0 ? st.b2 * kk[j] : st.acc2[o + j] + st.b2 * kk[j];
         }
-        float Un[C], Pn[C];
+        float Un[C];
         if (!st.last) {
 #pragma unroll
             for (int j = 0; j < C; ++j) {
@@ -567,6 +618,11 @@ k_flux(FvmMeshDev m, FvmPhys p, const float *__restrict__ prim, const float *__r
         state_to_prim<D>(Un, p.inv_cv, Pn);
 #pragma unroll
         for (int j = 0; j < C; ++j) st.prim_next[o + j] = Pn[j];
+    }
+    // boundary means of the next stage's primitives (block-uniform branch)
+    if (m.n_groups > 0) {
+        const int slot = m.blk_slot[blockIdx.x];
+        if (slot >= 0) bc_mean_block<D>(m, p, w, c, live, Pn[D + 1], slot);
     }
     if (st.last && st.acc2) {
         // fp64 block partials of the three column norms (Adaptive.update_stepsize)
@@ -633,37 +689,49 @@ __global__ void __launch_bounds__(kThreads) k_state_to_prim3(const float *__rest
 }
 
 static int check_mesh(const FvmMeshDev *m, const FvmPhys *p, const FvmWork *w, const char *who) {
-    if (!m || !p || !w || (m->dim != 2 && m->dim != 3) || m->n_cells <= 0 || m->n_groups < 0 || m->n_groups > 4 ||
-        m->n_bc < 0 || !w->gp || (m->n_bc > 0 && !w->ubc) || (m->n_groups > 0 && !w->bc_part) ||
-        p->lim_p < 1 || p->lim_p > 5 || m->n_cells > (int64_t(1) << 31) - 1) {
+    if (!m || !p || !w || (m->dim != 2 && m->dim != 3) || m->n_cells <= 0 || m->n_groups < 0 ||
+        m->n_groups > kMaxGroups || m->n_bc < 0 || !w->gp || (m->n_bc > 0 && !w->ubc) ||
+        (m->n_groups > 0 && (!w->bc_part || !w->bc_mean || !w->bc_counter || m->n_slots < 1 || !m->blk_slot ||
+                             !m->slot_blk)) ||
+        p->lim_p < 1 || p->lim_p > 5 || m->n_cells > (int64_t(1) << 30)) {
         set_error("%s: invalid mesh/work/phys arguments", who);
         return FVM_ERR_ARG;
     }
     return FVM_OK;
 }
 
+// parts: 1 = boundary-mean pre-pass, 4 = gradient + limiter (+ boundary values),
+// 8 = flux + divergence (+ stage update).  The pre-pass runs only when `need_mean`.
 template <int D>
 static int launch_rhs(const FvmMeshDev &m, const FvmPhys &p, const FvmWork &w, const float *prim, float *dudt,
-                      const FvmStage *st, hipStream_t s, int has_prev, int parts = 15) {
-    const int64_t nbp_full = (m.n_bc + kThreads - 1) / kThreads;
-    const int nbp = static_cast<int>(nbp_full < kBcPart ? nbp_full : kBcPart);
-    if ((parts & 1) && m.n_groups > 0 && nbp > 0)
-        hipLaunchKernelGGL(k_bc_sum<D>, dim3(nbp), dim3(kThreads), 0, s, m, p, prim, w.bc_part);
-    if ((parts & 2) && m.n_bc > 0)
-        hipLaunchKernelGGL(k_bc<D>, dim3(blocks_for(m.n_bc, kThreads)), dim3(kThreads), 0, s, m, p, prim, w.gp, w.ubc,
-                           w.bc_part, nbp, has_prev);
+                      const FvmStage *st, hipStream_t s, int has_prev, int need_mean, int parts = 15) {
+    if ((parts & 1) && need_mean && m.n_groups > 0)
+        hipLaunchKernelGGL(k_bc_mean<D>, dim3(m.n_slots), dim3(kThreads), 0, s, m, p, w, prim);
     const dim3 grid(blocks_for(m.n_cells, kThreads));
-    if (parts & 4) hipLaunchKernelGGL(k_grad<D>, grid, dim3(kThreads), 0, s, m, p, prim, w.ubc, w.gp);
-    if (!(parts & 8)) return check_launch("fvm_rk_stage_part_f32");
-    if (st) {
-        hipLaunchKernelGGL((k_flux<D, EPI_STAGE>), grid, dim3(kThreads), 0, s, m, p, prim, w.gp, w.ubc, nullptr, *st,
-                           w.red);
-    } else {
-        FvmStage none{};
-        hipLaunchKernelGGL((k_flux<D, EPI_DUDT>), grid, dim3(kThreads), 0, s, m, p, prim, w.gp, w.ubc, dudt, none,
-                           nullptr);
+    if (parts & 4) hipLaunchKernelGGL(k_grad<D>, grid, dim3(kThreads), 0, s, m, p, w, prim, has_prev);
+    if (parts & 8) {
+        if (st) {
+            hipLaunchKernelGGL((k_flux<D, EPI_STAGE>), grid, dim3(kThreads), 0, s, m, p, w, prim, nullptr, *st);
+        } else {
+            FvmStage none{};
+            hipLaunchKernelGGL((k_flux<D, EPI_DUDT>), grid, dim3(kThreads), 0, s, m, p, w, prim, dudt, none);
+        }
     }
     return check_launch(st ? "fvm_rk_stage_f32" : "fvm_rhs_f32");
+}
+
+static int check_stage(const FvmWork *w, const FvmStage *st, const char *who) {
+    if (!st || !st->prim || !st->U0 || !st->acc || !st->prim_next || !st->dt || (st->last && !st->U_out) ||
+        st->n_inc < 0 || st->n_inc > 10 || (st->store_k && !st->kbuf) || (st->last && st->acc2 && !w->red)) {
+        set_error("%s: invalid stage arguments", who);
+        return FVM_ERR_ARG;
+    }
+    for (int t = 0; t < st->n_inc; ++t)
+        if (st->inc_idx[t] < 0 || st->inc_idx[t] > st->stage || (st->inc_idx[t] != st->stage && !st->kbuf)) {
+            set_error("%s: increment refers to a stage that is not stored", who);
+            return FVM_ERR_ARG;
+        }
+    return FVM_OK;
 }
 
 }  // namespace fvm
@@ -680,42 +748,30 @@ int fvm_rhs_f32(const FvmMeshDev *m, const FvmPhys *p, const FvmWork *w, const f
         set_error("fvm_rhs_f32: null prim/dudt");
         return FVM_ERR_ARG;
     }
-    return m->dim == 2 ? launch_rhs<2>(*m, *p, *w, prim, dudt, nullptr, as_stream(stream), has_prev)
-                       : launch_rhs<3>(*m, *p, *w, prim, dudt, nullptr, as_stream(stream), has_prev);
+    return m->dim == 2 ? launch_rhs<2>(*m, *p, *w, prim, dudt, nullptr, as_stream(stream), has_prev, 1)
+                       : launch_rhs<3>(*m, *p, *w, prim, dudt, nullptr, as_stream(stream), has_prev, 1);
 }
 
 int fvm_rk_stage_f32(const FvmMeshDev *m, const FvmPhys *p, const FvmWork *w, const FvmStage *st, void *stream) {
     int e = check_mesh(m, p, w, "fvm_rk_stage_f32");
+    if (!e) e = check_stage(w, st, "fvm_rk_stage_f32");
     if (e) return e;
-    if (!st || !st->prim || !st->U0 || !st->acc || !st->prim_next || !st->dt || (st->last && !st->U_out) ||
-        st->n_inc < 0 || st->n_inc > 10 || (st->store_k && !st->kbuf) || (st->last && st->acc2 && !w->red)) {
-        set_error("fvm_rk_stage_f32: invalid stage arguments");
-        return FVM_ERR_ARG;
-    }
-    for (int t = 0; t < st->n_inc; ++t)
-        if (st->inc_idx[t] < 0 || st->inc_idx[t] > st->stage || (st->inc_idx[t] != st->stage && !st->kbuf)) {
-            set_error("fvm_rk_stage_f32: increment refers to a stage that is not stored");
-            return FVM_ERR_ARG;
-        }
-    return m->dim == 2 ? launch_rhs<2>(*m, *p, *w, st->prim, nullptr, st, as_stream(stream), st->has_prev)
-                       : launch_rhs<3>(*m, *p, *w, st->prim, nullptr, st, as_stream(stream), st->has_prev);
+    return m->dim == 2 ? launch_rhs<2>(*m, *p, *w, st->prim, nullptr, st, as_stream(stream), st->has_prev, !st->bc_ready)
+                       : launch_rhs<3>(*m, *p, *w, st->prim, nullptr, st, as_stream(stream), st->has_prev, !st->bc_ready);
 }
 
 int fvm_rk_stage_part_f32(const FvmMeshDev *m, const FvmPhys *p, const FvmWork *w, const FvmStage *st,
                           int32_t parts, void *stream) {
     if (parts < 1 || parts > 15) {
-        set_error("fvm_rk_stage_part_f32: parts must be a nonzero mask of 1|2|4|8");
+        set_error("fvm_rk_stage_part_f32: parts must be a nonzero mask of 1|4|8");
         return FVM_ERR_ARG;
     }
     int e = check_mesh(m, p, w, "fvm_rk_stage_part_f32");
+    if (!e) e = check_stage(w, st, "fvm_rk_stage_part_f32");
     if (e) return e;
-    if (!st || !st->prim || !st->U0 || !st->acc || !st->prim_next || !st->dt || (st->last && !st->U_out) ||
-        st->n_inc < 0 || st->n_inc > 10 || (st->store_k && !st->kbuf) || (st->last && st->acc2 && !w->red)) {
-        set_error("fvm_rk_stage_part_f32: invalid stage arguments");
-        return FVM_ERR_ARG;
-    }
-    return m->dim == 2 ? launch_rhs<2>(*m, *p, *w, st->prim, nullptr, st, as_stream(stream), st->has_prev, parts)
-                       : launch_rhs<3>(*m, *p, *w, st->prim, nullptr, st, as_stream(stream), st->has_prev, parts);
+    return m->dim == 2
+               ? launch_rhs<2>(*m, *p, *w, st->prim, nullptr, st, as_stream(stream), st->has_prev, !st->bc_ready, parts)
+               : launch_rhs<3>(*m, *p, *w, st->prim, nullptr, st, as_stream(stream), st->has_prev, !st->bc_ready, parts);
 }
 
 int fvm_update_stepsize(const FvmWork *w, int32_t C, int32_t nblk, float atol, float rtol, float mtol, float order,

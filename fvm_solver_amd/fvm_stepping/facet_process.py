@@ -197,6 +197,15 @@ class DeviceMesh:
                 bc_char[mask, D + 1] = vn
                 bc_char[mask, D + 3:] = vinf[None, :] - nn * vn[:, None]
         self.n_groups = len(modes)
+        gcount = np.zeros(4, np.float32)
+        for q in range(self.n_groups):
+            gcount[q] = np.float32((group == q).sum())
+        # 256-cell blocks holding interior cells of group facets: one partial-sum slot each
+        nblk = (N + 255) // 256
+        blocks = np.unique(bcell[group >= 0] // 256) if self.n_groups else np.zeros(0, np.int64)
+        blk_slot = np.full(nblk, -1, np.int32)
+        blk_slot[blocks] = np.arange(len(blocks), dtype=np.int32)
+        self.n_slots = len(blocks)
 
         # ---- upload ----
         dev = torch.device(device)
@@ -210,7 +219,8 @@ class DeviceMesh:
             cell_disp=up(disp_s.transpose(1, 2, 0), f32), cell_gval=up(gval.transpose(2, 1, 0), f32),
             cell_inv_vol=up(inv_vol, f32), cell_eps=up(eps, f32), fac_geo=up(geo, f32),
             bc_cell=up(bcell, i32), bc_group=up(group, i32), bc_flags=up(flags, i32), bc_val=up(bc_val, f32),
-            bc_l=up(l_vec, f32), bc_ga=up(bc_ga, f32), bc_gb=up(bc_gb, f32), bc_char=up(bc_char, f32))
+            bc_l=up(l_vec, f32), bc_ga=up(bc_ga, f32), bc_gb=up(bc_gb, f32), bc_char=up(bc_char, f32),
+            blk_slot=up(blk_slot, i32), slot_blk=up(np.maximum(blocks, 0) if len(blocks) else np.zeros(1), i32))
         self.host = dict(c2f_s=c2f_s, nbr_s=nbr_s, slot_s=slot_s, nbk=nbk, gval=gval, src=src, info=info,
                          inv_vol=inv_vol, eps=eps, geo=geo, bcell=bcell, group=group, flags=flags, bc_val=bc_val,
                          bc_l=l_vec.astype(np.float32), bc_ga=bc_ga, bc_gb=bc_gb, bc_char=bc_char, side=side)
@@ -221,8 +231,10 @@ class DeviceMesh:
         per = fs64[c2f].sum(1)
         self.h64 = up(D * g["volumes"] / per, torch.float64)     # CFL length d*vol/sum|n|
         self.struct = nat.FvmMeshDev(
-            dim=D, n_groups=self.n_groups, n_cells=N, n_facets=F, n_bc=Fb,
+            dim=D, n_groups=self.n_groups, n_cells=N, n_facets=F, n_bc=Fb, n_slots=self.n_slots,
             **{k: v.data_ptr() for k, v in self.t.items()})
+        for q in range(4):
+            self.struct.group_count[q] = float(gcount[q])
         # views used by tests and the Saver
         self.slots_sorted = slot_s
         self.bc_facet_side = side
@@ -263,11 +275,15 @@ class FacetCalc:
         self.nblk = (N + 255) // 256
         self.ubc = torch.zeros((max(Fb, 1), C), dtype=torch.float32, device=dev)
         self.gp = torch.zeros((N, (D + 1) * C), dtype=torch.float32, device=dev)
-        self.bc_part = torch.zeros(4 * 2 * _K_BC_PART, dtype=torch.float32, device=dev)
+        self.bc_part = torch.zeros(4 * max(self.mesh.n_slots, 1), dtype=torch.float32, device=dev)
+        self.bc_mean = torch.zeros(4, dtype=torch.float32, device=dev)
+        self.bc_counter = torch.zeros(1, dtype=torch.int32, device=dev)
         self.red = torch.zeros(self.nblk * 3 * C, dtype=torch.float64, device=dev)
         self.work = nat.FvmWork(ubc=self.ubc.data_ptr(), gp=self.gp.data_ptr(), bc_part=self.bc_part.data_ptr(),
+                                bc_mean=self.bc_mean.data_ptr(), bc_counter=self.bc_counter.data_ptr(),
                                 red=self.red.data_ptr())
         self.has_prev = False
+        self.bc_ready_ptr = None     # prim buffer whose boundary means bc_mean holds
 
     # The reference keeps the previous call's limited gradients in ``cell_grads``.
     @property
@@ -281,6 +297,7 @@ class FacetCalc:
     def reset_state(self):
         """Forget the previous call's gradients (the reference's ``cell_grads = None``)."""
         self.has_prev = False
+        self.bc_ready_ptr = None
 
     def rhs(self, prim: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
         nat.require_device(prim, "primatives")
@@ -292,13 +309,27 @@ class FacetCalc:
                                    int(self.has_prev), nat.stream_ptr(self.device))
         nat.check(rc, "fvm_rhs_f32")
         self.has_prev = True
+        self.bc_ready_ptr = None
         return dudt
 
-    def stage(self, st: "nat.FvmStage"):
+    def stage(self, st: "nat.FvmStage", parts: int | None = None):
+        """One fused RK stage; the boundary means are reused when the previous stage's
+        epilogue computed them for exactly this primitive buffer."""
         st.has_prev = int(self.has_prev)
-        rc = nat.lib().fvm_rk_stage_f32(self.mesh.struct, self.phys, self.work, st, nat.stream_ptr(self.device))
+        st.bc_ready = int(self.bc_ready_ptr is not None and self.bc_ready_ptr == st.prim)
+        if parts is None:
+            rc = nat.lib().fvm_rk_stage_f32(self.mesh.struct, self.phys, self.work, st, nat.stream_ptr(self.device))
+        else:
+            rc = nat.lib().fvm_rk_stage_part_f32(self.mesh.struct, self.phys, self.work, st, parts,
+                                                 nat.stream_ptr(self.device))
         nat.check(rc, "fvm_rk_stage_f32")
         self.has_prev = True
+        if parts is None or parts & 8:
+            self.bc_ready_ptr = st.prim_next
+
+    def invalidate_boundary_means(self):
+        """Call after writing primitives the stage epilogue did not produce."""
+        self.bc_ready_ptr = None
 
     def clear_temp(self):
         """The reference frees init-only tensors here (facet_process.py:290-295); the

@@ -82,6 +82,7 @@ class _FusedButcher(TSolver):
         if self._last_out is not None and U0.data_ptr() == self._last_out.data_ptr():
             return self._prim[self._flip]
         p = self._prim[self._flip]
+        self.eq.E_props.invalidate_boundary_means()
         nat.check(nat.lib().fvm_state_to_prim_f32(nat.ptr(U0), nat.ptr(p), U0.shape[0], U0.shape[1] - 2,
                                                   1 / self.cfg.C_v, nat.stream_ptr(U0.device)),
                   "fvm_state_to_prim_f32")

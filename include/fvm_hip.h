@@ -100,6 +100,12 @@ typedef struct FvmMeshDev {
     const float *bc_gb;         /* [Fb][C] b_face_grad                                */
     const float *bc_char;       /* [Fb][2d+3]: n_out[d], rho_inf, v_n_inf, p_inf,
                                    v_t_inf[d]                                         */
+    /* characteristic-BC group means (facet_boundary.py:427) ------------------------ */
+    int32_t n_slots;            /* 256-cell blocks holding cells of BC-group facets   */
+    int32_t pad0;
+    const int32_t *blk_slot;    /* [ceil(N/256)] partial slot of a block, or -1       */
+    const int32_t *slot_blk;    /* [n_slots] block of each slot                       */
+    float group_count[4];       /* facets per group                                   */
 } FvmMeshDev;
 
 typedef struct FvmPhys {      /* scalars of FluidConstitution / SlopeLimiter / KT      */
@@ -117,13 +123,16 @@ typedef struct FvmWork {      /* device scratch owned by the caller             
     float *ubc;       /* [Fb][C] boundary face values                                 */
     float *gp;        /* [N][(d+1)C] unlimited LSQ gradient [d][C] then limiter phi[C];
                          persists between calls (the next call's Neumann correction)  */
-    float *bc_part;   /* [4][2][64] per-group partial sums of c and facet counts     */
+    float *bc_part;   /* [n_groups][n_slots] partial sums of c per slot               */
+    float *bc_mean;   /* [4] group means of c of the primitives the next RHS reads    */
+    int32_t *bc_counter; /* [1] slot-completion counter, zero between launches        */
     double *red;      /* [ceil(N/256)][3C] fp64 partial column norms (adaptive dt)    */
 } FvmWork;
 
 /* One RHS evaluation dU/dt = FVMEquation.forward(prim) (fvm_equation.py:406-435).
  * has_prev != 0 applies the Neumann non-orthogonal correction with the gradients the
- * previous call left in work->gp (facet_boundary.py:135-136). */
+ * previous call left in work->gp (facet_boundary.py:135-136).  Up to three launches:
+ * boundary-mean pre-pass, gradient + limiter (+ boundary values), flux + divergence. */
 int fvm_rhs_f32(const FvmMeshDev *m, const FvmPhys *p, const FvmWork *w, const float *prim,
                 float *dudt, int32_t has_prev, void *stream);
 
@@ -143,6 +152,8 @@ typedef struct FvmStage {
     float *U_out;          /* [N][C] (last stage)                                     */
     const float *dt;       /* device scalar                                           */
     int32_t stage, last, store_k, has_prev;
+    int32_t bc_ready;      /* work->bc_mean already holds the boundary means of prim
+                              (the previous stage's epilogue wrote them)              */
     float b, b2;
     int32_t n_inc;
     int32_t inc_idx[10];
@@ -153,9 +164,9 @@ int fvm_rk_stage_f32(const FvmMeshDev *m, const FvmPhys *p, const FvmWork *w, co
                      void *stream);
 
 /* The same stage split into its launches, for per-kernel timing with stream events:
- * parts is a mask of 1 = boundary c-sum, 2 = boundary values, 4 = gradient + limiter,
- * 8 = flux + divergence + stage update.  Launching all four masks in order equals
- * fvm_rk_stage_f32.  No reference counterpart (measurement hook). */
+ * parts is a mask of 1 = boundary-mean pre-pass (only if !bc_ready), 4 = gradient +
+ * limiter + boundary values, 8 = flux + divergence + stage update.  Launching the masks
+ * in that order equals fvm_rk_stage_f32.  No reference counterpart (measurement hook). */
 int fvm_rk_stage_part_f32(const FvmMeshDev *m, const FvmPhys *p, const FvmWork *w, const FvmStage *st,
                           int32_t parts, void *stream);
 

@@ -44,8 +44,9 @@ struct Dims {
     static constexpr int CH = 2 * D + 3;
 };
 
-__device__ __forceinline__ float lim_phi(int lp, float delta, float dU, float eps) {
-    if (lp == 4) {
+template <int lp>
+__device__ __forceinline__ float lim_phi(float delta, float dU, float eps) {
+    if constexpr (lp == 4) {
         const float a = fabsf(delta), b = fabsf(dU);
         const float b2 = 2.0f * b;
         const float a2 = a * a;
@@ -55,27 +56,31 @@ __device__ __forceinline__ float lim_phi(int lp, float delta, float dU, float ep
         const float phi = (a_eps + a * S) / (a_eps + b * (d3 + S));
         return a < b2 ? phi : 1.0f;
     }
-    if (lp == 1) {
+    if constexpr (lp == 1) {
         const float s = 2.0f * ((dU > 0.0f ? 1.0f : 0.0f) - 0.5f);
         const float r = delta / (s * (fabsf(dU) + 1e-8f));
         return fminf(fmaxf(r, 0.0f), 1.0f);
     }
-    if (lp == 2) {
+    if constexpr (lp == 2) {
         const float d2 = delta * delta;
         return ((d2 + eps) + (2.0f * delta) * dU) / (((d2 + 2.0f * (dU * dU)) + delta * dU) + eps);
     }
-    if (lp == 3) {
+    if constexpr (lp == 3) {
         const float a = fabsf(delta), b = fabsf(dU);
         const float a_eps = (a * a) * a + eps;
         const float S = 4.0f * (b * b);
         const float phi = (a_eps + a * S) / (a_eps + b * (delta * delta + S));
         return a < 2.0f * b ? phi : 1.0f;
     }
-    const float a = fabsf(delta), b = fabsf(dU);
-    const float a_eps = powf(a, 5.0f) + eps;
-    const float S = (8.0f * (b * b)) * (a * a - (2.0f * b) * (a - b));
-    const float phi = (a_eps + a * S) / (a_eps + b * (powf(delta, 4.0f) + S));
-    return a < 2.0f * b ? phi : 1.0f;
+    if constexpr (lp == 5) {
+        const float a = fabsf(delta), b = fabsf(dU);
+        const float a_eps = powf(a, 5.0f) + eps;
+        const float S = (8.0f * (b * b)) * (a * a - (2.0f * b) * (a - b));
+        const float phi = (a_eps + a * S) / (a_eps + b * (powf(delta, 4.0f) + S));
+        return a < 2.0f * b ? phi : 1.0f;
+    }
+    static_assert(lp >= 1 && lp <= 5, "limiter order 1..5");
+    return 1.0f;
 }
 
 template <int D>
@@ -239,7 +244,7 @@ __global__ void __launch_bounds__(kThreads) k_bc_mean(FvmMeshDev m, FvmPhys p, F
 // ------------------------------------------------------------------------------------
 // LSQ gradient + slope limiter
 // ------------------------------------------------------------------------------------
-template <int D>
+template <int D, int LP>
 __global__ void __launch_bounds__(kThreads)
 k_grad(FvmMeshDev m, FvmPhys p, FvmWork w, const float *__restrict__ prim, int has_prev) {
     using Dm = Dims<D>;
@@ -307,7 +312,7 @@ k_grad(FvmMeshDev m, FvmPhys p, FvmWork w, const float *__restrict__ prim, int h
             float dU = g[0][j] * disp[k][0];
 #pragma unroll
             for (int i = 1; i < D; ++i) dU = dU + g[i][j] * disp[k][i];
-            const float lk = lim_phi(p.lim_p, dU > 0.0f ? upper : lower, dU, eps);
+            const float lk = lim_phi<LP>(dU > 0.0f ? upper : lower, dU, eps);
             ph = (k == 0) ? lk : fminf(ph, lk);
         }
         phi[j] = ph;
@@ -708,7 +713,15 @@ static int launch_rhs(const FvmMeshDev &m, const FvmPhys &p, const FvmWork &w, c
     if ((parts & 1) && need_mean && m.n_groups > 0)
         hipLaunchKernelGGL(k_bc_mean<D>, dim3(m.n_slots), dim3(kThreads), 0, s, m, p, w, prim);
     const dim3 grid(blocks_for(m.n_cells, kThreads));
-    if (parts & 4) hipLaunchKernelGGL(k_grad<D>, grid, dim3(kThreads), 0, s, m, p, w, prim, has_prev);
+    if (parts & 4) {
+        switch (p.lim_p) {
+            case 1: hipLaunchKernelGGL((k_grad<D, 1>), grid, dim3(kThreads), 0, s, m, p, w, prim, has_prev); break;
+            case 2: hipLaunchKernelGGL((k_grad<D, 2>), grid, dim3(kThreads), 0, s, m, p, w, prim, has_prev); break;
+            case 3: hipLaunchKernelGGL((k_grad<D, 3>), grid, dim3(kThreads), 0, s, m, p, w, prim, has_prev); break;
+            case 5: hipLaunchKernelGGL((k_grad<D, 5>), grid, dim3(kThreads), 0, s, m, p, w, prim, has_prev); break;
+            default: hipLaunchKernelGGL((k_grad<D, 4>), grid, dim3(kThreads), 0, s, m, p, w, prim, has_prev); break;
+        }
+    }
     if (parts & 8) {
         if (st) {
             hipLaunchKernelGGL((k_flux<D, EPI_STAGE>), grid, dim3(kThreads), 0, s, m, p, w, prim, nullptr, *st);

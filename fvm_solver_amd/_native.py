@@ -12,7 +12,7 @@ import torch
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libfvm_hip.so")
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 FVM_ERR_ROW_OVERFLOW = 2
 
@@ -46,7 +46,8 @@ class FvmStage(ctypes.Structure):
     _fields_ = [("prim", c_void_p), ("U0", c_void_p), ("acc", c_void_p), ("acc2", c_void_p), ("kbuf", c_void_p),
                 ("prim_next", c_void_p), ("U_out", c_void_p), ("dt", c_void_p),
                 ("stage", i32), ("last", i32), ("store_k", i32), ("has_prev", i32), ("bc_ready", i32),
-                ("b", f32), ("b2", f32), ("n_inc", i32), ("inc_idx", i32 * 10), ("inc_coef", f32 * 10)]
+                ("b", f32), ("b2", f32), ("n_inc", i32), ("inc_idx", i32 * 10), ("inc_coef", f32 * 10),
+                ("mode", i32), ("n_hist", i32), ("f_m1", c_void_p), ("f_m2", c_void_p), ("f_out", c_void_p)]
 
 
 _PROTOS = {

@@ -574,7 +574,25 @@ k_flux(FvmMeshDev m, FvmPhys p, FvmWork w, const float *__restrict__ prim, float
     float Pn[C];
 #pragma unroll
     for (int j = 0; j < C; ++j) Pn[j] = 0.0f;
-    if (live) {
+    if (live && st.mode == 1) {
+        // Adams-Bashforth 3 (fixed dt, one RHS per step)
+        const float c12 = *st.dt / 12.0f;
+        const int64_t o = c * C;
+        float U1[C];
+#pragma unroll
+        for (int j = 0; j < C; ++j) {
+            const float f0 = acc[j];
+            const float fm1 = st.n_hist >= 1 ? st.f_m1[o + j] : f0;
+            const float fm2 = st.n_hist >= 2 ? st.f_m2[o + j] : fm1;
+            const float comb = (23.0f * f0 - 16.0f * fm1) + 5.0f * fm2;
+            U1[j] = st.U0[o + j] + c12 * comb;
+            st.f_out[o + j] = f0;
+            st.U_out[o + j] = U1[j];
+        }
+        state_to_prim<D>(U1, p.inv_cv, Pn);
+#pragma unroll
+        for (int j = 0; j < C; ++j) st.prim_next[o + j] = Pn[j];
+    } else if (live) {
         const float dt = *st.dt;
         const int64_t o = c * C;
         float kk[C], a[C], U0[C];
@@ -734,7 +752,15 @@ static int launch_rhs(const FvmMeshDev &m, const FvmPhys &p, const FvmWork &w, c
 }
 
 static int check_stage(const FvmWork *w, const FvmStage *st, const char *who) {
-    if (!st || !st->prim || !st->U0 || !st->acc || !st->prim_next || !st->dt || (st->last && !st->U_out) ||
+    if (st && st->mode == 1) {
+        if (!st->prim || !st->U0 || !st->prim_next || !st->dt || !st->U_out || !st->f_out || st->n_hist < 0 ||
+            (st->n_hist >= 1 && !st->f_m1) || (st->n_hist >= 2 && !st->f_m2) || st->acc2) {
+            set_error("%s: invalid Adams-Bashforth stage arguments", who);
+            return FVM_ERR_ARG;
+        }
+        return FVM_OK;
+    }
+    if (!st || st->mode != 0 || !st->prim || !st->U0 || !st->acc || !st->prim_next || !st->dt || (st->last && !st->U_out) ||
         st->n_inc < 0 || st->n_inc > 10 || (st->store_k && !st->kbuf) || (st->last && st->acc2 && !w->red)) {
         set_error("%s: invalid stage arguments", who);
         return FVM_ERR_ARG;

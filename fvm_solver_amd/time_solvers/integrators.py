@@ -243,19 +243,43 @@ class Adams4PC(Adams3PC):
         return U0 + dU_high
 
 
-class AB3(TSolver):
-    """Fixed-dt Adams-Bashforth 3: U1 = U0 + dt/12 (23 f0 - 16 f-1 + 5 f-2)."""
+class AB3(_FusedButcher):
+    """Fixed-dt Adams-Bashforth 3, one fused RHS per step (build-defined: the reference
+    has no runnable Adams solver):  U1 = U0 + dt/12 (23 f0 - 16 f-1 + 5 f-2), with the
+    missing history of the first two steps replaced by f0 (so step 1 is forward Euler).
+    The history lives in a ring of three device buffers; nothing is allocated per step."""
 
     def __init__(self, cells, equation, cfg):
-        super().__init__(cells, equation, cfg)
-        self.hist = None
+        super().__init__(cells, equation, "Euler", cfg)
+        n = cells.state.shape
+        self._f = [torch.empty(n, device=cells.state.device) for _ in range(3)]
+        self._hist = []          # indices into _f, most recent first
+
+    def reset_history(self):
+        self._hist = []
 
     def _step(self, t):
         U0 = self.cells.state
-        f0 = self._forward_state(U0)
-        fm1, fm2 = self.hist if self.hist is not None else (f0, f0)
-        self.hist = (f0, fm1)
-        return U0 + self.dt / 12 * (23 * f0 - 16 * fm1 + 5 * fm2)
+        if not U0.is_contiguous() or U0.dtype != torch.float32:
+            U0 = U0.contiguous().float()
+        nat.require_device(U0, "state")
+        E = self.eq.E_props
+        out = self._out[1] if (self._last_out is not None and self._last_out.data_ptr() == self._out[0].data_ptr()) \
+            else self._out[0]
+        cur = self._stage_prim0(U0)
+        nxt = self._prim[1 - self._flip]
+        free = [i for i in range(3) if i not in self._hist[:2]][0]
+        st = nat.FvmStage(prim=cur.data_ptr(), U0=U0.data_ptr(), acc=None, prim_next=nxt.data_ptr(),
+                          U_out=out.data_ptr(), dt=self.dt.data_ptr(), stage=0, last=1, mode=1,
+                          n_hist=min(len(self._hist), 2),
+                          f_m1=self._f[self._hist[0]].data_ptr() if len(self._hist) >= 1 else None,
+                          f_m2=self._f[self._hist[1]].data_ptr() if len(self._hist) >= 2 else None,
+                          f_out=self._f[free].data_ptr())
+        E.stage(st)
+        self._flip = 1 - self._flip
+        self._hist = [free] + self._hist[:1]
+        self._last_out = out
+        return out
 
 
 def get_solver(cells: FVMCells, equation, cfg) -> TSolver:

@@ -158,6 +158,14 @@ typedef struct FvmStage {
     int32_t n_inc;
     int32_t inc_idx[10];
     float inc_coef[10];
+    /* mode 1 = fixed-dt Adams-Bashforth 3 (one RHS per step, build-defined):
+       U_out = U0 + dt/12 (23 f0 - 16 f_m1 + 5 f_m2), f0 written to f_out;
+       n_hist < 2 substitutes f0 for the missing history (f_m1, then f_m2) */
+    int32_t mode;
+    int32_t n_hist;
+    const float *f_m1;     /* [N][C]                                                  */
+    const float *f_m2;     /* [N][C]                                                  */
+    float *f_out;          /* [N][C]                                                  */
 } FvmStage;
 
 int fvm_rk_stage_f32(const FvmMeshDev *m, const FvmPhys *p, const FvmWork *w, const FvmStage *st,

@@ -219,3 +219,25 @@ def test_hip_graph_replay_matches_eager():
     _announce("graph replay vs eager")
     for a, b in zip(*runs):
         assert torch.equal(a, b)
+
+
+def test_ab3_fused_against_oracle():
+    """Fused Adams-Bashforth 3 (build-defined) against the oracle's AB3, 100 steps."""
+    from fvm_solver_amd.mesh_gen.channel import channel_mesh_2d
+    pts, cells, (fi, fb), tags = channel_mesh_2d(2000, seed=6)
+    cfg = make_cfg(2, solver_name="AB3", solver_extra="")
+    mesh, eq = product_equation(pts, cells, fi, fb, tags, cfg)
+    prim0, U0 = initial_state(mesh.centroids.numpy(), cfg, 2)
+    eq.cells.state = U0.to(DEV)
+    dt = 0.5 * float(eq.cfl_dt(prim0.to(DEV), cfg.cfl))
+    eq.t_solver.set_dt(dt)
+    _announce(f"ab3 100 steps N={mesh.n_cells}")
+    _, o32 = O.from_mesh_tuple(pts, cells, fi, fb, tags, oracle_phys(cfg))
+    st = O.OracleStepper(o32, "AB3", dt=dt)
+    U = U0.clone()
+    for i in range(100):
+        eq.t_solver._solve_step(None)
+        U = st.step(U)
+    err = rel_l2_cols(eq.cells.state.cpu().numpy(), U.numpy())
+    _announce(f"  rel L2 after 100 AB3 steps {err}")
+    assert np.all(err <= 1e-4), err

@@ -30,10 +30,14 @@ def _parse():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--cells", type=int, default=1_000_000)
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--cpu-steps", type=int, default=4)
+    ap.add_argument("--cpu-steps", type=int, default=30)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--kernel-timing-steps", type=int, default=20)
     ap.add_argument("--mesh-cache", default=None, help="npz file to reuse the generated mesh from (created if absent)")
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--traffic-file", default=os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                                                           "rk4_channel2d_1M_traffic.json"),
+                    help="measured HBM bytes per launch (rocprofv3 FETCH_SIZE/WRITE_SIZE, calibrated)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run one 256 MiB device-to-device copy (calibrates HBM byte counters)")
     return ap.parse_args()
@@ -112,6 +116,12 @@ def main():
 
     for _ in range(a.warmup):
         ts._solve_step(None)
+    tuned = None
+    if not a.no_autotune:
+        from fvm_solver_amd import autotune
+        tuned = autotune.tune(lambda: ts._solve_step(None), eq.E_props.mesh, N, D)
+        log(f"autotune: persist_blocks={tuned['persist_blocks']} "
+            + ", ".join(f"{k}:{v:.0f}us" for k, v in tuned["us_per_step"].items()))
     torch.cuda.synchronize()
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     wall0 = time.perf_counter()
@@ -204,6 +214,13 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None},
     }
+    out["config"]["persist_blocks"] = tuned["persist_blocks"] if tuned else 0
+    out["config"]["algorithmic_bytes_per_launch"] = dom_bytes
+    if a.traffic_file and os.path.exists(a.traffic_file):
+        tr = json.load(open(a.traffic_file))
+        if tr.get("kernel") == dom and tr.get("cells") == N:
+            out["roofline"]["traffic"] = tr["read_bytes"] + tr["write_bytes"]
+            out["roofline"]["traffic_source"] = os.path.basename(a.traffic_file)
 
     if not a.no_cpu_baseline:
         sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))

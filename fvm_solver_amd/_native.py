@@ -12,7 +12,7 @@ import torch
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libfvm_hip.so")
-ABI_VERSION = 3
+ABI_VERSION = 4
 
 FVM_ERR_ROW_OVERFLOW = 2
 
@@ -28,8 +28,10 @@ class FvmMeshDev(ctypes.Structure):
                 ("cell_eps", c_void_p), ("fac_geo", c_void_p),
                 ("bc_cell", c_void_p), ("bc_group", c_void_p), ("bc_flags", c_void_p), ("bc_val", c_void_p),
                 ("bc_l", c_void_p), ("bc_ga", c_void_p), ("bc_gb", c_void_p), ("bc_char", c_void_p),
-                ("n_slots", i32), ("pad0", i32), ("blk_slot", c_void_p), ("slot_blk", c_void_p),
-                ("group_count", f32 * 4)]
+                ("n_slots", i32), ("n_meshes", i32), ("blk_slot", c_void_p), ("slot_blk", c_void_p),
+                ("slot_g0", c_void_p), ("slot_ng", c_void_p), ("group_s0", c_void_p), ("group_s1", c_void_p),
+                ("group_count", c_void_p), ("cell_mesh", c_void_p), ("mesh_phys", c_void_p),
+                ("persist_blocks", i32), ("pad1", i32)]
 
 
 class FvmPhys(ctypes.Structure):

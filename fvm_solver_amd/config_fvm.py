@@ -127,3 +127,15 @@ class ConfigChannel3D(ConfigChannel):
     n_iter: int | None = 500
     n_cells: int = 4_000_000
     bump_centre: tuple = (0.25, 0.5, 0.5)
+
+
+def channel_config(mach: float = MACH, diameter: float = DIAMETER, dim: int = 2, **kw) -> ConfigChannel:
+    """Channel preset at another freestream Mach number / obstacle diameter: Re 200 on
+    the diameter and Prandtl 0.72 fix viscosity and conduction."""
+    u = mach * math.sqrt(GAMMA * R_GAS * T_INF)
+    mu = RHO_INF * u * diameter / REYNOLDS
+    v3 = (u, 0.0, 0.0)
+    base = dict(viscosity=mu, visc_bulk=mu, thermal_cond=mu * GAMMA * C_V / 0.72,
+                inlet_cfg=ChannelInlet(v_n_inf=u, v_inf=v3), exit_cfg=ChannelFarfield(v_n_inf=-u, v_inf=v3))
+    base.update(kw)
+    return ConfigChannel(**base) if dim == 2 else ConfigChannel3D(**base)

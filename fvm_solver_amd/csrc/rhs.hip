@@ -40,7 +40,7 @@ struct Dims {
     static constexpr int C = D + 2;
     static constexpr int NL = D + 1;
     static constexpr int REC = (D + 1) * (D + 2);  // gp record: g[D][C] then phi[C]
-    static constexpr int GEO = D == 2 ? 8 : 12;
+    static constexpr int GEO = D == 2 ? 12 : 20;  // n, n_hat, |n|, 1/(n.d), X, d_slot0, d_slot1
     static constexpr int CH = 2 * D + 3;
 };
 
@@ -81,6 +81,15 @@ __device__ __forceinline__ float lim_phi(float delta, float dU, float eps) {
     }
     static_assert(lp >= 1 && lp <= 5, "limiter order 1..5");
     return 1.0f;
+}
+
+// Workgroups are dealt round-robin to the 8 XCDs (workgroup w runs on XCD w % 8), each
+// with its own L2.  Give each XCD a contiguous range of tiles instead, so that the
+// neighbour gathers of Morton-adjacent tiles hit the L2 that already holds them.
+// Bijective on [0, T) for virtual ids v < T; a persistent grid must be a multiple of 8.
+__device__ __forceinline__ int64_t xcd_tile(int64_t v, int64_t T) {
+    const int64_t x = v & 7, local = v >> 3, q = T >> 3, r = T & 7;
+    return x * q + (x < r ? x : r) + local;
 }
 
 template <int D>
@@ -179,40 +188,43 @@ __device__ __forceinline__ void bc_value(const FvmMeshDev &m, const FvmPhys &p, 
 }
 
 // Block-level part of the group means of c over the characteristic-BC facets: every
-// thread of the block calls it (uniform control flow).  Thread contributions go through
-// a fixed tree into partial slot `slot`; the last slot block to finish sums the slots in
-// slot order and writes the means.  Identical partials whichever kernel computes them.
+// thread of the block calls it (uniform control flow).  A slot is a 256-cell tile whose
+// cells own group facets; all its cells belong to one mesh, whose groups are
+// slot_g0 .. slot_g0 + slot_ng - 1.  Thread contributions go through a fixed tree into
+// bc_part[slot][q]; the last slot tile to finish sums, per group, the slots of its mesh
+// in slot order and writes the means.  Identical partials whichever kernel computes them.
 template <int D>
 __device__ void bc_mean_block(const FvmMeshDev &m, const FvmPhys &p, const FvmWork &w, int64_t c, bool live,
                               float T, int slot) {
     constexpr int NL = D + 1;
     const int64_t N = m.n_cells;
+    const int g0 = m.slot_g0[slot], ng = m.slot_ng[slot];
     float cs[kMaxGroups] = {0.0f, 0.0f, 0.0f, 0.0f};
     if (live) {
         const float cc = sqrtf(p.gR * T);
         for (int k = 0; k < NL; ++k) {
             const int64_t nb = m.cell_nbr[k * N + c];
             if (nb >= N) {
-                const int g = m.bc_group[nb - N];
+                const int q0 = m.bc_group[nb - N] - g0;
 #pragma unroll
                 for (int q = 0; q < kMaxGroups; ++q)
-                    if (g == q) cs[q] += cc;
+                    if (q0 == q) cs[q] += cc;
             }
         }
     }
     __shared__ float red[kThreads];
     __shared__ int is_last;
-    for (int g = 0; g < m.n_groups; ++g) {
+    for (int q = 0; q < ng; ++q) {
         float v = cs[0];
 #pragma unroll
-        for (int q = 1; q < kMaxGroups; ++q) v = (g == q) ? cs[q] : v;
+        for (int r = 1; r < kMaxGroups; ++r) v = (q == r) ? cs[r] : v;
         red[threadIdx.x] = v;
         __syncthreads();
         for (int h = kThreads / 2; h > 0; h >>= 1) {
             if (threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
             __syncthreads();
         }
-        if (threadIdx.x == 0) w.bc_part[g * m.n_slots + slot] = red[0];
+        if (threadIdx.x == 0) w.bc_part[slot * kMaxGroups + q] = red[0];
         __syncthreads();
     }
     if (threadIdx.x == 0) {
@@ -220,15 +232,17 @@ __device__ void bc_mean_block(const FvmMeshDev &m, const FvmPhys &p, const FvmWo
         is_last = atomicAdd(w.bc_counter, 1) == m.n_slots - 1;
     }
     __syncthreads();
-    if (is_last && threadIdx.x < m.n_groups) {
+    if (is_last) {
         __threadfence();
-        const int g = threadIdx.x;
-        float sum = 0.0f;
-        for (int i = 0; i < m.n_slots; ++i)
-            sum += __hip_atomic_load(w.bc_part + g * m.n_slots + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        w.bc_mean[g] = sum / m.group_count[g];
+        for (int g = threadIdx.x; g < m.n_groups; g += kThreads) {
+            float sum = 0.0f;
+            for (int i = m.group_s0[g]; i < m.group_s1[g]; ++i)
+                sum += __hip_atomic_load(w.bc_part + i * kMaxGroups + (g - m.slot_g0[i]), __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_AGENT);
+            w.bc_mean[g] = sum / m.group_count[g];
+        }
+        if (threadIdx.x == 0) atomicExch(w.bc_counter, 0);
     }
-    if (is_last && threadIdx.x == 0) atomicExch(w.bc_counter, 0);
 }
 
 // Pre-pass: boundary means of `prim` when no previous stage produced them.
@@ -250,9 +264,12 @@ k_grad(FvmMeshDev m, FvmPhys p, FvmWork w, const float *__restrict__ prim, int h
     using Dm = Dims<D>;
     constexpr int C = Dm::C, NL = Dm::NL;
     const int64_t N = m.n_cells;
-    const int64_t c = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
-    if (c >= N) return;
+    const int64_t ntiles = (N + kThreads - 1) / kThreads;
     float *__restrict__ gp = w.gp;
+    for (int64_t v = blockIdx.x; v < ntiles; v += gridDim.x) {
+    const int64_t tile = xcd_tile(v, ntiles);
+    const int64_t c = tile * kThreads + threadIdx.x;
+    if (c >= N) continue;
     float P[C];
     load_prim<D>(prim + c * C, P);
     float Un[NL][C];
@@ -324,13 +341,15 @@ k_grad(FvmMeshDev m, FvmPhys p, FvmWork w, const float *__restrict__ prim, int h
         for (int j = 0; j < C; ++j) o[i * C + j] = g[i][j];
 #pragma unroll
     for (int j = 0; j < C; ++j) o[D * C + j] = phi[j];
+    }
 }
 
 // ------------------------------------------------------------------------------------
 // Facet flux (canonical slot order) -- identical for both neighbours of a facet
 // ------------------------------------------------------------------------------------
 template <int D>
-__device__ __forceinline__ void facet_flux(const FvmPhys &p, const float (&U0)[D + 2], const float (&U1)[D + 2],
+__device__ __forceinline__ void facet_flux(const FvmPhys &p, float mu0, float mub0, float kappa,
+                                           const float (&U0)[D + 2], const float (&U1)[D + 2],
                                            const float (&gl0)[D][D + 1], const float (&gl1)[D][D + 1],
                                            const float (&gdn)[D + 1], const float *__restrict__ geo,
                                            float (&F)[D + 2]) {
@@ -391,8 +410,8 @@ __device__ __forceinline__ void facet_flux(const FvmPhys &p, const float (&U0)[D
 #pragma unroll
         for (int j = 0; j < D; ++j) Dm_[i][j] = 0.5f * (gV[i][j] + gV[j][i]);
     const float I1 = Dm_[0][0] + Dm_[1][1];
-    const float mu = ((p.mu0 * powf(Tm / p.T0, 1.5f)) * p.T0pS) / (Tm + p.S);
-    const float mub = (p.mub0 * (Tm * Tm)) / p.T0sq;
+    const float mu = ((mu0 * powf(Tm / p.T0, 1.5f)) * p.T0pS) / (Tm + p.S);
+    const float mub = (mub0 * (Tm * Tm)) / p.T0sq;
     const float a0 = (-mub) * I1;
     const float a1 = -2.0f * mu;
     float tau[D][D];
@@ -432,7 +451,7 @@ __device__ __forceinline__ void facet_flux(const FvmPhys &p, const float (&U0)[D
             const float t = (tau[i][j] * Vb[j]) * n[i];
             heat = (i == 0 && j == 0) ? t : heat + t;
         }
-    heat = heat + ((-p.kappa) * gTn) * fs;
+    heat = heat + ((-kappa) * gTn) * fs;
     F[D + 1] = F[D + 1] + heat;
     // Kurganov-Tadmor dissipation (:337-368)
     const float Vn0 = sqrtf(sq0), Vn1 = sqrtf(sq1);
@@ -480,8 +499,18 @@ k_flux(FvmMeshDev m, FvmPhys p, FvmWork w, const float *__restrict__ prim, float
     using Dm = Dims<D>;
     constexpr int C = Dm::C, NL = Dm::NL;
     const int64_t N = m.n_cells;
-    const int64_t c = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
+    const int64_t ntiles = (N + kThreads - 1) / kThreads;
+    for (int64_t v = blockIdx.x; v < ntiles; v += gridDim.x) {
+    const int64_t tile = xcd_tile(v, ntiles);
+    const int64_t c = tile * kThreads + threadIdx.x;
     const bool live = c < N;
+    const int mesh = (m.cell_mesh && live) ? m.cell_mesh[c] : 0;
+    float mu0 = p.mu0, mub0 = p.mub0, kappa = p.kappa;
+    if (m.mesh_phys) {
+        mu0 = m.mesh_phys[4 * mesh];
+        mub0 = m.mesh_phys[4 * mesh + 1];
+        kappa = m.mesh_phys[4 * mesh + 2];
+    }
     float acc[C];
 #pragma unroll
     for (int j = 0; j < C; ++j) acc[j] = 0.0f;
@@ -498,20 +527,21 @@ k_flux(FvmMeshDev m, FvmPhys p, FvmWork w, const float *__restrict__ prim, float
             const int64_t f = m.cell_facet[k * N + c];
             const int64_t nb = m.cell_nbr[k * N + c];
             const int own_slot = (info >> k) & 1;
-            float dc[D];
+            const float *geo = m.fac_geo + f * Dm::GEO;
+            // centroid -> midpoint displacements of both sides come with the facet record
+            float dc[D], dn[D];
 #pragma unroll
-            for (int i = 0; i < D; ++i) dc[i] = m.cell_disp[(k * D + i) * N + c];
+            for (int i = 0; i < D; ++i) {
+                const float d0 = geo[3 * D + 2 + i], d1 = geo[4 * D + 2 + i];
+                dc[i] = own_slot ? d1 : d0;
+                dn[i] = own_slot ? d0 : d1;
+            }
             float Uo[C], glo[D][D + 1];
             side_values<D>(Pc, gc, dc, Uo, glo);
             float Ux[C], glx[D][D + 1], gdn[D + 1];
-            const float *geo = m.fac_geo + f * Dm::GEO;
             if (nb < N) {
                 float Pn[C];
                 load_prim<D>(prim + nb * C, Pn);
-                const int kn = (info >> (8 + 2 * k)) & 3;
-                float dn[D];
-#pragma unroll
-                for (int i = 0; i < D; ++i) dn[i] = m.cell_disp[(kn * D + i) * N + nb];
                 side_values<D>(Pn, gp + nb * Dm::REC, dn, Ux, glx);
                 // A_face_grad row: (-1/d, +1/d) on (slot-0, slot-1) cells, ascending columns
                 const float inv_d = geo[2 * D + 1];
@@ -555,7 +585,7 @@ k_flux(FvmMeshDev m, FvmPhys p, FvmWork w, const float *__restrict__ prim, float
                     g1s[i][j] = sw ? glo[i][j] : glx[i][j];
                 }
             float F[C];
-            facet_flux<D>(p, U0s, U1s, g0s, g1s, gdn, geo, F);
+            facet_flux<D>(p, mu0, mub0, kappa, U0s, U1s, g0s, g1s, gdn, geo, F);
             const float v = own_slot ? inv_vol : -inv_vol;
 #pragma unroll
             for (int j = 0; j < C; ++j) acc[j] = acc[j] + v * F[j];
@@ -565,7 +595,7 @@ k_flux(FvmMeshDev m, FvmPhys p, FvmWork w, const float *__restrict__ prim, float
         if (live)
 #pragma unroll
             for (int j = 0; j < C; ++j) dudt[c * C + j] = acc[j];
-        return;
+        continue;
     }
     // ---------------- explicit RK stage epilogue ----------------
     double sq[3 * C];
@@ -576,7 +606,7 @@ k_flux(FvmMeshDev m, FvmPhys p, FvmWork w, const float *__restrict__ prim, float
     for (int j = 0; j < C; ++j) Pn[j] = 0.0f;
     if (live && st.mode == 1) {
         // Adams-Bashforth 3 (fixed dt, one RHS per step)
-        const float c12 = *st.dt / 12.0f;
+        const float c12 = st.dt[mesh] / 12.0f;
         const int64_t o = c * C;
         float U1[C];
 #pragma unroll
@@ -593,7 +623,7 @@ k_flux(FvmMeshDev m, FvmPhys p, FvmWork w, const float *__restrict__ prim, float
 #pragma unroll
         for (int j = 0; j < C; ++j) st.prim_next[o + j] = Pn[j];
     } else if (live) {
-        const float dt = *st.dt;
+        const float dt = st.dt[mesh];
         const int64_t o = c * C;
         float kk[C], a[C], U0[C];
 #pragma unroll
@@ -644,7 +674,7 @@ k_flux(FvmMeshDev m, FvmPhys p, FvmWork w, const float *__restrict__ prim, float
     }
     // boundary means of the next stage's primitives (block-uniform branch)
     if (m.n_groups > 0) {
-        const int slot = m.blk_slot[blockIdx.x];
+        const int slot = m.blk_slot[tile];
         if (slot >= 0) bc_mean_block<D>(m, p, w, c, live, Pn[D + 1], slot);
     }
     if (st.last && st.acc2) {
@@ -657,9 +687,10 @@ k_flux(FvmMeshDev m, FvmPhys p, FvmWork w, const float *__restrict__ prim, float
                 if (threadIdx.x < w) sred[threadIdx.x] += sred[threadIdx.x + w];
                 __syncthreads();
             }
-            if (threadIdx.x == 0) red[static_cast<int64_t>(blockIdx.x) * 3 * C + q] = sred[0];
+            if (threadIdx.x == 0) red[tile * 3 * C + q] = sred[0];
             __syncthreads();
         }
+    }
     }
 }
 
@@ -713,9 +744,11 @@ __global__ void __launch_bounds__(kThreads) k_state_to_prim3(const float *__rest
 
 static int check_mesh(const FvmMeshDev *m, const FvmPhys *p, const FvmWork *w, const char *who) {
     if (!m || !p || !w || (m->dim != 2 && m->dim != 3) || m->n_cells <= 0 || m->n_groups < 0 ||
-        m->n_groups > kMaxGroups || m->n_bc < 0 || !w->gp || (m->n_bc > 0 && !w->ubc) ||
+        m->n_groups > (1 << 16) || m->n_bc < 0 || !w->gp || (m->n_bc > 0 && !w->ubc) ||
         (m->n_groups > 0 && (!w->bc_part || !w->bc_mean || !w->bc_counter || m->n_slots < 1 || !m->blk_slot ||
-                             !m->slot_blk)) ||
+                             !m->slot_blk || !m->slot_g0 || !m->slot_ng || !m->group_s0 || !m->group_s1 ||
+                             !m->group_count)) ||
+        (m->mesh_phys && !m->cell_mesh) || m->n_meshes < 1 ||
         p->lim_p < 1 || p->lim_p > 5 || m->n_cells > (int64_t(1) << 30)) {
         set_error("%s: invalid mesh/work/phys arguments", who);
         return FVM_ERR_ARG;
@@ -730,7 +763,9 @@ static int launch_rhs(const FvmMeshDev &m, const FvmPhys &p, const FvmWork &w, c
                       const FvmStage *st, hipStream_t s, int has_prev, int need_mean, int parts = 15) {
     if ((parts & 1) && need_mean && m.n_groups > 0)
         hipLaunchKernelGGL(k_bc_mean<D>, dim3(m.n_slots), dim3(kThreads), 0, s, m, p, w, prim);
-    const dim3 grid(blocks_for(m.n_cells, kThreads));
+    const unsigned int ntiles = blocks_for(m.n_cells, kThreads);
+    const dim3 grid(m.persist_blocks > 0 && static_cast<unsigned int>(m.persist_blocks) < ntiles
+                        ? static_cast<unsigned int>(m.persist_blocks) : ntiles);
     if (parts & 4) {
         switch (p.lim_p) {
             case 1: hipLaunchKernelGGL((k_grad<D, 1>), grid, dim3(kThreads), 0, s, m, p, w, prim, has_prev); break;

@@ -54,191 +54,275 @@ def _bc_tables(bc_tags, C):
     return dm, nm, dv, nv, ff, il
 
 
+def host_layout(mesh, n_comp: int, bc_tags, cfg) -> dict:
+    """Per-mesh FvmMeshDev arrays (numpy, mesh-local indices) from an FVMMesh and its BC
+    tags; ``DeviceMesh`` concatenates one or more of these and uploads them."""
+    g = mesh.geo64
+    D = mesh.dim
+    C = n_comp
+    if C != D + 2:
+        raise ValueError(f"n_comp must be dim + 2 (got {C} for dim {D})")
+    NL = D + 1
+    N, F, Fb = mesh.n_cells, mesh.n_facets, mesh.n_bc_facet
+    if N >= 2 ** 31 - 1 or F >= 2 ** 31 - 1:
+        raise ValueError("mesh too large for int32 indices")
+    bcm = mesh.bc_facet_mask.numpy()
+    c2f, signs, comb = g["c2f"], g["signs"], g["combined"]
+    slots = (signs == -1).astype(np.int64)           # int(-s + 1/2), facet_process.py:71
+
+    # ---- per-cell lists in ascending facet order (flux_mat ELL order) ----
+    perm = np.argsort(c2f, axis=1, kind="stable")
+    invp = np.argsort(perm, axis=1, kind="stable")
+    c2f_s = np.take_along_axis(c2f, perm, 1)
+    nbr_s = np.take_along_axis(comb, perm, 1)
+    slot_s = np.take_along_axis(slots, perm, 1)
+    disp_s = np.take_along_axis(g["cent_to_facet_disp"], perm[..., None], 1)
+    flat = c2f_s.reshape(-1)
+    o = np.argsort(flat, kind="stable")
+    counts = np.bincount(flat, minlength=F)
+    start = np.concatenate([[0], np.cumsum(counts)[:-1]])
+    two = counts == 2
+    pa = o[start[two]]
+    pb = o[start[two] + 1]
+    nbk = np.zeros(N * NL, np.int64)
+    nbk[pa] = pb % NL
+    nbk[pb] = pa % NL
+    nbk = nbk.reshape(N, NL)
+
+    # ---- LSQ rows in ascending column order (G_mats ELL order) ----
+    G32 = g["G"].astype(np.float32)                   # [N, D, NL], reference local order
+    diag = G32[:, :, 0].copy()
+    for k in range(1, NL):
+        diag = (diag + G32[:, :, k]).astype(np.float32)
+    cols = np.concatenate([comb, np.arange(N)[:, None]], 1)
+    vals = np.concatenate([G32, -diag[:, :, None]], 2)    # [N, D, NL+1]
+    eo = np.argsort(cols, axis=1, kind="stable")
+    gval = np.take_along_axis(vals, eo[:, None, :], 2)    # [N, D, NL+1]
+    src = np.where(eo == NL, 7, np.take_along_axis(invp, np.minimum(eo, NL - 1), 1))
+
+    info = np.zeros(N, np.int64)
+    for k in range(NL):
+        info |= slot_s[:, k] << k
+        info |= nbk[:, k] << (8 + 2 * k)
+    for j in range(NL + 1):
+        info |= src[:, j].astype(np.int64) << (16 + 3 * j)
+
+    vol32 = g["volumes"].astype(np.float32)
+    inv_vol = (np.float32(1.0) / vol32).astype(np.float32)
+    eps = np.power(np.float32(cfg.lim_K) * np.sqrt(vol32), np.float32(cfg.lim_p + 1)).astype(np.float32)
+
+    # ---- per-facet geometry ----
+    n64, nh64 = g["normals"], g["normals_hat"]
+    fs64 = np.linalg.norm(n64, axis=1)
+    cen = g["centroids"]
+    GEO = 12 if D == 2 else 20
+    geo = np.zeros((F, GEO), np.float32)
+    geo[:, :D] = n64
+    geo[:, D:2 * D] = nh64
+    geo[:, 2 * D] = fs64
+    inter = ~bcm
+    cd = cen[g["right"][inter]] - cen[g["left"][inter]]
+    dproj = np.einsum("nd,nd->n", nh64[inter], cd)
+    geo[inter, 2 * D + 1] = 1.0 / dproj
+    geo[inter, 2 * D + 2:3 * D + 2] = cd / dproj[:, None] - nh64[inter]
+    # centroid -> facet-midpoint displacement of the slot-0 and slot-1 cells
+    disp64 = g["cent_to_facet_disp"]
+    for k in range(NL):
+        f_k = c2f[:, k]
+        sl = slots[:, k]
+        for sv in (0, 1):
+            sel = sl == sv
+            geo[f_k[sel], 3 * D + 2 + sv * D:3 * D + 2 + (sv + 1) * D] = disp64[sel, k]
+
+    # ---- boundary facets ----
+    dm, nm, dv, nv, ffm, ilm = _bc_tables(bc_tags, C)
+    if len(dm) != Fb:
+        raise ValueError(f"{len(dm)} BC tags for {Fb} boundary facets")
+    fidx = np.flatnonzero(bcm)
+    bcell = g["left"][bcm]
+    kloc = np.argmax(c2f[bcell] == fidx[:, None], axis=1)
+    cell_slot = slots[bcell, kloc]
+    side = 1 - cell_slot                                   # the free slot (facet_process.py:98-101)
+    mid = g["midpoints"][bcm]
+    dist64 = np.einsum("nd,nd->n", mid - cen[bcell], nh64[bcm])
+    dist32 = dist64.astype(np.float32)
+    nv_masked = nv[nm]                                      # Neumann values, row-major entry order
+    comp = np.broadcast_to(np.arange(C), (Fb, C))
+    bc_val = np.zeros((Fb, C), np.float32)
+    bc_val[dm] = dv[dm]
+    bc_ga = np.zeros((Fb, C), np.float32)
+    bc_gb = np.zeros((Fb, C), np.float32)
+    if dm.any():
+        dd = np.broadcast_to(dist32[:, None], (Fb, C))[dm]
+        bc_ga[dm] = np.float32(-1.0) / dd
+        bc_gb[dm] = dv[dm] / dd
+    if nm.any():
+        # b_bc and b_face_grad index the masked Neumann values by component
+        # (facet_boundary.py:278, facet_process.py:192); kept as the reference does it.
+        if comp[nm].max() >= len(nv_masked):
+            raise IndexError("Neumann values indexed by component out of range")
+        bc_val[nm] = nv_masked[comp[nm]] * np.broadcast_to(dist32[:, None], (Fb, C))[nm]
+        bc_gb[nm] = nv_masked[comp[nm]]
+    flags = (nm * (1 << np.arange(C))).sum(1).astype(np.int32)
+    r = g["cent_to_facet_disp"][bcell, kloc]
+    nhb = nh64[bcm]
+    l_vec = r - nhb * np.einsum("nd,nd->n", r, nhb)[:, None]
+    l_vec[~nm.any(1)] = 0.0
+
+    # ---- characteristic BC groups (facet_boundary.py:32-80, 289-385) ----
+    nh32 = nhb.astype(np.float32)
+    sgn = (np.float32(2.0) * (side.astype(np.float32) - np.float32(0.5))).astype(np.float32)
+    n_out = (nh32 * sgn[:, None]).astype(np.float32)
+    group = np.full(Fb, -1, np.int32)
+    CH = 2 * D + 3
+    bc_char = np.zeros((Fb, CH), np.float32)
+    modes = []
+    R32 = np.float32(cfg.C_v * (cfg.gamma - 1))
+    for mask, bcfg in ((ffm, cfg.exit_cfg), (ilm, cfg.inlet_cfg)):
+        if not mask.any():
+            continue
+        if bcfg.mode != BCMode.Characteristic:
+            raise NotImplementedError(f"Unknown BCMode {bcfg.mode}: only the characteristic BC is supported "
+                                      "(the reference's Farfield mode is broken and Isentropic is unimplemented)")
+        if bcfg.mode not in modes:
+            modes.append(bcfg.mode)
+        group[mask] = modes.index(bcfg.mode)
+        nn = n_out[mask]
+        rho = np.float32(bcfg.rho_inf)
+        T = np.float32(bcfg.T_inf)
+        bc_char[mask, :D] = nn
+        bc_char[mask, D] = rho
+        bc_char[mask, D + 2] = (R32 * rho) * T
+        if D == 2:
+            bc_char[mask, D + 1] = np.float32(-bcfg.v_n_inf)
+            tang = np.stack([-nn[:, 1], nn[:, 0]], 1)
+            bc_char[mask, D + 3:] = tang * np.float32(bcfg.v_t_inf)
+        else:
+            vinf = _f32(bcfg.v_inf)
+            vn = (nn[:, 0] * vinf[0] + nn[:, 1] * vinf[1]) + nn[:, 2] * vinf[2]
+            bc_char[mask, D + 1] = vn
+            bc_char[mask, D + 3:] = vinf[None, :] - nn * vn[:, None]
+    per = fs64[c2f].sum(1)
+    return dict(D=D, C=C, N=N, F=F, Fb=Fb, n_groups=len(modes), c2f_s=c2f_s, nbr_s=nbr_s, slot_s=slot_s, nbk=nbk,
+                gval=gval, src=src, info=info, disp_s=disp_s, inv_vol=inv_vol, eps=eps, geo=geo, bcell=bcell,
+                group=group, flags=flags, bc_val=bc_val, bc_l=l_vec.astype(np.float32), bc_ga=bc_ga, bc_gb=bc_gb,
+                bc_char=bc_char, side=side, volumes64=g["volumes"], h64=D * g["volumes"] / per,
+                phys=(float(cfg.viscosity), float(cfg.visc_bulk), float(cfg.thermal_cond)))
+
+
+_TILE = 256
+
+
 class DeviceMesh:
-    """Builds the FvmMeshDev arrays on ``device`` from an FVMMesh and the BC tags."""
+    """The FvmMeshDev arrays of one mesh, or of several meshes concatenated
+    block-diagonally (``DeviceMesh.batch``), uploaded to ``device``.
 
-    def __init__(self, mesh, n_comp: int, bc_tags, cfg, device):
-        g = mesh.geo64
-        D = mesh.dim
-        C = n_comp
-        if C != D + 2:
-            raise ValueError(f"n_comp must be dim + 2 (got {C} for dim {D})")
+    Meshes are not padded to 256-cell tiles: a tile may hold cells of two meshes, and
+    the boundary-mean slot tables list, per tile, the groups of every mesh the tile
+    touches (at most 4)."""
+
+    def __init__(self, mesh, n_comp: int, bc_tags, cfg, device, persist_blocks: int = 0):
+        self._build([host_layout(mesh, n_comp, bc_tags, cfg)], device, persist_blocks)
+
+    @classmethod
+    def batch(cls, items, device, persist_blocks: int = 0):
+        """items: iterable of (mesh, n_comp, bc_tags, cfg)."""
+        obj = cls.__new__(cls)
+        obj._build([host_layout(*it) for it in items], device, persist_blocks)
+        return obj
+
+    def _build(self, L, device, persist_blocks):
+        D, C = L[0]["D"], L[0]["C"]
+        if any(l["D"] != D for l in L):
+            raise ValueError("all meshes of a batch must have the same dimension")
         NL = D + 1
-        N, F, Fb = mesh.n_cells, mesh.n_facets, mesh.n_bc_facet
-        if N >= 2 ** 31 - 1 or F >= 2 ** 31 - 1:
+        nm = len(L)
+        N = sum(l["N"] for l in L)
+        F = sum(l["F"] for l in L)
+        Fb = sum(l["Fb"] for l in L)
+        if N >= 2 ** 30 or F >= 2 ** 31 - 1:
             raise ValueError("mesh too large for int32 indices")
-        self.dim, self.C, self.N, self.F, self.Fb = D, C, N, F, Fb
-        bcm = mesh.bc_facet_mask.numpy()
-        c2f, signs, comb = g["c2f"], g["signs"], g["combined"]
-        slots = (signs == -1).astype(np.int64)           # int(-s + 1/2), facet_process.py:71
+        self.dim, self.C, self.N, self.F, self.Fb, self.n_meshes = D, C, N, F, Fb, nm
+        co = np.cumsum([0] + [l["N"] for l in L])
+        fo = np.cumsum([0] + [l["F"] for l in L])
+        bo = np.cumsum([0] + [l["Fb"] for l in L])
+        go = np.cumsum([0] + [l["n_groups"] for l in L])
+        self.cell_offsets, self.facet_offsets, self.bc_offsets, self.group_offsets = co, fo, bo, go
+        self.n_groups = int(go[-1])
 
-        # ---- per-cell lists in ascending facet order (flux_mat ELL order) ----
-        perm = np.argsort(c2f, axis=1, kind="stable")
-        invp = np.argsort(perm, axis=1, kind="stable")
-        c2f_s = np.take_along_axis(c2f, perm, 1)
-        nbr_s = np.take_along_axis(comb, perm, 1)
-        slot_s = np.take_along_axis(slots, perm, 1)
-        disp_s = np.take_along_axis(g["cent_to_facet_disp"], perm[..., None], 1)
-        flat = c2f_s.reshape(-1)
-        o = np.argsort(flat, kind="stable")
-        counts = np.bincount(flat, minlength=F)
-        start = np.concatenate([[0], np.cumsum(counts)[:-1]])
-        two = counts == 2
-        pa = o[start[two]]
-        pb = o[start[two] + 1]
-        nbk = np.zeros(N * NL, np.int64)
-        nbk[pa] = pb % NL
-        nbk[pb] = pa % NL
-        nbk = nbk.reshape(N, NL)
+        def cat(key, fn=lambda a, i: a):
+            return np.concatenate([fn(l[key], i) for i, l in enumerate(L)])
 
-        # ---- LSQ rows in ascending column order (G_mats ELL order) ----
-        G32 = g["G"].astype(np.float32)                   # [N, D, NL], reference local order
-        diag = G32[:, :, 0].copy()
-        for k in range(1, NL):
-            diag = (diag + G32[:, :, k]).astype(np.float32)
-        cols = np.concatenate([comb, np.arange(N)[:, None]], 1)
-        vals = np.concatenate([G32, -diag[:, :, None]], 2)    # [N, D, NL+1]
-        eo = np.argsort(cols, axis=1, kind="stable")
-        gval = np.take_along_axis(vals, eo[:, None, :], 2)    # [N, D, NL+1]
-        src = np.where(eo == NL, 7, np.take_along_axis(invp, np.minimum(eo, NL - 1), 1))
+        c2f_s = cat("c2f_s", lambda a, i: a + fo[i])
+        nbr_s = cat("nbr_s", lambda a, i: np.where(a < L[i]["N"], a + co[i], a - L[i]["N"] + bo[i] + N))
+        bcell = cat("bcell", lambda a, i: a + co[i])
+        group = cat("group", lambda a, i: np.where(a >= 0, a + go[i], -1)).astype(np.int32)
+        cell_mesh = np.repeat(np.arange(nm, dtype=np.int32), [l["N"] for l in L])
 
-        info = np.zeros(N, np.int64)
-        for k in range(NL):
-            info |= slot_s[:, k] << k
-            info |= nbk[:, k] << (8 + 2 * k)
-        for j in range(NL + 1):
-            info |= src[:, j].astype(np.int64) << (16 + 3 * j)
+        # boundary-mean slots: tiles holding the interior cells of group facets
+        ntiles = (N + _TILE - 1) // _TILE
+        tiles = np.unique(bcell[group >= 0] // _TILE) if self.n_groups else np.zeros(0, np.int64)
+        blk_slot = np.full(ntiles, -1, np.int32)
+        blk_slot[tiles] = np.arange(len(tiles), dtype=np.int32)
+        m_lo = cell_mesh[tiles * _TILE] if len(tiles) else np.zeros(0, np.int32)
+        m_hi = cell_mesh[np.minimum(tiles * _TILE + _TILE - 1, N - 1)] if len(tiles) else np.zeros(0, np.int32)
+        slot_g0 = go[m_lo].astype(np.int32)
+        slot_ng = (go[m_hi + 1] - go[m_lo]).astype(np.int32)
+        if len(tiles) and slot_ng.max() > 4:
+            raise ValueError("a 256-cell tile touches more than 4 boundary groups (meshes too small to batch)")
+        gmesh = np.repeat(np.arange(nm), [l["n_groups"] for l in L])
+        group_s0 = np.searchsorted(m_hi, gmesh, side="left").astype(np.int32)
+        group_s1 = np.searchsorted(m_lo, gmesh, side="right").astype(np.int32)
+        group_count = np.array([np.sum(group == q) for q in range(self.n_groups)], np.float32)
+        self.n_slots = len(tiles)
+        self.host = dict(c2f_s=c2f_s, nbr_s=nbr_s, bcell=bcell, group=group, cell_mesh=cell_mesh,
+                         blk_slot=blk_slot, slot_g0=slot_g0, slot_ng=slot_ng, group_s0=group_s0,
+                         group_s1=group_s1, group_count=group_count)
+        for k in ("slot_s", "nbk", "gval", "src", "info", "disp_s", "inv_vol", "eps", "geo", "flags", "bc_val",
+                  "bc_l", "bc_ga", "bc_gb", "bc_char", "side"):
+            self.host[k] = cat(k)
+        h = self.host
+        self.mesh_phys = np.array([[*l["phys"], 0.0] for l in L], np.float32)
 
-        vol32 = g["volumes"].astype(np.float32)
-        inv_vol = (np.float32(1.0) / vol32).astype(np.float32)
-        eps = np.power(np.float32(cfg.lim_K) * np.sqrt(vol32), np.float32(cfg.lim_p + 1)).astype(np.float32)
-
-        # ---- per-facet geometry ----
-        n64, nh64 = g["normals"], g["normals_hat"]
-        fs64 = np.linalg.norm(n64, axis=1)
-        cen = g["centroids"]
-        GEO = 8 if D == 2 else 12
-        geo = np.zeros((F, GEO), np.float32)
-        geo[:, :D] = n64
-        geo[:, D:2 * D] = nh64
-        geo[:, 2 * D] = fs64
-        inter = ~bcm
-        cd = cen[g["right"][inter]] - cen[g["left"][inter]]
-        dproj = np.einsum("nd,nd->n", nh64[inter], cd)
-        geo[inter, 2 * D + 1] = 1.0 / dproj
-        geo[inter, 2 * D + 2:3 * D + 2] = cd / dproj[:, None] - nh64[inter]
-
-        # ---- boundary facets ----
-        dm, nm, dv, nv, ffm, ilm = _bc_tables(bc_tags, C)
-        if len(dm) != Fb:
-            raise ValueError(f"{len(dm)} BC tags for {Fb} boundary facets")
-        fidx = np.flatnonzero(bcm)
-        bcell = g["left"][bcm]
-        kloc = np.argmax(c2f[bcell] == fidx[:, None], axis=1)
-        cell_slot = slots[bcell, kloc]
-        side = 1 - cell_slot                                   # the free slot (facet_process.py:98-101)
-        mid = g["midpoints"][bcm]
-        dist64 = np.einsum("nd,nd->n", mid - cen[bcell], nh64[bcm])
-        dist32 = dist64.astype(np.float32)
-        nv_masked = nv[nm]                                      # Neumann values, row-major entry order
-        comp = np.broadcast_to(np.arange(C), (Fb, C))
-        bc_val = np.zeros((Fb, C), np.float32)
-        bc_val[dm] = dv[dm]
-        bc_ga = np.zeros((Fb, C), np.float32)
-        bc_gb = np.zeros((Fb, C), np.float32)
-        if dm.any():
-            dd = np.broadcast_to(dist32[:, None], (Fb, C))[dm]
-            bc_ga[dm] = np.float32(-1.0) / dd
-            bc_gb[dm] = dv[dm] / dd
-        if nm.any():
-            # b_bc and b_face_grad index the masked Neumann values by component
-            # (facet_boundary.py:278, facet_process.py:192); kept as the reference does it.
-            if comp[nm].max() >= len(nv_masked):
-                raise IndexError("Neumann values indexed by component out of range")
-            bc_val[nm] = nv_masked[comp[nm]] * np.broadcast_to(dist32[:, None], (Fb, C))[nm]
-            bc_gb[nm] = nv_masked[comp[nm]]
-        flags = (nm * (1 << np.arange(C))).sum(1).astype(np.int32)
-        r = g["cent_to_facet_disp"][bcell, kloc]
-        nhb = nh64[bcm]
-        l_vec = r - nhb * np.einsum("nd,nd->n", r, nhb)[:, None]
-        l_vec[~nm.any(1)] = 0.0
-
-        # ---- characteristic BC groups (facet_boundary.py:32-80, 289-385) ----
-        nh32 = nhb.astype(np.float32)
-        sgn = (np.float32(2.0) * (side.astype(np.float32) - np.float32(0.5))).astype(np.float32)
-        n_out = (nh32 * sgn[:, None]).astype(np.float32)
-        group = np.full(Fb, -1, np.int32)
-        CH = 2 * D + 3
-        bc_char = np.zeros((Fb, CH), np.float32)
-        modes = []
-        R32 = np.float32(cfg.C_v * (cfg.gamma - 1))
-        for mask, bcfg in ((ffm, cfg.exit_cfg), (ilm, cfg.inlet_cfg)):
-            if not mask.any():
-                continue
-            if bcfg.mode != BCMode.Characteristic:
-                raise NotImplementedError(f"Unknown BCMode {bcfg.mode}: only the characteristic BC is supported "
-                                          "(the reference's Farfield mode is broken and Isentropic is unimplemented)")
-            if bcfg.mode not in modes:
-                modes.append(bcfg.mode)
-            group[mask] = modes.index(bcfg.mode)
-            nn = n_out[mask]
-            rho = np.float32(bcfg.rho_inf)
-            T = np.float32(bcfg.T_inf)
-            bc_char[mask, :D] = nn
-            bc_char[mask, D] = rho
-            bc_char[mask, D + 2] = (R32 * rho) * T
-            if D == 2:
-                bc_char[mask, D + 1] = np.float32(-bcfg.v_n_inf)
-                tang = np.stack([-nn[:, 1], nn[:, 0]], 1)
-                bc_char[mask, D + 3:] = tang * np.float32(bcfg.v_t_inf)
-            else:
-                vinf = _f32(bcfg.v_inf)
-                vn = (nn[:, 0] * vinf[0] + nn[:, 1] * vinf[1]) + nn[:, 2] * vinf[2]
-                bc_char[mask, D + 1] = vn
-                bc_char[mask, D + 3:] = vinf[None, :] - nn * vn[:, None]
-        self.n_groups = len(modes)
-        gcount = np.zeros(4, np.float32)
-        for q in range(self.n_groups):
-            gcount[q] = np.float32((group == q).sum())
-        # 256-cell blocks holding interior cells of group facets: one partial-sum slot each
-        nblk = (N + 255) // 256
-        blocks = np.unique(bcell[group >= 0] // 256) if self.n_groups else np.zeros(0, np.int64)
-        blk_slot = np.full(nblk, -1, np.int32)
-        blk_slot[blocks] = np.arange(len(blocks), dtype=np.int32)
-        self.n_slots = len(blocks)
-
-        # ---- upload ----
         dev = torch.device(device)
 
         def up(a, dt):
             return torch.from_numpy(np.ascontiguousarray(a)).to(dtype=dt, device=dev)
 
+        def up_nz(a, dt):
+            return up(a if len(a) else np.zeros(1), dt)
+
         i32, f32 = torch.int32, torch.float32
         self.t = dict(
-            cell_facet=up(c2f_s.T, i32), cell_nbr=up(nbr_s.T, i32), cell_info=up(info, i32),
-            cell_disp=up(disp_s.transpose(1, 2, 0), f32), cell_gval=up(gval.transpose(2, 1, 0), f32),
-            cell_inv_vol=up(inv_vol, f32), cell_eps=up(eps, f32), fac_geo=up(geo, f32),
-            bc_cell=up(bcell, i32), bc_group=up(group, i32), bc_flags=up(flags, i32), bc_val=up(bc_val, f32),
-            bc_l=up(l_vec, f32), bc_ga=up(bc_ga, f32), bc_gb=up(bc_gb, f32), bc_char=up(bc_char, f32),
-            blk_slot=up(blk_slot, i32), slot_blk=up(np.maximum(blocks, 0) if len(blocks) else np.zeros(1), i32))
-        self.host = dict(c2f_s=c2f_s, nbr_s=nbr_s, slot_s=slot_s, nbk=nbk, gval=gval, src=src, info=info,
-                         inv_vol=inv_vol, eps=eps, geo=geo, bcell=bcell, group=group, flags=flags, bc_val=bc_val,
-                         bc_l=l_vec.astype(np.float32), bc_ga=bc_ga, bc_gb=bc_gb, bc_char=bc_char, side=side)
+            cell_facet=up(c2f_s.T, i32), cell_nbr=up(nbr_s.T, i32), cell_info=up(h["info"], i32),
+            cell_disp=up(h["disp_s"].transpose(1, 2, 0), f32), cell_gval=up(h["gval"].transpose(2, 1, 0), f32),
+            cell_inv_vol=up(h["inv_vol"], f32), cell_eps=up(h["eps"], f32), fac_geo=up(h["geo"], f32),
+            bc_cell=up_nz(bcell, i32), bc_group=up_nz(group, i32), bc_flags=up_nz(h["flags"], i32),
+            bc_val=up_nz(h["bc_val"], f32), bc_l=up_nz(h["bc_l"], f32), bc_ga=up_nz(h["bc_ga"], f32),
+            bc_gb=up_nz(h["bc_gb"], f32), bc_char=up_nz(h["bc_char"], f32),
+            blk_slot=up(blk_slot, i32), slot_blk=up_nz(tiles, i32), slot_g0=up_nz(slot_g0, i32),
+            slot_ng=up_nz(slot_ng, i32), group_s0=up_nz(group_s0, i32), group_s1=up_nz(group_s1, i32),
+            group_count=up_nz(group_count, f32))
+        if nm > 1:
+            self.t["cell_mesh"] = up(cell_mesh, i32)
+            self.t["mesh_phys"] = up(self.mesh_phys, f32)
         if dev.type != "cuda":      # host-only layout (inspection and tests)
             self.t, self.struct = None, None
             return
-        self.volumes64 = up(g["volumes"], torch.float64)
-        per = fs64[c2f].sum(1)
-        self.h64 = up(D * g["volumes"] / per, torch.float64)     # CFL length d*vol/sum|n|
+        self.volumes64 = up(cat("volumes64"), torch.float64)
+        self.h64 = up(cat("h64"), torch.float64)
         self.struct = nat.FvmMeshDev(
-            dim=D, n_groups=self.n_groups, n_cells=N, n_facets=F, n_bc=Fb, n_slots=self.n_slots,
-            **{k: v.data_ptr() for k, v in self.t.items()})
-        for q in range(4):
-            self.struct.group_count[q] = float(gcount[q])
-        # views used by tests and the Saver
-        self.slots_sorted = slot_s
-        self.bc_facet_side = side
+            dim=D, n_groups=self.n_groups, n_cells=N, n_facets=F, n_bc=Fb, n_slots=self.n_slots, n_meshes=nm,
+            persist_blocks=int(persist_blocks), **{k: v.data_ptr() for k, v in self.t.items()})
+        self.bc_facet_side = h["side"]
         self.bc_cell_np = bcell
+
+    def set_persist_blocks(self, n: int):
+        """Workgroups of the per-cell kernels (0 = one per 256-cell tile); must be a
+        multiple of 8 so the XCD-aware tile mapping stays a bijection."""
+        if n < 0 or n % 8:
+            raise ValueError("persist_blocks must be a non-negative multiple of 8")
+        self.struct.persist_blocks = int(n)
 
 
 def phys_struct(cfg) -> "nat.FvmPhys":
@@ -254,29 +338,44 @@ class FacetCalc:
     """Face/flux pipeline owner (facet_process.py:229-450) on the GPU."""
 
     def __init__(self, phy_setup, cfg, mesh_setup, n_comp: int, bc_tags, device="cuda"):
-        self.device = torch.device(device)
-        if self.device.type != "cuda":
+        device = torch.device(device)
+        if device.type != "cuda":
             raise RuntimeError("FacetCalc runs on a GPU device only")
         nat.load_library()
-        self.cfg = cfg
         self.phy_setup = phy_setup
-        self.n_comp = n_comp
-        self.dim = mesh_setup.dim
-        self.n_local = self.dim + 1
-        self.n_cells = mesh_setup.n_cells
-        self.n_facets = mesh_setup.n_facets
-        self.n_facets_bc = mesh_setup.n_bc_facet
-        self.mesh = DeviceMesh(mesh_setup, n_comp, bc_tags, cfg, self.device)
         self.fvm_mesh = mesh_setup
         self.bc_type_str = [e.tag for e in bc_tags.values()]
+        self._init(DeviceMesh(mesh_setup, n_comp, bc_tags, cfg, device), cfg, device)
+
+    @classmethod
+    def from_device_mesh(cls, dm: "DeviceMesh", cfg, device="cuda"):
+        """A FacetCalc over an already built (e.g. batched) device layout."""
+        obj = cls.__new__(cls)
+        nat.load_library()
+        obj.phy_setup = None
+        obj.fvm_mesh = None
+        obj.bc_type_str = []
+        obj._init(dm, cfg, torch.device(device))
+        return obj
+
+    def _init(self, dm, cfg, device):
+        self.device = device
+        self.cfg = cfg
+        self.mesh = dm
+        self.n_comp = dm.C
+        self.dim = dm.dim
+        self.n_local = self.dim + 1
+        self.n_cells = dm.N
+        self.n_facets = dm.F
+        self.n_facets_bc = dm.Fb
         self.phys = phys_struct(cfg)
-        D, C, N, Fb = self.dim, n_comp, self.n_cells, self.n_facets_bc
+        D, C, N, Fb = self.dim, self.n_comp, self.n_cells, self.n_facets_bc
         dev = self.device
         self.nblk = (N + 255) // 256
         self.ubc = torch.zeros((max(Fb, 1), C), dtype=torch.float32, device=dev)
         self.gp = torch.zeros((N, (D + 1) * C), dtype=torch.float32, device=dev)
         self.bc_part = torch.zeros(4 * max(self.mesh.n_slots, 1), dtype=torch.float32, device=dev)
-        self.bc_mean = torch.zeros(4, dtype=torch.float32, device=dev)
+        self.bc_mean = torch.zeros(max(self.mesh.n_groups, 1), dtype=torch.float32, device=dev)
         self.bc_counter = torch.zeros(1, dtype=torch.int32, device=dev)
         self.red = torch.zeros(self.nblk * 3 * C, dtype=torch.float64, device=dev)
         self.work = nat.FvmWork(ubc=self.ubc.data_ptr(), gp=self.gp.data_ptr(), bc_part=self.bc_part.data_ptr(),

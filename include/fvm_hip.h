@@ -88,8 +88,9 @@ typedef struct FvmMeshDev {
     const float *cell_inv_vol;  /* [N] fp32(1/vol): |flux_mat| entries                */
     const float *cell_eps;      /* [N] limiter epsilon (K sqrt(vol))^(p+1)            */
     /* per facet ------------------------------------------------------------------- */
-    const float *fac_geo;       /* [F][G] G = 8 (2-D) / 12 (3-D): n[d], n_hat[d],
-                                   |n|, 1/(n_hat.d), X_orthog[d] (0 on the boundary)  */
+    const float *fac_geo;       /* [F][G] G = 12 (2-D) / 20 (3-D): n[d], n_hat[d],
+                                   |n|, 1/(n_hat.d), X_orthog[d] (0 on the boundary),
+                                   centroid->midpoint of the slot-0, slot-1 cells  */
     /* per boundary facet (boundary-local index b) ---------------------------------- */
     const int32_t *bc_cell;     /* [Fb] interior cell                                 */
     const int32_t *bc_group;    /* [Fb] characteristic group, or -1                   */
@@ -100,12 +101,23 @@ typedef struct FvmMeshDev {
     const float *bc_gb;         /* [Fb][C] b_face_grad                                */
     const float *bc_char;       /* [Fb][2d+3]: n_out[d], rho_inf, v_n_inf, p_inf,
                                    v_t_inf[d]                                         */
-    /* characteristic-BC group means (facet_boundary.py:427) ------------------------ */
-    int32_t n_slots;            /* 256-cell blocks holding cells of BC-group facets   */
-    int32_t pad0;
-    const int32_t *blk_slot;    /* [ceil(N/256)] partial slot of a block, or -1       */
-    const int32_t *slot_blk;    /* [n_slots] block of each slot                       */
-    float group_count[4];       /* facets per group                                   */
+    /* characteristic-BC group means (facet_boundary.py:427); a group = one BC mode of
+       one mesh ----------------------------------------------------------------------- */
+    int32_t n_slots;            /* 256-cell tiles holding cells of group facets       */
+    int32_t n_meshes;           /* meshes concatenated block-diagonally (>= 1)        */
+    const int32_t *blk_slot;    /* [ceil(N/256)] partial slot of a tile, or -1        */
+    const int32_t *slot_blk;    /* [n_slots] tile of each slot                        */
+    const int32_t *slot_g0;     /* [n_slots] first group of the slot's mesh           */
+    const int32_t *slot_ng;     /* [n_slots] groups of that mesh (<= 4)               */
+    const int32_t *group_s0;    /* [n_groups] first slot of the group's mesh          */
+    const int32_t *group_s1;    /* [n_groups] one past its last slot                  */
+    const float *group_count;   /* [n_groups] facets per group                        */
+    /* batches of meshes (NULL for one mesh) ----------------------------------------- */
+    const int32_t *cell_mesh;   /* [N] mesh of each cell; tiles never straddle meshes */
+    const float *mesh_phys;     /* [n_meshes][4] mu0, mub0, kappa, 0 per mesh         */
+    int32_t persist_blocks;     /* workgroups of the per-cell kernels (0 = one per
+                                   256-cell tile); tiles are walked grid-stride       */
+    int32_t pad1;
 } FvmMeshDev;
 
 typedef struct FvmPhys {      /* scalars of FluidConstitution / SlopeLimiter / KT      */
@@ -123,8 +135,8 @@ typedef struct FvmWork {      /* device scratch owned by the caller             
     float *ubc;       /* [Fb][C] boundary face values                                 */
     float *gp;        /* [N][(d+1)C] unlimited LSQ gradient [d][C] then limiter phi[C];
                          persists between calls (the next call's Neumann correction)  */
-    float *bc_part;   /* [n_groups][n_slots] partial sums of c per slot               */
-    float *bc_mean;   /* [4] group means of c of the primitives the next RHS reads    */
+    float *bc_part;   /* [n_slots][4] partial sums of c per slot and mesh-local group */
+    float *bc_mean;   /* [n_groups] means of c of the primitives the next RHS reads   */
     int32_t *bc_counter; /* [1] slot-completion counter, zero between launches        */
     double *red;      /* [ceil(N/256)][3C] fp64 partial column norms (adaptive dt)    */
 } FvmWork;
@@ -150,7 +162,7 @@ typedef struct FvmStage {
     float *kbuf;           /* [s][N][C] stored stages or NULL                         */
     float *prim_next;      /* [N][C]                                                  */
     float *U_out;          /* [N][C] (last stage)                                     */
-    const float *dt;       /* device scalar                                           */
+    const float *dt;       /* device time step; [n_meshes] for a batch                */
     int32_t stage, last, store_k, has_prev;
     int32_t bc_ready;      /* work->bc_mean already holds the boundary means of prim
                               (the previous stage's epilogue wrote them)              */

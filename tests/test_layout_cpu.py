@@ -156,3 +156,52 @@ def test_facet_geometry_layout(case):
     X = of.X_orthog.squeeze(-1).numpy()
     assert np.max(np.abs(geo[:, 2 * D + 2:3 * D + 2] - X)) < 1e-4 * np.abs(X).max()
     assert _rel(h["eps"], of.eps_p.reshape(-1)) < VAL_RTOL
+
+
+def test_facet_record_side_displacements(case):
+    d, mesh, dm, om, of = case
+    D = mesh.dim
+    geo = dm.host["geo"]
+    disp = d["cent_to_facet_disp"]
+    c2f = d["cell_to_facet"]
+    slots = d["slots"].reshape(c2f.shape)
+    for s_ in (0, 1):
+        c, k = np.nonzero(slots == s_)
+        got = geo[c2f[c, k], 3 * D + 2 + s_ * D:3 * D + 2 + (s_ + 1) * D]
+        assert np.max(np.abs(got - disp[c, k])) < 5e-7
+
+
+def test_batch_layout_offsets_and_group_tables():
+    from fvm_solver_amd.config_fvm import channel_config
+    from fvm_solver_amd.mesh_gen.channel import ChannelSpec, channel_mesh_2d
+    items, meshes = [], []
+    for i, (n, mach, r) in enumerate(((600, 0.2, 0.08), (300, 0.5, 0.12), (900, 0.35, 0.1))):
+        pts, cells, (fi, fb), tags = channel_mesh_2d(n, seed=20 + i, spec=ChannelSpec(centre=(0.5, 0.5), radius=r))
+        mesh = mesh_from_tuple(pts, cells, fi, fb, device="cpu")
+        meshes.append((mesh, tags))
+        items.append((mesh, 4, channel_bc_tags(tags, 2), channel_config(mach, 2 * r, device="cpu")))
+    dm = DeviceMesh.batch(items, "cpu")
+    singles = [DeviceMesh(*it, "cpu") for it in items]
+    h = dm.host
+    co, fo, bo = dm.cell_offsets, dm.facet_offsets, dm.bc_offsets
+    for i, sd in enumerate(singles):
+        a, b = co[i], co[i + 1]
+        sh = sd.host
+        assert np.array_equal(h["c2f_s"][a:b], sh["c2f_s"] + fo[i])
+        nb = sh["nbr_s"]
+        exp = np.where(nb < sd.N, nb + co[i], nb - sd.N + bo[i] + dm.N)
+        assert np.array_equal(h["nbr_s"][a:b], exp)
+        assert np.array_equal(h["info"][a:b], sh["info"])
+        assert np.array_equal(h["geo"][fo[i]:fo[i + 1]], sh["geo"])
+        assert np.array_equal(h["cell_mesh"][a:b], np.full(b - a, i))
+        assert np.allclose(dm.mesh_phys[i, :3], [items[i][3].viscosity, items[i][3].visc_bulk,
+                                                  items[i][3].thermal_cond])
+    # every group facet's tile is a slot whose group window contains the group, and the
+    # group's slot range is exactly the slots whose tiles touch its mesh
+    g = h["group"]
+    tiles = h["bcell"] // 256
+    for b in np.flatnonzero(g >= 0):
+        sl = h["blk_slot"][tiles[b]]
+        assert sl >= 0 and h["slot_g0"][sl] <= g[b] < h["slot_g0"][sl] + h["slot_ng"][sl]
+        assert h["group_s0"][g[b]] <= sl < h["group_s1"][g[b]]
+    assert np.all(h["group_count"] == np.bincount(g[g >= 0]))

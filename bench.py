@@ -40,6 +40,19 @@ def _parse():
                     help="measured HBM bytes per launch (rocprofv3 FETCH_SIZE/WRITE_SIZE, calibrated)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run one 256 MiB device-to-device copy (calibrates HBM byte counters)")
+    g = ap.add_argument_group("learned corrector (--learned)")
+    g.add_argument("--learned", action="store_true",
+                   help="measure the learned window corrector against the plain RK4 solver instead")
+    g.add_argument("--train-meshes", type=int, default=64)
+    g.add_argument("--train-windows", type=int, default=157, help="windows per training mesh (64 x 157 = 10k pairs)")
+    g.add_argument("--train-iters", type=int, default=2000)
+    g.add_argument("--eval-meshes", type=int, default=64)
+    g.add_argument("--eval-windows", type=int, default=16)
+    g.add_argument("--window", type=int, default=8, help="solver steps per predicted window")
+    g.add_argument("--n-min", type=int, default=5000)
+    g.add_argument("--n-max", type=int, default=200_000)
+    g.add_argument("--oracle-meshes", type=int, default=2, help="evaluation meshes re-run by the FP64 CPU oracle")
+    g.add_argument("--tol", type=float, default=1e-3)
     return ap.parse_args()
 
 
@@ -47,19 +60,24 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def flux_kernel_bytes(N, F, Fb, D, stage, stages):
-    """Algorithmic (compulsory) bytes of one k_flux launch: every array read or written
-    once, fp32 values, int32 indices.  Per cell: prim, the gradient record
-    ((D+1)C floats), the facet/neighbour lists, the packed info word, centroid->facet
-    displacements, 1/vol, U0, the RK accumulator (read after stage 0; written, or the
-    new state written on the last stage) and the next-stage primitives; per facet the
-    geometry record (n, n_hat, |n|, 1/(n.d), X_orthog); per boundary facet the boundary
-    values and the two face-gradient coefficients."""
+def facet_kernel_bytes(N, F, Fb, D):
+    """Algorithmic (compulsory) bytes of one k_facet_flux launch, every array read or
+    written once (fp32 values, int32 indices).  Per facet: its two cells (2 indices),
+    the geometry record (n, n_hat, |n|, 1/(n.d), X_orthog, both centroid->midpoint
+    displacements) and the flux record written; per cell: prim and the gradient record
+    ((D+1)C floats); per boundary facet: the boundary values and the two face-gradient
+    coefficients."""
+    C = D + 2
+    geo = 12 if D == 2 else 20
+    return F * 4 * (2 + geo + C) + N * 4 * (C + (D + 1) * C) + Fb * 4 * (3 * C)
+
+
+def div_kernel_bytes(N, F, D, stage):
+    """k_div (stage epilogue): per cell the facet list, info word, 1/vol, U0, the RK
+    accumulator (read after stage 0), the accumulator or new state written, and the
+    next-stage primitives written; per facet its flux record, read once."""
     C, NL = D + 2, D + 1
-    cell = 4 * (C + (D + 1) * C + NL + NL + 1 + NL * D + 1 + C + (C if stage > 0 else 0) + C + C)
-    facet = 4 * (3 * D + 2)
-    bc = 4 * (3 * C)
-    return N * cell + F * facet + Fb * bc
+    return N * 4 * (NL + 1 + 1 + C + (C if stage > 0 else 0) + 2 * C) + F * 4 * C
 
 
 def grad_kernel_bytes(N, Fb, D):
@@ -148,9 +166,9 @@ def main():
 
     # ---- per-kernel durations: the same stages, launched part by part between events ----
     E = eq.E_props
-    parts = {1: "k_bc_mean", 4: "k_grad", 8: "k_flux"}
+    parts = {1: "k_bc_mean", 2: "k_grad", 4: "k_facet_flux", 8: "k_div"}
     dur = {k: [] for k in parts.values()}
-    flux_bytes = []
+    div_bytes = []
     U = eq.cells.state
     st_prim = ts._stage_prim0(U)
     s = ts.stages
@@ -173,7 +191,7 @@ def main():
                 E.stage(st, parts=mask)
                 e1.record()
                 dur[name].append((e0, e1))
-            flux_bytes.append(flux_kernel_bytes(N, F, Fb, D, i, s))
+            div_bytes.append(div_kernel_bytes(N, F, D, i))
             ts._flip = 1 - ts._flip
             cur = nxt
         ts._last_out = out
@@ -183,17 +201,15 @@ def main():
     torch.cuda.synchronize()
     avg_us = {k: 1e3 * float(np.mean([e0.elapsed_time(e1) for e0, e1 in v])) for k, v in dur.items()}
     log("kernel averages (us): " + ", ".join(f"{k}={v:.1f}" for k, v in avg_us.items()))
+    kbytes = {"k_grad": float(grad_kernel_bytes(N, Fb, D)), "k_facet_flux": float(facet_kernel_bytes(N, F, Fb, D)),
+              "k_div": float(np.mean(div_bytes)), "k_bc_mean": float(max(E.mesh.n_slots, 1) * 256 * 4 * 2)}
+    kgbs = {k: kbytes[k] / (avg_us[k] * 1e-6) / 1e9 for k in avg_us}
     dom = max(avg_us, key=avg_us.get)
-    if dom == "k_flux":
-        dom_bytes = float(np.mean(flux_bytes))
-    elif dom == "k_grad":
-        dom_bytes = float(grad_kernel_bytes(N, Fb, D))
-    else:
-        dom_bytes = float(max(E.mesh.n_slots, 1) * 256 * 4 * 2)
-    achieved = dom_bytes / (avg_us[dom] * 1e-6) / 1e9
-    stage_bytes = float(np.mean(flux_bytes)) + grad_kernel_bytes(N, Fb, D)
-    log(f"dominant {dom}: {dom_bytes / 1e6:.1f} MB/launch -> {achieved:.0f} GB/s; "
-        f"grad+flux {stage_bytes / N:.0f} B/cell/stage")
+    dom_bytes = kbytes[dom]
+    achieved = kgbs[dom]
+    stage_bytes = kbytes["k_grad"] + kbytes["k_facet_flux"] + kbytes["k_div"]
+    log(f"dominant {dom}: {dom_bytes / 1e6:.1f} MB/launch -> {achieved:.0f} GB/s; per kernel GB/s: "
+        + ", ".join(f"{k}={v:.0f}" for k, v in kgbs.items()) + f"; {stage_bytes / N:.0f} B/cell/stage")
 
     out = {
         "metric": "cell-updates/s (fused HIP RK4 step, 2-D NS channel)",
@@ -210,7 +226,8 @@ def main():
         "data": "synthetic (seeded Poisson-disk Delaunay mesh, freestream + Gaussian bump)",
         "config": {"workload": "2-D compressible NS channel with circular obstacle, RK4 at CFL 0.5",
                    "cells": N, "facets": F, "boundary_facets": Fb, "integrator": "Butcher RK4",
-                   "dt": dt, "finite": finite, "kernel_avg_us": avg_us},
+                   "dt": dt, "finite": finite, "kernel_avg_us": avg_us,
+                   "kernel_algorithmic_bytes": kbytes, "kernel_gbs": kgbs},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None},
     }
@@ -252,5 +269,115 @@ def main():
     print(json.dumps(out), flush=True)
 
 
+def _oracle_phys(cfg):
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from oracle import fvm_oracle as O
+    ins, ex = cfg.inlet_cfg, cfg.exit_cfg
+    return O.Phys(T_0=cfg.T_0, viscosity=cfg.viscosity, visc_bulk=cfg.visc_bulk, thermal_cond=cfg.thermal_cond,
+                  S_const=cfg.S_const, gamma=cfg.gamma, C_v=cfg.C_v, v_factor=cfg.v_factor, lim_p=cfg.lim_p,
+                  lim_K=cfg.lim_K, inlet=O.BCSpec(ins.v_n_inf, ins.v_t_inf, ins.rho_inf, ins.T_inf, tuple(ins.v_inf)),
+                  exit=O.BCSpec(ex.v_n_inf, ex.v_t_inf, ex.rho_inf, ex.T_inf, tuple(ex.v_inf)))
+
+
+def learned_main(a):
+    """Learned corrector vs plain RK4 on a batch of 2-D channel meshes: simulated time
+    per wall second, acceptance, and error against the FP64 CPU oracle (the checker)."""
+    from fvm_solver_amd.batch import BatchedSolver, random_channel_batch
+    from fvm_solver_amd.learned.corrector import LearnedCorrector
+    dev = torch.device("cuda:0")
+    t0 = time.time()
+    tp, tparams = random_channel_batch(a.train_meshes, seed=a.seed + 1000, n_min=a.n_min, n_max=a.n_max, device=dev)
+    ts = BatchedSolver(tp, device=dev)
+    ts.set_dt(ts.cfl_dt(0.5))
+    lc = LearnedCorrector(ts, k=a.window, seed=a.seed, tol=a.tol)
+    data = lc.generate(a.train_windows, log=log)
+    gen_s = time.time() - t0
+    log(f"training data: {a.train_meshes} meshes x {a.train_windows} windows ({ts.dm.N} cells) in {gen_s:.1f}s")
+    t1 = time.time()
+    hist = lc.train(data[:-4], a.train_iters, log=log)
+    torch.cuda.synchronize()
+    train_s = time.time() - t1
+    held = lc.eval_loss(data[-4:])
+    zero = float(np.mean([float((T.double() ** 2).mean()) for _, T in data[-4:]]))
+    log(f"trained {a.train_iters} iters in {train_s:.1f}s; held-out loss {held:.4f} (Euler window {zero:.4f})")
+    stats, weights = lc.stats, lc.net.state_dict()
+    del data, lc, ts
+    torch.cuda.empty_cache()
+
+    ep, eparams = random_channel_batch(a.eval_meshes, seed=a.seed + 2000, n_min=a.n_min, n_max=a.n_max, device=dev)
+    plain = BatchedSolver(ep, device=dev)
+    dts = plain.cfl_dt(0.5)
+    plain.set_dt(dts)
+    learn = BatchedSolver(ep, device=dev)
+    learn.set_dt(dts)
+    lc = LearnedCorrector(learn, k=a.window, tol=a.tol)
+    lc.stats = stats
+    lc._apply_stats()
+    lc.net.load_state_dict(weights)
+    U0 = plain.state.clone()
+    g = plain.capture_graph(a.window)                 # one window of plain steps as a HIP graph
+    plain.set_state(U0)
+    torch.cuda.synchronize()
+    tp0 = time.perf_counter()
+    for _ in range(a.eval_windows):
+        g.replay()
+    torch.cuda.synchronize()
+    plain_s = time.perf_counter() - tp0
+    learn_s, acc = lc.run(a.eval_windows)
+    sim_t = a.eval_windows * a.window * dts          # per mesh
+    N = plain.dm.N
+    log(f"eval: {a.eval_meshes} meshes, {N} cells, {a.eval_windows} windows of {a.window} steps: plain {plain_s:.3f}s, "
+        f"learned {learn_s:.3f}s, accepted {acc}/{a.eval_windows}")
+
+    # checker: the FP64 CPU oracle re-runs the smallest evaluation meshes
+    from oracle import fvm_oracle as O
+    order = np.argsort([p["cells"] for p in eparams])[:a.oracle_meshes]
+    errs = {}
+    for i in order:
+        pts, cells, fi, fb, tags = eparams[i]["mesh_tuple"]
+        cfg = ep[i][2]
+        _, ofvm = O.from_mesh_tuple(pts, cells, fi, fb, tags, _oracle_phys(cfg), dtype=torch.float64)
+        st = O.OracleStepper(ofvm, "Butcher", "RK4", dt=float(dts[i]))
+        U = ep[i][3].double()
+        for _ in range(a.eval_windows * a.window):
+            U = st.step(U)
+        ref = U.numpy()
+        rel = lambda x: (np.linalg.norm(x.cpu().double().numpy() - ref, axis=0) / np.linalg.norm(ref, axis=0)).tolist()
+        errs[int(i)] = {"cells": int(eparams[i]["cells"]), "learned": rel(learn.state_of(i)),
+                        "plain": rel(plain.state_of(i))}
+        log(f"mesh {i} ({eparams[i]['cells']} cells): rel-L2 vs FP64 learned {errs[int(i)]['learned']}, "
+            f"plain {errs[int(i)]['plain']}")
+    worst = max(max(e["learned"]) for e in errs.values())
+    out = {
+        "metric": "simulated time per wall second, learned window corrector vs plain RK4 (batched 2-D channel)",
+        "value": float(np.mean(sim_t) / learn_s),
+        "unit": "sim-s/s",
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "vs_plain": plain_s / learn_s,
+        "plain_value": float(np.mean(sim_t) / plain_s),
+        "accepted_windows": int(acc),
+        "windows": a.eval_windows,
+        "max_rel_l2_vs_fp64": worst,
+        "tol": a.tol,
+        "dtype": "bf16 network (fp32 accumulation), f32 solver",
+        "data": "synthetic: trajectories of the batched solver on seeded random channel meshes",
+        "config": {"workload": "learned corrector, 3 message-passing layers, hidden 128, window of "
+                               f"{a.window} RK4 steps at CFL 0.5", "train_meshes": a.train_meshes,
+                   "train_pairs": a.train_meshes * a.train_windows, "train_iters": a.train_iters,
+                   "eval_meshes": a.eval_meshes, "eval_cells": N, "cells_range": [a.n_min, a.n_max],
+                   "held_out_loss": held, "euler_window_loss": zero, "train_seconds": train_s,
+                   "loss_history": hist[-3:], "errors_vs_fp64": errs},
+    }
+    print(json.dumps(out), flush=True)
+
+
 if __name__ == "__main__":
-    main()
+    _a = _parse()
+    if _a.learned:
+        if _a.gpus != 1:
+            print("bench.py runs on one GPU only", file=sys.stderr)
+            sys.exit(2)
+        learned_main(_a)
+    else:
+        main()

@@ -12,7 +12,7 @@ import torch
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libfvm_hip.so")
-ABI_VERSION = 4
+ABI_VERSION = 5
 
 FVM_ERR_ROW_OVERFLOW = 2
 
@@ -25,7 +25,7 @@ class FvmMeshDev(ctypes.Structure):
     _fields_ = [("dim", i32), ("n_groups", i32), ("n_cells", i64), ("n_facets", i64), ("n_bc", i64),
                 ("cell_facet", c_void_p), ("cell_nbr", c_void_p), ("cell_info", c_void_p),
                 ("cell_disp", c_void_p), ("cell_gval", c_void_p), ("cell_inv_vol", c_void_p),
-                ("cell_eps", c_void_p), ("fac_geo", c_void_p),
+                ("cell_eps", c_void_p), ("fac_geo", c_void_p), ("fac_cell", c_void_p),
                 ("bc_cell", c_void_p), ("bc_group", c_void_p), ("bc_flags", c_void_p), ("bc_val", c_void_p),
                 ("bc_l", c_void_p), ("bc_ga", c_void_p), ("bc_gb", c_void_p), ("bc_char", c_void_p),
                 ("n_slots", i32), ("n_meshes", i32), ("blk_slot", c_void_p), ("slot_blk", c_void_p),
@@ -40,8 +40,8 @@ class FvmPhys(ctypes.Structure):
 
 
 class FvmWork(ctypes.Structure):
-    _fields_ = [("ubc", c_void_p), ("gp", c_void_p), ("bc_part", c_void_p), ("bc_mean", c_void_p),
-                ("bc_counter", c_void_p), ("red", c_void_p)]
+    _fields_ = [("ubc", c_void_p), ("gp", c_void_p), ("bc_part", c_void_p), ("red", c_void_p),
+                ("fflux", c_void_p)]
 
 
 class FvmStage(ctypes.Structure):
@@ -71,6 +71,23 @@ _PROTOS = {
     "fvm_state_to_prim_f32": (ctypes.c_int, [c_void_p, c_void_p, i64, i32, f32, c_void_p]),
     "fvm_conservation_f64": (ctypes.c_int, [c_void_p, c_void_p, i64, i32, c_void_p, c_void_p, c_void_p]),
     "fvm_cfl_dt": (ctypes.c_int, [c_void_p, c_void_p, i64, i32, f64, f64, c_void_p, c_void_p, c_void_p]),
+    "fvm_gemm_bf16": (ctypes.c_int, [c_void_p, i64, i64, c_void_p, i64, i64, i64, i64, i64, i32, c_void_p,
+                                     c_void_p, i64, c_void_p, i32, c_void_p, c_void_p, c_void_p]),
+    "fvm_mp_aggregate": (ctypes.c_int, [c_void_p, i32, c_void_p, i64, i32, c_void_p, c_void_p]),
+    "fvm_mp_aggregate_bwd": (ctypes.c_int, [c_void_p, i32, c_void_p, i64, i32, c_void_p, c_void_p, c_void_p]),
+    "fvm_relu_residual": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, i64, c_void_p]),
+    "fvm_relu_bwd": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p, i64, c_void_p]),
+    "fvm_colsum_f32": (ctypes.c_int, [c_void_p, i64, i32, c_void_p, c_void_p, c_void_p]),
+    "fvm_mse_loss": (ctypes.c_int, [c_void_p, i64, c_void_p, i64, i32, c_void_p, c_void_p, c_void_p]),
+    "fvm_adam_f32": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, i64, f32, f32, f32, f32, i32,
+                                    c_void_p]),
+    "fvm_cast_f32_bf16": (ctypes.c_int, [c_void_p, c_void_p, i64, c_void_p]),
+    "fvm_mp_features": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p, i32, i64, i32, c_void_p, c_void_p,
+                                       c_void_p, c_void_p, i32, c_void_p]),
+    "fvm_mp_apply": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p, i64, c_void_p, i64, i32, c_void_p,
+                                    c_void_p]),
+    "fvm_mp_defect": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, i32, i32, c_void_p,
+                                     c_void_p, c_void_p]),
 }
 
 _lib = None

@@ -1,6 +1,6 @@
 """Launch-shape autotuning of the per-cell kernels.
 
-The tunable is the workgroup count of k_grad / k_flux (``FvmMeshDev.persist_blocks``):
+The tunable is the workgroup count of k_grad / k_facet_flux / k_div (``FvmMeshDev.persist_blocks``):
 0 launches one workgroup per 256-cell tile; a positive multiple of 8 launches that many
 workgroups which walk the tiles grid-stride (persistent), each XCD over a contiguous
 tile range.  Candidates are timed with stream events on the live problem (the steps it

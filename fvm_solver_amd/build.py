@@ -23,6 +23,7 @@ SOURCES = {
     "ell.hip": [],
     "rhs.hip": ["-ffp-contract=off"],
     "reduce.hip": [],
+    "mpnn.hip": [],
 }
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang",

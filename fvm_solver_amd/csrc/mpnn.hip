@@ -1,0 +1,475 @@
+// Message-passing corrector on the mesh graph: BF16 MFMA GEMMs with FP32 accumulation,
+// neighbour aggregation forward/backward, activations, loss and Adam, for gfx950.
+//
+// The network (the corrector of the learned path) is
+//   h0 = relu(x W_in + b_in),  h_{l+1} = h_l + relu([h_l | mean_nb h_l] W_l + b_l)  (l < 3),
+//   y  = h3 W_out + b_out,
+// with hidden width H = 128 and the mean over a cell's interior face neighbours.  The
+// mesh adjacency is symmetric, so the aggregation's backward is a gather as well
+// (deterministic, no atomics).
+#include "fvm_common.hpp"
+
+namespace fvm {
+
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+
+constexpr int BM = 64, BN = 64, BK = 32;
+
+// C(i, j) = sum_k A(i, k) B(k, j) over k in this split's range, with A(i, k) at
+// A[i * a_rs + k * a_cs] and B(k, j) at B[k * b_rs + j * b_cs] (any bf16 layout: tiles
+// are staged through LDS).  One 256-thread workgroup computes a 64 x 64 tile; wave w
+// owns rows 16w..16w+15 and issues four 16x16x32 MFMAs per 32-wide K step.
+// Split-K (gridDim.z > 1) writes fp32 partials part[z][M][N]; otherwise the epilogue
+// adds the bias and writes the pre-activation z (fp32, Cf, row stride c_rs) and/or
+// res + relu(z) (bf16, Cb; relu and res optional).
+__global__ void __launch_bounds__(256)
+k_gemm_bf16(const __bf16 *__restrict__ A, int64_t a_rs, int64_t a_cs, const __bf16 *__restrict__ B, int64_t b_rs,
+            int64_t b_cs, int64_t M, int64_t N, int64_t K, int64_t k_chunk, const float *__restrict__ bias,
+            float *__restrict__ Cf, int64_t c_rs, __bf16 *__restrict__ Cb, int relu, const __bf16 *__restrict__ res,
+            float *__restrict__ part) {
+    // rows padded to 40 elements (80 bytes): 16-byte fragment reads stay aligned and
+    // consecutive rows start in different banks
+    __shared__ __attribute__((aligned(16))) __bf16 As[BM][BK + 8];
+    __shared__ __attribute__((aligned(16))) __bf16 Bs[BN][BK + 8];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int64_t m0 = static_cast<int64_t>(blockIdx.x) * BM, n0 = static_cast<int64_t>(blockIdx.y) * BN;
+    const int64_t kb = static_cast<int64_t>(blockIdx.z) * k_chunk;
+    const int64_t ke = kb + k_chunk < K ? kb + k_chunk : K;
+    f32x4_t acc[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) acc[q] = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
+    const int lr = tid >> 2, lk = (tid & 3) * 8;
+    for (int64_t k0 = kb; k0 < ke; k0 += BK) {
+        {
+            const int64_t gi = m0 + lr;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const int64_t gk = k0 + lk + e;
+                As[lr][lk + e] = (gi < M && gk < ke) ? A[gi * a_rs + gk * a_cs] : __bf16(0.0f);
+            }
+            const int64_t gj = n0 + lr;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const int64_t gk = k0 + lk + e;
+                Bs[lr][lk + e] = (gj < N && gk < ke) ? B[gk * b_rs + gj * b_cs] : __bf16(0.0f);
+            }
+        }
+        __syncthreads();
+        const bf16x8_t a = *reinterpret_cast<const bf16x8_t *>(&As[16 * wave + (lane & 15)][8 * (lane >> 4)]);
+#pragma unroll
+        for (int cb = 0; cb < 4; ++cb) {
+            const bf16x8_t b = *reinterpret_cast<const bf16x8_t *>(&Bs[16 * cb + (lane & 15)][8 * (lane >> 4)]);
+            acc[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[cb], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    const bool split = gridDim.z > 1;
+#pragma unroll
+    for (int cb = 0; cb < 4; ++cb) {
+        const int64_t j = n0 + 16 * cb + (lane & 15);
+        if (j >= N) continue;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int64_t i = m0 + 16 * wave + 4 * (lane >> 4) + r;
+            if (i >= M) continue;
+            float v = acc[cb][r];
+            if (split) {
+                part[(static_cast<int64_t>(blockIdx.z) * M + i) * N + j] = v;
+                continue;
+            }
+            if (bias) v += bias[j];
+            if (Cf) Cf[i * c_rs + j] = v;
+            if (relu) v = fmaxf(v, 0.0f);
+            if (res) v += static_cast<float>(res[i * N + j]);
+            if (Cb) Cb[i * N + j] = __bf16(v);
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256)
+k_splitk_reduce(const float *__restrict__ part, int S, int64_t M, int64_t N, const float *__restrict__ bias,
+                float *__restrict__ Cf, int64_t c_rs, __bf16 *__restrict__ Cb, int relu,
+                const __bf16 *__restrict__ res) {
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= M * N) return;
+    const int64_t i = t / N, j = t % N;
+    float v = 0.0f;
+    for (int z = 0; z < S; ++z) v += part[z * M * N + t];
+    if (bias) v += bias[j];
+    if (Cf) Cf[i * c_rs + j] = v;
+    if (relu) v = fmaxf(v, 0.0f);
+    if (res) v += static_cast<float>(res[t]);
+    if (Cb) Cb[t] = __bf16(v);
+}
+
+// A[c] = [h[c] | mean of h over the interior face neighbours of c]   (bf16, [N][2H])
+__global__ void __launch_bounds__(256)
+k_mp_aggregate(const __bf16 *__restrict__ h, int H, const int32_t *__restrict__ nbr, int64_t N, int NL,
+               __bf16 *__restrict__ A) {
+    const int chunks = H / 8;
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= N * chunks) return;
+    const int64_t c = t / chunks;
+    const int q = static_cast<int>(t % chunks);
+    const bf16x8_t own = *reinterpret_cast<const bf16x8_t *>(h + c * H + 8 * q);
+    *reinterpret_cast<bf16x8_t *>(A + c * 2 * H + 8 * q) = own;
+    float s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int cnt = 0;
+    for (int k = 0; k < NL; ++k) {
+        const int64_t nb = nbr[k * N + c];
+        if (nb < N) {
+            const bf16x8_t v = *reinterpret_cast<const bf16x8_t *>(h + nb * H + 8 * q);
+#pragma unroll
+            for (int e = 0; e < 8; ++e) s[e] += static_cast<float>(v[e]);
+            ++cnt;
+        }
+    }
+    const float inv = cnt ? 1.0f / static_cast<float>(cnt) : 0.0f;
+    bf16x8_t m;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) m[e] = __bf16(s[e] * inv);
+    *reinterpret_cast<bf16x8_t *>(A + c * 2 * H + H + 8 * q) = m;
+}
+
+// dh[c] = dA[c][:H] + sum_{nb interior neighbour of c} dA[nb][H:] / n_nb  (+ dres[c])
+__global__ void __launch_bounds__(256)
+k_mp_aggregate_bwd(const float *__restrict__ dA, int H, const int32_t *__restrict__ nbr, int64_t N, int NL,
+                   const float *__restrict__ dres, float *__restrict__ dh) {
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= N * H) return;
+    const int64_t c = t / H;
+    const int e = static_cast<int>(t % H);
+    float v = dA[c * 2 * H + e];
+    for (int k = 0; k < NL; ++k) {
+        const int64_t nb = nbr[k * N + c];
+        if (nb < N) {
+            int cnt = 0;
+            for (int kk = 0; kk < NL; ++kk) cnt += nbr[kk * N + nb] < N;
+            v += dA[nb * 2 * H + H + e] / static_cast<float>(cnt);
+        }
+    }
+    if (dres) v += dres[t];
+    dh[t] = v;
+}
+
+// h_out = h_in + relu(z) (bf16 out); with h_in == NULL: h_out = relu(z)
+__global__ void __launch_bounds__(256)
+k_relu_residual(const float *__restrict__ z, const __bf16 *__restrict__ h_in, __bf16 *__restrict__ h_out, int64_t n) {
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const float r = fmaxf(z[t], 0.0f);
+    h_out[t] = __bf16(h_in ? static_cast<float>(h_in[t]) + r : r);
+}
+
+// dz = dh * (z > 0), written as bf16 (GEMM operand) and fp32
+__global__ void __launch_bounds__(256)
+k_relu_bwd(const float *__restrict__ dh, const float *__restrict__ z, __bf16 *__restrict__ dz_b,
+           float *__restrict__ dz_f, int64_t n) {
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const float g = z[t] > 0.0f ? dh[t] : 0.0f;
+    dz_b[t] = __bf16(g);
+    if (dz_f) dz_f[t] = g;
+}
+
+// Column sums of a [M][N] fp32 matrix: per-block partials [gridDim.x][N], summed in block
+// order by k_colsum_final.
+__global__ void __launch_bounds__(256)
+k_colsum_partial(const float *__restrict__ X, int64_t M, int N, float *__restrict__ part) {
+    for (int j = threadIdx.x; j < N; j += blockDim.x) {
+        float s = 0.0f;
+        for (int64_t i = blockIdx.x; i < M; i += gridDim.x) s += X[i * N + j];
+        part[static_cast<int64_t>(blockIdx.x) * N + j] = s;
+    }
+}
+
+__global__ void k_colsum_final(const float *__restrict__ part, int nblk, int N, float *__restrict__ out) {
+    for (int j = threadIdx.x; j < N; j += blockDim.x) {
+        float s = 0.0f;
+        for (int b = 0; b < nblk; ++b) s += part[static_cast<int64_t>(b) * N + j];
+        out[j] = s;
+    }
+}
+
+// Masked MSE over the first n_out of `ld` columns: loss partials (fp64) and
+// dy = scale * 2 (y - t), scale = 1 / (rows * n_out).
+__global__ void __launch_bounds__(256)
+k_mse(const float *__restrict__ y, int64_t ld_y, const float *__restrict__ tgt, int64_t M, int n_out,
+      float *__restrict__ dy, double *__restrict__ part) {
+    __shared__ double red[256];
+    double s = 0.0;
+    const float scale = 1.0f / static_cast<float>(M * n_out);
+    for (int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < M * ld_y;
+         t += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const int64_t i = t / ld_y;
+        const int j = static_cast<int>(t % ld_y);
+        float g = 0.0f;
+        if (j < n_out) {
+            const float d = y[t] - tgt[i * n_out + j];
+            s += static_cast<double>(d) * d;
+            g = 2.0f * d * scale;
+        }
+        dy[t] = g;
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// Adam with bias correction on fp32 master weights; refreshes the bf16 copy.
+__global__ void __launch_bounds__(256)
+k_adam(float *__restrict__ w, const float *__restrict__ g, float *__restrict__ m, float *__restrict__ v,
+       __bf16 *__restrict__ wb, int64_t n, float lr, float b1, float b2, float eps, float bc1, float bc2) {
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const float gt = g[t];
+    const float mt = b1 * m[t] + (1.0f - b1) * gt;
+    const float vt = b2 * v[t] + (1.0f - b2) * gt * gt;
+    m[t] = mt;
+    v[t] = vt;
+    const float wt = w[t] - lr * (mt / bc1) / (sqrtf(vt / bc2) + eps);
+    w[t] = wt;
+    if (wb) wb[t] = __bf16(wt);
+}
+
+__global__ void __launch_bounds__(256) k_cast_f32_bf16(const float *__restrict__ x, __bf16 *__restrict__ y, int64_t n) {
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t < n) y[t] = __bf16(x[t]);
+}
+
+__global__ void __launch_bounds__(256)
+k_mp_features(const float *__restrict__ prim, const float *__restrict__ dudt, const float *__restrict__ win,
+              const float *__restrict__ geo, int G, int64_t N, int C, const float *__restrict__ mu_p,
+              const float *__restrict__ sd_p, const float *__restrict__ sd_d, __bf16 *__restrict__ X, int KX) {
+    const int64_t c = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (c >= N) return;
+    __bf16 *x = X + c * KX;
+    const float w = win[c];
+    for (int j = 0; j < C; ++j) {
+        x[j] = __bf16((prim[c * C + j] - mu_p[j]) / sd_p[j]);
+        x[C + j] = __bf16(w * dudt[c * C + j] / sd_d[j]);
+    }
+    for (int g = 0; g < G; ++g) x[2 * C + g] = __bf16(geo[c * G + g]);
+    for (int j = 2 * C + G; j < KX; ++j) x[j] = __bf16(0.0f);
+}
+
+__global__ void __launch_bounds__(256)
+k_mp_apply(const float *__restrict__ U, const float *__restrict__ dudt, const float *__restrict__ win,
+           const float *__restrict__ y, int64_t ld_y, const float *__restrict__ scale, int64_t N, int C,
+           float *__restrict__ U_out) {
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= N * C) return;
+    const int64_t c = t / C;
+    const int j = static_cast<int>(t % C);
+    U_out[t] = (U[t] + win[c] * dudt[t]) + scale[j] * y[c * ld_y + j];
+}
+
+// grid (64, n_meshes): block b of mesh q sums cells off[q] + b, + 64 * 256 ... strided
+__global__ void __launch_bounds__(256)
+k_mp_defect_partial(const float *__restrict__ U0, const float *__restrict__ U1, const float *__restrict__ f0,
+                    const float *__restrict__ f1, const float *__restrict__ win, const int64_t *__restrict__ off,
+                    int C, double *__restrict__ part) {
+    __shared__ double red[256];
+    const int q = blockIdx.y;
+    const int64_t a = off[q], b = off[q + 1];
+    double s[16];
+    for (int j = 0; j < 2 * C; ++j) s[j] = 0.0;
+    for (int64_t c = a + static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; c < b;
+         c += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const float hw = 0.5f * win[c];
+        for (int j = 0; j < C; ++j) {
+            const int64_t o = c * C + j;
+            const float d = (U1[o] - U0[o]) - hw * (f0[o] + f1[o]);
+            s[j] += static_cast<double>(d) * d;
+            s[C + j] += static_cast<double>(U1[o]) * U1[o];
+        }
+    }
+    for (int j = 0; j < 2 * C; ++j) {
+        red[threadIdx.x] = s[j];
+        __syncthreads();
+        for (int h = 128; h > 0; h >>= 1) {
+            if (threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) part[(static_cast<int64_t>(q) * gridDim.x + blockIdx.x) * 2 * C + j] = red[0];
+        __syncthreads();
+    }
+}
+
+__global__ void k_mp_defect_final(const double *__restrict__ part, int nblk, int n_meshes, int C,
+                                  double *__restrict__ out) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_meshes * 2 * C) return;
+    const int q = t / (2 * C), j = t % (2 * C);
+    double s = 0.0;
+    for (int b = 0; b < nblk; ++b) s += part[(static_cast<int64_t>(q) * nblk + b) * 2 * C + j];
+    out[t] = s;
+}
+
+}  // namespace fvm
+
+using namespace fvm;
+
+extern "C" {
+
+int fvm_gemm_bf16(const void *A, int64_t a_rs, int64_t a_cs, const void *B, int64_t b_rs, int64_t b_cs, int64_t M,
+                  int64_t N, int64_t K, int32_t splits, const float *bias, float *Cf, int64_t c_rs, void *Cb,
+                  int32_t relu, const void *res, float *part, void *stream) {
+    if (!A || !B || M < 0 || N < 0 || K < 0 || splits < 1 || splits > 65535 || (!Cf && !Cb) ||
+        (splits > 1 && !part) || M > (int64_t(1) << 36) || N > (int64_t(65535) * BN)) {
+        set_error("fvm_gemm_bf16: invalid arguments");
+        return FVM_ERR_ARG;
+    }
+    if (M == 0 || N == 0) return FVM_OK;
+    int64_t k_chunk = (K + splits - 1) / splits;
+    k_chunk = ((k_chunk + BK - 1) / BK) * BK;
+    const int S = static_cast<int>((K + k_chunk - 1) / k_chunk > 0 ? (K + k_chunk - 1) / k_chunk : 1);
+    hipStream_t s = as_stream(stream);
+    dim3 grid(static_cast<unsigned int>((M + BM - 1) / BM), static_cast<unsigned int>((N + BN - 1) / BN), S);
+    const __bf16 *r = static_cast<const __bf16 *>(res);
+    hipLaunchKernelGGL(k_gemm_bf16, grid, dim3(256), 0, s, static_cast<const __bf16 *>(A), a_rs, a_cs,
+                       static_cast<const __bf16 *>(B), b_rs, b_cs, M, N, K, k_chunk, bias, Cf, c_rs,
+                       static_cast<__bf16 *>(Cb), relu, r, part);
+    if (S > 1)
+        hipLaunchKernelGGL(k_splitk_reduce, dim3(blocks_for(M * N, 256)), dim3(256), 0, s, part, S, M, N, bias, Cf,
+                           c_rs, static_cast<__bf16 *>(Cb), relu, r);
+    return check_launch("fvm_gemm_bf16");
+}
+
+int fvm_mp_aggregate(const void *h, int32_t H, const int32_t *nbr, int64_t N, int32_t NL, void *A, void *stream) {
+    if (!h || !nbr || !A || H % 8 || N < 0 || NL < 1) {
+        set_error("fvm_mp_aggregate: invalid arguments");
+        return FVM_ERR_ARG;
+    }
+    if (N == 0) return FVM_OK;
+    hipLaunchKernelGGL(k_mp_aggregate, dim3(blocks_for(N * (H / 8), 256)), dim3(256), 0, as_stream(stream),
+                       static_cast<const __bf16 *>(h), H, nbr, N, NL, static_cast<__bf16 *>(A));
+    return check_launch("fvm_mp_aggregate");
+}
+
+int fvm_mp_aggregate_bwd(const float *dA, int32_t H, const int32_t *nbr, int64_t N, int32_t NL, const float *dres,
+                         float *dh, void *stream) {
+    if (!dA || !nbr || !dh || N < 0 || NL < 1 || H < 1) {
+        set_error("fvm_mp_aggregate_bwd: invalid arguments");
+        return FVM_ERR_ARG;
+    }
+    if (N == 0) return FVM_OK;
+    hipLaunchKernelGGL(k_mp_aggregate_bwd, dim3(blocks_for(N * H, 256)), dim3(256), 0, as_stream(stream), dA, H, nbr,
+                       N, NL, dres, dh);
+    return check_launch("fvm_mp_aggregate_bwd");
+}
+
+int fvm_relu_residual(const float *z, const void *h_in, void *h_out, int64_t n, void *stream) {
+    if (!z || !h_out || n < 0) {
+        set_error("fvm_relu_residual: invalid arguments");
+        return FVM_ERR_ARG;
+    }
+    if (n == 0) return FVM_OK;
+    hipLaunchKernelGGL(k_relu_residual, dim3(blocks_for(n, 256)), dim3(256), 0, as_stream(stream), z,
+                       static_cast<const __bf16 *>(h_in), static_cast<__bf16 *>(h_out), n);
+    return check_launch("fvm_relu_residual");
+}
+
+int fvm_relu_bwd(const float *dh, const float *z, void *dz_b, float *dz_f, int64_t n, void *stream) {
+    if (!dh || !z || !dz_b || n < 0) {
+        set_error("fvm_relu_bwd: invalid arguments");
+        return FVM_ERR_ARG;
+    }
+    if (n == 0) return FVM_OK;
+    hipLaunchKernelGGL(k_relu_bwd, dim3(blocks_for(n, 256)), dim3(256), 0, as_stream(stream), dh, z,
+                       static_cast<__bf16 *>(dz_b), dz_f, n);
+    return check_launch("fvm_relu_bwd");
+}
+
+int fvm_colsum_f32(const float *X, int64_t M, int32_t N, float *out, float *part /* [256][N] */, void *stream) {
+    if (!X || !out || !part || M < 0 || N < 1) {
+        set_error("fvm_colsum_f32: invalid arguments");
+        return FVM_ERR_ARG;
+    }
+    hipStream_t s = as_stream(stream);
+    hipLaunchKernelGGL(k_colsum_partial, dim3(256), dim3(256), 0, s, X, M, N, part);
+    hipLaunchKernelGGL(k_colsum_final, dim3(1), dim3(256), 0, s, part, 256, N, out);
+    return check_launch("fvm_colsum_f32");
+}
+
+int fvm_mse_loss(const float *y, int64_t ld_y, const float *tgt, int64_t M, int32_t n_out, float *dy,
+                 double *part /* [256] */, void *stream) {
+    if (!y || !tgt || !dy || !part || M < 1 || n_out < 1 || ld_y < n_out) {
+        set_error("fvm_mse_loss: invalid arguments");
+        return FVM_ERR_ARG;
+    }
+    hipLaunchKernelGGL(k_mse, dim3(256), dim3(256), 0, as_stream(stream), y, ld_y, tgt, M, n_out, dy, part);
+    return check_launch("fvm_mse_loss");
+}
+
+int fvm_adam_f32(float *w, const float *g, float *m, float *v, void *wb, int64_t n, float lr, float b1, float b2,
+                 float eps, int32_t step, void *stream) {
+    if (!w || !g || !m || !v || n < 0 || step < 1) {
+        set_error("fvm_adam_f32: invalid arguments");
+        return FVM_ERR_ARG;
+    }
+    if (n == 0) return FVM_OK;
+    const float bc1 = 1.0f - powf(b1, static_cast<float>(step));
+    const float bc2 = 1.0f - powf(b2, static_cast<float>(step));
+    hipLaunchKernelGGL(k_adam, dim3(blocks_for(n, 256)), dim3(256), 0, as_stream(stream), w, g, m, v,
+                       static_cast<__bf16 *>(wb), n, lr, b1, b2, eps, bc1, bc2);
+    return check_launch("fvm_adam_f32");
+}
+
+int fvm_cast_f32_bf16(const float *x, void *y, int64_t n, void *stream) {
+    if (!x || !y || n < 0) {
+        set_error("fvm_cast_f32_bf16: invalid arguments");
+        return FVM_ERR_ARG;
+    }
+    if (n == 0) return FVM_OK;
+    hipLaunchKernelGGL(k_cast_f32_bf16, dim3(blocks_for(n, 256)), dim3(256), 0, as_stream(stream), x,
+                       static_cast<__bf16 *>(y), n);
+    return check_launch("fvm_cast_f32_bf16");
+}
+
+int fvm_mp_features(const float *prim, const float *dudt, const float *win, const float *geo, int32_t G, int64_t N,
+                    int32_t C, const float *mu_p, const float *sd_p, const float *sd_d, void *X, int32_t KX,
+                    void *stream) {
+    if (!prim || !dudt || !win || (G > 0 && !geo) || G < 0 || !mu_p || !sd_p || !sd_d || !X || N < 0 || C < 1 ||
+        2 * C + G > KX) {
+        set_error("fvm_mp_features: invalid arguments");
+        return FVM_ERR_ARG;
+    }
+    if (N == 0) return FVM_OK;
+    hipLaunchKernelGGL(k_mp_features, dim3(blocks_for(N, 256)), dim3(256), 0, as_stream(stream), prim, dudt, win, geo,
+                       G, N, C, mu_p, sd_p, sd_d, static_cast<__bf16 *>(X), KX);
+    return check_launch("fvm_mp_features");
+}
+
+int fvm_mp_apply(const float *U, const float *dudt, const float *win, const float *y, int64_t ld_y,
+                 const float *scale, int64_t N, int32_t C, float *U_out, void *stream) {
+    if (!U || !dudt || !win || !y || !scale || !U_out || N < 0 || C < 1 || ld_y < C) {
+        set_error("fvm_mp_apply: invalid arguments");
+        return FVM_ERR_ARG;
+    }
+    if (N == 0) return FVM_OK;
+    hipLaunchKernelGGL(k_mp_apply, dim3(blocks_for(N * C, 256)), dim3(256), 0, as_stream(stream), U, dudt, win, y,
+                       ld_y, scale, N, C, U_out);
+    return check_launch("fvm_mp_apply");
+}
+
+int fvm_mp_defect(const float *U0, const float *U1, const float *f0, const float *f1, const float *win,
+                  const int64_t *off, int32_t n_meshes, int32_t C, double *out, double *part, void *stream) {
+    if (!U0 || !U1 || !f0 || !f1 || !win || !off || !out || !part || n_meshes < 1 || n_meshes > 65535 || C < 1 ||
+        C > 8) {
+        set_error("fvm_mp_defect: invalid arguments");
+        return FVM_ERR_ARG;
+    }
+    hipStream_t s = as_stream(stream);
+    hipLaunchKernelGGL(k_mp_defect_partial, dim3(64, n_meshes), dim3(256), 0, s, U0, U1, f0, f1, win, off, C, part);
+    hipLaunchKernelGGL(k_mp_defect_final, dim3(blocks_for(static_cast<int64_t>(n_meshes) * 2 * C, 256)), dim3(256), 0,
+                       s, part, 64, n_meshes, C, out);
+    return check_launch("fvm_mp_defect");
+}
+
+}  // extern "C"

@@ -1,29 +1,28 @@
 // Fused right-hand side of the compressible Navier-Stokes FVM step for gfx950.
 //
-// One RHS evaluation (FVMEquation.forward, fvm_equation.py:406-435) is two launches over
-// the cells (plus, only when the boundary means are not already known, a small pre-pass):
-//   k_bc_mean  (pre-pass) per characteristic-BC group, the mean speed of sound of the
-//              boundary cells (the group-wide mean in facet_boundary.py:427), reduced
-//              deterministically: per-block partials in fixed slots, summed in slot
-//              order by the last block to finish
-//   k_grad     per cell: the values of its boundary facets -- A_bc.x + b_bc, the Neumann
-//              non-orthogonal correction with the previous call's limited gradients,
-//              the characteristic BC (facet_boundary.py:124-168, 418-512) -- then the
-//              LSQ gradient G.[U; U_bc] and the slope limiter (facet_process.py:387-418,
-//              limiter.py); stores the unlimited gradient and the limiter factor phi,
-//              from which every consumer rebuilds the limited face values and gradients
-//              with the reference's own operation order
-//   k_flux     per cell: for each of its facets (ascending facet index) the complete
-//              facet flux -- face reconstruction, normal-gradient correction, stress,
-//              advection, pressure, viscosity, heating, KT dissipation
-//              (facet_process.py:332-385, fvm_equation.py:62-368) -- summed into the
-//              divergence (flux_mat row), then an epilogue: write dU/dt, or apply an
-//              explicit Runge-Kutta stage update, emit the next stage's primitives and
-//              the boundary means of those primitives (so the next stage needs no
-//              pre-pass).
-// Interior facet fluxes are computed by both neighbours with identical inputs and
-// identical instruction sequences, so the two contributions cancel exactly and no
-// atomics or facet-flux round trip through HBM are needed.
+// One RHS evaluation (FVMEquation.forward, fvm_equation.py:406-435) is three launches
+// (plus, only when the boundary means are not already known, a small pre-pass):
+//   k_bc_mean    (pre-pass) per characteristic-BC group, partial sums of the speed of
+//                sound of the boundary cells (for the group-wide mean of
+//                facet_boundary.py:427) in fixed per-tile slots; k_grad reduces them in
+//                a fixed order, so the mean is deterministic
+//   k_grad       per cell: the values of its boundary facets -- A_bc.x + b_bc, the
+//                Neumann non-orthogonal correction with the previous call's limited
+//                gradients, the characteristic BC (facet_boundary.py:124-168, 418-512)
+//                -- then the LSQ gradient G.[U; U_bc] and the slope limiter
+//                (facet_process.py:387-418, limiter.py); stores the unlimited gradient
+//                and the limiter factor phi, from which the facet kernel rebuilds the
+//                limited face values and gradients with the reference's operation order
+//   k_facet_flux per facet: face reconstruction, normal-gradient correction, stress,
+//                advection, pressure, viscosity, heating, KT dissipation
+//                (facet_process.py:332-385, fvm_equation.py:62-368) -> one flux record
+//   k_div        per cell: the divergence (flux_mat row) of its facets' fluxes, then an
+//                epilogue: write dU/dt, or apply an explicit Runge-Kutta stage update,
+//                emit the next stage's primitives and the boundary means of those
+//                primitives (so the next stage needs no pre-pass).
+// The facet flux costs ~500 VALU operations; evaluating it once per facet and passing it
+// through HBM (16-20 bytes per facet, written once, read twice) is cheaper than
+// evaluating it from both sides.
 //
 // This file is compiled with -ffp-contract=off: every product and sum is rounded
 // separately, in the order the reference's ATen op sequence performs them, so that the
@@ -92,10 +91,36 @@ __device__ __forceinline__ int64_t xcd_tile(int64_t v, int64_t T) {
     return x * q + (x < r ? x : r) + local;
 }
 
+
+// Per-cell records of C floats; 2-D records (C = 4) are 16-byte aligned (the host
+// wrappers check the base pointers) and move as one dwordx4.
+template <int C>
+__device__ __forceinline__ void load_vec(const float *__restrict__ src, float (&u)[C]) {
+    if constexpr (C == 4) {
+        const float4 q = *reinterpret_cast<const float4 *>(src);
+        u[0] = q.x;
+        u[1] = q.y;
+        u[2] = q.z;
+        u[3] = q.w;
+    } else {
+#pragma unroll
+        for (int j = 0; j < C; ++j) u[j] = src[j];
+    }
+}
+
+template <int C>
+__device__ __forceinline__ void store_vec(float *__restrict__ dst, const float (&u)[C]) {
+    if constexpr (C == 4) {
+        *reinterpret_cast<float4 *>(dst) = make_float4(u[0], u[1], u[2], u[3]);
+    } else {
+#pragma unroll
+        for (int j = 0; j < C; ++j) dst[j] = u[j];
+    }
+}
+
 template <int D>
 __device__ __forceinline__ void load_prim(const float *__restrict__ src, float (&u)[D + 2]) {
-#pragma unroll
-    for (int j = 0; j < D + 2; ++j) u[j] = src[j];
+    load_vec<D + 2>(src, u);
 }
 
 template <int D>
@@ -118,8 +143,8 @@ __device__ __forceinline__ void state_to_prim(const float (&U)[D + 2], float inv
 // gprev of the previous call (facet_boundary.py:124-168, characteristic BC :436-512).
 template <int D>
 __device__ __forceinline__ void bc_value(const FvmMeshDev &m, const FvmPhys &p, int64_t b, const float (&P)[D + 2],
-                                         const float *__restrict__ gprev, int has_prev,
-                                         const float *__restrict__ bc_mean, float (&u)[D + 2]) {
+                                         const float *__restrict__ gprev, int has_prev, const float *gmean,
+                                         int gbase, float (&u)[D + 2]) {
     using Dm = Dims<D>;
     constexpr int C = Dm::C;
     const int flags = m.bc_flags[b];
@@ -142,7 +167,7 @@ __device__ __forceinline__ void bc_value(const FvmMeshDev &m, const FvmPhys &p, 
     }
     const int grp = m.bc_group[b];
     if (grp < 0) return;
-    const float cmean = bc_mean[grp];
+    const float cmean = gmean[grp - gbase];
     const float *ch = m.bc_char + b * Dm::CH;
     float n[D], vt[D];
     float vn = 0.0f;
@@ -187,12 +212,13 @@ __device__ __forceinline__ void bc_value(const FvmMeshDev &m, const FvmPhys &p, 
     u[D + 1] = p_b / (p.R * rho_b);
 }
 
-// Block-level part of the group means of c over the characteristic-BC facets: every
-// thread of the block calls it (uniform control flow).  A slot is a 256-cell tile whose
-// cells own group facets; all its cells belong to one mesh, whose groups are
-// slot_g0 .. slot_g0 + slot_ng - 1.  Thread contributions go through a fixed tree into
-// bc_part[slot][q]; the last slot tile to finish sums, per group, the slots of its mesh
-// in slot order and writes the means.  Identical partials whichever kernel computes them.
+// Block-level partial sums of the speed of sound over the characteristic-BC facets:
+// every thread of the block calls it (uniform control flow).  A slot is a 256-cell tile
+// holding interior cells of group facets; the groups of the meshes its cells belong to
+// are slot_g0 .. slot_g0 + slot_ng - 1 (<= 4).  Thread contributions go through a fixed
+// tree into bc_part[slot][q].  The consumer (k_grad of the next evaluation) sums the
+// partials of a group in a fixed order, so no grid-wide fence or counter is needed and
+// the kernel boundary orders the writes before the reads.
 template <int D>
 __device__ void bc_mean_block(const FvmMeshDev &m, const FvmPhys &p, const FvmWork &w, int64_t c, bool live,
                               float T, int slot) {
@@ -213,7 +239,6 @@ __device__ void bc_mean_block(const FvmMeshDev &m, const FvmPhys &p, const FvmWo
         }
     }
     __shared__ float red[kThreads];
-    __shared__ int is_last;
     for (int q = 0; q < ng; ++q) {
         float v = cs[0];
 #pragma unroll
@@ -227,21 +252,28 @@ __device__ void bc_mean_block(const FvmMeshDev &m, const FvmPhys &p, const FvmWo
         if (threadIdx.x == 0) w.bc_part[slot * kMaxGroups + q] = red[0];
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
-        __threadfence();
-        is_last = atomicAdd(w.bc_counter, 1) == m.n_slots - 1;
-    }
-    __syncthreads();
-    if (is_last) {
-        __threadfence();
-        for (int g = threadIdx.x; g < m.n_groups; g += kThreads) {
-            float sum = 0.0f;
-            for (int i = m.group_s0[g]; i < m.group_s1[g]; ++i)
-                sum += __hip_atomic_load(w.bc_part + i * kMaxGroups + (g - m.slot_g0[i]), __ATOMIC_RELAXED,
-                                         __HIP_MEMORY_SCOPE_AGENT);
-            w.bc_mean[g] = sum / m.group_count[g];
+}
+
+// Group means of the tile's slot (block-uniform call): mean[q] for groups slot_g0 + q,
+// each the sum of its mesh's slot partials in a fixed strided-then-tree order (the same
+// bits in every block) divided by the group's facet count (facet_boundary.py:427).
+__device__ void block_group_means(const FvmMeshDev &m, const FvmWork &w, int slot, float *gmean) {
+    __shared__ float red[kThreads];
+    __syncthreads();                       // the previous tile's readers are done
+    const int g0 = m.slot_g0[slot], ng = m.slot_ng[slot];
+    for (int q = 0; q < ng; ++q) {
+        const int g = g0 + q;
+        float v = 0.0f;
+        for (int i = m.group_s0[g] + threadIdx.x; i < m.group_s1[g]; i += kThreads)
+            v += w.bc_part[i * kMaxGroups + (g - m.slot_g0[i])];
+        red[threadIdx.x] = v;
+        __syncthreads();
+        for (int h = kThreads / 2; h > 0; h >>= 1) {
+            if (threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+            __syncthreads();
         }
-        if (threadIdx.x == 0) atomicExch(w.bc_counter, 0);
+        if (threadIdx.x == 0) gmean[q] = red[0] / m.group_count[g];
+        __syncthreads();
     }
 }
 
@@ -266,9 +298,13 @@ k_grad(FvmMeshDev m, FvmPhys p, FvmWork w, const float *__restrict__ prim, int h
     const int64_t N = m.n_cells;
     const int64_t ntiles = (N + kThreads - 1) / kThreads;
     float *__restrict__ gp = w.gp;
+    __shared__ float gmean[kMaxGroups];
     for (int64_t v = blockIdx.x; v < ntiles; v += gridDim.x) {
     const int64_t tile = xcd_tile(v, ntiles);
     const int64_t c = tile * kThreads + threadIdx.x;
+    const int slot = m.n_groups > 0 ? m.blk_slot[tile] : -1;
+    if (slot >= 0) block_group_means(m, w, slot, gmean);
+    const int g0 = slot >= 0 ? m.slot_g0[slot] : 0;
     if (c >= N) continue;
     float P[C];
     load_prim<D>(prim + c * C, P);
@@ -282,7 +318,7 @@ k_grad(FvmMeshDev m, FvmPhys p, FvmWork w, const float *__restrict__ prim, int h
             // this cell is the boundary facet's only cell: it computes the facet's values
             // (reading its own previous gradient record before overwriting it below)
             const int64_t b = nb - N;
-            bc_value<D>(m, p, b, P, gp + c * Dm::REC, has_prev, w.bc_mean, Un[k]);
+            bc_value<D>(m, p, b, P, gp + c * Dm::REC, has_prev, gmean, g0, Un[k]);
 #pragma unroll
             for (int j = 0; j < C; ++j) w.ubc[b * C + j] = Un[k][j];
         }
@@ -488,13 +524,100 @@ __device__ __forceinline__ void side_values(const float (&P)[D + 2], const float
     }
 }
 
+// Per facet (device facet order): both sides' limited face values and gradients, the
+// two-point normal gradient, and the complete facet flux -> fflux[f][C].  Each facet is
+// evaluated once; the divergence kernel adds it to one cell and subtracts it from the
+// other, so the two contributions cancel exactly.
+template <int D>
+__global__ void __launch_bounds__(kThreads)
+k_facet_flux(FvmMeshDev m, FvmPhys p, FvmWork w, const float *__restrict__ prim) {
+    const float *__restrict__ gp = w.gp;
+    using Dm = Dims<D>;
+    constexpr int C = Dm::C;
+    const int64_t N = m.n_cells, NF = m.n_facets;
+    const int64_t ntiles = (NF + kThreads - 1) / kThreads;
+    for (int64_t v = blockIdx.x; v < ntiles; v += gridDim.x) {
+        const int64_t f = xcd_tile(v, ntiles) * kThreads + threadIdx.x;
+        if (f >= NF) continue;
+        const int64_t c0 = m.fac_cell[f], c1 = m.fac_cell[NF + f];
+        const float *geo = m.fac_geo + f * Dm::GEO;
+        const bool outer0 = c0 >= N;                 // slot 0 is a boundary facet's outer side
+        const int64_t own = outer0 ? c1 : c0;
+        float mu0 = p.mu0, mub0 = p.mub0, kappa = p.kappa;
+        if (m.mesh_phys) {
+            const int mesh = m.cell_mesh[own];
+            mu0 = m.mesh_phys[4 * mesh];
+            mub0 = m.mesh_phys[4 * mesh + 1];
+            kappa = m.mesh_phys[4 * mesh + 2];
+        }
+        float d0[D], d1[D];
+#pragma unroll
+        for (int i = 0; i < D; ++i) {
+            d0[i] = geo[3 * D + 2 + i];
+            d1[i] = geo[4 * D + 2 + i];
+        }
+        float U0s[C], U1s[C], g0s[D][D + 1], g1s[D][D + 1], gdn[D + 1];
+        if (c0 < N && c1 < N) {
+            float P0[C], P1[C];
+            load_prim<D>(prim + c0 * C, P0);
+            load_prim<D>(prim + c1 * C, P1);
+            side_values<D>(P0, gp + c0 * Dm::REC, d0, U0s, g0s);
+            side_values<D>(P1, gp + c1 * Dm::REC, d1, U1s, g1s);
+            // A_face_grad row: (-1/d, +1/d) on (slot-0, slot-1) cells, ascending columns
+            const float inv_d = geo[2 * D + 1];
+            const bool left_first = c0 < c1;
+#pragma unroll
+            for (int jg = 0; jg <= D; ++jg) {
+                const int j = jg < D ? jg : D + 1;
+                const float tl = (-inv_d) * P0[j], tr = inv_d * P1[j];
+                gdn[jg] = (left_first ? tl + tr : tr + tl) + 0.0f;
+            }
+        } else {
+            // boundary facet: the outer side takes the boundary values and the interior
+            // cell's limited gradient; the normal gradient is A_face_grad.x + b_face_grad
+            const int64_t b = (outer0 ? c0 : c1) - N;
+            float Pc[C], Uo[C], glo[D][D + 1], Ux[C];
+            load_prim<D>(prim + own * C, Pc);
+            side_values<D>(Pc, gp + own * Dm::REC, outer0 ? d1 : d0, Uo, glo);
+            load_prim<D>(w.ubc + b * C, Ux);
+#pragma unroll
+            for (int j = 0; j < C; ++j) {
+                U0s[j] = outer0 ? Ux[j] : Uo[j];
+                U1s[j] = outer0 ? Uo[j] : Ux[j];
+            }
+#pragma unroll
+            for (int i = 0; i < D; ++i)
+#pragma unroll
+                for (int j = 0; j <= D; ++j) {
+                    g0s[i][j] = glo[i][j];
+                    g1s[i][j] = glo[i][j];
+                }
+#pragma unroll
+            for (int jg = 0; jg <= D; ++jg) {
+                const int j = jg < D ? jg : D + 1;
+                gdn[jg] = m.bc_ga[b * C + j] * Pc[j] + m.bc_gb[b * C + j];
+            }
+        }
+        float Fx[C];
+        facet_flux<D>(p, mu0, mub0, kappa, U0s, U1s, g0s, g1s, gdn, geo, Fx);
+        float *out = w.fflux + f * C;
+        if constexpr (C == 4) {
+            *reinterpret_cast<float4 *>(out) = make_float4(Fx[0], Fx[1], Fx[2], Fx[3]);
+        } else {
+#pragma unroll
+            for (int j = 0; j < C; ++j) out[j] = Fx[j];
+        }
+    }
+}
+
 enum { EPI_DUDT = 0, EPI_STAGE = 1 };
 
+// Per cell: the divergence (flux_mat row: +-1/vol on its facets in ascending reference
+// facet order) of the facet fluxes, then the epilogue -- dU/dt, or an explicit stage
+// update with the next stage's primitives and their boundary means.
 template <int D, int EPI>
 __global__ void __launch_bounds__(kThreads)
-k_flux(FvmMeshDev m, FvmPhys p, FvmWork w, const float *__restrict__ prim, float *__restrict__ dudt, FvmStage st) {
-    const float *__restrict__ gp = w.gp;
-    const float *__restrict__ ubc = w.ubc;
+k_div(FvmMeshDev m, FvmPhys p, FvmWork w, float *__restrict__ dudt, FvmStage st) {
     double *__restrict__ red = w.red;
     using Dm = Dims<D>;
     constexpr int C = Dm::C, NL = Dm::NL;
@@ -505,99 +628,42 @@ k_flux(FvmMeshDev m, FvmPhys p, FvmWork w, const float *__restrict__ prim, float
     const int64_t c = tile * kThreads + threadIdx.x;
     const bool live = c < N;
     const int mesh = (m.cell_mesh && live) ? m.cell_mesh[c] : 0;
-    float mu0 = p.mu0, mub0 = p.mub0, kappa = p.kappa;
-    if (m.mesh_phys) {
-        mu0 = m.mesh_phys[4 * mesh];
-        mub0 = m.mesh_phys[4 * mesh + 1];
-        kappa = m.mesh_phys[4 * mesh + 2];
-    }
     float acc[C];
 #pragma unroll
     for (int j = 0; j < C; ++j) acc[j] = 0.0f;
     if (live) {
-        float Pc[C];
-        load_prim<D>(prim + c * C, Pc);
-        const float *gc = gp + c * Dm::REC;
         const int info = m.cell_info[c];
         const float inv_vol = m.cell_inv_vol[c];
-        // not unrolled: one facet's flux at a time keeps the register footprint (and so
-        // the waves in flight per SIMD) at what a single facet needs
-#pragma unroll 1
+        float Fk[NL][C];
+#pragma unroll
         for (int k = 0; k < NL; ++k) {
             const int64_t f = m.cell_facet[k * N + c];
-            const int64_t nb = m.cell_nbr[k * N + c];
-            const int own_slot = (info >> k) & 1;
-            const float *geo = m.fac_geo + f * Dm::GEO;
-            // centroid -> midpoint displacements of both sides come with the facet record
-            float dc[D], dn[D];
-#pragma unroll
-            for (int i = 0; i < D; ++i) {
-                const float d0 = geo[3 * D + 2 + i], d1 = geo[4 * D + 2 + i];
-                dc[i] = own_slot ? d1 : d0;
-                dn[i] = own_slot ? d0 : d1;
-            }
-            float Uo[C], glo[D][D + 1];
-            side_values<D>(Pc, gc, dc, Uo, glo);
-            float Ux[C], glx[D][D + 1], gdn[D + 1];
-            if (nb < N) {
-                float Pn[C];
-                load_prim<D>(prim + nb * C, Pn);
-                side_values<D>(Pn, gp + nb * Dm::REC, dn, Ux, glx);
-                // A_face_grad row: (-1/d, +1/d) on (slot-0, slot-1) cells, ascending columns
-                const float inv_d = geo[2 * D + 1];
-                const bool own_left = own_slot == 0;
-                const bool left_first = own_left ? (c < nb) : (nb < c);
-#pragma unroll
-                for (int jg = 0; jg <= D; ++jg) {
-                    const int j = jg < D ? jg : D + 1;
-                    const float pl = own_left ? Pc[j] : Pn[j];
-                    const float pr = own_left ? Pn[j] : Pc[j];
-                    const float tl = (-inv_d) * pl, tr = inv_d * pr;
-                    gdn[jg] = (left_first ? tl + tr : tr + tl) + 0.0f;
-                }
+            const float *src = w.fflux + f * C;
+            if constexpr (C == 4) {
+                const float4 q = *reinterpret_cast<const float4 *>(src);
+                Fk[k][0] = q.x;
+                Fk[k][1] = q.y;
+                Fk[k][2] = q.z;
+                Fk[k][3] = q.w;
             } else {
-                const int64_t b = nb - N;
-                load_prim<D>(ubc + b * C, Ux);
 #pragma unroll
-                for (int i = 0; i < D; ++i)
-#pragma unroll
-                    for (int j = 0; j <= D; ++j) glx[i][j] = glo[i][j];
-#pragma unroll
-                for (int jg = 0; jg <= D; ++jg) {
-                    const int j = jg < D ? jg : D + 1;
-                    gdn[jg] = m.bc_ga[b * C + j] * Pc[j] + m.bc_gb[b * C + j];
-                }
+                for (int j = 0; j < C; ++j) Fk[k][j] = src[j];
             }
-            // canonical slot order by selects, not branches: lanes of a wavefront
-            // disagree on the slot, and a branch would execute the flux twice
-            const bool sw = own_slot != 0;
-            float U0s[C], U1s[C], g0s[D][D + 1], g1s[D][D + 1];
+        }
 #pragma unroll
-            for (int j = 0; j < C; ++j) {
-                U0s[j] = sw ? Ux[j] : Uo[j];
-                U1s[j] = sw ? Uo[j] : Ux[j];
-            }
+        for (int k = 0; k < NL; ++k) {
+            const float vk = ((info >> k) & 1) ? inv_vol : -inv_vol;
 #pragma unroll
-            for (int i = 0; i < D; ++i)
-#pragma unroll
-                for (int j = 0; j <= D; ++j) {
-                    g0s[i][j] = sw ? glx[i][j] : glo[i][j];
-                    g1s[i][j] = sw ? glo[i][j] : glx[i][j];
-                }
-            float F[C];
-            facet_flux<D>(p, mu0, mub0, kappa, U0s, U1s, g0s, g1s, gdn, geo, F);
-            const float v = own_slot ? inv_vol : -inv_vol;
-#pragma unroll
-            for (int j = 0; j < C; ++j) acc[j] = acc[j] + v * F[j];
+            for (int j = 0; j < C; ++j) acc[j] = acc[j] + vk * Fk[k][j];
         }
     }
     if (EPI == EPI_DUDT) {
-        if (live)
-#pragma unroll
-            for (int j = 0; j < C; ++j) dudt[c * C + j] = acc[j];
+        if (live) store_vec<C>(dudt + c * C, acc);
         continue;
     }
     // ---------------- explicit RK stage epilogue ----------------
+    // Stage flags are uniform; every per-cell load is issued before the first use so
+    // that a cell's memory round trips overlap.
     double sq[3 * C];
 #pragma unroll
     for (int j = 0; j < 3 * C; ++j) sq[j] = 0.0;
@@ -606,71 +672,83 @@ k_flux(FvmMeshDev m, FvmPhys p, FvmWork w, const float *__restrict__ prim, float
     for (int j = 0; j < C; ++j) Pn[j] = 0.0f;
     if (live && st.mode == 1) {
         // Adams-Bashforth 3 (fixed dt, one RHS per step)
-        const float c12 = st.dt[mesh] / 12.0f;
         const int64_t o = c * C;
+        float U0v[C], fm1[C], fm2[C];
+        load_vec<C>(st.U0 + o, U0v);
+        if (st.n_hist >= 1) load_vec<C>(st.f_m1 + o, fm1);
+        if (st.n_hist >= 2) load_vec<C>(st.f_m2 + o, fm2);
+        const float c12 = st.dt[mesh] / 12.0f;
         float U1[C];
 #pragma unroll
         for (int j = 0; j < C; ++j) {
             const float f0 = acc[j];
-            const float fm1 = st.n_hist >= 1 ? st.f_m1[o + j] : f0;
-            const float fm2 = st.n_hist >= 2 ? st.f_m2[o + j] : fm1;
-            const float comb = (23.0f * f0 - 16.0f * fm1) + 5.0f * fm2;
-            U1[j] = st.U0[o + j] + c12 * comb;
-            st.f_out[o + j] = f0;
-            st.U_out[o + j] = U1[j];
+            const float g1 = st.n_hist >= 1 ? fm1[j] : f0;
+            const float g2 = st.n_hist >= 2 ? fm2[j] : g1;
+            const float comb = (23.0f * f0 - 16.0f * g1) + 5.0f * g2;
+            U1[j] = U0v[j] + c12 * comb;
         }
+        store_vec<C>(st.f_out + o, acc);
+        store_vec<C>(st.U_out + o, U1);
         state_to_prim<D>(U1, p.inv_cv, Pn);
-#pragma unroll
-        for (int j = 0; j < C; ++j) st.prim_next[o + j] = Pn[j];
+        store_vec<C>(st.prim_next + o, Pn);
     } else if (live) {
-        const float dt = st.dt[mesh];
         const int64_t o = c * C;
-        float kk[C], a[C], U0[C];
+        const bool first = st.stage == 0;
+        const bool has2 = st.acc2 != nullptr;
+        float U0v[C], ap[C], a2p[C];
+        load_vec<C>(st.U0 + o, U0v);
+        if (!first) load_vec<C>(st.acc + o, ap);
+        if (!first && has2) load_vec<C>(st.acc2 + o, a2p);
+        const float dt = st.dt[mesh];
+        float kk[C], a[C], a2[C];
 #pragma unroll
         for (int j = 0; j < C; ++j) {
             kk[j] = dt * acc[j];
-            U0[j] = st.U0[o + j];
-            a[j] = st.stage == 0 ? st.b * kk[j] : st.acc[o + j] + st.b * kk[j];
+            a[j] = first ? st.b * kk[j] : ap[j] + st.b * kk[j];
+            a2[j] = first ? st.b2 * kk[j] : a2p[j] + st.b2 * kk[j];
         }
-        if (st.store_k)
-#pragma unroll
-            for (int j = 0; j < C; ++j) st.kbuf[static_cast<int64_t>(st.stage) * N * C + o + j] = kk[j];
-        float a2[C];
-        if (st.acc2) {
-#pragma unroll
-            for (int j = 0; j < C; ++j) a2[j] = st.stage == 0 ? st.b2 * kk[j] : st.acc2[o + j] + st.b2 * kk[j];
-        }
+        if (st.store_k) store_vec<C>(st.kbuf + static_cast<int64_t>(st.stage) * N * C + o, kk);
         float Un[C];
         if (!st.last) {
+            float inc[C];
 #pragma unroll
-            for (int j = 0; j < C; ++j) {
-                float inc = 0.0f;
-                for (int t = 0; t < st.n_inc; ++t) {
-                    const int sidx = st.inc_idx[t];
-                    const float kv = sidx == st.stage ? kk[j] : st.kbuf[static_cast<int64_t>(sidx) * N * C + o + j];
-                    const float term = st.inc_coef[t] * kv;
-                    inc = t == 0 ? term : inc + term;
+            for (int j = 0; j < C; ++j) inc[j] = 0.0f;
+            for (int t = 0; t < st.n_inc; ++t) {
+                const int sidx = st.inc_idx[t];
+                const float cf = st.inc_coef[t];
+                float kv[C];
+                if (sidx == st.stage) {
+#pragma unroll
+                    for (int j = 0; j < C; ++j) kv[j] = kk[j];
+                } else {
+                    load_vec<C>(st.kbuf + static_cast<int64_t>(sidx) * N * C + o, kv);
                 }
-                Un[j] = U0[j] + inc;
-                st.acc[o + j] = a[j];
-                if (st.acc2) st.acc2[o + j] = a2[j];
+#pragma unroll
+                for (int j = 0; j < C; ++j) {
+                    const float term = cf * kv[j];
+                    inc[j] = t == 0 ? term : inc[j] + term;
+                }
             }
+#pragma unroll
+            for (int j = 0; j < C; ++j) Un[j] = U0v[j] + inc[j];
+            store_vec<C>(st.acc + o, a);
+            if (has2) store_vec<C>(st.acc2 + o, a2);
         } else {
 #pragma unroll
-            for (int j = 0; j < C; ++j) {
-                Un[j] = U0[j] + a[j];
-                st.U_out[o + j] = Un[j];
-                if (st.acc2) {
+            for (int j = 0; j < C; ++j) Un[j] = U0v[j] + a[j];
+            store_vec<C>(st.U_out + o, Un);
+            if (has2) {
+#pragma unroll
+                for (int j = 0; j < C; ++j) {
                     const double dd = static_cast<double>(a[j] - a2[j]);
                     sq[j] = dd * dd;
                     sq[C + j] = static_cast<double>(a[j]) * a[j];
-                    sq[2 * C + j] = static_cast<double>(U0[j]) * U0[j];
+                    sq[2 * C + j] = static_cast<double>(U0v[j]) * U0v[j];
                 }
             }
         }
         state_to_prim<D>(Un, p.inv_cv, Pn);
-#pragma unroll
-        for (int j = 0; j < C; ++j) st.prim_next[o + j] = Pn[j];
+        store_vec<C>(st.prim_next + o, Pn);
     }
     // boundary means of the next stage's primitives (block-uniform branch)
     if (m.n_groups > 0) {
@@ -745,7 +823,8 @@ __global__ void __launch_bounds__(kThreads) k_state_to_prim3(const float *__rest
 static int check_mesh(const FvmMeshDev *m, const FvmPhys *p, const FvmWork *w, const char *who) {
     if (!m || !p || !w || (m->dim != 2 && m->dim != 3) || m->n_cells <= 0 || m->n_groups < 0 ||
         m->n_groups > (1 << 16) || m->n_bc < 0 || !w->gp || (m->n_bc > 0 && !w->ubc) ||
-        (m->n_groups > 0 && (!w->bc_part || !w->bc_mean || !w->bc_counter || m->n_slots < 1 || !m->blk_slot ||
+        !m->fac_cell || !m->fac_geo || !w->fflux || m->n_facets < 1 || m->n_facets >= (int64_t(1) << 31) ||
+        (m->n_groups > 0 && (!w->bc_part || m->n_slots < 1 || !m->blk_slot ||
                              !m->slot_blk || !m->slot_g0 || !m->slot_ng || !m->group_s0 || !m->group_s1 ||
                              !m->group_count)) ||
         (m->mesh_phys && !m->cell_mesh) || m->n_meshes < 1 ||
@@ -756,8 +835,9 @@ static int check_mesh(const FvmMeshDev *m, const FvmPhys *p, const FvmWork *w, c
     return FVM_OK;
 }
 
-// parts: 1 = boundary-mean pre-pass, 4 = gradient + limiter (+ boundary values),
-// 8 = flux + divergence (+ stage update).  The pre-pass runs only when `need_mean`.
+// parts: 1 = boundary-mean pre-pass, 2 = gradient + limiter (+ boundary values),
+// 4 = facet fluxes, 8 = divergence (+ stage update).  The pre-pass runs only when
+// `need_mean`.
 template <int D>
 static int launch_rhs(const FvmMeshDev &m, const FvmPhys &p, const FvmWork &w, const float *prim, float *dudt,
                       const FvmStage *st, hipStream_t s, int has_prev, int need_mean, int parts = 15) {
@@ -766,7 +846,7 @@ static int launch_rhs(const FvmMeshDev &m, const FvmPhys &p, const FvmWork &w, c
     const unsigned int ntiles = blocks_for(m.n_cells, kThreads);
     const dim3 grid(m.persist_blocks > 0 && static_cast<unsigned int>(m.persist_blocks) < ntiles
                         ? static_cast<unsigned int>(m.persist_blocks) : ntiles);
-    if (parts & 4) {
+    if (parts & 2) {
         switch (p.lim_p) {
             case 1: hipLaunchKernelGGL((k_grad<D, 1>), grid, dim3(kThreads), 0, s, m, p, w, prim, has_prev); break;
             case 2: hipLaunchKernelGGL((k_grad<D, 2>), grid, dim3(kThreads), 0, s, m, p, w, prim, has_prev); break;
@@ -775,15 +855,39 @@ static int launch_rhs(const FvmMeshDev &m, const FvmPhys &p, const FvmWork &w, c
             default: hipLaunchKernelGGL((k_grad<D, 4>), grid, dim3(kThreads), 0, s, m, p, w, prim, has_prev); break;
         }
     }
+    if (parts & 4) {
+        const unsigned int ftiles = blocks_for(m.n_facets, kThreads);
+        const dim3 fgrid(m.persist_blocks > 0 && static_cast<unsigned int>(m.persist_blocks) < ftiles
+                             ? static_cast<unsigned int>(m.persist_blocks) : ftiles);
+        hipLaunchKernelGGL(k_facet_flux<D>, fgrid, dim3(kThreads), 0, s, m, p, w, prim);
+    }
     if (parts & 8) {
         if (st) {
-            hipLaunchKernelGGL((k_flux<D, EPI_STAGE>), grid, dim3(kThreads), 0, s, m, p, w, prim, nullptr, *st);
+            hipLaunchKernelGGL((k_div<D, EPI_STAGE>), grid, dim3(kThreads), 0, s, m, p, w, nullptr, *st);
         } else {
             FvmStage none{};
-            hipLaunchKernelGGL((k_flux<D, EPI_DUDT>), grid, dim3(kThreads), 0, s, m, p, w, prim, dudt, none);
+            hipLaunchKernelGGL((k_div<D, EPI_DUDT>), grid, dim3(kThreads), 0, s, m, p, w, dudt, none);
         }
     }
     return check_launch(st ? "fvm_rk_stage_f32" : "fvm_rhs_f32");
+}
+
+// 2-D per-cell records move as 16-byte vectors: their base pointers must be aligned.
+static bool aligned16(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
+
+static int check_align(const FvmMeshDev *m, const FvmWork *w, const FvmStage *st, const float *dudt,
+                       const float *prim, const char *who) {
+    if (m->dim != 2) return FVM_OK;
+    bool ok = aligned16(prim) && aligned16(w->ubc) && aligned16(w->fflux) && aligned16(dudt);
+    if (st)
+        ok = ok && aligned16(st->U0) && aligned16(st->acc) && aligned16(st->acc2) && aligned16(st->kbuf) &&
+             aligned16(st->prim_next) && aligned16(st->U_out) && aligned16(st->f_m1) && aligned16(st->f_m2) &&
+             aligned16(st->f_out);
+    if (!ok) {
+        set_error("%s: 2-D state/primitive/flux buffers must be 16-byte aligned", who);
+        return FVM_ERR_ARG;
+    }
+    return FVM_OK;
 }
 
 static int check_stage(const FvmWork *w, const FvmStage *st, const char *who) {
@@ -822,6 +926,8 @@ int fvm_rhs_f32(const FvmMeshDev *m, const FvmPhys *p, const FvmWork *w, const f
         set_error("fvm_rhs_f32: null prim/dudt");
         return FVM_ERR_ARG;
     }
+    e = check_align(m, w, nullptr, dudt, prim, "fvm_rhs_f32");
+    if (e) return e;
     return m->dim == 2 ? launch_rhs<2>(*m, *p, *w, prim, dudt, nullptr, as_stream(stream), has_prev, 1)
                        : launch_rhs<3>(*m, *p, *w, prim, dudt, nullptr, as_stream(stream), has_prev, 1);
 }
@@ -829,6 +935,7 @@ int fvm_rhs_f32(const FvmMeshDev *m, const FvmPhys *p, const FvmWork *w, const f
 int fvm_rk_stage_f32(const FvmMeshDev *m, const FvmPhys *p, const FvmWork *w, const FvmStage *st, void *stream) {
     int e = check_mesh(m, p, w, "fvm_rk_stage_f32");
     if (!e) e = check_stage(w, st, "fvm_rk_stage_f32");
+    if (!e) e = check_align(m, w, st, nullptr, st->prim, "fvm_rk_stage_f32");
     if (e) return e;
     return m->dim == 2 ? launch_rhs<2>(*m, *p, *w, st->prim, nullptr, st, as_stream(stream), st->has_prev, !st->bc_ready)
                        : launch_rhs<3>(*m, *p, *w, st->prim, nullptr, st, as_stream(stream), st->has_prev, !st->bc_ready);
@@ -837,11 +944,12 @@ int fvm_rk_stage_f32(const FvmMeshDev *m, const FvmPhys *p, const FvmWork *w, co
 int fvm_rk_stage_part_f32(const FvmMeshDev *m, const FvmPhys *p, const FvmWork *w, const FvmStage *st,
                           int32_t parts, void *stream) {
     if (parts < 1 || parts > 15) {
-        set_error("fvm_rk_stage_part_f32: parts must be a nonzero mask of 1|4|8");
+        set_error("fvm_rk_stage_part_f32: parts must be a nonzero mask of 1|2|4|8");
         return FVM_ERR_ARG;
     }
     int e = check_mesh(m, p, w, "fvm_rk_stage_part_f32");
     if (!e) e = check_stage(w, st, "fvm_rk_stage_part_f32");
+    if (!e) e = check_align(m, w, st, nullptr, st->prim, "fvm_rk_stage_part_f32");
     if (e) return e;
     return m->dim == 2
                ? launch_rhs<2>(*m, *p, *w, st->prim, nullptr, st, as_stream(stream), st->has_prev, !st->bc_ready, parts)
@@ -860,8 +968,8 @@ int fvm_update_stepsize(const FvmWork *w, int32_t C, int32_t nblk, float atol, f
 }
 
 int fvm_state_to_prim_f32(const float *state, float *prim, int64_t N, int32_t dim, float inv_cv, void *stream) {
-    if (!state || !prim || N < 0 || (dim != 2 && dim != 3)) {
-        set_error("fvm_state_to_prim_f32: invalid arguments");
+    if (!state || !prim || N < 0 || (dim != 2 && dim != 3) || (dim == 2 && !(aligned16(state) && aligned16(prim)))) {
+        set_error("fvm_state_to_prim_f32: invalid arguments (2-D buffers must be 16-byte aligned)");
         return FVM_ERR_ARG;
     }
     if (N == 0) return FVM_OK;

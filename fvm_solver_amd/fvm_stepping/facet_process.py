@@ -284,6 +284,21 @@ class DeviceMesh:
         h = self.host
         self.mesh_phys = np.array([[*l["phys"], 0.0] for l in L], np.float32)
 
+        # the two sides of every facet, and the device facet order: by lowest adjacent
+        # cell, so a run of facets touches a run of cells (cells are space-filling-curve
+        # ordered) and each cell finds its facets' fluxes close together
+        slot_s = h["slot_s"]
+        fac_cell = np.full((2, F), -1, np.int64)
+        fac_cell[slot_s, c2f_s] = np.broadcast_to(np.arange(N)[:, None], c2f_s.shape)
+        outer = nbr_s >= N
+        fac_cell[1 - slot_s[outer], c2f_s[outer]] = nbr_s[outer]
+        if (fac_cell < 0).any():
+            raise ValueError("a facet is missing one of its two sides")
+        fperm = np.argsort(fac_cell.min(0), kind="stable")
+        fpos = np.empty(F, np.int64)
+        fpos[fperm] = np.arange(F)
+        h["fac_cell"], h["fperm"], h["fpos"] = fac_cell, fperm, fpos
+
         dev = torch.device(device)
 
         def up(a, dt):
@@ -294,9 +309,10 @@ class DeviceMesh:
 
         i32, f32 = torch.int32, torch.float32
         self.t = dict(
-            cell_facet=up(c2f_s.T, i32), cell_nbr=up(nbr_s.T, i32), cell_info=up(h["info"], i32),
+            cell_facet=up(fpos[c2f_s].T, i32), cell_nbr=up(nbr_s.T, i32), cell_info=up(h["info"], i32),
             cell_disp=up(h["disp_s"].transpose(1, 2, 0), f32), cell_gval=up(h["gval"].transpose(2, 1, 0), f32),
-            cell_inv_vol=up(h["inv_vol"], f32), cell_eps=up(h["eps"], f32), fac_geo=up(h["geo"], f32),
+            cell_inv_vol=up(h["inv_vol"], f32), cell_eps=up(h["eps"], f32), fac_geo=up(h["geo"][fperm], f32),
+            fac_cell=up(fac_cell[:, fperm], i32),
             bc_cell=up_nz(bcell, i32), bc_group=up_nz(group, i32), bc_flags=up_nz(h["flags"], i32),
             bc_val=up_nz(h["bc_val"], f32), bc_l=up_nz(h["bc_l"], f32), bc_ga=up_nz(h["bc_ga"], f32),
             bc_gb=up_nz(h["bc_gb"], f32), bc_char=up_nz(h["bc_char"], f32),
@@ -375,14 +391,12 @@ class FacetCalc:
         self.ubc = torch.zeros((max(Fb, 1), C), dtype=torch.float32, device=dev)
         self.gp = torch.zeros((N, (D + 1) * C), dtype=torch.float32, device=dev)
         self.bc_part = torch.zeros(4 * max(self.mesh.n_slots, 1), dtype=torch.float32, device=dev)
-        self.bc_mean = torch.zeros(max(self.mesh.n_groups, 1), dtype=torch.float32, device=dev)
-        self.bc_counter = torch.zeros(1, dtype=torch.int32, device=dev)
         self.red = torch.zeros(self.nblk * 3 * C, dtype=torch.float64, device=dev)
+        self.fflux = torch.zeros((self.n_facets, C), dtype=torch.float32, device=dev)
         self.work = nat.FvmWork(ubc=self.ubc.data_ptr(), gp=self.gp.data_ptr(), bc_part=self.bc_part.data_ptr(),
-                                bc_mean=self.bc_mean.data_ptr(), bc_counter=self.bc_counter.data_ptr(),
-                                red=self.red.data_ptr())
+                                red=self.red.data_ptr(), fflux=self.fflux.data_ptr())
         self.has_prev = False
-        self.bc_ready_ptr = None     # prim buffer whose boundary means bc_mean holds
+        self.bc_ready_ptr = None     # prim buffer whose boundary sums bc_part holds
 
     # The reference keeps the previous call's limited gradients in ``cell_grads``.
     @property
@@ -422,7 +436,8 @@ class FacetCalc:
             rc = nat.lib().fvm_rk_stage_part_f32(self.mesh.struct, self.phys, self.work, st, parts,
                                                  nat.stream_ptr(self.device))
         nat.check(rc, "fvm_rk_stage_f32")
-        self.has_prev = True
+        if parts is None or parts & 2:
+            self.has_prev = True
         if parts is None or parts & 8:
             self.bc_ready_ptr = st.prim_next
 

@@ -1,0 +1,211 @@
+"""Learned window predictor with the plain Runge-Kutta solver as fallback and judge.
+
+No reference counterpart.  Over a window of k solver steps of a (batched) channel
+problem the corrector predicts
+
+    U(t + k dt) = U + W f(U) + s * MPNN(features)         W = k dt of the cell's mesh
+
+i.e. an explicit Euler step over the whole window plus a learned, per-component scaled
+correction.  The network's inputs per cell are the normalised primitives, the
+normalised Euler increment W f(U) and two geometry features (log cell size, share of
+boundary facets); its graph is the mesh's face adjacency.
+
+Every predicted window is checked before it is accepted: the trapezoid defect
+
+    d = U1 - U - W/2 (f(U) + f(U1))
+
+costs one more RHS evaluation, and a window is accepted only if, on every mesh of the
+batch and for every conserved variable j, ||d_j|| <= tol ||U1_j||.  Otherwise the window
+is discarded and re-run with the plain fp32 RK4 solver.
+
+Training data are the batched solver's own trajectories: (U_t, U_{t+k}) pairs of every
+mesh of the batch, stored as the network's inputs and the normalised residual of the
+Euler window.
+"""
+from __future__ import annotations
+
+import math
+import time
+
+import numpy as np
+import torch
+
+from fvm_solver_amd import _native as nat
+from fvm_solver_amd.fvm_stepping.facet_process import FacetCalc
+from fvm_solver_amd.learned.mpnn import MPNN
+
+_BF = torch.bfloat16
+
+
+class LearnedCorrector:
+    """Corrector for one BatchedSolver (whose meshes, physics and dt it uses)."""
+
+    KX = 16
+
+    def __init__(self, solver, k: int = 8, hidden: int = 128, layers: int = 3, seed: int = 0, tol: float = 1e-3):
+        self.s = solver
+        self.k = k
+        self.tol = tol
+        dm = solver.dm
+        self.device = solver.device
+        self.C = solver.C
+        self.N = dm.N
+        # a private RHS evaluator on the same device mesh: the solver's own carried
+        # gradient state is not touched by the corrector's evaluations
+        self.E = FacetCalc.from_device_mesh(dm, solver.E.cfg, self.device)
+        self.nbr = dm.t["cell_nbr"]                                   # int32 [NL, N]
+        self.cell_mesh = torch.from_numpy(dm.host["cell_mesh"].astype(np.int64)).to(self.device)
+        self.off = torch.from_numpy(np.asarray(dm.cell_offsets, np.int64)).to(self.device)
+        nb = dm.host["nbr_s"]
+        bfrac = (nb >= self.N).sum(1) / nb.shape[1]
+        logh = np.log(dm.h64.cpu().numpy())
+        self.geo_raw = np.stack([logh, bfrac], 1)
+        self.net = MPNN(self.KX, self.C, hidden, layers, device=self.device, seed=seed)
+        self.stats = None
+        z = lambda *sh: torch.empty(*sh, device=self.device)
+        self._prim = z(self.N, self.C)
+        self._prim1 = z(self.N, self.C)
+        self._f0 = z(self.N, self.C)
+        self._f1 = z(self.N, self.C)
+        self._U1 = z(self.N, self.C)
+        self._X = torch.empty(self.N, self.KX, dtype=_BF, device=self.device)
+        self._def = torch.empty(solver.n_meshes, 2 * self.C, dtype=torch.float64, device=self.device)
+        self._def_part = torch.empty(solver.n_meshes * 64 * 2 * self.C, dtype=torch.float64, device=self.device)
+        self.n_accept = 0
+        self.n_reject = 0
+
+    # ------------------------------------------------------------------ pieces
+    def window(self) -> torch.Tensor:
+        return self.k * self.s.dt[self.cell_mesh]
+
+    def _state_to_prim(self, U, out):
+        nat.check(nat.lib().fvm_state_to_prim_f32(nat.ptr(U), nat.ptr(out), self.N, self.C - 2, self.s.inv_cv,
+                                                  nat.stream_ptr(self.device)), "fvm_state_to_prim_f32")
+
+    def _rhs(self, U, prim, f):
+        self._state_to_prim(U, prim)
+        self.E.rhs(prim, out=f)
+
+    def set_stats(self, U):
+        """Feature normalisation from a representative state."""
+        self._rhs(U, self._prim, self._f0)
+        W = self.window()[:, None]
+        inc = (W * self._f0).double()
+        p = self._prim.double()
+        g = torch.from_numpy(self.geo_raw)
+        self.stats = {
+            "mu_p": p.mean(0).float(), "sd_p": (p.std(0) + 1e-6 * p.abs().mean(0) + 1e-12).float(),
+            "sd_d": (inc.std(0) + 1e-12).float(),
+            "mu_g": g.mean(0).float(), "sd_g": (g.std(0) + 1e-6).float(),
+            "scale": torch.ones(self.C),
+        }
+        self._apply_stats()
+
+    def _apply_stats(self):
+        st = self.stats
+        self._mu_p = st["mu_p"].to(self.device)
+        self._sd_p = st["sd_p"].to(self.device)
+        self._sd_d = st["sd_d"].to(self.device)
+        self._scale = st["scale"].to(self.device)
+        g = (torch.from_numpy(self.geo_raw).float() - st["mu_g"]) / st["sd_g"]
+        self._geo = g.contiguous().to(self.device)
+
+    def features(self, U):
+        """prim, f(U) and the network input of state U (into the corrector's buffers)."""
+        self._rhs(U, self._prim, self._f0)
+        W = self.window().float().contiguous()
+        nat.check(nat.lib().fvm_mp_features(nat.ptr(self._prim), nat.ptr(self._f0), nat.ptr(W), nat.ptr(self._geo),
+                                            2, self.N, self.C, nat.ptr(self._mu_p), nat.ptr(self._sd_p),
+                                            nat.ptr(self._sd_d), nat.ptr(self._X), self.KX,
+                                            nat.stream_ptr(self.device)), "fvm_mp_features")
+        return self._X, self._f0, W
+
+    # ------------------------------------------------------------------ data
+    def generate(self, n_windows: int, log=None):
+        """Run the solver for n_windows windows and keep (inputs, normalised residual)
+        of every window; the solver's state advances.  Returns the dataset list."""
+        if self.stats is None:
+            self.set_stats(self.s.state)
+        data, raw = [], []
+        for i in range(n_windows):
+            U0 = self.s.state.clone()
+            X, f0, W = self.features(U0)
+            X = X.clone()
+            base = U0 + W[:, None] * f0
+            for _ in range(self.k):
+                self.s.step()
+            raw.append((X, (self.s.state - base).float()))
+            if log and (i + 1) % max(1, n_windows // 5) == 0:
+                log(f"generated {i + 1}/{n_windows} windows")
+        allr = torch.cat([r for _, r in raw])
+        scale = (allr.double().std(0) + 1e-30).float()
+        self.stats["scale"] = scale.cpu()
+        self._apply_stats()
+        for X, r in raw:
+            data.append((X, (r / scale).contiguous()))
+        return data
+
+    # ------------------------------------------------------------------ training
+    def train(self, data, iters: int, lr: float = 2e-3, lr_min: float = 1e-4, seed: int = 0, log=None):
+        """Adam on random windows with a cosine learning-rate schedule; returns the loss
+        history (device values are fetched every 50 iterations)."""
+        rng = np.random.default_rng(seed)
+        hist = []
+        for it in range(iters):
+            X, T = data[int(rng.integers(len(data)))]
+            cur = lr_min + 0.5 * (lr - lr_min) * (1 + math.cos(math.pi * it / max(iters - 1, 1)))
+            loss = self.net.train_step(X, self.nbr, T, lr=cur)
+            if it % 50 == 0 or it == iters - 1:
+                hist.append((it, float(loss)))
+                if log:
+                    log(f"iter {it}: loss {hist[-1][1]:.4e}")
+        return hist
+
+    def eval_loss(self, data) -> float:
+        tot = 0.0
+        for X, T in data:
+            self.net.forward(X, self.nbr, train=True)
+            tot += float(self.net.loss(T))
+        return tot / len(data)
+
+    # ------------------------------------------------------------------ inference
+    def predict(self, U, out):
+        X, f0, W = self.features(U)
+        y = self.net.forward(X, self.nbr)
+        nat.check(nat.lib().fvm_mp_apply(nat.ptr(U), nat.ptr(f0), nat.ptr(W), nat.ptr(y), self.C,
+                                         nat.ptr(self._scale), self.N, self.C, nat.ptr(out),
+                                         nat.stream_ptr(self.device)), "fvm_mp_apply")
+        return out, f0, W
+
+    def defect(self, U0, U1, f0, W) -> np.ndarray:
+        """Per mesh and variable, ||d_j|| / ||U1_j|| of the trapezoid defect (host array)."""
+        self._rhs(U1, self._prim1, self._f1)
+        nat.check(nat.lib().fvm_mp_defect(nat.ptr(U0), nat.ptr(U1), nat.ptr(f0), nat.ptr(self._f1), nat.ptr(W),
+                                          nat.ptr(self.off), self.s.n_meshes, self.C, nat.ptr(self._def),
+                                          nat.ptr(self._def_part), nat.stream_ptr(self.device)), "fvm_mp_defect")
+        d = self._def.cpu().numpy()
+        return np.sqrt(d[:, :self.C] / np.maximum(d[:, self.C:], 1e-300))
+
+    def advance(self, use_net: bool = True) -> bool:
+        """One window: predict and check, or (if the check fails or use_net is False)
+        k plain RK4 steps.  Returns True when the prediction was accepted."""
+        if use_net:
+            U0 = self.s.state
+            U1, f0, W = self.predict(U0, self._U1)
+            rel = self.defect(U0, U1, f0, W)
+            if np.all(rel <= self.tol) and np.all(np.isfinite(rel)):
+                self.s.set_state(U1)
+                self.n_accept += 1
+                return True
+            self.n_reject += 1
+        for _ in range(self.k):
+            self.s.step()
+        return False
+
+    def run(self, n_windows: int, use_net: bool = True):
+        """n_windows windows; returns (seconds, accepted windows)."""
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        acc = sum(self.advance(use_net) for _ in range(n_windows))
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0, acc

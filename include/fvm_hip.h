@@ -88,9 +88,13 @@ typedef struct FvmMeshDev {
     const float *cell_inv_vol;  /* [N] fp32(1/vol): |flux_mat| entries                */
     const float *cell_eps;      /* [N] limiter epsilon (K sqrt(vol))^(p+1)            */
     /* per facet ------------------------------------------------------------------- */
+    /* Facets are stored in a device order of their own (by lowest adjacent cell), and
+       cell_facet holds positions in that order.                                        */
     const float *fac_geo;       /* [F][G] G = 12 (2-D) / 20 (3-D): n[d], n_hat[d],
                                    |n|, 1/(n_hat.d), X_orthog[d] (0 on the boundary),
                                    centroid->midpoint of the slot-0, slot-1 cells  */
+    const int32_t *fac_cell;    /* [2][F] slot-0 and slot-1 cell; N + boundary index
+                                   for the outer side of a boundary facet             */
     /* per boundary facet (boundary-local index b) ---------------------------------- */
     const int32_t *bc_cell;     /* [Fb] interior cell                                 */
     const int32_t *bc_group;    /* [Fb] characteristic group, or -1                   */
@@ -135,16 +139,17 @@ typedef struct FvmWork {      /* device scratch owned by the caller             
     float *ubc;       /* [Fb][C] boundary face values                                 */
     float *gp;        /* [N][(d+1)C] unlimited LSQ gradient [d][C] then limiter phi[C];
                          persists between calls (the next call's Neumann correction)  */
-    float *bc_part;   /* [n_slots][4] partial sums of c per slot and mesh-local group */
-    float *bc_mean;   /* [n_groups] means of c of the primitives the next RHS reads   */
-    int32_t *bc_counter; /* [1] slot-completion counter, zero between launches        */
+    float *bc_part;   /* [n_slots][4] partial sums of c per slot and group, of the
+                         primitives the next gradient launch reads                    */
     double *red;      /* [ceil(N/256)][3C] fp64 partial column norms (adaptive dt)    */
+    float *fflux;     /* [F][C] facet fluxes of the current evaluation (device order) */
 } FvmWork;
 
 /* One RHS evaluation dU/dt = FVMEquation.forward(prim) (fvm_equation.py:406-435).
  * has_prev != 0 applies the Neumann non-orthogonal correction with the gradients the
- * previous call left in work->gp (facet_boundary.py:135-136).  Up to three launches:
- * boundary-mean pre-pass, gradient + limiter (+ boundary values), flux + divergence. */
+ * previous call left in work->gp (facet_boundary.py:135-136).  Up to four launches:
+ * boundary-mean pre-pass, gradient + limiter (+ boundary values), facet fluxes,
+ * divergence. */
 int fvm_rhs_f32(const FvmMeshDev *m, const FvmPhys *p, const FvmWork *w, const float *prim,
                 float *dudt, int32_t has_prev, void *stream);
 
@@ -164,7 +169,7 @@ typedef struct FvmStage {
     float *U_out;          /* [N][C] (last stage)                                     */
     const float *dt;       /* device time step; [n_meshes] for a batch                */
     int32_t stage, last, store_k, has_prev;
-    int32_t bc_ready;      /* work->bc_mean already holds the boundary means of prim
+    int32_t bc_ready;      /* work->bc_part already holds the boundary sums of prim
                               (the previous stage's epilogue wrote them)              */
     float b, b2;
     int32_t n_inc;
@@ -184,9 +189,10 @@ int fvm_rk_stage_f32(const FvmMeshDev *m, const FvmPhys *p, const FvmWork *w, co
                      void *stream);
 
 /* The same stage split into its launches, for per-kernel timing with stream events:
- * parts is a mask of 1 = boundary-mean pre-pass (only if !bc_ready), 4 = gradient +
- * limiter + boundary values, 8 = flux + divergence + stage update.  Launching the masks
- * in that order equals fvm_rk_stage_f32.  No reference counterpart (measurement hook). */
+ * parts is a mask of 1 = boundary-mean pre-pass (only if !bc_ready), 2 = gradient +
+ * limiter + boundary values, 4 = facet fluxes, 8 = divergence + stage update.  Launching
+ * the masks 1, 2, 4, 8 in turn equals fvm_rk_stage_f32.  No reference counterpart
+ * (measurement hook). */
 int fvm_rk_stage_part_f32(const FvmMeshDev *m, const FvmPhys *p, const FvmWork *w, const FvmStage *st,
                           int32_t parts, void *stream);
 
@@ -209,6 +215,72 @@ int fvm_conservation_f64(const float *state, const double *vol, int64_t N, int32
  * dt = cfl * min_c h_c / (|V_c| + c_c), h_c = d vol_c / sum_f |n_f|.  out (device fp64). */
 int fvm_cfl_dt(const float *prim, const double *h, int64_t N, int32_t dim, double gR, double cfl,
                double *out, double *partial /* [256] */, void *stream);
+
+/* ---------------------------------------------------------------------------------
+ * Learned corrector on the mesh graph (no reference counterpart: the reference has no
+ * neural network).  A message-passing network predicts the state a window of k solver
+ * steps ahead; these are the kernels of its forward pass, backward pass and Adam update.
+ * bf16 operands are passed as void* (2-byte bfloat16), fp32 accumulation throughout.
+ * --------------------------------------------------------------------------------- */
+
+/* z = A.B + bias[N] with A(i,k) = A[i*a_rs + k*a_cs], B(k,j) = B[k*b_rs + j*b_cs]
+ * (any strides, so transposed operands need no copy); 64x64 tiles on
+ * v_mfma_f32_16x16x32_bf16.  Cf (fp32, row stride c_rs) receives z; Cb (bf16, row stride
+ * N) receives res + relu(z) (relu if relu != 0, res bf16 [M][N] if non-NULL).
+ * splits > 1 splits K over workgroups into fp32 partials part[splits][M][N] that a
+ * second launch sums in split order. */
+int fvm_gemm_bf16(const void *A, int64_t a_rs, int64_t a_cs, const void *B, int64_t b_rs, int64_t b_cs,
+                  int64_t M, int64_t N, int64_t K, int32_t splits, const float *bias, float *Cf, int64_t c_rs,
+                  void *Cb, int32_t relu, const void *res, float *part, void *stream);
+
+/* A[c] = [h[c] | mean of h over c's interior face neighbours] (bf16 [N][2H]); nbr is
+ * FvmMeshDev.cell_nbr ([NL][N], >= N marks a boundary facet). H % 8 == 0. */
+int fvm_mp_aggregate(const void *h, int32_t H, const int32_t *nbr, int64_t N, int32_t NL, void *A,
+                     void *stream);
+
+/* Backward of fvm_mp_aggregate (symmetric adjacency, so a gather):
+ * dh[c] = dA[c][:H] + sum_{n in nbr(c)} dA[n][H:] / |nbr(n)|  (+ dres[c] if non-NULL). */
+int fvm_mp_aggregate_bwd(const float *dA, int32_t H, const int32_t *nbr, int64_t N, int32_t NL,
+                         const float *dres, float *dh, void *stream);
+
+/* h_out = h_in + relu(z) (bf16), or relu(z) when h_in is NULL. */
+int fvm_relu_residual(const float *z, const void *h_in, void *h_out, int64_t n, void *stream);
+
+/* dz = dh * (z > 0) as bf16 (dz_b) and, if dz_f is non-NULL, fp32. */
+int fvm_relu_bwd(const float *dh, const float *z, void *dz_b, float *dz_f, int64_t n, void *stream);
+
+/* out[j] = sum_i X[i][j] for X fp32 [M][N]; part is [256][N] scratch; fixed order. */
+int fvm_colsum_f32(const float *X, int64_t M, int32_t N, float *out, float *part, void *stream);
+
+/* Mean squared error over the first n_out columns of y ([M][ld_y]) against tgt
+ * ([M][n_out]): dy = 2 (y - tgt) / (M n_out) (0 in the padding columns), and fp64
+ * per-block sums of squares in part[256] (loss = sum(part) / (M n_out)). */
+int fvm_mse_loss(const float *y, int64_t ld_y, const float *tgt, int64_t M, int32_t n_out, float *dy,
+                 double *part, void *stream);
+
+/* Adam with bias correction (step >= 1) on fp32 weights w; wb (bf16 copy) optional. */
+int fvm_adam_f32(float *w, const float *g, float *m, float *v, void *wb, int64_t n, float lr, float b1,
+                 float b2, float eps, int32_t step, void *stream);
+
+int fvm_cast_f32_bf16(const float *x, void *y, int64_t n, void *stream);
+
+/* Node features of the corrector: X[c] (bf16, row stride KX) =
+ * [(prim[c] - mu_p) / sd_p, win[c] * dudt[c] / sd_d, geo[c][0..G), 0 ...]  (2C + G <= KX),
+ * win[c] = the window length (k dt of the cell's mesh). */
+int fvm_mp_features(const float *prim, const float *dudt, const float *win, const float *geo, int32_t G,
+                    int64_t N, int32_t C, const float *mu_p, const float *sd_p, const float *sd_d, void *X,
+                    int32_t KX, void *stream);
+
+/* Window update: U_out = U + win * dudt + scale * y[:, :C] (y fp32, row stride ld_y). */
+int fvm_mp_apply(const float *U, const float *dudt, const float *win, const float *y, int64_t ld_y,
+                 const float *scale, int64_t N, int32_t C, float *U_out, void *stream);
+
+/* Trapezoid defect of a predicted window, per mesh:
+ * d = U1 - U0 - win/2 (f0 + f1); out[mesh][j] = sum d_j^2, out[mesh][C + j] = sum U1_j^2
+ * (fp64, cells of each mesh in [off[mesh], off[mesh+1])); part is
+ * [n_meshes][64][2C] fp64 scratch. */
+int fvm_mp_defect(const float *U0, const float *U1, const float *f0, const float *f1, const float *win,
+                  const int64_t *off, int32_t n_meshes, int32_t C, double *out, double *part, void *stream);
 
 #ifdef __cplusplus
 }

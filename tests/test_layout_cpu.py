@@ -171,6 +171,31 @@ def test_facet_record_side_displacements(case):
         assert np.max(np.abs(got - disp[c, k])) < 5e-7
 
 
+def test_facet_sides_and_device_order(case):
+    """fac_cell names each facet's slot-0 / slot-1 cell as the reference's facet_to_cell
+    tables and facet signs do (outer side of a boundary facet: N + boundary index), and
+    the device facet order is a permutation sorted by lowest adjacent cell."""
+    d, mesh, dm, om, of = case
+    h = dm.host
+    N, F = dm.N, dm.F
+    fc = h["fac_cell"]
+    c2f = d["cell_to_facet"]
+    slots = d["slots"].reshape(c2f.shape)
+    for s_ in (0, 1):
+        c, k = np.nonzero(slots == s_)
+        assert np.array_equal(fc[s_, c2f[c, k]], c)
+    bmask = mesh.bc_facet_mask.numpy()
+    fb = np.flatnonzero(bmask)
+    outer = np.where(fc[0, fb] >= N, fc[0, fb], fc[1, fb])
+    assert np.array_equal(outer, N + np.arange(len(fb)))
+    inter = np.flatnonzero(~bmask)
+    assert (fc[:, inter] < N).all() and (fc[0, inter] != fc[1, inter]).all()
+    perm, pos = h["fperm"], h["fpos"]
+    assert np.array_equal(np.sort(perm), np.arange(F)) and np.array_equal(pos[perm], np.arange(F))
+    key = fc.min(0)[perm]
+    assert (np.diff(key) >= 0).all()
+
+
 def test_batch_layout_offsets_and_group_tables():
     from fvm_solver_amd.config_fvm import channel_config
     from fvm_solver_amd.mesh_gen.channel import ChannelSpec, channel_mesh_2d

@@ -235,8 +235,9 @@ def main():
     out["config"]["algorithmic_bytes_per_launch"] = dom_bytes
     if a.traffic_file and os.path.exists(a.traffic_file):
         tr = json.load(open(a.traffic_file))
-        if tr.get("kernel") == dom and tr.get("cells") == N:
-            out["roofline"]["traffic"] = tr["read_bytes"] + tr["write_bytes"]
+        k = tr.get("kernels", {}).get(dom)
+        if k and tr.get("cells") == N:
+            out["roofline"]["traffic"] = k["read_bytes"] + k["write_bytes"]
             out["roofline"]["traffic_source"] = os.path.basename(a.traffic_file)
 
     if not a.no_cpu_baseline:

@@ -75,6 +75,9 @@ _PROTOS = {
                                      c_void_p, i64, c_void_p, i32, c_void_p, c_void_p, c_void_p]),
     "fvm_mp_aggregate": (ctypes.c_int, [c_void_p, i32, c_void_p, i64, i32, c_void_p, c_void_p]),
     "fvm_mp_aggregate_bwd": (ctypes.c_int, [c_void_p, i32, c_void_p, i64, i32, c_void_p, c_void_p, c_void_p]),
+    "fvm_mp_layer": (ctypes.c_int, [c_void_p, i32, c_void_p, i64, i32, c_void_p, c_void_p, c_void_p, c_void_p,
+                                    c_void_p, c_void_p]),
+    "fvm_transpose_bf16": (ctypes.c_int, [c_void_p, i64, i64, c_void_p, c_void_p]),
     "fvm_relu_residual": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, i64, c_void_p]),
     "fvm_relu_bwd": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p, i64, c_void_p]),
     "fvm_colsum_f32": (ctypes.c_int, [c_void_p, i64, i32, c_void_p, c_void_p, c_void_p]),

@@ -311,6 +311,94 @@ __global__ void k_mp_defect_final(const double *__restrict__ part, int nblk, int
     out[t] = s;
 }
 
+// One message-passing layer, aggregation fused into the GEMM's A-tile staging:
+//   h_out = h_in + relu([h_in | mean_nb h_in] W + b)       (H = 128, K = 2H = 256)
+// One 256-thread workgroup per 64 cells computes the full 64 x 128 output (the A rows
+// are read once); wave w owns rows 16w..16w+15 and all 8 column blocks.  W is passed
+// transposed (WT[j][k], bf16 [H][2H]) so B fragments stage with 16-byte loads.  For
+// training, Z (pre-activation, fp32) and the staged A rows (bf16 [N][2H]) can be kept.
+template <int NLT>
+__global__ void __launch_bounds__(256)
+k_mp_layer(const __bf16 *__restrict__ h, const int32_t *__restrict__ nbr, int64_t N, const __bf16 *__restrict__ WT,
+           const float *__restrict__ bias, __bf16 *__restrict__ h_out, float *__restrict__ Z,
+           __bf16 *__restrict__ A_out) {
+    constexpr int H = 128, K = 2 * H;
+    __shared__ __attribute__((aligned(16))) __bf16 As[BM][BK + 8];
+    __shared__ __attribute__((aligned(16))) __bf16 Bs[H][BK + 8];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int64_t m0 = static_cast<int64_t>(blockIdx.x) * BM;
+    const int lr = tid >> 2, lk = (tid & 3) * 8;
+    const int64_t row = m0 + lr;
+    const bool rlive = row < N;
+    int64_t nb[NLT];
+    int cnt = 0;
+#pragma unroll
+    for (int k = 0; k < NLT; ++k) {
+        nb[k] = rlive ? static_cast<int64_t>(nbr[k * N + row]) : N;
+        cnt += nb[k] < N;
+    }
+    const float inv = cnt ? 1.0f / static_cast<float>(cnt) : 0.0f;
+    f32x4_t acc[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) acc[q] = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
+    const int bj = tid >> 1, bk = (tid & 1) * 16;
+    for (int k0 = 0; k0 < K; k0 += BK) {
+        bf16x8_t av;
+        if (!rlive) {
+#pragma unroll
+            for (int e = 0; e < 8; ++e) av[e] = __bf16(0.0f);
+        } else if (k0 < H) {
+            av = *reinterpret_cast<const bf16x8_t *>(h + row * H + k0 + lk);
+        } else {
+            float sacc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+            for (int k = 0; k < NLT; ++k) {
+                if (nb[k] < N) {
+                    const bf16x8_t v = *reinterpret_cast<const bf16x8_t *>(h + nb[k] * H + (k0 - H) + lk);
+#pragma unroll
+                    for (int e = 0; e < 8; ++e) sacc[e] += static_cast<float>(v[e]);
+                }
+            }
+#pragma unroll
+            for (int e = 0; e < 8; ++e) av[e] = __bf16(sacc[e] * inv);
+        }
+        *reinterpret_cast<bf16x8_t *>(&As[lr][lk]) = av;
+        if (A_out && rlive) *reinterpret_cast<bf16x8_t *>(A_out + row * K + k0 + lk) = av;
+        *reinterpret_cast<bf16x8_t *>(&Bs[bj][bk]) = *reinterpret_cast<const bf16x8_t *>(WT + bj * K + k0 + bk);
+        *reinterpret_cast<bf16x8_t *>(&Bs[bj][bk + 8]) =
+            *reinterpret_cast<const bf16x8_t *>(WT + bj * K + k0 + bk + 8);
+        __syncthreads();
+        const bf16x8_t a = *reinterpret_cast<const bf16x8_t *>(&As[16 * wave + (lane & 15)][8 * (lane >> 4)]);
+#pragma unroll
+        for (int cb = 0; cb < 8; ++cb) {
+            const bf16x8_t b = *reinterpret_cast<const bf16x8_t *>(&Bs[16 * cb + (lane & 15)][8 * (lane >> 4)]);
+            acc[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[cb], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int cb = 0; cb < 8; ++cb) {
+        const int j = 16 * cb + (lane & 15);
+        const float bj_ = bias[j];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int64_t i = m0 + 16 * wave + 4 * (lane >> 4) + r;
+            if (i >= N) continue;
+            const float z = acc[cb][r] + bj_;
+            if (Z) Z[i * H + j] = z;
+            h_out[i * H + j] = __bf16(static_cast<float>(h[i * H + j]) + fmaxf(z, 0.0f));
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256)
+k_transpose_bf16(const float *__restrict__ W, int64_t nrow, int64_t ncol, __bf16 *__restrict__ WT) {
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= nrow * ncol) return;
+    const int64_t r = t / ncol, c = t % ncol;
+    WT[c * nrow + r] = __bf16(W[t]);
+}
+
 }  // namespace fvm
 
 using namespace fvm;
@@ -470,6 +558,35 @@ int fvm_mp_defect(const float *U0, const float *U1, const float *f0, const float
     hipLaunchKernelGGL(k_mp_defect_final, dim3(blocks_for(static_cast<int64_t>(n_meshes) * 2 * C, 256)), dim3(256), 0,
                        s, part, 64, n_meshes, C, out);
     return check_launch("fvm_mp_defect");
+}
+
+int fvm_mp_layer(const void *h_in, int32_t H, const int32_t *nbr, int64_t N, int32_t NL, const void *WT,
+                 const float *bias, void *h_out, float *Z, void *A_out, void *stream) {
+    if (!h_in || !nbr || !WT || !bias || !h_out || H != 128 || N < 0 || (NL != 3 && NL != 4) || h_in == h_out) {
+        set_error("fvm_mp_layer: invalid arguments (H must be 128, NL 3 or 4, h_out distinct from h_in)");
+        return FVM_ERR_ARG;
+    }
+    if (N == 0) return FVM_OK;
+    const dim3 grid(static_cast<unsigned int>((N + BM - 1) / BM));
+    const __bf16 *hi = static_cast<const __bf16 *>(h_in);
+    const __bf16 *wt = static_cast<const __bf16 *>(WT);
+    __bf16 *ho = static_cast<__bf16 *>(h_out), *ao = static_cast<__bf16 *>(A_out);
+    if (NL == 3)
+        hipLaunchKernelGGL(k_mp_layer<3>, grid, dim3(256), 0, as_stream(stream), hi, nbr, N, wt, bias, ho, Z, ao);
+    else
+        hipLaunchKernelGGL(k_mp_layer<4>, grid, dim3(256), 0, as_stream(stream), hi, nbr, N, wt, bias, ho, Z, ao);
+    return check_launch("fvm_mp_layer");
+}
+
+int fvm_transpose_bf16(const float *W, int64_t nrow, int64_t ncol, void *WT, void *stream) {
+    if (!W || !WT || nrow < 0 || ncol < 0) {
+        set_error("fvm_transpose_bf16: invalid arguments");
+        return FVM_ERR_ARG;
+    }
+    if (nrow * ncol == 0) return FVM_OK;
+    hipLaunchKernelGGL(k_transpose_bf16, dim3(blocks_for(nrow * ncol, 256)), dim3(256), 0, as_stream(stream), W, nrow,
+                       ncol, static_cast<__bf16 *>(WT));
+    return check_launch("fvm_transpose_bf16");
 }
 
 }  // extern "C"

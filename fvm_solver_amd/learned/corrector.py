@@ -86,20 +86,37 @@ class LearnedCorrector:
         self._state_to_prim(U, prim)
         self.E.rhs(prim, out=f)
 
-    def set_stats(self, U):
-        """Feature normalisation from a representative state."""
-        self._rhs(U, self._prim, self._f0)
-        W = self.window()[:, None]
-        inc = (W * self._f0).double()
-        p = self._prim.double()
-        g = torch.from_numpy(self.geo_raw)
-        self.stats = {
-            "mu_p": p.mean(0).float(), "sd_p": (p.std(0) + 1e-6 * p.abs().mean(0) + 1e-12).float(),
-            "sd_d": (inc.std(0) + 1e-12).float(),
-            "mu_g": g.mean(0).float(), "sd_g": (g.std(0) + 1e-6).float(),
-            "scale": torch.ones(self.C),
-        }
+    @staticmethod
+    def _moments(chunks, floor_rel=1e-3, floor_abs=1e-8):
+        """Per-column mean and (floored) standard deviation over a list of [n, C]
+        tensors, accumulated in fp64."""
+        n, s1, s2 = 0, 0.0, 0.0
+        for x in chunks:
+            xd = x.double()
+            n += xd.shape[0]
+            s1 = s1 + xd.sum(0)
+            s2 = s2 + (xd * xd).sum(0)
+        mu = s1 / n
+        sd = torch.sqrt(torch.clamp(s2 / n - mu * mu, min=0.0))
+        sd = torch.maximum(sd, floor_rel * (mu.abs() + sd))
+        sd = torch.clamp(sd, min=floor_abs)
+        return mu.float().cpu(), sd.float().cpu()
+
+    def set_stats_from(self, prims, incs, residuals=None):
+        """Normalisation of the inputs (primitives, Euler increments) and of the
+        target (window residual) from samples of each."""
+        mu_p, sd_p = self._moments(prims)
+        _, sd_d = self._moments(incs)
+        g = torch.from_numpy(self.geo_raw).float()
+        scale = self._moments(residuals)[1] if residuals is not None else torch.ones(self.C)
+        self.stats = {"mu_p": mu_p, "sd_p": sd_p, "sd_d": sd_d, "mu_g": g.mean(0),
+                      "sd_g": g.std(0).clamp(min=1e-6), "scale": scale}
         self._apply_stats()
+
+    def set_stats(self, U):
+        """Normalisation from a single representative state (no target scale)."""
+        self._rhs(U, self._prim, self._f0)
+        self.set_stats_from([self._prim], [self.window()[:, None] * self._f0])
 
     def _apply_stats(self):
         st = self.stats
@@ -110,39 +127,42 @@ class LearnedCorrector:
         g = (torch.from_numpy(self.geo_raw).float() - st["mu_g"]) / st["sd_g"]
         self._geo = g.contiguous().to(self.device)
 
+    def _features_from(self, prim, f0, W, X):
+        nat.check(nat.lib().fvm_mp_features(nat.ptr(prim), nat.ptr(f0), nat.ptr(W), nat.ptr(self._geo), 2, self.N,
+                                            self.C, nat.ptr(self._mu_p), nat.ptr(self._sd_p), nat.ptr(self._sd_d),
+                                            nat.ptr(X), self.KX, nat.stream_ptr(self.device)), "fvm_mp_features")
+        return X
+
     def features(self, U):
         """prim, f(U) and the network input of state U (into the corrector's buffers)."""
         self._rhs(U, self._prim, self._f0)
         W = self.window().float().contiguous()
-        nat.check(nat.lib().fvm_mp_features(nat.ptr(self._prim), nat.ptr(self._f0), nat.ptr(W), nat.ptr(self._geo),
-                                            2, self.N, self.C, nat.ptr(self._mu_p), nat.ptr(self._sd_p),
-                                            nat.ptr(self._sd_d), nat.ptr(self._X), self.KX,
-                                            nat.stream_ptr(self.device)), "fvm_mp_features")
+        self._features_from(self._prim, self._f0, W, self._X)
         return self._X, self._f0, W
 
     # ------------------------------------------------------------------ data
     def generate(self, n_windows: int, log=None):
-        """Run the solver for n_windows windows and keep (inputs, normalised residual)
-        of every window; the solver's state advances.  Returns the dataset list."""
-        if self.stats is None:
-            self.set_stats(self.s.state)
-        data, raw = [], []
+        """Run the solver for n_windows windows and keep, per window, the network input
+        and the normalised residual of the Euler window; the solver's state advances.
+        The normalisation comes from the whole trajectory.  Returns the dataset list."""
+        W = self.window().float().contiguous()
+        raw = []
         for i in range(n_windows):
             U0 = self.s.state.clone()
-            X, f0, W = self.features(U0)
-            X = X.clone()
+            self._rhs(U0, self._prim, self._f0)
+            prim, f0 = self._prim.clone(), self._f0.clone()
             base = U0 + W[:, None] * f0
             for _ in range(self.k):
                 self.s.step()
-            raw.append((X, (self.s.state - base).float()))
+            raw.append((prim, f0, (self.s.state - base).float()))
             if log and (i + 1) % max(1, n_windows // 5) == 0:
                 log(f"generated {i + 1}/{n_windows} windows")
-        allr = torch.cat([r for _, r in raw])
-        scale = (allr.double().std(0) + 1e-30).float()
-        self.stats["scale"] = scale.cpu()
-        self._apply_stats()
-        for X, r in raw:
-            data.append((X, (r / scale).contiguous()))
+        self.set_stats_from([r[0] for r in raw], [W[:, None] * r[1] for r in raw], [r[2] for r in raw])
+        data = []
+        while raw:
+            prim, f0, r = raw.pop(0)
+            X = torch.empty(self.N, self.KX, dtype=_BF, device=self.device)
+            data.append((self._features_from(prim, f0, W, X), (r / self._scale).contiguous()))
         return data
 
     # ------------------------------------------------------------------ training

@@ -8,8 +8,10 @@ No reference counterpart: the reference defines no neural network.  The network 
     y       = h_L W_out + b_out
 
 with bfloat16 weights and activations and fp32 accumulation on the MFMA units
-(csrc/mpnn.hip).  The graph is the mesh's face adjacency (``FvmMeshDev.cell_nbr``), so a
-batch of meshes concatenated block-diagonally is one graph.  Forward, backward and the
+(csrc/mpnn.hip); for hidden width 128 each message layer is one fused kernel (the
+neighbour mean is formed while the GEMM stages its operand tile).  The graph is the
+mesh's face adjacency (``FvmMeshDev.cell_nbr``), so a batch of meshes concatenated
+block-diagonally is one graph.  Forward, backward and the
 Adam update are the library's own kernels; fp32 master weights are kept beside their
 bf16 copies.
 """
@@ -76,6 +78,9 @@ class MPNN:
         self.names = list(w)
         self.w = {k: v.detach().to(self.device, torch.float32).contiguous() for k, v in w.items()}
         self.wb = {k: v.to(_BF) for k, v in self.w.items() if k.startswith("W")}
+        # message-layer weights also kept transposed ([H][2H], bf16) for the fused layer
+        self.wt = {l: torch.empty(hidden, 2 * hidden, dtype=_BF, device=self.device) for l in range(layers)}
+        self._refresh_transposed()
         self.g = {k: torch.zeros_like(v) for k, v in self.w.items()}
         self.m = {k: torch.zeros_like(v) for k, v in self.w.items()}
         self.v = {k: torch.zeros_like(v) for k, v in self.w.items()}
@@ -92,6 +97,13 @@ class MPNN:
             self.w[k].copy_(d[k].to(self.device, torch.float32))
             if k in self.wb:
                 self.wb[k].copy_(self.w[k].to(_BF))
+        self._refresh_transposed()
+
+    def _refresh_transposed(self):
+        H = self.H
+        for l in range(self.L):
+            nat.check(nat.lib().fvm_transpose_bf16(nat.ptr(self.w[f"W_{l}"]), 2 * H, H, nat.ptr(self.wt[l]),
+                                                   nat.stream_ptr(self.device)), "fvm_transpose_bf16")
 
     # ------------------------------------------------------------------ workspace
     def _work(self, M: int, train: bool):
@@ -117,7 +129,8 @@ class MPNN:
                                          zip(sp, ((self.kx, H), (2 * H, H), (H, self.n_out)))), device=dev)
         else:
             ws["Hs"] = [torch.empty(M, H, dtype=_BF, device=dev) for _ in range(2)]
-            ws["A"] = [torch.empty(M, 2 * H, dtype=_BF, device=dev)]
+            if H != 128:
+                ws["A"] = [torch.empty(M, 2 * H, dtype=_BF, device=dev)]
         self._ws[key] = ws
         return ws
 
@@ -140,6 +153,12 @@ class MPNN:
         for l in range(L):
             h_in = Hs[l] if train else Hs[l % 2]
             h_out = Hs[l + 1] if train else Hs[(l + 1) % 2]
+            if H == 128:
+                nat.check(lib.fvm_mp_layer(nat.ptr(h_in), H, nat.ptr(nbr), M, nbr.shape[0], nat.ptr(self.wt[l]),
+                                           nat.ptr(self.w[f"b_{l}"]), nat.ptr(h_out),
+                                           nat.ptr(ws["Z"][l + 1]) if train else None,
+                                           nat.ptr(ws["A"][l]) if train else None, sp), "fvm_mp_layer")
+                continue
             A = ws["A"][l] if train else ws["A"][0]
             nat.check(lib.fvm_mp_aggregate(nat.ptr(h_in), H, nat.ptr(nbr), M, nbr.shape[0], nat.ptr(A), sp),
                       "fvm_mp_aggregate")
@@ -205,6 +224,7 @@ class MPNN:
             nat.check(lib.fvm_adam_f32(nat.ptr(self.w[k]), nat.ptr(self.g[k]), nat.ptr(self.m[k]), nat.ptr(self.v[k]),
                                        nat.ptr(self.wb.get(k)), self.w[k].numel(), lr, b1, b2, eps, self.step_count,
                                        sp), "fvm_adam_f32")
+        self._refresh_transposed()
 
     def train_step(self, X, nbr, target, lr=1e-3) -> torch.Tensor:
         self.forward(X, nbr, train=True)

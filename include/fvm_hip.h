@@ -243,6 +243,16 @@ int fvm_mp_aggregate(const void *h, int32_t H, const int32_t *nbr, int64_t N, in
 int fvm_mp_aggregate_bwd(const float *dA, int32_t H, const int32_t *nbr, int64_t N, int32_t NL,
                          const float *dres, float *dh, void *stream);
 
+/* One message-passing layer with the aggregation fused into the GEMM:
+ * h_out = h_in + relu([h_in | mean_nb h_in] W + bias), H = 128, NL = 3 or 4.  WT is W
+ * transposed, bf16 [H][2H]; Z (fp32 [N][H], pre-activation) and A_out (bf16 [N][2H],
+ * the aggregated operand) are written when non-NULL (training).  h_out != h_in. */
+int fvm_mp_layer(const void *h_in, int32_t H, const int32_t *nbr, int64_t N, int32_t NL, const void *WT,
+                 const float *bias, void *h_out, float *Z, void *A_out, void *stream);
+
+/* WT[c][r] = bf16(W[r][c]) for fp32 W [nrow][ncol]. */
+int fvm_transpose_bf16(const float *W, int64_t nrow, int64_t ncol, void *WT, void *stream);
+
 /* h_out = h_in + relu(z) (bf16), or relu(z) when h_in is NULL. */
 int fvm_relu_residual(const float *z, const void *h_in, void *h_out, int64_t n, void *stream);
 

@@ -152,3 +152,34 @@ def test_training_lowers_loss():
     losses = [float(net.train_step(xd, nb, td, lr=2e-3)) for _ in range(300)]
     print(f"[gpu-test]   loss {losses[0]:.4f} -> {losses[-1]:.4f}")
     assert losses[-1] < 0.3 * losses[0]
+
+
+def test_fused_layer_matches_aggregate_then_gemm():
+    """fvm_mp_layer (aggregation inside the GEMM's tile staging) equals the two-kernel
+    form bit for bit: same operand values, same MFMA accumulation order."""
+    nbr = _graph(2500, 9)
+    NL, N = nbr.shape
+    H = 128
+    g = torch.Generator().manual_seed(4)
+    h = torch.randn(N, H, generator=g).to(torch.bfloat16).to(DEV)
+    W = (torch.randn(2 * H, H, generator=g) * 0.05).to(DEV)
+    b = (torch.randn(H, generator=g) * 0.1).to(DEV)
+    nb = torch.from_numpy(nbr.astype(np.int32)).to(DEV)
+    lib, sp = nat.lib(), nat.stream_ptr(DEV)
+    WT = torch.empty(H, 2 * H, dtype=torch.bfloat16, device=DEV)
+    nat.check(lib.fvm_transpose_bf16(nat.ptr(W), 2 * H, H, nat.ptr(WT), sp), "transpose")
+    assert torch.equal(WT, W.t().contiguous().to(torch.bfloat16))
+    h1 = torch.empty_like(h)
+    Z1 = torch.empty(N, H, device=DEV)
+    A1 = torch.empty(N, 2 * H, dtype=torch.bfloat16, device=DEV)
+    nat.check(lib.fvm_mp_layer(nat.ptr(h), H, nat.ptr(nb), N, NL, nat.ptr(WT), nat.ptr(b), nat.ptr(h1), nat.ptr(Z1),
+                               nat.ptr(A1), sp), "mp_layer")
+    A2 = torch.empty_like(A1)
+    nat.check(lib.fvm_mp_aggregate(nat.ptr(h), H, nat.ptr(nb), N, NL, nat.ptr(A2), sp), "agg")
+    h2 = torch.empty_like(h)
+    Z2 = torch.empty_like(Z1)
+    gemm(A2, 2 * H, 1, W.to(torch.bfloat16), H, 1, N, H, 2 * H, bias=b, Cf=Z2, Cb=h2, relu=True, res=h)
+    torch.cuda.synchronize()
+    assert torch.equal(A1, A2)
+    assert torch.equal(Z1, Z2)
+    assert torch.equal(h1, h2)

@@ -244,29 +244,84 @@ __global__ void __launch_bounds__(256) k_cast_f32_bf16(const float *__restrict__
 
 __global__ void __launch_bounds__(256)
 k_mp_features(const float *__restrict__ prim, const float *__restrict__ dudt, const float *__restrict__ win,
-              const float *__restrict__ geo, int G, int64_t N, int C, const float *__restrict__ mu_p,
-              const float *__restrict__ sd_p, const float *__restrict__ sd_d, __bf16 *__restrict__ X, int KX) {
+              const float *__restrict__ geo, int G, const float *__restrict__ grad, const float *__restrict__ hc,
+              int D, int64_t N, int C, const float *__restrict__ mu_p, const float *__restrict__ sd_p,
+              const float *__restrict__ sd_d, const int32_t *__restrict__ cell_mesh, __bf16 *__restrict__ X,
+              int KX) {
     const int64_t c = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (c >= N) return;
     __bf16 *x = X + c * KX;
     const float w = win[c];
+    const float *sd = sd_d + (cell_mesh ? cell_mesh[c] : 0) * C;
     for (int j = 0; j < C; ++j) {
         x[j] = __bf16((prim[c * C + j] - mu_p[j]) / sd_p[j]);
-        x[C + j] = __bf16(w * dudt[c * C + j] / sd_d[j]);
+        x[C + j] = __bf16(w * dudt[c * C + j] / sd[j]);
     }
     for (int g = 0; g < G; ++g) x[2 * C + g] = __bf16(geo[c * G + g]);
-    for (int j = 2 * C + G; j < KX; ++j) x[j] = __bf16(0.0f);
+    int n = 2 * C + G;
+    if (grad) {
+        // limited gradient (unlimited gradient times limiter factor) times the cell size,
+        // in units of the primitive's spread: the local variation across one cell
+        const float *r = grad + c * (D + 1) * C;
+        const float h = hc[c];
+        for (int i = 0; i < D; ++i)
+            for (int j = 0; j < C; ++j) x[n + i * C + j] = __bf16(h * r[i * C + j] * r[D * C + j] / sd_p[j]);
+        n += D * C;
+    }
+    for (int j = n; j < KX; ++j) x[j] = __bf16(0.0f);
 }
 
 __global__ void __launch_bounds__(256)
 k_mp_apply(const float *__restrict__ U, const float *__restrict__ dudt, const float *__restrict__ win,
-           const float *__restrict__ y, int64_t ld_y, const float *__restrict__ scale, int64_t N, int C,
-           float *__restrict__ U_out) {
+           const float *__restrict__ y, int64_t ld_y, const float *__restrict__ scale,
+           const int32_t *__restrict__ cell_mesh, int64_t N, int C, float *__restrict__ U_out) {
     const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (t >= N * C) return;
     const int64_t c = t / C;
     const int j = static_cast<int>(t % C);
-    U_out[t] = (U[t] + win[c] * dudt[t]) + scale[j] * y[c * ld_y + j];
+    const float sc = scale[(cell_mesh ? cell_mesh[c] : 0) * C + j];
+    U_out[t] = (U[t] + win[c] * dudt[t]) + sc * y[c * ld_y + j];
+}
+
+// grid (64, n_meshes): per mesh q, fp64 partial sums of (w x_j)^2 over its cells
+__global__ void __launch_bounds__(256)
+k_mesh_sumsq_partial(const float *__restrict__ X, const float *__restrict__ win, const int64_t *__restrict__ off,
+                     int C, double *__restrict__ part) {
+    __shared__ double red[256];
+    const int q = blockIdx.y;
+    const int64_t a = off[q], b = off[q + 1];
+    double s[8];
+    for (int j = 0; j < C; ++j) s[j] = 0.0;
+    for (int64_t c = a + static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; c < b;
+         c += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const float w = win ? win[c] : 1.0f;
+        for (int j = 0; j < C; ++j) {
+            const float v = w * X[c * C + j];
+            s[j] += static_cast<double>(v) * v;
+        }
+    }
+    for (int j = 0; j < C; ++j) {
+        red[threadIdx.x] = s[j];
+        __syncthreads();
+        for (int h = 128; h > 0; h >>= 1) {
+            if (threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) part[(static_cast<int64_t>(q) * gridDim.x + blockIdx.x) * C + j] = red[0];
+        __syncthreads();
+    }
+}
+
+__global__ void k_mesh_rms_final(const double *__restrict__ part, int nblk, int n_meshes, int C,
+                                 const int64_t *__restrict__ off, float floor_abs, float *__restrict__ out) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_meshes * C) return;
+    const int q = t / C, j = t % C;
+    double s = 0.0;
+    for (int b = 0; b < nblk; ++b) s += part[(static_cast<int64_t>(q) * nblk + b) * C + j];
+    const double n = static_cast<double>(off[q + 1] - off[q]);
+    const float r = static_cast<float>(sqrt(s / (n > 0 ? n : 1.0)));
+    out[t] = r > floor_abs ? r : floor_abs;
 }
 
 // grid (64, n_meshes): block b of mesh q sums cells off[q] + b, + 64 * 256 ... strided
@@ -311,70 +366,99 @@ __global__ void k_mp_defect_final(const double *__restrict__ part, int nblk, int
     out[t] = s;
 }
 
-// One message-passing layer, aggregation fused into the GEMM's A-tile staging:
+// One message-passing layer, aggregation fused into the GEMM's operand staging:
 //   h_out = h_in + relu([h_in | mean_nb h_in] W + b)       (H = 128, K = 2H = 256)
-// One 256-thread workgroup per 64 cells computes the full 64 x 128 output (the A rows
-// are read once); wave w owns rows 16w..16w+15 and all 8 column blocks.  W is passed
-// transposed (WT[j][k], bf16 [H][2H]) so B fragments stage with 16-byte loads.  For
-// training, Z (pre-activation, fp32) and the staged A rows (bf16 [N][2H]) can be kept.
+// One 256-thread workgroup per 64 cells computes the full 64 x 128 output, so each A
+// row is formed once.  Every thread first issues all of its operand loads (4 own
+// 16-byte chunks and 4 x NL neighbour chunks), forms the neighbour means, and writes
+// the 64 x 256 operand to LDS; after one barrier each wave (16 rows) runs the 8 K-steps
+// x 8 column blocks of v_mfma_f32_16x16x32_bf16 with the B fragments read straight from
+// W^T (64 KB, cache-resident), the next K-step's fragments prefetched into registers
+// while the current step's MFMAs issue.  Z (pre-activation, fp32) and the operand A
+// (bf16 [N][2H]) are written when non-NULL (training).
 template <int NLT>
 __global__ void __launch_bounds__(256)
 k_mp_layer(const __bf16 *__restrict__ h, const int32_t *__restrict__ nbr, int64_t N, const __bf16 *__restrict__ WT,
            const float *__restrict__ bias, __bf16 *__restrict__ h_out, float *__restrict__ Z,
            __bf16 *__restrict__ A_out) {
-    constexpr int H = 128, K = 2 * H;
-    __shared__ __attribute__((aligned(16))) __bf16 As[BM][BK + 8];
-    __shared__ __attribute__((aligned(16))) __bf16 Bs[H][BK + 8];
+    constexpr int H = 128, K = 2 * H, KP = K + 8;
+    __shared__ __attribute__((aligned(16))) __bf16 As[BM][KP];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int64_t m0 = static_cast<int64_t>(blockIdx.x) * BM;
-    const int lr = tid >> 2, lk = (tid & 3) * 8;
-    const int64_t row = m0 + lr;
-    const bool rlive = row < N;
-    int64_t nb[NLT];
-    int cnt = 0;
+    {
+        const int lr = tid >> 2, q = tid & 3;
+        const int64_t row = m0 + lr;
+        const bool rlive = row < N;
+        const int64_t rs = rlive ? row : 0;
+        int64_t nb[NLT];
+        bool val[NLT];
+        int cnt = 0;
 #pragma unroll
-    for (int k = 0; k < NLT; ++k) {
-        nb[k] = rlive ? static_cast<int64_t>(nbr[k * N + row]) : N;
-        cnt += nb[k] < N;
-    }
-    const float inv = cnt ? 1.0f / static_cast<float>(cnt) : 0.0f;
-    f32x4_t acc[8];
+        for (int k = 0; k < NLT; ++k) {
+            const int64_t v = rlive ? static_cast<int64_t>(nbr[k * N + row]) : N;
+            val[k] = v < N;
+            cnt += val[k];
+            nb[k] = val[k] ? v : rs;
+        }
+        bf16x8_t own[4], nv[NLT][4];
 #pragma unroll
-    for (int q = 0; q < 8; ++q) acc[q] = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
-    const int bj = tid >> 1, bk = (tid & 1) * 16;
-    for (int k0 = 0; k0 < K; k0 += BK) {
-        bf16x8_t av;
-        if (!rlive) {
+        for (int i = 0; i < 4; ++i) {
+            const int kk = 8 * (q + 4 * i);
+            own[i] = *reinterpret_cast<const bf16x8_t *>(h + rs * H + kk);
 #pragma unroll
-            for (int e = 0; e < 8; ++e) av[e] = __bf16(0.0f);
-        } else if (k0 < H) {
-            av = *reinterpret_cast<const bf16x8_t *>(h + row * H + k0 + lk);
-        } else {
+            for (int k = 0; k < NLT; ++k) nv[k][i] = *reinterpret_cast<const bf16x8_t *>(h + nb[k] * H + kk);
+        }
+        const float inv = cnt ? 1.0f / static_cast<float>(cnt) : 0.0f;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int kk = 8 * (q + 4 * i);
+            bf16x8_t mv;
             float sacc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
             for (int k = 0; k < NLT; ++k) {
-                if (nb[k] < N) {
-                    const bf16x8_t v = *reinterpret_cast<const bf16x8_t *>(h + nb[k] * H + (k0 - H) + lk);
+                if (val[k]) {
 #pragma unroll
-                    for (int e = 0; e < 8; ++e) sacc[e] += static_cast<float>(v[e]);
+                    for (int e = 0; e < 8; ++e) sacc[e] += static_cast<float>(nv[k][i][e]);
                 }
             }
 #pragma unroll
-            for (int e = 0; e < 8; ++e) av[e] = __bf16(sacc[e] * inv);
-        }
-        *reinterpret_cast<bf16x8_t *>(&As[lr][lk]) = av;
-        if (A_out && rlive) *reinterpret_cast<bf16x8_t *>(A_out + row * K + k0 + lk) = av;
-        *reinterpret_cast<bf16x8_t *>(&Bs[bj][bk]) = *reinterpret_cast<const bf16x8_t *>(WT + bj * K + k0 + bk);
-        *reinterpret_cast<bf16x8_t *>(&Bs[bj][bk + 8]) =
-            *reinterpret_cast<const bf16x8_t *>(WT + bj * K + k0 + bk + 8);
-        __syncthreads();
-        const bf16x8_t a = *reinterpret_cast<const bf16x8_t *>(&As[16 * wave + (lane & 15)][8 * (lane >> 4)]);
+            for (int e = 0; e < 8; ++e) mv[e] = __bf16(sacc[e] * inv);
+            bf16x8_t ov = own[i];
+            if (!rlive) {
 #pragma unroll
-        for (int cb = 0; cb < 8; ++cb) {
-            const bf16x8_t b = *reinterpret_cast<const bf16x8_t *>(&Bs[16 * cb + (lane & 15)][8 * (lane >> 4)]);
-            acc[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[cb], 0, 0, 0);
+                for (int e = 0; e < 8; ++e) ov[e] = __bf16(0.0f);
+            }
+            *reinterpret_cast<bf16x8_t *>(&As[lr][kk]) = ov;
+            *reinterpret_cast<bf16x8_t *>(&As[lr][H + kk]) = mv;
+            if (A_out && rlive) {
+                *reinterpret_cast<bf16x8_t *>(A_out + row * K + kk) = ov;
+                *reinterpret_cast<bf16x8_t *>(A_out + row * K + H + kk) = mv;
+            }
         }
-        __syncthreads();
+    }
+    __syncthreads();
+    f32x4_t acc[8];
+#pragma unroll
+    for (int cb = 0; cb < 8; ++cb) acc[cb] = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
+    const __bf16 *wcol = WT + static_cast<int64_t>(lane & 15) * K + 8 * (lane >> 4);
+    bf16x8_t bcur[8], bnext[8];
+#pragma unroll
+    for (int cb = 0; cb < 8; ++cb) bcur[cb] = *reinterpret_cast<const bf16x8_t *>(wcol + 16 * cb * K);
+#pragma unroll
+    for (int ks = 0; ks < K / BK; ++ks) {
+        if (ks + 1 < K / BK) {
+#pragma unroll
+            for (int cb = 0; cb < 8; ++cb)
+                bnext[cb] = *reinterpret_cast<const bf16x8_t *>(wcol + 16 * cb * K + BK * (ks + 1));
+        }
+        const bf16x8_t a =
+            *reinterpret_cast<const bf16x8_t *>(&As[16 * wave + (lane & 15)][BK * ks + 8 * (lane >> 4)]);
+#pragma unroll
+        for (int cb = 0; cb < 8; ++cb) acc[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bcur[cb], acc[cb], 0, 0, 0);
+        if (ks + 1 < K / BK) {
+#pragma unroll
+            for (int cb = 0; cb < 8; ++cb) bcur[cb] = bnext[cb];
+        }
     }
 #pragma unroll
     for (int cb = 0; cb < 8; ++cb) {
@@ -382,11 +466,12 @@ k_mp_layer(const __bf16 *__restrict__ h, const int32_t *__restrict__ nbr, int64_
         const float bj_ = bias[j];
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            const int64_t i = m0 + 16 * wave + 4 * (lane >> 4) + r;
+            const int lrow = 16 * wave + 4 * (lane >> 4) + r;
+            const int64_t i = m0 + lrow;
             if (i >= N) continue;
             const float z = acc[cb][r] + bj_;
             if (Z) Z[i * H + j] = z;
-            h_out[i * H + j] = __bf16(static_cast<float>(h[i * H + j]) + fmaxf(z, 0.0f));
+            h_out[i * H + j] = __bf16(static_cast<float>(As[lrow][j]) + fmaxf(z, 0.0f));
         }
     }
 }
@@ -520,30 +605,44 @@ int fvm_cast_f32_bf16(const float *x, void *y, int64_t n, void *stream) {
     return check_launch("fvm_cast_f32_bf16");
 }
 
-int fvm_mp_features(const float *prim, const float *dudt, const float *win, const float *geo, int32_t G, int64_t N,
-                    int32_t C, const float *mu_p, const float *sd_p, const float *sd_d, void *X, int32_t KX,
+int fvm_mp_features(const float *prim, const float *dudt, const float *win, const float *geo, int32_t G,
+                    const float *grad, const float *hc, int32_t D, int64_t N, int32_t C, const float *mu_p,
+                    const float *sd_p, const float *sd_d, const int32_t *cell_mesh, void *X, int32_t KX,
                     void *stream) {
-    if (!prim || !dudt || !win || (G > 0 && !geo) || G < 0 || !mu_p || !sd_p || !sd_d || !X || N < 0 || C < 1 ||
-        2 * C + G > KX) {
+    if (!prim || !dudt || !win || (G > 0 && !geo) || G < 0 || (grad && (!hc || D < 1)) || !mu_p || !sd_p || !sd_d ||
+        !X || N < 0 || C < 1 || 2 * C + G + (grad ? D * C : 0) > KX) {
         set_error("fvm_mp_features: invalid arguments");
         return FVM_ERR_ARG;
     }
     if (N == 0) return FVM_OK;
     hipLaunchKernelGGL(k_mp_features, dim3(blocks_for(N, 256)), dim3(256), 0, as_stream(stream), prim, dudt, win, geo,
-                       G, N, C, mu_p, sd_p, sd_d, static_cast<__bf16 *>(X), KX);
+                       G, grad, hc, D, N, C, mu_p, sd_p, sd_d, cell_mesh, static_cast<__bf16 *>(X), KX);
     return check_launch("fvm_mp_features");
 }
 
 int fvm_mp_apply(const float *U, const float *dudt, const float *win, const float *y, int64_t ld_y,
-                 const float *scale, int64_t N, int32_t C, float *U_out, void *stream) {
+                 const float *scale, const int32_t *cell_mesh, int64_t N, int32_t C, float *U_out, void *stream) {
     if (!U || !dudt || !win || !y || !scale || !U_out || N < 0 || C < 1 || ld_y < C) {
         set_error("fvm_mp_apply: invalid arguments");
         return FVM_ERR_ARG;
     }
     if (N == 0) return FVM_OK;
     hipLaunchKernelGGL(k_mp_apply, dim3(blocks_for(N * C, 256)), dim3(256), 0, as_stream(stream), U, dudt, win, y,
-                       ld_y, scale, N, C, U_out);
+                       ld_y, scale, cell_mesh, N, C, U_out);
     return check_launch("fvm_mp_apply");
+}
+
+int fvm_mesh_rms(const float *X, const float *win, const int64_t *off, int32_t n_meshes, int32_t C, float floor_abs,
+                 float *out, double *part, void *stream) {
+    if (!X || !off || !out || !part || n_meshes < 1 || n_meshes > 65535 || C < 1 || C > 8) {
+        set_error("fvm_mesh_rms: invalid arguments");
+        return FVM_ERR_ARG;
+    }
+    hipStream_t s = as_stream(stream);
+    hipLaunchKernelGGL(k_mesh_sumsq_partial, dim3(64, n_meshes), dim3(256), 0, s, X, win, off, C, part);
+    hipLaunchKernelGGL(k_mesh_rms_final, dim3(blocks_for(static_cast<int64_t>(n_meshes) * C, 256)), dim3(256), 0, s,
+                       part, 64, n_meshes, C, off, floor_abs, out);
+    return check_launch("fvm_mesh_rms");
 }
 
 int fvm_mp_defect(const float *U0, const float *U1, const float *f0, const float *f1, const float *win,

@@ -306,24 +306,39 @@ k_grad(FvmMeshDev m, FvmPhys p, FvmWork w, const float *__restrict__ prim, int h
     if (slot >= 0) block_group_means(m, w, slot, gmean);
     const int g0 = slot >= 0 ? m.slot_g0[slot] : 0;
     if (c >= N) continue;
+    // every per-cell load is issued before the neighbour gathers that depend on them,
+    // and the gathers are branch-free (a boundary slot gathers the cell itself and is
+    // overwritten below), so a cell costs two memory round trips
     float P[C];
     load_prim<D>(prim + c * C, P);
+    int64_t nbk[NL];
+#pragma unroll
+    for (int k = 0; k < NL; ++k) nbk[k] = m.cell_nbr[k * N + c];
+    const int info = m.cell_info[c];
+    float gv[NL + 1][D];
+#pragma unroll
+    for (int jj = 0; jj <= NL; ++jj)
+#pragma unroll
+        for (int i = 0; i < D; ++i) gv[jj][i] = m.cell_gval[(jj * D + i) * N + c];
+    float disp[NL][D];
+#pragma unroll
+    for (int k = 0; k < NL; ++k)
+#pragma unroll
+        for (int i = 0; i < D; ++i) disp[k][i] = m.cell_disp[(k * D + i) * N + c];
+    const float eps = m.cell_eps[c];
     float Un[NL][C];
 #pragma unroll
+    for (int k = 0; k < NL; ++k) load_prim<D>(prim + (nbk[k] < N ? nbk[k] : c) * C, Un[k]);
+#pragma unroll
     for (int k = 0; k < NL; ++k) {
-        const int64_t nb = m.cell_nbr[k * N + c];
-        if (nb < N) {
-            load_prim<D>(prim + nb * C, Un[k]);
-        } else {
+        if (nbk[k] >= N) {
             // this cell is the boundary facet's only cell: it computes the facet's values
             // (reading its own previous gradient record before overwriting it below)
-            const int64_t b = nb - N;
+            const int64_t b = nbk[k] - N;
             bc_value<D>(m, p, b, P, gp + c * Dm::REC, has_prev, gmean, g0, Un[k]);
-#pragma unroll
-            for (int j = 0; j < C; ++j) w.ubc[b * C + j] = Un[k][j];
+            store_vec<C>(w.ubc + b * C, Un[k]);
         }
     }
-    const int info = m.cell_info[c];
     float g[D][C];
 #pragma unroll
     for (int jj = 0; jj <= NL; ++jj) {
@@ -338,17 +353,10 @@ k_grad(FvmMeshDev m, FvmPhys p, FvmWork w, const float *__restrict__ prim, int h
         }
 #pragma unroll
         for (int i = 0; i < D; ++i) {
-            const float gv = m.cell_gval[(jj * D + i) * N + c];
 #pragma unroll
-            for (int j = 0; j < C; ++j) g[i][j] = (jj == 0) ? gv * U[j] : g[i][j] + gv * U[j];
+            for (int j = 0; j < C; ++j) g[i][j] = (jj == 0) ? gv[jj][i] * U[j] : g[i][j] + gv[jj][i] * U[j];
         }
     }
-    float disp[NL][D];
-#pragma unroll
-    for (int k = 0; k < NL; ++k)
-#pragma unroll
-        for (int i = 0; i < D; ++i) disp[k][i] = m.cell_disp[(k * D + i) * N + c];
-    const float eps = m.cell_eps[c];
     float phi[C];
 #pragma unroll
     for (int j = 0; j < C; ++j) {
@@ -556,13 +564,27 @@ k_facet_flux(FvmMeshDev m, FvmPhys p, FvmWork w, const float *__restrict__ prim)
             d0[i] = geo[3 * D + 2 + i];
             d1[i] = geo[4 * D + 2 + i];
         }
+        // both sides' records are gathered unconditionally (a boundary facet gathers its
+        // interior cell twice) so the loads of a facet overlap
+        const int64_t ca = outer0 ? c1 : c0, cb = c1 < N ? c1 : c0;
+        float P0[C], P1[C], ga[Dm::REC], gb[Dm::REC];
+        load_prim<D>(prim + ca * C, P0);
+        load_prim<D>(prim + cb * C, P1);
+#pragma unroll
+        for (int q = 0; q < Dm::REC; q += C) {
+            float t0[C], t1[C];
+            load_vec<C>(gp + ca * Dm::REC + q, t0);
+            load_vec<C>(gp + cb * Dm::REC + q, t1);
+#pragma unroll
+            for (int j = 0; j < C; ++j) {
+                ga[q + j] = t0[j];
+                gb[q + j] = t1[j];
+            }
+        }
         float U0s[C], U1s[C], g0s[D][D + 1], g1s[D][D + 1], gdn[D + 1];
         if (c0 < N && c1 < N) {
-            float P0[C], P1[C];
-            load_prim<D>(prim + c0 * C, P0);
-            load_prim<D>(prim + c1 * C, P1);
-            side_values<D>(P0, gp + c0 * Dm::REC, d0, U0s, g0s);
-            side_values<D>(P1, gp + c1 * Dm::REC, d1, U1s, g1s);
+            side_values<D>(P0, ga, d0, U0s, g0s);
+            side_values<D>(P1, gb, d1, U1s, g1s);
             // A_face_grad row: (-1/d, +1/d) on (slot-0, slot-1) cells, ascending columns
             const float inv_d = geo[2 * D + 1];
             const bool left_first = c0 < c1;
@@ -576,9 +598,8 @@ k_facet_flux(FvmMeshDev m, FvmPhys p, FvmWork w, const float *__restrict__ prim)
             // boundary facet: the outer side takes the boundary values and the interior
             // cell's limited gradient; the normal gradient is A_face_grad.x + b_face_grad
             const int64_t b = (outer0 ? c0 : c1) - N;
-            float Pc[C], Uo[C], glo[D][D + 1], Ux[C];
-            load_prim<D>(prim + own * C, Pc);
-            side_values<D>(Pc, gp + own * Dm::REC, outer0 ? d1 : d0, Uo, glo);
+            float Uo[C], glo[D][D + 1], Ux[C];
+            side_values<D>(P0, ga, outer0 ? d1 : d0, Uo, glo);
             load_prim<D>(w.ubc + b * C, Ux);
 #pragma unroll
             for (int j = 0; j < C; ++j) {
@@ -595,7 +616,7 @@ k_facet_flux(FvmMeshDev m, FvmPhys p, FvmWork w, const float *__restrict__ prim)
 #pragma unroll
             for (int jg = 0; jg <= D; ++jg) {
                 const int j = jg < D ? jg : D + 1;
-                gdn[jg] = m.bc_ga[b * C + j] * Pc[j] + m.bc_gb[b * C + j];
+                gdn[jg] = m.bc_ga[b * C + j] * P0[j] + m.bc_gb[b * C + j];
             }
         }
         float Fx[C];
@@ -631,25 +652,28 @@ k_div(FvmMeshDev m, FvmPhys p, FvmWork w, float *__restrict__ dudt, FvmStage st)
     float acc[C];
 #pragma unroll
     for (int j = 0; j < C; ++j) acc[j] = 0.0f;
+    // the stage epilogue's per-cell inputs are loaded first, so they are in flight
+    // together with the facet lists and the flux gathers
+    const bool first = st.stage == 0;
+    const bool has2 = st.acc2 != nullptr;
+    float U0v[C], ap[C], a2p[C];
+    if (EPI == EPI_STAGE && live) {
+        const int64_t o = c * C;
+        load_vec<C>(st.U0 + o, U0v);
+        if (st.mode == 1) {
+            if (st.n_hist >= 1) load_vec<C>(st.f_m1 + o, ap);
+            if (st.n_hist >= 2) load_vec<C>(st.f_m2 + o, a2p);
+        } else {
+            if (!first) load_vec<C>(st.acc + o, ap);
+            if (!first && has2) load_vec<C>(st.acc2 + o, a2p);
+        }
+    }
     if (live) {
         const int info = m.cell_info[c];
         const float inv_vol = m.cell_inv_vol[c];
         float Fk[NL][C];
 #pragma unroll
-        for (int k = 0; k < NL; ++k) {
-            const int64_t f = m.cell_facet[k * N + c];
-            const float *src = w.fflux + f * C;
-            if constexpr (C == 4) {
-                const float4 q = *reinterpret_cast<const float4 *>(src);
-                Fk[k][0] = q.x;
-                Fk[k][1] = q.y;
-                Fk[k][2] = q.z;
-                Fk[k][3] = q.w;
-            } else {
-#pragma unroll
-                for (int j = 0; j < C; ++j) Fk[k][j] = src[j];
-            }
-        }
+        for (int k = 0; k < NL; ++k) load_vec<C>(w.fflux + static_cast<int64_t>(m.cell_facet[k * N + c]) * C, Fk[k]);
 #pragma unroll
         for (int k = 0; k < NL; ++k) {
             const float vk = ((info >> k) & 1) ? inv_vol : -inv_vol;
@@ -671,12 +695,10 @@ k_div(FvmMeshDev m, FvmPhys p, FvmWork w, float *__restrict__ dudt, FvmStage st)
 #pragma unroll
     for (int j = 0; j < C; ++j) Pn[j] = 0.0f;
     if (live && st.mode == 1) {
-        // Adams-Bashforth 3 (fixed dt, one RHS per step)
+        // Adams-Bashforth 3 (fixed dt, one RHS per step); ap / a2p hold f_m1 / f_m2
         const int64_t o = c * C;
-        float U0v[C], fm1[C], fm2[C];
-        load_vec<C>(st.U0 + o, U0v);
-        if (st.n_hist >= 1) load_vec<C>(st.f_m1 + o, fm1);
-        if (st.n_hist >= 2) load_vec<C>(st.f_m2 + o, fm2);
+        const float(&fm1)[C] = ap;
+        const float(&fm2)[C] = a2p;
         const float c12 = st.dt[mesh] / 12.0f;
         float U1[C];
 #pragma unroll
@@ -693,12 +715,6 @@ k_div(FvmMeshDev m, FvmPhys p, FvmWork w, float *__restrict__ dudt, FvmStage st)
         store_vec<C>(st.prim_next + o, Pn);
     } else if (live) {
         const int64_t o = c * C;
-        const bool first = st.stage == 0;
-        const bool has2 = st.acc2 != nullptr;
-        float U0v[C], ap[C], a2p[C];
-        load_vec<C>(st.U0 + o, U0v);
-        if (!first) load_vec<C>(st.acc + o, ap);
-        if (!first && has2) load_vec<C>(st.acc2 + o, a2p);
         const float dt = st.dt[mesh];
         float kk[C], a[C], a2[C];
 #pragma unroll

@@ -5,22 +5,27 @@ problem the corrector predicts
 
     U(t + k dt) = U + W f(U) + s * MPNN(features)         W = k dt of the cell's mesh
 
-i.e. an explicit Euler step over the whole window plus a learned, per-component scaled
-correction.  The network's inputs per cell are the normalised primitives, the
-normalised Euler increment W f(U) and two geometry features (log cell size, share of
-boundary facets); its graph is the mesh's face adjacency.
+i.e. an explicit Euler step over the whole window plus a learned correction, where s
+is, per mesh and conserved variable, the RMS of the Euler increment W f(U) over the
+mesh's cells.  The network therefore learns the window's residual relative to the size
+of the window's change, which is what varies least between an impulsive start and a
+developed flow.  Inputs per cell: the normalised primitives, the Euler increment
+divided by s, two geometry features (log cell size, share of boundary facets) and the
+limited gradient of the RHS evaluation times the cell size; the graph is the mesh's
+face adjacency.
 
 Every predicted window is checked before it is accepted: the trapezoid defect
 
     d = U1 - U - W/2 (f(U) + f(U1))
 
 costs one more RHS evaluation, and a window is accepted only if, on every mesh of the
-batch and for every conserved variable j, ||d_j|| <= tol ||U1_j||.  Otherwise the window
-is discarded and re-run with the plain fp32 RK4 solver.
+batch, the relative L2 norm of d over the mesh's whole conserved state,
+||d|| / ||U1||, is at most tol.  Otherwise the window is discarded and re-run with the
+plain fp32 RK4 solver.
 
 Training data are the batched solver's own trajectories: (U_t, U_{t+k}) pairs of every
-mesh of the batch, stored as the network's inputs and the normalised residual of the
-Euler window.
+mesh of the batch, stored as the network's inputs and the scaled residual of the Euler
+window.
 """
 from __future__ import annotations
 
@@ -40,7 +45,7 @@ _BF = torch.bfloat16
 class LearnedCorrector:
     """Corrector for one BatchedSolver (whose meshes, physics and dt it uses)."""
 
-    KX = 16
+    KX = 32
 
     def __init__(self, solver, k: int = 8, hidden: int = 128, layers: int = 3, seed: int = 0, tol: float = 1e-3):
         self.s = solver
@@ -50,16 +55,20 @@ class LearnedCorrector:
         self.device = solver.device
         self.C = solver.C
         self.N = dm.N
+        self.M = solver.n_meshes
         # a private RHS evaluator on the same device mesh: the solver's own carried
         # gradient state is not touched by the corrector's evaluations
         self.E = FacetCalc.from_device_mesh(dm, solver.E.cfg, self.device)
         self.nbr = dm.t["cell_nbr"]                                   # int32 [NL, N]
-        self.cell_mesh = torch.from_numpy(dm.host["cell_mesh"].astype(np.int64)).to(self.device)
+        cm = dm.host["cell_mesh"].astype(np.int32)
+        self.cell_mesh = torch.from_numpy(cm).to(self.device)
+        self._cm_long = self.cell_mesh.long()
         self.off = torch.from_numpy(np.asarray(dm.cell_offsets, np.int64)).to(self.device)
         nb = dm.host["nbr_s"]
         bfrac = (nb >= self.N).sum(1) / nb.shape[1]
         logh = np.log(dm.h64.cpu().numpy())
         self.geo_raw = np.stack([logh, bfrac], 1)
+        self._h = dm.h64.float().contiguous()
         self.net = MPNN(self.KX, self.C, hidden, layers, device=self.device, seed=seed)
         self.stats = None
         z = lambda *sh: torch.empty(*sh, device=self.device)
@@ -68,15 +77,17 @@ class LearnedCorrector:
         self._f0 = z(self.N, self.C)
         self._f1 = z(self.N, self.C)
         self._U1 = z(self.N, self.C)
+        self._sc = z(self.M, self.C)
         self._X = torch.empty(self.N, self.KX, dtype=_BF, device=self.device)
-        self._def = torch.empty(solver.n_meshes, 2 * self.C, dtype=torch.float64, device=self.device)
-        self._def_part = torch.empty(solver.n_meshes * 64 * 2 * self.C, dtype=torch.float64, device=self.device)
+        self._def = torch.empty(self.M, 2 * self.C, dtype=torch.float64, device=self.device)
+        self._part = torch.empty(self.M * 64 * 2 * self.C, dtype=torch.float64, device=self.device)
         self.n_accept = 0
         self.n_reject = 0
+        self.last_defect = None
 
     # ------------------------------------------------------------------ pieces
     def window(self) -> torch.Tensor:
-        return self.k * self.s.dt[self.cell_mesh]
+        return (self.k * self.s.dt[self._cm_long]).float().contiguous()
 
     def _state_to_prim(self, U, out):
         nat.check(nat.lib().fvm_state_to_prim_f32(nat.ptr(U), nat.ptr(out), self.N, self.C - 2, self.s.inv_cv,
@@ -85,6 +96,13 @@ class LearnedCorrector:
     def _rhs(self, U, prim, f):
         self._state_to_prim(U, prim)
         self.E.rhs(prim, out=f)
+
+    def _inc_scale(self, f0, W, out):
+        """Per mesh and variable, the RMS of the Euler increment W f over the mesh."""
+        nat.check(nat.lib().fvm_mesh_rms(nat.ptr(f0), nat.ptr(W), nat.ptr(self.off), self.M, self.C, 1e-30,
+                                         nat.ptr(out), nat.ptr(self._part), nat.stream_ptr(self.device)),
+                  "fvm_mesh_rms")
+        return out
 
     @staticmethod
     def _moments(chunks, floor_rel=1e-3, floor_abs=1e-8):
@@ -102,67 +120,66 @@ class LearnedCorrector:
         sd = torch.clamp(sd, min=floor_abs)
         return mu.float().cpu(), sd.float().cpu()
 
-    def set_stats_from(self, prims, incs, residuals=None):
-        """Normalisation of the inputs (primitives, Euler increments) and of the
-        target (window residual) from samples of each."""
+    def set_stats_from(self, prims):
+        """Normalisation of the primitive inputs and the geometry features."""
         mu_p, sd_p = self._moments(prims)
-        _, sd_d = self._moments(incs)
         g = torch.from_numpy(self.geo_raw).float()
-        scale = self._moments(residuals)[1] if residuals is not None else torch.ones(self.C)
-        self.stats = {"mu_p": mu_p, "sd_p": sd_p, "sd_d": sd_d, "mu_g": g.mean(0),
-                      "sd_g": g.std(0).clamp(min=1e-6), "scale": scale}
+        self.stats = {"mu_p": mu_p, "sd_p": sd_p, "mu_g": g.mean(0), "sd_g": g.std(0).clamp(min=1e-6)}
         self._apply_stats()
 
     def set_stats(self, U):
-        """Normalisation from a single representative state (no target scale)."""
+        """Normalisation from a single representative state."""
         self._rhs(U, self._prim, self._f0)
-        self.set_stats_from([self._prim], [self.window()[:, None] * self._f0])
+        self.set_stats_from([self._prim])
 
     def _apply_stats(self):
         st = self.stats
         self._mu_p = st["mu_p"].to(self.device)
         self._sd_p = st["sd_p"].to(self.device)
-        self._sd_d = st["sd_d"].to(self.device)
-        self._scale = st["scale"].to(self.device)
         g = (torch.from_numpy(self.geo_raw).float() - st["mu_g"]) / st["sd_g"]
         self._geo = g.contiguous().to(self.device)
 
-    def _features_from(self, prim, f0, W, X):
-        nat.check(nat.lib().fvm_mp_features(nat.ptr(prim), nat.ptr(f0), nat.ptr(W), nat.ptr(self._geo), 2, self.N,
-                                            self.C, nat.ptr(self._mu_p), nat.ptr(self._sd_p), nat.ptr(self._sd_d),
-                                            nat.ptr(X), self.KX, nat.stream_ptr(self.device)), "fvm_mp_features")
+    def _features_from(self, prim, f0, grad, W, sc, X):
+        nat.check(nat.lib().fvm_mp_features(nat.ptr(prim), nat.ptr(f0), nat.ptr(W), nat.ptr(self._geo), 2,
+                                            nat.ptr(grad), nat.ptr(self._h), self.C - 2, self.N, self.C,
+                                            nat.ptr(self._mu_p), nat.ptr(self._sd_p), nat.ptr(sc),
+                                            nat.ptr(self.cell_mesh), nat.ptr(X), self.KX,
+                                            nat.stream_ptr(self.device)), "fvm_mp_features")
         return X
 
     def features(self, U):
-        """prim, f(U) and the network input of state U (into the corrector's buffers)."""
+        """prim, f(U), the increment scale and the network input of state U (into the
+        corrector's buffers)."""
         self._rhs(U, self._prim, self._f0)
-        W = self.window().float().contiguous()
-        self._features_from(self._prim, self._f0, W, self._X)
-        return self._X, self._f0, W
+        W = self.window()
+        self._inc_scale(self._f0, W, self._sc)
+        self._features_from(self._prim, self._f0, self.E.gp, W, self._sc, self._X)
+        return self._X, self._f0, W, self._sc
 
     # ------------------------------------------------------------------ data
     def generate(self, n_windows: int, log=None):
         """Run the solver for n_windows windows and keep, per window, the network input
-        and the normalised residual of the Euler window; the solver's state advances.
-        The normalisation comes from the whole trajectory.  Returns the dataset list."""
-        W = self.window().float().contiguous()
+        and the scaled residual of the Euler window; the solver's state advances.
+        Returns the dataset list of (X, target)."""
+        W = self.window()
         raw = []
         for i in range(n_windows):
             U0 = self.s.state.clone()
             self._rhs(U0, self._prim, self._f0)
-            prim, f0 = self._prim.clone(), self._f0.clone()
+            prim, f0, grad = self._prim.clone(), self._f0.clone(), self.E.gp.clone()
+            sc = self._inc_scale(f0, W, torch.empty(self.M, self.C, device=self.device))
             base = U0 + W[:, None] * f0
             for _ in range(self.k):
                 self.s.step()
-            raw.append((prim, f0, (self.s.state - base).float()))
+            raw.append((prim, f0, grad, sc, (self.s.state - base).float()))
             if log and (i + 1) % max(1, n_windows // 5) == 0:
                 log(f"generated {i + 1}/{n_windows} windows")
-        self.set_stats_from([r[0] for r in raw], [W[:, None] * r[1] for r in raw], [r[2] for r in raw])
+        self.set_stats_from([r[0] for r in raw])
         data = []
         while raw:
-            prim, f0, r = raw.pop(0)
+            prim, f0, grad, sc, r = raw.pop(0)
             X = torch.empty(self.N, self.KX, dtype=_BF, device=self.device)
-            data.append((self._features_from(prim, f0, W, X), (r / self._scale).contiguous()))
+            data.append((self._features_from(prim, f0, grad, W, sc, X), (r / sc[self._cm_long]).contiguous()))
         return data
 
     # ------------------------------------------------------------------ training
@@ -190,21 +207,23 @@ class LearnedCorrector:
 
     # ------------------------------------------------------------------ inference
     def predict(self, U, out):
-        X, f0, W = self.features(U)
+        X, f0, W, sc = self.features(U)
         y = self.net.forward(X, self.nbr)
-        nat.check(nat.lib().fvm_mp_apply(nat.ptr(U), nat.ptr(f0), nat.ptr(W), nat.ptr(y), self.C,
-                                         nat.ptr(self._scale), self.N, self.C, nat.ptr(out),
+        nat.check(nat.lib().fvm_mp_apply(nat.ptr(U), nat.ptr(f0), nat.ptr(W), nat.ptr(y), self.C, nat.ptr(sc),
+                                         nat.ptr(self.cell_mesh), self.N, self.C, nat.ptr(out),
                                          nat.stream_ptr(self.device)), "fvm_mp_apply")
         return out, f0, W
 
     def defect(self, U0, U1, f0, W) -> np.ndarray:
-        """Per mesh and variable, ||d_j|| / ||U1_j|| of the trapezoid defect (host array)."""
+        """Per mesh, ||d|| / ||U1|| of the trapezoid defect over the mesh's whole state
+        (host array); per-variable ratios are kept in ``last_defect``."""
         self._rhs(U1, self._prim1, self._f1)
         nat.check(nat.lib().fvm_mp_defect(nat.ptr(U0), nat.ptr(U1), nat.ptr(f0), nat.ptr(self._f1), nat.ptr(W),
-                                          nat.ptr(self.off), self.s.n_meshes, self.C, nat.ptr(self._def),
-                                          nat.ptr(self._def_part), nat.stream_ptr(self.device)), "fvm_mp_defect")
+                                          nat.ptr(self.off), self.M, self.C, nat.ptr(self._def), nat.ptr(self._part),
+                                          nat.stream_ptr(self.device)), "fvm_mp_defect")
         d = self._def.cpu().numpy()
-        return np.sqrt(d[:, :self.C] / np.maximum(d[:, self.C:], 1e-300))
+        self.last_defect = np.sqrt(d[:, :self.C] / np.maximum(d[:, self.C:], 1e-300))
+        return np.sqrt(d[:, :self.C].sum(1) / np.maximum(d[:, self.C:].sum(1), 1e-300))
 
     def advance(self, use_net: bool = True) -> bool:
         """One window: predict and check, or (if the check fails or use_net is False)
@@ -213,7 +232,7 @@ class LearnedCorrector:
             U0 = self.s.state
             U1, f0, W = self.predict(U0, self._U1)
             rel = self.defect(U0, U1, f0, W)
-            if np.all(rel <= self.tol) and np.all(np.isfinite(rel)):
+            if np.all(rel <= self.tol):
                 self.s.set_state(U1)
                 self.n_accept += 1
                 return True

@@ -275,15 +275,25 @@ int fvm_adam_f32(float *w, const float *g, float *m, float *v, void *wb, int64_t
 int fvm_cast_f32_bf16(const float *x, void *y, int64_t n, void *stream);
 
 /* Node features of the corrector: X[c] (bf16, row stride KX) =
- * [(prim[c] - mu_p) / sd_p, win[c] * dudt[c] / sd_d, geo[c][0..G), 0 ...]  (2C + G <= KX),
- * win[c] = the window length (k dt of the cell's mesh). */
+ * [(prim[c] - mu_p) / sd_p, win[c] * dudt[c] / sd_d[m(c)], geo[c][0..G),
+ *  hc[c] * g[c][i][j] * phi[c][j] / sd_p[j] (i < D, j < C; only if grad), 0 ...]
+ * with win[c] the window length (k dt of the cell's mesh), m(c) = cell_mesh[c] (0 when
+ * cell_mesh is NULL), sd_d [n_meshes][C], grad the gradient record of FvmWork.gp
+ * ([N][(D+1)C]: g[D][C] then phi[C]) and hc a cell size. */
 int fvm_mp_features(const float *prim, const float *dudt, const float *win, const float *geo, int32_t G,
-                    int64_t N, int32_t C, const float *mu_p, const float *sd_p, const float *sd_d, void *X,
-                    int32_t KX, void *stream);
+                    const float *grad, const float *hc, int32_t D, int64_t N, int32_t C, const float *mu_p,
+                    const float *sd_p, const float *sd_d, const int32_t *cell_mesh, void *X, int32_t KX,
+                    void *stream);
 
-/* Window update: U_out = U + win * dudt + scale * y[:, :C] (y fp32, row stride ld_y). */
+/* Window update: U_out = U + win * dudt + scale[m(c)] * y[:, :C] (y fp32, row stride
+ * ld_y; scale [n_meshes][C]). */
 int fvm_mp_apply(const float *U, const float *dudt, const float *win, const float *y, int64_t ld_y,
-                 const float *scale, int64_t N, int32_t C, float *U_out, void *stream);
+                 const float *scale, const int32_t *cell_mesh, int64_t N, int32_t C, float *U_out, void *stream);
+
+/* Per mesh and column, out[mesh][j] = max(floor_abs, sqrt(mean over the mesh's cells of
+ * (win[c] X[c][j])^2)) (win NULL = 1), fp64 sums; part is [n_meshes][64][C] scratch. */
+int fvm_mesh_rms(const float *X, const float *win, const int64_t *off, int32_t n_meshes, int32_t C,
+                 float floor_abs, float *out, double *part, void *stream);
 
 /* Trapezoid defect of a predicted window, per mesh:
  * d = U1 - U0 - win/2 (f0 + f1); out[mesh][j] = sum d_j^2, out[mesh][C + j] = sum U1_j^2

@@ -28,7 +28,10 @@ def _parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--cells", type=int, default=1_000_000)
+    ap.add_argument("--config", choices=("channel2d", "channel3d"), default="channel2d",
+                    help="channel2d: the 1 M-cell 2-D channel (the headline); channel3d: the 3-D channel "
+                         "(tetrahedra, C = 5; default 4 M cells)")
+    ap.add_argument("--cells", type=int, default=None, help="cells (default 1 M in 2-D, 4 M in 3-D)")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--cpu-steps", type=int, default=30)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -96,11 +99,16 @@ def main():
               "DESIGN.md and no multi-GPU figure is measured", file=sys.stderr)
         sys.exit(2)
     from fvm_solver_amd import _native as nat
-    from fvm_solver_amd.config_fvm import ConfigChannel
-    from fvm_solver_amd.fvm_equation import FluidConstitution2D, FVMEquation
-    from fvm_solver_amd.mesh_gen.channel import channel_mesh_2d
+    from fvm_solver_amd.config_fvm import ConfigChannel, ConfigChannel3D
+    from fvm_solver_amd.fvm_equation import FluidConstitution2D, FluidConstitution3D, FVMEquation
+    from fvm_solver_amd.mesh_gen.channel import channel_mesh_2d, channel_mesh_3d
     from fvm_solver_amd.mesh_utils.fvm_mesh import mesh_from_tuple
     from fvm_solver_amd.mesh_utils.mesh_store import channel_bc_tags
+
+    D = 3 if a.config == "channel3d" else 2
+    C = D + 2
+    if a.cells is None:
+        a.cells = 4_000_000 if D == 3 else 1_000_000
 
     if not torch.cuda.is_available():
         raise RuntimeError("bench.py needs a GPU")
@@ -110,25 +118,27 @@ def main():
         z = np.load(a.mesh_cache, allow_pickle=False)
         pts, cells, fi, fb, tags = z["pts"], z["cells"], z["fi"], z["fb"], [str(t) for t in z["tags"]]
     else:
-        pts, cells, (fi, fb), tags = channel_mesh_2d(a.cells, seed=a.seed)
+        gen = channel_mesh_3d if D == 3 else channel_mesh_2d
+        pts, cells, (fi, fb), tags = gen(a.cells, seed=a.seed)
         if a.mesh_cache:
             np.savez(a.mesh_cache, pts=pts, cells=cells, fi=fi, fb=fb, tags=np.array(tags))
     log(f"mesh: {len(cells)} cells, {len(fi) + len(fb)} facets ({len(fb)} boundary) in {time.time() - t0:.1f}s")
-    cfg = ConfigChannel(device="cuda:0", n_cells=a.cells, solver_name="Butcher", solver_extra="RK4")
+    Cfg = ConfigChannel3D if D == 3 else ConfigChannel
+    cfg = Cfg(device="cuda:0", n_cells=a.cells, solver_name="Butcher", solver_extra="RK4")
     t0 = time.time()
     mesh = mesh_from_tuple(pts, cells, fi, fb, device=dev)
-    phy = FluidConstitution2D(cfg)
-    N, F, Fb, D = mesh.n_cells, mesh.n_facets, mesh.n_bc_facet, 2
+    phy = FluidConstitution3D(cfg) if D == 3 else FluidConstitution2D(cfg)
+    N, F, Fb = mesh.n_cells, mesh.n_facets, mesh.n_bc_facet
     c = mesh.geo64["centroids"]
-    prim = np.zeros((N, 4))
+    prim = np.zeros((N, C))
     prim[:, 0] = cfg.inlet_cfg.v_n_inf
-    r2 = ((c - np.asarray(cfg.bump_centre)) ** 2).sum(1)
-    prim[:, 2] = 1.0 + cfg.bump_amp * np.exp(-r2 / (2 * cfg.bump_sigma ** 2))
-    prim[:, 3] = cfg.inlet_cfg.T_inf
+    r2 = ((c - np.asarray(cfg.bump_centre)[:D]) ** 2).sum(1)
+    prim[:, D] = 1.0 + cfg.bump_amp * np.exp(-r2 / (2 * cfg.bump_sigma ** 2))
+    prim[:, D + 1] = cfg.inlet_cfg.T_inf
     prim = torch.from_numpy(prim).float()
-    V, rho, T = prim[:, :2], prim[:, 2:3], prim[:, 3:]
+    V, rho, T = prim[:, :D], prim[:, D:D + 1], prim[:, D + 1:]
     U0 = torch.cat(phy.primatives_to_state(V, rho, T), -1)
-    eq = FVMEquation(cfg, phy, mesh, 4, channel_bc_tags(tags, 2), us_init=U0)
+    eq = FVMEquation(cfg, phy, mesh, C, channel_bc_tags(tags, D), us_init=U0)
     dt = float(eq.cfl_dt(prim.to(dev), cfg.cfl))
     eq.t_solver.set_dt(dt)
     log(f"set-up {time.time() - t0:.1f}s, dt={dt:.4e}")
@@ -214,7 +224,7 @@ def main():
         + ", ".join(f"{k}={v:.0f}" for k, v in kgbs.items()) + f"; {stage_bytes / N:.0f} B/cell/stage")
 
     out = {
-        "metric": "cell-updates/s (fused HIP RK4 step, 2-D NS channel)",
+        "metric": f"cell-updates/s (fused HIP RK4 step, {D}-D NS channel)",
         "value": value,
         "unit": "cell-updates/s",
         "n_gpus": 1,
@@ -226,7 +236,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded Poisson-disk Delaunay mesh, freestream + Gaussian bump)",
-        "config": {"workload": "2-D compressible NS channel with circular obstacle, RK4 at CFL 0.5",
+        "config": {"workload": ("3-D compressible NS channel with spherical obstacle (tetrahedra)" if D == 3 else
+                                "2-D compressible NS channel with circular obstacle") + ", RK4 at CFL 0.5",
                    "cells": N, "facets": F, "boundary_facets": Fb, "integrator": "Butcher RK4",
                    "dt": dt, "finite": finite, "kernel_avg_us": avg_us,
                    "kernel_algorithmic_bytes": kbytes, "kernel_gbs": kgbs},
@@ -257,6 +268,8 @@ def main():
             pass
         torch.set_num_threads(threads)
         _, ofvm = O.from_mesh_tuple(pts, cells, fi, fb, tags, ph)
+        if D == 3:
+            a.cpu_steps = max(1, a.cpu_steps // 6)
         stepper = O.OracleStepper(ofvm, "Butcher", "RK4", dt=dt)
         Uc = U0.clone()
         Uc = stepper.step(Uc)            # warm-up

@@ -90,6 +90,8 @@ _PROTOS = {
     "fvm_mp_apply": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p, i64, c_void_p, c_void_p, i64, i32,
                                     c_void_p, c_void_p]),
     "fvm_mesh_rms": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, i32, i32, f32, c_void_p, c_void_p, c_void_p]),
+    "fvm_mp_defect3": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                      i32, i32, c_void_p, c_void_p, c_void_p]),
     "fvm_mp_defect": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, i32, i32, c_void_p,
                                      c_void_p, c_void_p]),
 }

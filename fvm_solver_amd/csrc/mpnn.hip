@@ -23,6 +23,7 @@ constexpr int BM = 64, BN = 64, BK = 32;
 // Split-K (gridDim.z > 1) writes fp32 partials part[z][M][N]; otherwise the epilogue
 // adds the bias and writes the pre-activation z (fp32, Cf, row stride c_rs) and/or
 // res + relu(z) (bf16, Cb; relu and res optional).
+template <bool AV, bool BV>
 __global__ void __launch_bounds__(256)
 k_gemm_bf16(const __bf16 *__restrict__ A, int64_t a_rs, int64_t a_cs, const __bf16 *__restrict__ B, int64_t b_rs,
             int64_t b_cs, int64_t M, int64_t N, int64_t K, int64_t k_chunk, const float *__restrict__ bias,
@@ -42,17 +43,28 @@ k_gemm_bf16(const __bf16 *__restrict__ A, int64_t a_rs, int64_t a_cs, const __bf
     const int lr = tid >> 2, lk = (tid & 3) * 8;
     for (int64_t k0 = kb; k0 < ke; k0 += BK) {
         {
+            // a K-contiguous operand stages 8 elements with one 16-byte load
             const int64_t gi = m0 + lr;
+            if (AV && gi < M && k0 + lk + 8 <= ke) {
+                *reinterpret_cast<bf16x8_t *>(&As[lr][lk]) =
+                    *reinterpret_cast<const bf16x8_t *>(A + gi * a_rs + k0 + lk);
+            } else {
 #pragma unroll
-            for (int e = 0; e < 8; ++e) {
-                const int64_t gk = k0 + lk + e;
-                As[lr][lk + e] = (gi < M && gk < ke) ? A[gi * a_rs + gk * a_cs] : __bf16(0.0f);
+                for (int e = 0; e < 8; ++e) {
+                    const int64_t gk = k0 + lk + e;
+                    As[lr][lk + e] = (gi < M && gk < ke) ? A[gi * a_rs + gk * a_cs] : __bf16(0.0f);
+                }
             }
             const int64_t gj = n0 + lr;
+            if (BV && gj < N && k0 + lk + 8 <= ke) {
+                *reinterpret_cast<bf16x8_t *>(&Bs[lr][lk]) =
+                    *reinterpret_cast<const bf16x8_t *>(B + gj * b_cs + k0 + lk);
+            } else {
 #pragma unroll
-            for (int e = 0; e < 8; ++e) {
-                const int64_t gk = k0 + lk + e;
-                Bs[lr][lk + e] = (gj < N && gk < ke) ? B[gk * b_rs + gj * b_cs] : __bf16(0.0f);
+                for (int e = 0; e < 8; ++e) {
+                    const int64_t gk = k0 + lk + e;
+                    Bs[lr][lk + e] = (gj < N && gk < ke) ? B[gk * b_rs + gj * b_cs] : __bf16(0.0f);
+                }
             }
         }
         __syncthreads();
@@ -242,33 +254,45 @@ __global__ void __launch_bounds__(256) k_cast_f32_bf16(const float *__restrict__
     if (t < n) y[t] = __bf16(x[t]);
 }
 
+template <int D, int G, int KX>
 __global__ void __launch_bounds__(256)
 k_mp_features(const float *__restrict__ prim, const float *__restrict__ dudt, const float *__restrict__ win,
-              const float *__restrict__ geo, int G, const float *__restrict__ grad, const float *__restrict__ hc,
-              int D, int64_t N, int C, const float *__restrict__ mu_p, const float *__restrict__ sd_p,
-              const float *__restrict__ sd_d, const int32_t *__restrict__ cell_mesh, __bf16 *__restrict__ X,
-              int KX) {
+              const float *__restrict__ geo, const float *__restrict__ grad, const float *__restrict__ hc, int64_t N,
+              const float *__restrict__ mu_p, const float *__restrict__ sd_p, const float *__restrict__ sd_d,
+              const int32_t *__restrict__ cell_mesh, __bf16 *__restrict__ X) {
+    constexpr int C = D + 2;
     const int64_t c = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (c >= N) return;
-    __bf16 *x = X + c * KX;
+    float x[KX];
+#pragma unroll
+    for (int j = 0; j < KX; ++j) x[j] = 0.0f;
     const float w = win[c];
     const float *sd = sd_d + (cell_mesh ? cell_mesh[c] : 0) * C;
+#pragma unroll
     for (int j = 0; j < C; ++j) {
-        x[j] = __bf16((prim[c * C + j] - mu_p[j]) / sd_p[j]);
-        x[C + j] = __bf16(w * dudt[c * C + j] / sd[j]);
+        x[j] = (prim[c * C + j] - mu_p[j]) / sd_p[j];
+        x[C + j] = w * dudt[c * C + j] / sd[j];
     }
-    for (int g = 0; g < G; ++g) x[2 * C + g] = __bf16(geo[c * G + g]);
-    int n = 2 * C + G;
+#pragma unroll
+    for (int g = 0; g < G; ++g) x[2 * C + g] = geo[c * G + g];
     if (grad) {
         // limited gradient (unlimited gradient times limiter factor) times the cell size,
         // in units of the primitive's spread: the local variation across one cell
         const float *r = grad + c * (D + 1) * C;
         const float h = hc[c];
+#pragma unroll
         for (int i = 0; i < D; ++i)
-            for (int j = 0; j < C; ++j) x[n + i * C + j] = __bf16(h * r[i * C + j] * r[D * C + j] / sd_p[j]);
-        n += D * C;
+#pragma unroll
+            for (int j = 0; j < C; ++j) x[2 * C + G + i * C + j] = h * r[i * C + j] * r[D * C + j] / sd_p[j];
     }
-    for (int j = n; j < KX; ++j) x[j] = __bf16(0.0f);
+    // the row leaves as 16-byte vectors
+#pragma unroll
+    for (int q = 0; q < KX; q += 8) {
+        bf16x8_t v;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) v[e] = __bf16(x[q + e]);
+        *reinterpret_cast<bf16x8_t *>(X + c * KX + q) = v;
+    }
 }
 
 __global__ void __launch_bounds__(256)
@@ -356,40 +380,59 @@ k_mp_defect_partial(const float *__restrict__ U0, const float *__restrict__ U1, 
     }
 }
 
-__global__ void k_mp_defect_final(const double *__restrict__ part, int nblk, int n_meshes, int C,
+// per mesh, the ncol columns of the block partials summed in block order
+__global__ void k_mp_defect_final(const double *__restrict__ part, int nblk, int n_meshes, int ncol,
                                   double *__restrict__ out) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n_meshes * 2 * C) return;
-    const int q = t / (2 * C), j = t % (2 * C);
+    if (t >= n_meshes * ncol) return;
+    const int q = t / ncol, j = t % ncol;
     double s = 0.0;
-    for (int b = 0; b < nblk; ++b) s += part[(static_cast<int64_t>(q) * nblk + b) * 2 * C + j];
+    for (int b = 0; b < nblk; ++b) s += part[(static_cast<int64_t>(q) * nblk + b) * ncol + j];
     out[t] = s;
 }
 
-// One message-passing layer, aggregation fused into the GEMM's operand staging:
+// One message-passing layer, aggregation fused into the GEMM's operand loads:
 //   h_out = h_in + relu([h_in | mean_nb h_in] W + b)       (H = 128, K = 2H = 256)
-// One 256-thread workgroup per 64 cells computes the full 64 x 128 output, so each A
-// row is formed once.  Every thread first issues all of its operand loads (4 own
-// 16-byte chunks and 4 x NL neighbour chunks), forms the neighbour means, and writes
-// the 64 x 256 operand to LDS; after one barrier each wave (16 rows) runs the 8 K-steps
-// x 8 column blocks of v_mfma_f32_16x16x32_bf16 with the B fragments read straight from
-// W^T (64 KB, cache-resident), the next K-step's fragments prefetched into registers
-// while the current step's MFMAs issue.  Z (pre-activation, fp32) and the operand A
-// (bf16 [N][2H]) are written when non-NULL (training).
+// Persistent 512-thread workgroups stage W^T (bf16 [128][256], 66 KB) in LDS once; each
+// wave then takes 16-cell tiles.  The MFMA computes features x cells (W^T is the A
+// operand, read from LDS), so a lane's B operand is 8 consecutive k of one cell's row:
+// it is loaded straight from h into registers -- the cell's own row for k < 128, the
+// fp32 mean of its neighbours' chunks for k >= 128 -- with every load of the tile
+// issued before the first MFMA, and each lane ends with 4 consecutive output features
+// of one cell per column block (8-byte epilogue loads and stores).  8 K-steps x 8
+// feature blocks of v_mfma_f32_16x16x32_bf16, in the generic GEMM's K-step order.
+// Z (pre-activation, fp32) and the operand A (bf16 [N][2H]) are written when non-NULL
+// (training).
+constexpr int kLayerThreads = 512;
+
 template <int NLT>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(kLayerThreads) __attribute__((amdgpu_waves_per_eu(4)))
 k_mp_layer(const __bf16 *__restrict__ h, const int32_t *__restrict__ nbr, int64_t N, const __bf16 *__restrict__ WT,
            const float *__restrict__ bias, __bf16 *__restrict__ h_out, float *__restrict__ Z,
            __bf16 *__restrict__ A_out) {
     constexpr int H = 128, K = 2 * H, KP = K + 8;
-    __shared__ __attribute__((aligned(16))) __bf16 As[BM][KP];
+    __shared__ __attribute__((aligned(16))) __bf16 Ws[H][KP];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    const int64_t m0 = static_cast<int64_t>(blockIdx.x) * BM;
-    {
-        const int lr = tid >> 2, q = tid & 3;
-        const int64_t row = m0 + lr;
+    for (int t = tid; t < H * (K / 8); t += kLayerThreads) {
+        const int j = t / (K / 8), kc = (t % (K / 8)) * 8;
+        *reinterpret_cast<bf16x8_t *>(&Ws[j][kc]) = *reinterpret_cast<const bf16x8_t *>(WT + j * K + kc);
+    }
+    __syncthreads();
+    constexpr int nwaves = kLayerThreads / 64;
+    const int64_t ntiles = (N + 15) / 16;
+    const int kq = 8 * (lane >> 4);
+    const int col = lane & 15;
+    for (int64_t t = static_cast<int64_t>(blockIdx.x) * nwaves + wave; t < ntiles;
+         t += static_cast<int64_t>(gridDim.x) * nwaves) {
+        const int64_t r0 = t * 16;
+        const int64_t row = r0 + col;
         const bool rlive = row < N;
         const int64_t rs = rlive ? row : 0;
+        // the B fragments do not depend on the tile; an opaque per-tile offset keeps the
+        // compiler from hoisting all 64 of them out of the loop (256 registers)
+        int wofs = 0;
+        asm volatile("" : "+v"(wofs));
+        const __bf16 *wsb = &Ws[col][kq] + wofs;
         int64_t nb[NLT];
         bool val[NLT];
         int cnt = 0;
@@ -403,16 +446,28 @@ k_mp_layer(const __bf16 *__restrict__ h, const int32_t *__restrict__ nbr, int64_
         bf16x8_t own[4], nv[NLT][4];
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            const int kk = 8 * (q + 4 * i);
-            own[i] = *reinterpret_cast<const bf16x8_t *>(h + rs * H + kk);
+            own[i] = *reinterpret_cast<const bf16x8_t *>(h + rs * H + 32 * i + kq);
 #pragma unroll
-            for (int k = 0; k < NLT; ++k) nv[k][i] = *reinterpret_cast<const bf16x8_t *>(h + nb[k] * H + kk);
+            for (int k = 0; k < NLT; ++k) nv[k][i] = *reinterpret_cast<const bf16x8_t *>(h + nb[k] * H + 32 * i + kq);
         }
         const float inv = cnt ? 1.0f / static_cast<float>(cnt) : 0.0f;
+        f32x4_t acc[8];
+#pragma unroll
+        for (int cb = 0; cb < 8; ++cb) acc[cb] = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            const int kk = 8 * (q + 4 * i);
-            bf16x8_t mv;
+            if (!rlive) {
+#pragma unroll
+                for (int e = 0; e < 8; ++e) own[i][e] = __bf16(0.0f);
+            }
+#pragma unroll
+            for (int cb = 0; cb < 8; ++cb) {
+                const bf16x8_t wf = *reinterpret_cast<const bf16x8_t *>(wsb + 16 * cb * KP + 32 * i);
+                acc[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf, own[i], acc[cb], 0, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
             float sacc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
             for (int k = 0; k < NLT; ++k) {
@@ -421,57 +476,40 @@ k_mp_layer(const __bf16 *__restrict__ h, const int32_t *__restrict__ nbr, int64_
                     for (int e = 0; e < 8; ++e) sacc[e] += static_cast<float>(nv[k][i][e]);
                 }
             }
+            bf16x8_t mv;
 #pragma unroll
             for (int e = 0; e < 8; ++e) mv[e] = __bf16(sacc[e] * inv);
-            bf16x8_t ov = own[i];
-            if (!rlive) {
-#pragma unroll
-                for (int e = 0; e < 8; ++e) ov[e] = __bf16(0.0f);
-            }
-            *reinterpret_cast<bf16x8_t *>(&As[lr][kk]) = ov;
-            *reinterpret_cast<bf16x8_t *>(&As[lr][H + kk]) = mv;
             if (A_out && rlive) {
-                *reinterpret_cast<bf16x8_t *>(A_out + row * K + kk) = ov;
-                *reinterpret_cast<bf16x8_t *>(A_out + row * K + H + kk) = mv;
+                *reinterpret_cast<bf16x8_t *>(A_out + row * K + 32 * i + kq) = own[i];
+                *reinterpret_cast<bf16x8_t *>(A_out + row * K + H + 32 * i + kq) = mv;
+            }
+#pragma unroll
+            for (int cb = 0; cb < 8; ++cb) {
+                const bf16x8_t wf = *reinterpret_cast<const bf16x8_t *>(wsb + 16 * cb * KP + H + 32 * i);
+                acc[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf, mv, acc[cb], 0, 0, 0);
             }
         }
-    }
-    __syncthreads();
-    f32x4_t acc[8];
+        // W^T is the MFMA's A operand and the cell rows its B operand, so the 16 x 16
+        // output block is features x cells: this lane holds features 16cb + 4g .. +3 of
+        // cell `row` -- 4 consecutive values that load and store as one 8-byte access
+        if (rlive) {
+            const int g4 = 4 * (lane >> 4);
 #pragma unroll
-    for (int cb = 0; cb < 8; ++cb) acc[cb] = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
-    const __bf16 *wcol = WT + static_cast<int64_t>(lane & 15) * K + 8 * (lane >> 4);
-    bf16x8_t bcur[8], bnext[8];
-#pragma unroll
-    for (int cb = 0; cb < 8; ++cb) bcur[cb] = *reinterpret_cast<const bf16x8_t *>(wcol + 16 * cb * K);
-#pragma unroll
-    for (int ks = 0; ks < K / BK; ++ks) {
-        if (ks + 1 < K / BK) {
-#pragma unroll
-            for (int cb = 0; cb < 8; ++cb)
-                bnext[cb] = *reinterpret_cast<const bf16x8_t *>(wcol + 16 * cb * K + BK * (ks + 1));
-        }
-        const bf16x8_t a =
-            *reinterpret_cast<const bf16x8_t *>(&As[16 * wave + (lane & 15)][BK * ks + 8 * (lane >> 4)]);
-#pragma unroll
-        for (int cb = 0; cb < 8; ++cb) acc[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bcur[cb], acc[cb], 0, 0, 0);
-        if (ks + 1 < K / BK) {
-#pragma unroll
-            for (int cb = 0; cb < 8; ++cb) bcur[cb] = bnext[cb];
-        }
-    }
-#pragma unroll
-    for (int cb = 0; cb < 8; ++cb) {
-        const int j = 16 * cb + (lane & 15);
-        const float bj_ = bias[j];
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int lrow = 16 * wave + 4 * (lane >> 4) + r;
-            const int64_t i = m0 + lrow;
-            if (i >= N) continue;
-            const float z = acc[cb][r] + bj_;
-            if (Z) Z[i * H + j] = z;
-            h_out[i * H + j] = __bf16(static_cast<float>(As[lrow][j]) + fmaxf(z, 0.0f));
+            for (int cb = 0; cb < 8; ++cb) {
+                const int j0 = 16 * cb + g4;
+                const float4 bb = *reinterpret_cast<const float4 *>(bias + j0);
+                const float z0 = acc[cb][0] + bb.x, z1 = acc[cb][1] + bb.y, z2 = acc[cb][2] + bb.z,
+                            z3 = acc[cb][3] + bb.w;
+                if (Z) *reinterpret_cast<float4 *>(Z + row * H + j0) = make_float4(z0, z1, z2, z3);
+                typedef __bf16 bf16x4_t __attribute__((ext_vector_type(4)));
+                const bf16x4_t hr = *reinterpret_cast<const bf16x4_t *>(h + row * H + j0);
+                bf16x4_t o;
+                o[0] = __bf16(static_cast<float>(hr[0]) + fmaxf(z0, 0.0f));
+                o[1] = __bf16(static_cast<float>(hr[1]) + fmaxf(z1, 0.0f));
+                o[2] = __bf16(static_cast<float>(hr[2]) + fmaxf(z2, 0.0f));
+                o[3] = __bf16(static_cast<float>(hr[3]) + fmaxf(z3, 0.0f));
+                *reinterpret_cast<bf16x4_t *>(h_out + row * H + j0) = o;
+            }
         }
     }
 }
@@ -482,6 +520,43 @@ k_transpose_bf16(const float *__restrict__ W, int64_t nrow, int64_t ncol, __bf16
     if (t >= nrow * ncol) return;
     const int64_t r = t / ncol, c = t % ncol;
     WT[c * nrow + r] = __bf16(W[t]);
+}
+
+// grid (64, n_meshes): per mesh, fp64 partial sums of the Simpson defect of the window
+// d = U1 - U0 - w/6 (f0 + 4 fm + f1), of the Hermite midpoint defect
+// e = Um - (U0 + U1)/2 - w/8 (f0 - f1), and of U1^2, per variable
+__global__ void __launch_bounds__(256)
+k_mp_defect3_partial(const float *__restrict__ U0, const float *__restrict__ Um, const float *__restrict__ U1,
+                     const float *__restrict__ f0, const float *__restrict__ fm, const float *__restrict__ f1,
+                     const float *__restrict__ win, const int64_t *__restrict__ off, int C,
+                     double *__restrict__ part) {
+    __shared__ double red[256];
+    const int q = blockIdx.y;
+    const int64_t a = off[q], b = off[q + 1];
+    double s[24];
+    for (int j = 0; j < 3 * C; ++j) s[j] = 0.0;
+    for (int64_t c = a + static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; c < b;
+         c += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const float w = win[c];
+        for (int j = 0; j < C; ++j) {
+            const int64_t o = c * C + j;
+            const float d = (U1[o] - U0[o]) - (w / 6.0f) * ((f0[o] + 4.0f * fm[o]) + f1[o]);
+            const float e = (Um[o] - 0.5f * (U0[o] + U1[o])) - (w * 0.125f) * (f0[o] - f1[o]);
+            s[j] += static_cast<double>(d) * d;
+            s[C + j] += static_cast<double>(e) * e;
+            s[2 * C + j] += static_cast<double>(U1[o]) * U1[o];
+        }
+    }
+    for (int j = 0; j < 3 * C; ++j) {
+        red[threadIdx.x] = s[j];
+        __syncthreads();
+        for (int h = 128; h > 0; h >>= 1) {
+            if (threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) part[(static_cast<int64_t>(q) * gridDim.x + blockIdx.x) * 3 * C + j] = red[0];
+        __syncthreads();
+    }
 }
 
 }  // namespace fvm
@@ -505,9 +580,22 @@ int fvm_gemm_bf16(const void *A, int64_t a_rs, int64_t a_cs, const void *B, int6
     hipStream_t s = as_stream(stream);
     dim3 grid(static_cast<unsigned int>((M + BM - 1) / BM), static_cast<unsigned int>((N + BN - 1) / BN), S);
     const __bf16 *r = static_cast<const __bf16 *>(res);
-    hipLaunchKernelGGL(k_gemm_bf16, grid, dim3(256), 0, s, static_cast<const __bf16 *>(A), a_rs, a_cs,
-                       static_cast<const __bf16 *>(B), b_rs, b_cs, M, N, K, k_chunk, bias, Cf, c_rs,
-                       static_cast<__bf16 *>(Cb), relu, r, part);
+    const __bf16 *a = static_cast<const __bf16 *>(A), *b = static_cast<const __bf16 *>(B);
+    __bf16 *cb = static_cast<__bf16 *>(Cb);
+    const bool av = a_cs == 1 && a_rs % 8 == 0 && (reinterpret_cast<uintptr_t>(A) & 15u) == 0;
+    const bool bv = b_rs == 1 && b_cs % 8 == 0 && (reinterpret_cast<uintptr_t>(B) & 15u) == 0;
+    if (av && bv)
+        hipLaunchKernelGGL((k_gemm_bf16<true, true>), grid, dim3(256), 0, s, a, a_rs, a_cs, b, b_rs, b_cs, M, N, K,
+                           k_chunk, bias, Cf, c_rs, cb, relu, r, part);
+    else if (av)
+        hipLaunchKernelGGL((k_gemm_bf16<true, false>), grid, dim3(256), 0, s, a, a_rs, a_cs, b, b_rs, b_cs, M, N, K,
+                           k_chunk, bias, Cf, c_rs, cb, relu, r, part);
+    else if (bv)
+        hipLaunchKernelGGL((k_gemm_bf16<false, true>), grid, dim3(256), 0, s, a, a_rs, a_cs, b, b_rs, b_cs, M, N, K,
+                           k_chunk, bias, Cf, c_rs, cb, relu, r, part);
+    else
+        hipLaunchKernelGGL((k_gemm_bf16<false, false>), grid, dim3(256), 0, s, a, a_rs, a_cs, b, b_rs, b_cs, M, N, K,
+                           k_chunk, bias, Cf, c_rs, cb, relu, r, part);
     if (S > 1)
         hipLaunchKernelGGL(k_splitk_reduce, dim3(blocks_for(M * N, 256)), dim3(256), 0, s, part, S, M, N, bias, Cf,
                            c_rs, static_cast<__bf16 *>(Cb), relu, r);
@@ -609,14 +697,19 @@ int fvm_mp_features(const float *prim, const float *dudt, const float *win, cons
                     const float *grad, const float *hc, int32_t D, int64_t N, int32_t C, const float *mu_p,
                     const float *sd_p, const float *sd_d, const int32_t *cell_mesh, void *X, int32_t KX,
                     void *stream) {
-    if (!prim || !dudt || !win || (G > 0 && !geo) || G < 0 || (grad && (!hc || D < 1)) || !mu_p || !sd_p || !sd_d ||
-        !X || N < 0 || C < 1 || 2 * C + G + (grad ? D * C : 0) > KX) {
-        set_error("fvm_mp_features: invalid arguments");
+    if (!prim || !dudt || !win || !geo || G != 2 || (grad && !hc) || (D != 2 && D != 3) || C != D + 2 || !mu_p ||
+        !sd_p || !sd_d || !X || N < 0 || KX != 32 || (reinterpret_cast<uintptr_t>(X) & 15u)) {
+        set_error("fvm_mp_features: invalid arguments (this build takes G = 2, KX = 32, C = D + 2)");
         return FVM_ERR_ARG;
     }
     if (N == 0) return FVM_OK;
-    hipLaunchKernelGGL(k_mp_features, dim3(blocks_for(N, 256)), dim3(256), 0, as_stream(stream), prim, dudt, win, geo,
-                       G, grad, hc, D, N, C, mu_p, sd_p, sd_d, cell_mesh, static_cast<__bf16 *>(X), KX);
+    __bf16 *x = static_cast<__bf16 *>(X);
+    if (D == 2)
+        hipLaunchKernelGGL((k_mp_features<2, 2, 32>), dim3(blocks_for(N, 256)), dim3(256), 0, as_stream(stream), prim,
+                           dudt, win, geo, grad, hc, N, mu_p, sd_p, sd_d, cell_mesh, x);
+    else
+        hipLaunchKernelGGL((k_mp_features<3, 2, 32>), dim3(blocks_for(N, 256)), dim3(256), 0, as_stream(stream), prim,
+                           dudt, win, geo, grad, hc, N, mu_p, sd_p, sd_d, cell_mesh, x);
     return check_launch("fvm_mp_features");
 }
 
@@ -655,7 +748,7 @@ int fvm_mp_defect(const float *U0, const float *U1, const float *f0, const float
     hipStream_t s = as_stream(stream);
     hipLaunchKernelGGL(k_mp_defect_partial, dim3(64, n_meshes), dim3(256), 0, s, U0, U1, f0, f1, win, off, C, part);
     hipLaunchKernelGGL(k_mp_defect_final, dim3(blocks_for(static_cast<int64_t>(n_meshes) * 2 * C, 256)), dim3(256), 0,
-                       s, part, 64, n_meshes, C, out);
+                       s, part, 64, n_meshes, 2 * C, out);
     return check_launch("fvm_mp_defect");
 }
 
@@ -666,14 +759,29 @@ int fvm_mp_layer(const void *h_in, int32_t H, const int32_t *nbr, int64_t N, int
         return FVM_ERR_ARG;
     }
     if (N == 0) return FVM_OK;
-    const dim3 grid(static_cast<unsigned int>((N + BM - 1) / BM));
+    // persistent: two workgroups per CU (the LDS copy of W^T is 66 KB) or fewer if
+    // there are fewer tiles
+    static int n_cu = 0;
+    if (n_cu == 0) {
+        int dev = 0, v = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0)
+            n_cu = v;
+        else
+            n_cu = 256;
+    }
+    const int64_t wave_tiles = (N + 15) / 16;
+    const int64_t blocks = (wave_tiles + (kLayerThreads / 64) - 1) / (kLayerThreads / 64);
+    const dim3 grid(static_cast<unsigned int>(blocks < 2 * n_cu ? blocks : 2 * n_cu));
     const __bf16 *hi = static_cast<const __bf16 *>(h_in);
     const __bf16 *wt = static_cast<const __bf16 *>(WT);
     __bf16 *ho = static_cast<__bf16 *>(h_out), *ao = static_cast<__bf16 *>(A_out);
     if (NL == 3)
-        hipLaunchKernelGGL(k_mp_layer<3>, grid, dim3(256), 0, as_stream(stream), hi, nbr, N, wt, bias, ho, Z, ao);
+        hipLaunchKernelGGL(k_mp_layer<3>, grid, dim3(kLayerThreads), 0, as_stream(stream), hi, nbr, N, wt, bias, ho, Z,
+                           ao);
     else
-        hipLaunchKernelGGL(k_mp_layer<4>, grid, dim3(256), 0, as_stream(stream), hi, nbr, N, wt, bias, ho, Z, ao);
+        hipLaunchKernelGGL(k_mp_layer<4>, grid, dim3(kLayerThreads), 0, as_stream(stream), hi, nbr, N, wt, bias, ho, Z,
+                           ao);
     return check_launch("fvm_mp_layer");
 }
 
@@ -686,6 +794,22 @@ int fvm_transpose_bf16(const float *W, int64_t nrow, int64_t ncol, void *WT, voi
     hipLaunchKernelGGL(k_transpose_bf16, dim3(blocks_for(nrow * ncol, 256)), dim3(256), 0, as_stream(stream), W, nrow,
                        ncol, static_cast<__bf16 *>(WT));
     return check_launch("fvm_transpose_bf16");
+}
+
+int fvm_mp_defect3(const float *U0, const float *Um, const float *U1, const float *f0, const float *fm, const float *f1,
+                   const float *win, const int64_t *off, int32_t n_meshes, int32_t C, double *out, double *part,
+                   void *stream) {
+    if (!U0 || !Um || !U1 || !f0 || !fm || !f1 || !win || !off || !out || !part || n_meshes < 1 ||
+        n_meshes > 65535 || C < 1 || C > 8) {
+        set_error("fvm_mp_defect3: invalid arguments");
+        return FVM_ERR_ARG;
+    }
+    hipStream_t s = as_stream(stream);
+    hipLaunchKernelGGL(k_mp_defect3_partial, dim3(64, n_meshes), dim3(256), 0, s, U0, Um, U1, f0, fm, f1, win, off, C,
+                       part);
+    hipLaunchKernelGGL(k_mp_defect_final, dim3(blocks_for(static_cast<int64_t>(n_meshes) * 3 * C, 256)), dim3(256), 0,
+                       s, part, 64, n_meshes, 3 * C, out);
+    return check_launch("fvm_mp_defect3");
 }
 
 }  // extern "C"

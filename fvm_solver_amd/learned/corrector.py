@@ -1,31 +1,33 @@
 """Learned window predictor with the plain Runge-Kutta solver as fallback and judge.
 
 No reference counterpart.  Over a window of k solver steps of a (batched) channel
-problem the corrector predicts
+problem the corrector predicts the state at the window's midpoint and end,
 
-    U(t + k dt) = U + W f(U) + s * MPNN(features)         W = k dt of the cell's mesh
+    U(t + W/2) = U + (W/2) f(U) + s * y_mid,     U(t + W) = U + W f(U) + s * y_end,
 
-i.e. an explicit Euler step over the whole window plus a learned correction, where s
-is, per mesh and conserved variable, the RMS of the Euler increment W f(U) over the
-mesh's cells.  The network therefore learns the window's residual relative to the size
-of the window's change, which is what varies least between an impulsive start and a
-developed flow.  Inputs per cell: the normalised primitives, the Euler increment
-divided by s, two geometry features (log cell size, share of boundary facets) and the
-limited gradient of the RHS evaluation times the cell size; the graph is the mesh's
-face adjacency.
+with W = k dt of the cell's mesh and y = MPNN(features): explicit Euler steps plus a
+learned correction, where s is, per mesh and conserved variable, the RMS of the Euler
+increment W f(U) over the mesh's cells.  The network therefore learns the window's
+residual relative to the size of the window's change, which is what varies least
+between an impulsive start and a developed flow.  Inputs per cell: the normalised
+primitives, the Euler increment divided by s, two geometry features (log cell size,
+share of boundary facets) and the limited gradient of the RHS evaluation times the cell
+size; the graph is the mesh's face adjacency.
 
-Every predicted window is checked before it is accepted: the trapezoid defect
+Every predicted window is checked before it is accepted, with two more RHS evaluations
+(at the predicted midpoint and end): the Simpson defect
 
-    d = U1 - U - W/2 (f(U) + f(U1))
+    d = U1 - U - W/6 (f(U) + 4 f(Um) + f(U1))
 
-costs one more RHS evaluation, and a window is accepted only if, on every mesh of the
-batch, the relative L2 norm of d over the mesh's whole conserved state,
-||d|| / ||U1||, is at most tol.  Otherwise the window is discarded and re-run with the
-plain fp32 RK4 solver.
+and the Hermite-midpoint defect e = Um - (U + U1)/2 - W/8 (f(U) - f(U1)).  Both vanish to
+fourth order in W for a smooth exact trajectory.  A window is accepted only if, on every
+mesh of the batch, ||d|| / ||U1|| and ||e|| / ||U1|| over the mesh's whole conserved
+state are at most tol.  Otherwise it is discarded and re-run with the plain fp32 RK4
+solver.
 
-Training data are the batched solver's own trajectories: (U_t, U_{t+k}) pairs of every
-mesh of the batch, stored as the network's inputs and the scaled residual of the Euler
-window.
+Training data are the batched solver's own trajectories: (U_t, U_{t+k/2}, U_{t+k})
+triples of every mesh of the batch, stored as the network's inputs and the scaled
+residuals of the two Euler steps.
 """
 from __future__ import annotations
 
@@ -69,18 +71,22 @@ class LearnedCorrector:
         logh = np.log(dm.h64.cpu().numpy())
         self.geo_raw = np.stack([logh, bfrac], 1)
         self._h = dm.h64.float().contiguous()
-        self.net = MPNN(self.KX, self.C, hidden, layers, device=self.device, seed=seed)
+        if k % 2:
+            raise ValueError("the window must hold an even number of steps (its midpoint is predicted)")
+        self.net = MPNN(self.KX, 2 * self.C, hidden, layers, device=self.device, seed=seed)
         self.stats = None
         z = lambda *sh: torch.empty(*sh, device=self.device)
         self._prim = z(self.N, self.C)
         self._prim1 = z(self.N, self.C)
         self._f0 = z(self.N, self.C)
+        self._fm = z(self.N, self.C)
         self._f1 = z(self.N, self.C)
+        self._Um = z(self.N, self.C)
         self._U1 = z(self.N, self.C)
         self._sc = z(self.M, self.C)
         self._X = torch.empty(self.N, self.KX, dtype=_BF, device=self.device)
-        self._def = torch.empty(self.M, 2 * self.C, dtype=torch.float64, device=self.device)
-        self._part = torch.empty(self.M * 64 * 2 * self.C, dtype=torch.float64, device=self.device)
+        self._def = torch.empty(self.M, 3 * self.C, dtype=torch.float64, device=self.device)
+        self._part = torch.empty(self.M * 64 * 3 * self.C, dtype=torch.float64, device=self.device)
         self.n_accept = 0
         self.n_reject = 0
         self.last_defect = None
@@ -159,8 +165,8 @@ class LearnedCorrector:
     # ------------------------------------------------------------------ data
     def generate(self, n_windows: int, log=None):
         """Run the solver for n_windows windows and keep, per window, the network input
-        and the scaled residual of the Euler window; the solver's state advances.
-        Returns the dataset list of (X, target)."""
+        and the scaled residuals of the half- and full-window Euler steps; the solver's
+        state advances.  Returns the dataset list of (X, target [N, 2C])."""
         W = self.window()
         raw = []
         for i in range(n_windows):
@@ -168,10 +174,13 @@ class LearnedCorrector:
             self._rhs(U0, self._prim, self._f0)
             prim, f0, grad = self._prim.clone(), self._f0.clone(), self.E.gp.clone()
             sc = self._inc_scale(f0, W, torch.empty(self.M, self.C, device=self.device))
-            base = U0 + W[:, None] * f0
-            for _ in range(self.k):
+            for _ in range(self.k // 2):
                 self.s.step()
-            raw.append((prim, f0, grad, sc, (self.s.state - base).float()))
+            rm = (self.s.state - U0 - 0.5 * W[:, None] * f0).float()
+            for _ in range(self.k // 2):
+                self.s.step()
+            r1 = (self.s.state - U0 - W[:, None] * f0).float()
+            raw.append((prim, f0, grad, sc, torch.cat([rm, r1], 1)))
             if log and (i + 1) % max(1, n_windows // 5) == 0:
                 log(f"generated {i + 1}/{n_windows} windows")
         self.set_stats_from([r[0] for r in raw])
@@ -179,7 +188,8 @@ class LearnedCorrector:
         while raw:
             prim, f0, grad, sc, r = raw.pop(0)
             X = torch.empty(self.N, self.KX, dtype=_BF, device=self.device)
-            data.append((self._features_from(prim, f0, grad, W, sc, X), (r / sc[self._cm_long]).contiguous()))
+            sc2 = sc[self._cm_long].repeat(1, 2)
+            data.append((self._features_from(prim, f0, grad, W, sc, X), (r / sc2).contiguous()))
         return data
 
     # ------------------------------------------------------------------ training
@@ -206,32 +216,45 @@ class LearnedCorrector:
         return tot / len(data)
 
     # ------------------------------------------------------------------ inference
-    def predict(self, U, out):
+    def _apply(self, U, f0, W, y, col, out):
+        nat.check(nat.lib().fvm_mp_apply(nat.ptr(U), nat.ptr(f0), nat.ptr(W), nat.ptr(y[:, col:]), 2 * self.C,
+                                         nat.ptr(self._sc), nat.ptr(self.cell_mesh), self.N, self.C, nat.ptr(out),
+                                         nat.stream_ptr(self.device)), "fvm_mp_apply")
+
+    def predict(self, U, out_mid=None, out_end=None):
+        """Predicted midpoint and end states of the window from U (into the corrector's
+        buffers unless given), with f(U) and the window lengths."""
         X, f0, W, sc = self.features(U)
         y = self.net.forward(X, self.nbr)
-        nat.check(nat.lib().fvm_mp_apply(nat.ptr(U), nat.ptr(f0), nat.ptr(W), nat.ptr(y), self.C, nat.ptr(sc),
-                                         nat.ptr(self.cell_mesh), self.N, self.C, nat.ptr(out),
-                                         nat.stream_ptr(self.device)), "fvm_mp_apply")
-        return out, f0, W
+        Um = self._Um if out_mid is None else out_mid
+        U1 = self._U1 if out_end is None else out_end
+        self._apply(U, f0, 0.5 * W, y, 0, Um)
+        self._apply(U, f0, W, y, self.C, U1)
+        return Um, U1, f0, W
 
-    def defect(self, U0, U1, f0, W) -> np.ndarray:
-        """Per mesh, ||d|| / ||U1|| of the trapezoid defect over the mesh's whole state
-        (host array); per-variable ratios are kept in ``last_defect``."""
+    def defect(self, U0, Um, U1, f0, W) -> np.ndarray:
+        """Per mesh, max(||d||, ||e||) / ||U1|| of the Simpson and Hermite-midpoint
+        defects over the mesh's whole state (host array); per-variable ratios are kept
+        in ``last_defect``."""
+        self._rhs(Um, self._prim1, self._fm)
         self._rhs(U1, self._prim1, self._f1)
-        nat.check(nat.lib().fvm_mp_defect(nat.ptr(U0), nat.ptr(U1), nat.ptr(f0), nat.ptr(self._f1), nat.ptr(W),
-                                          nat.ptr(self.off), self.M, self.C, nat.ptr(self._def), nat.ptr(self._part),
-                                          nat.stream_ptr(self.device)), "fvm_mp_defect")
+        nat.check(nat.lib().fvm_mp_defect3(nat.ptr(U0), nat.ptr(Um), nat.ptr(U1), nat.ptr(f0), nat.ptr(self._fm),
+                                           nat.ptr(self._f1), nat.ptr(W), nat.ptr(self.off), self.M, self.C,
+                                           nat.ptr(self._def), nat.ptr(self._part), nat.stream_ptr(self.device)),
+                  "fvm_mp_defect3")
         d = self._def.cpu().numpy()
-        self.last_defect = np.sqrt(d[:, :self.C] / np.maximum(d[:, self.C:], 1e-300))
-        return np.sqrt(d[:, :self.C].sum(1) / np.maximum(d[:, self.C:].sum(1), 1e-300))
+        C = self.C
+        u2 = np.maximum(d[:, 2 * C:].sum(1), 1e-300)
+        self.last_defect = np.sqrt(d[:, :C] / np.maximum(d[:, 2 * C:], 1e-300))
+        return np.sqrt(np.maximum(d[:, :C].sum(1), d[:, C:2 * C].sum(1)) / u2)
 
     def advance(self, use_net: bool = True) -> bool:
         """One window: predict and check, or (if the check fails or use_net is False)
         k plain RK4 steps.  Returns True when the prediction was accepted."""
         if use_net:
             U0 = self.s.state
-            U1, f0, W = self.predict(U0, self._U1)
-            rel = self.defect(U0, U1, f0, W)
+            Um, U1, f0, W = self.predict(U0)
+            rel = self.defect(U0, Um, U1, f0, W)
             if np.all(rel <= self.tol):
                 self.s.set_state(U1)
                 self.n_accept += 1

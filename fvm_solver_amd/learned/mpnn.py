@@ -78,8 +78,10 @@ class MPNN:
         self.names = list(w)
         self.w = {k: v.detach().to(self.device, torch.float32).contiguous() for k, v in w.items()}
         self.wb = {k: v.to(_BF) for k, v in self.w.items() if k.startswith("W")}
-        # message-layer weights also kept transposed ([H][2H], bf16) for the fused layer
-        self.wt = {l: torch.empty(hidden, 2 * hidden, dtype=_BF, device=self.device) for l in range(layers)}
+        # every weight is also kept transposed ([out][in], bf16): K-contiguous B operands
+        # stage with 16-byte loads, and the fused layer reads W_l^T
+        self.wt = {k: torch.empty(v.shape[1], v.shape[0], dtype=_BF, device=self.device)
+                   for k, v in self.w.items() if k.startswith("W")}
         self._refresh_transposed()
         self.g = {k: torch.zeros_like(v) for k, v in self.w.items()}
         self.m = {k: torch.zeros_like(v) for k, v in self.w.items()}
@@ -100,10 +102,10 @@ class MPNN:
         self._refresh_transposed()
 
     def _refresh_transposed(self):
-        H = self.H
-        for l in range(self.L):
-            nat.check(nat.lib().fvm_transpose_bf16(nat.ptr(self.w[f"W_{l}"]), 2 * H, H, nat.ptr(self.wt[l]),
-                                                   nat.stream_ptr(self.device)), "fvm_transpose_bf16")
+        for k, t in self.wt.items():
+            r, c = self.w[k].shape
+            nat.check(nat.lib().fvm_transpose_bf16(nat.ptr(self.w[k]), r, c, nat.ptr(t), nat.stream_ptr(self.device)),
+                      "fvm_transpose_bf16")
 
     # ------------------------------------------------------------------ workspace
     def _work(self, M: int, train: bool):
@@ -148,13 +150,13 @@ class MPNN:
         ws = self._work(M, train)
         lib, sp = nat.lib(), nat.stream_ptr(self.device)
         Hs = ws["Hs"]
-        gemm(X, self.kx, 1, self.wb["W_in"], H, 1, M, H, self.kx, bias=self.w["b_in"],
+        gemm(X, self.kx, 1, self.wt["W_in"], 1, self.kx, M, H, self.kx, bias=self.w["b_in"],
              Cf=ws["Z"][0] if train else None, Cb=Hs[0], relu=True)
         for l in range(L):
             h_in = Hs[l] if train else Hs[l % 2]
             h_out = Hs[l + 1] if train else Hs[(l + 1) % 2]
             if H == 128:
-                nat.check(lib.fvm_mp_layer(nat.ptr(h_in), H, nat.ptr(nbr), M, nbr.shape[0], nat.ptr(self.wt[l]),
+                nat.check(lib.fvm_mp_layer(nat.ptr(h_in), H, nat.ptr(nbr), M, nbr.shape[0], nat.ptr(self.wt[f"W_{l}"]),
                                            nat.ptr(self.w[f"b_{l}"]), nat.ptr(h_out),
                                            nat.ptr(ws["Z"][l + 1]) if train else None,
                                            nat.ptr(ws["A"][l]) if train else None, sp), "fvm_mp_layer")
@@ -165,7 +167,7 @@ class MPNN:
             gemm(A, 2 * H, 1, self.wb[f"W_{l}"], H, 1, M, H, 2 * H, bias=self.w[f"b_{l}"],
                  Cf=ws["Z"][l + 1] if train else None, Cb=h_out, relu=True, res=h_in)
         h_last = Hs[L] if train else Hs[L % 2]
-        gemm(h_last, H, 1, self.wb["W_out"], self.n_out, 1, M, self.n_out, H, bias=self.w["b_out"], Cf=ws["Y"])
+        gemm(h_last, H, 1, self.wt["W_out"], 1, H, M, self.n_out, H, bias=self.w["b_out"], Cf=ws["Y"])
         self._last = (X, nbr, ws)
         return ws["Y"]
 
@@ -202,7 +204,7 @@ class MPNN:
                                        M * H, sp), "fvm_relu_bwd")
             gemm(ws["A"][l], 1, 2 * H, ws["dZb"], H, 1, 2 * H, H, M, Cf=self.g[f"W_{l}"], splits=s_mid, part=part)
             self._colsum(ws["dZf"], M, H, self.g[f"b_{l}"])
-            gemm(ws["dZb"], H, 1, self.wb[f"W_{l}"], 1, H, M, 2 * H, H, Cf=ws["dA"])
+            gemm(ws["dZb"], H, 1, self.wb[f"W_{l}"], 1, H, M, 2 * H, H, Cf=ws["dA"])  # B(k,j) = W_l[j][k]
             dH_new = ws["dH"][1] if dH is ws["dH"][0] else ws["dH"][0]
             nat.check(lib.fvm_mp_aggregate_bwd(nat.ptr(ws["dA"]), H, nat.ptr(nbr), M, nbr.shape[0], nat.ptr(dH),
                                                nat.ptr(dH_new), sp), "fvm_mp_aggregate_bwd")

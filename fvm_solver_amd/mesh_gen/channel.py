@@ -128,6 +128,25 @@ def _poisson_disk(rng: np.random.Generator, lo, hi, h: float, inside, fixed: np.
     return pts[~np.isnan(pts[:, 0])]
 
 
+def _jittered_lattice(rng: np.random.Generator, lo, hi, h: float, inside, fixed: np.ndarray,
+                      jitter: float = 0.15) -> np.ndarray:
+    """Points of a cubic lattice with the Poisson-disk sampler's density (spacing
+    1.16 h), each moved by a uniform jitter of +-jitter spacings, kept if admissible and
+    at least 0.6 h from every fixed (boundary) point.  O(n) and vectorised: the interior
+    sampler for large 3-D meshes."""
+    lo = np.asarray(lo, float)
+    hi = np.asarray(hi, float)
+    a = h / 0.64 ** (1.0 / 3.0)
+    axes = [np.arange(l + 0.5 * a, u - 0.25 * a, a) for l, u in zip(lo, hi)]
+    g = np.stack(np.meshgrid(*axes, indexing="ij"), -1).reshape(-1, len(lo))
+    g = g + (rng.random(g.shape) * 2.0 - 1.0) * (jitter * a)
+    g = g[inside(g) & np.all(g > lo + 0.3 * h, 1) & np.all(g < hi - 0.3 * h, 1)]
+    if len(fixed):
+        dmin, _ = cKDTree(fixed).query(g, k=1)
+        g = g[dmin >= 0.6 * h]
+    return g
+
+
 def _segment_points(a, b, h):
     a = np.asarray(a, float)
     b = np.asarray(b, float)
@@ -196,16 +215,25 @@ def _fibonacci_sphere(n: int) -> np.ndarray:
     return np.stack([np.cos(th) * np.sin(phi), np.sin(th) * np.sin(phi), np.cos(phi)], 1)
 
 
-def channel_mesh_3d(n_cells: int, seed: int = 1, spec: ChannelSpec | None = None, h: float | None = None):
-    """Tetrahedral mesh of [0,Lx]x[0,Ly]x[0,Lz] minus a ball, with about ``n_cells`` cells."""
+def channel_mesh_3d(n_cells: int, seed: int = 1, spec: ChannelSpec | None = None, h: float | None = None,
+                    sampler: str = "auto"):
+    """Tetrahedral mesh of [0,Lx]x[0,Ly]x[0,Lz] minus a ball, with about ``n_cells`` cells.
+
+    sampler: "poisson" (dart throwing, slow above a few 100k cells), "lattice" (jittered
+    cubic lattice inside, Poisson-disk on the faces) or "auto" (lattice above 200k)."""
+    if sampler == "auto":
+        sampler = "lattice" if n_cells > 200_000 else "poisson"
+    if sampler not in ("poisson", "lattice"):
+        raise ValueError(f"unknown sampler {sampler!r}")
     spec = spec or ChannelSpec(lengths=(2.0, 1.0, 1.0), centre=(0.5, 0.5, 0.5), radius=0.1)
     L = np.asarray(spec.lengths, float)
     c = np.asarray(spec.centre, float)
     R = spec.radius
     if h is None:
         vol = float(np.prod(L)) - 4.0 / 3.0 * math.pi * R ** 3
-        # ~0.64/h^3 points per unit volume for this sampler, ~6.3 tets per point.
-        h = (0.64 * 6.3 * vol / n_cells) ** (1.0 / 3.0)
+        # ~0.64/h^3 points per unit volume for this sampler, ~6.3 tets per point (the
+        # jittered lattice gives ~10 % more tets per point).
+        h = (0.64 * 6.3 * vol / n_cells) ** (1.0 / 3.0) * (1.031 if sampler == "lattice" else 1.0)
     rng = np.random.default_rng(seed)
 
     # Box edges, then face interiors (2-D Poisson disk per face), then the sphere.
@@ -238,7 +266,10 @@ def channel_mesh_3d(n_cells: int, seed: int = 1, spec: ChannelSpec | None = None
     def inside(p):
         return np.all(p > 0, 1) & np.all(p < L, 1) & (np.sum((p - c) ** 2, 1) > R * R)
 
-    interior = _poisson_disk(rng, (0, 0, 0), L, h, inside, fixed)
+    if sampler == "lattice":
+        interior = _jittered_lattice(rng, (0, 0, 0), L, h, inside, fixed)
+    else:
+        interior = _poisson_disk(rng, (0, 0, 0), L, h, inside, fixed)
     pts = np.concatenate([fixed, interior])
     # The box is the convex hull; joggled input keeps every coplanar face point a vertex.
     tri = Delaunay(pts, qhull_options="QJ Qbb")

@@ -302,6 +302,14 @@ int fvm_mesh_rms(const float *X, const float *win, const int64_t *off, int32_t n
 int fvm_mp_defect(const float *U0, const float *U1, const float *f0, const float *f1, const float *win,
                   const int64_t *off, int32_t n_meshes, int32_t C, double *out, double *part, void *stream);
 
+/* Simpson and Hermite-midpoint defects of a window predicted at its midpoint and end, per
+ * mesh: d = U1 - U0 - win/6 (f0 + 4 fm + f1), e = Um - (U0 + U1)/2 - win/8 (f0 - f1);
+ * out[mesh][j] = sum d_j^2, out[mesh][C + j] = sum e_j^2, out[mesh][2C + j] = sum U1_j^2
+ * (fp64); part is [n_meshes][64][3C] scratch. */
+int fvm_mp_defect3(const float *U0, const float *Um, const float *U1, const float *f0, const float *fm,
+                   const float *f1, const float *win, const int64_t *off, int32_t n_meshes, int32_t C,
+                   double *out, double *part, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -155,8 +155,9 @@ def test_training_lowers_loss():
 
 
 def test_fused_layer_matches_aggregate_then_gemm():
-    """fvm_mp_layer (aggregation inside the GEMM's tile staging) equals the two-kernel
-    form bit for bit: same operand values, same MFMA accumulation order."""
+    """fvm_mp_layer (aggregation fused into the GEMM's operand loads, features x cells
+    orientation) equals the two-kernel form: the aggregated operand bit for bit, the
+    pre-activation to fp32 accumulation order, the bf16 output to one rounding step."""
     nbr = _graph(2500, 9)
     NL, N = nbr.shape
     H = 128
@@ -181,5 +182,7 @@ def test_fused_layer_matches_aggregate_then_gemm():
     gemm(A2, 2 * H, 1, W.to(torch.bfloat16), H, 1, N, H, 2 * H, bias=b, Cf=Z2, Cb=h2, relu=True, res=h)
     torch.cuda.synchronize()
     assert torch.equal(A1, A2)
-    assert torch.equal(Z1, Z2)
-    assert torch.equal(h1, h2)
+    assert _rel(Z1, Z2) < 1e-6
+    diff = (h1.float() - h2.float()).abs()
+    assert float((diff > 0).float().mean()) < 1e-3
+    assert float(diff.max()) <= float(h2.float().abs().max()) * 2 ** -7

@@ -71,6 +71,12 @@ _PROTOS = {
     "fvm_state_to_prim_f32": (ctypes.c_int, [c_void_p, c_void_p, i64, i32, f32, c_void_p]),
     "fvm_conservation_f64": (ctypes.c_int, [c_void_p, c_void_p, i64, i32, c_void_p, c_void_p, c_void_p]),
     "fvm_cfl_dt": (ctypes.c_int, [c_void_p, c_void_p, i64, i32, f64, f64, c_void_p, c_void_p, c_void_p]),
+    "fvm_geom_facets_f64": (ctypes.c_int, [c_void_p, i32, c_void_p, i64, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "fvm_geom_cells_f64": (ctypes.c_int, [c_void_p, i32, c_void_p, i64, c_void_p, c_void_p, c_void_p]),
+    "fvm_geom_disp_signs_f64": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, i32, c_void_p, i64, c_void_p, c_void_p,
+                                               c_void_p]),
+    "fvm_geom_lsq_f64": (ctypes.c_int, [c_void_p, c_void_p, i32, c_void_p, c_void_p, c_void_p, i64, c_void_p,
+                                        c_void_p]),
     "fvm_gemm_bf16": (ctypes.c_int, [c_void_p, i64, i64, c_void_p, i64, i64, i64, i64, i64, i32, c_void_p,
                                      c_void_p, i64, c_void_p, i32, c_void_p, c_void_p, c_void_p]),
     "fvm_mp_aggregate": (ctypes.c_int, [c_void_p, i32, c_void_p, i64, i32, c_void_p, c_void_p]),

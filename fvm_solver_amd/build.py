@@ -24,6 +24,7 @@ SOURCES = {
     "rhs.hip": ["-ffp-contract=off"],
     "reduce.hip": [],
     "mpnn.hip": [],
+    "geom.hip": [],
 }
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang",

@@ -362,6 +362,9 @@ class FacetCalc:
         self.fvm_mesh = mesh_setup
         self.bc_type_str = [e.tag for e in bc_tags.values()]
         self._init(DeviceMesh(mesh_setup, n_comp, bc_tags, cfg, device), cfg, device)
+        self.fvm_mesh = mesh_setup            # the reference keeps it as E_props.mesh.fvm_mesh
+        self.mesh.fvm_mesh = mesh_setup
+        self.mesh.bc_facet_mask = mesh_setup.bc_facet_mask
 
     @classmethod
     def from_device_mesh(cls, dm: "DeviceMesh", cfg, device="cuda"):
@@ -397,6 +400,7 @@ class FacetCalc:
                                 red=self.red.data_ptr(), fflux=self.fflux.data_ptr())
         self.has_prev = False
         self.bc_ready_ptr = None     # prim buffer whose boundary sums bc_part holds
+        self.last_prim = None        # primitives of the last evaluation, if known (saving)
 
     # The reference keeps the previous call's limited gradients in ``cell_grads``.
     @property
@@ -421,13 +425,16 @@ class FacetCalc:
         rc = nat.lib().fvm_rhs_f32(self.mesh.struct, self.phys, self.work, nat.ptr(prim), nat.ptr(dudt),
                                    int(self.has_prev), nat.stream_ptr(self.device))
         nat.check(rc, "fvm_rhs_f32")
+        self.last_prim = prim
         self.has_prev = True
         self.bc_ready_ptr = None
         return dudt
 
-    def stage(self, st: "nat.FvmStage", parts: int | None = None):
+    def stage(self, st: "nat.FvmStage", parts: int | None = None, prim: torch.Tensor | None = None):
         """One fused RK stage; the boundary means are reused when the previous stage's
-        epilogue computed them for exactly this primitive buffer."""
+        epilogue computed them for exactly this primitive buffer.  ``prim`` (the tensor
+        behind st.prim) is remembered for saving."""
+        self.last_prim = prim
         st.has_prev = int(self.has_prev)
         st.bc_ready = int(self.bc_ready_ptr is not None and self.bc_ready_ptr == st.prim)
         if parts is None:

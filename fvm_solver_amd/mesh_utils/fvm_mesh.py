@@ -11,11 +11,13 @@
   weighted least-squares gradient G = A^-1 d^T W^2, w = |d|^-1/4 (x7.5 on boundary
   offsets) and the diagonal -sum_k G accumulated in fp32 in local order.
 
-Differences from the reference, by design: the work is vectorised (numpy, no per-cell
-Python loops, so 1M-4M cells take seconds), and every geometric quantity -- normals,
-volumes, centroids, displacements, the LSQ system and its inverse -- is computed in
-float64 from the given vertices and rounded to float32 only where the reference stores
-float32.  The float64 arrays are kept in ``geo64`` for the device-layout builder.
+Differences from the reference, by design: the work is vectorised (no per-cell Python
+loops, so 1M-4M cells take seconds), and every geometric quantity -- normals, volumes,
+centroids, displacements, the LSQ system and its inverse -- is computed in float64
+from the given vertices and rounded to float32 only where the reference stores float32.
+For a mesh on a GPU device the float64 arithmetic runs in the library's kernels
+(csrc/geom.hip); the integer topology (facet lookup, left/right ordering) stays in
+numpy.  The float64 arrays are kept in ``geo64`` for the device-layout builder.
 Quirk kept: the 3-D normal is the full cross product, |n| = 2 x facet area
 (fvm_mesh.py:418).
 """
@@ -66,22 +68,28 @@ class FVMMesh:
         d = self.dim
         nl = d + 1
 
-        # ---- geometry (fp64) ----
-        fv = V[Fv]
-        if d == 2:
-            e = fv[:, 1] - fv[:, 0]
-            n64 = np.stack([e[:, 1], -e[:, 0]], 1)
+        # ---- geometry (fp64; on the GPU when the mesh lives there) ----
+        gpu = _GpuGeometry(V, d, device) if _on_gpu(device) else None
+        if gpu is not None:
+            n64, nh64, mid64 = gpu.facets(Fv)
+            vol64, cen64 = gpu.cells(C)
         else:
-            n64 = np.cross(fv[:, 1] - fv[:, 0], fv[:, 2] - fv[:, 0])
-        mid64 = fv.mean(axis=1)
-        cp = V[C]
-        if d == 2:
-            ab, ac = cp[:, 1] - cp[:, 0], cp[:, 2] - cp[:, 0]
-            vol64 = 0.5 * np.abs(ab[:, 0] * ac[:, 1] - ab[:, 1] * ac[:, 0])
-        else:
-            a = cp[:, 0]
-            vol64 = np.abs(np.einsum("ij,ij->i", cp[:, 1] - a, np.cross(cp[:, 2] - a, cp[:, 3] - a))) / 6.0
-        cen64 = cp.mean(axis=1)
+            fv = V[Fv]
+            if d == 2:
+                e = fv[:, 1] - fv[:, 0]
+                n64 = np.stack([e[:, 1], -e[:, 0]], 1)
+            else:
+                n64 = np.cross(fv[:, 1] - fv[:, 0], fv[:, 2] - fv[:, 0])
+            mid64 = fv.mean(axis=1)
+            cp = V[C]
+            if d == 2:
+                ab, ac = cp[:, 1] - cp[:, 0], cp[:, 2] - cp[:, 0]
+                vol64 = 0.5 * np.abs(ab[:, 0] * ac[:, 1] - ab[:, 1] * ac[:, 0])
+            else:
+                a = cp[:, 0]
+                vol64 = np.abs(np.einsum("ij,ij->i", cp[:, 1] - a, np.cross(cp[:, 2] - a, cp[:, 3] - a))) / 6.0
+            cen64 = cp.mean(axis=1)
+            nh64 = n64 / np.linalg.norm(n64, axis=1, keepdims=True)
         if np.any(vol64 <= 0):
             raise ValueError("degenerate cell (zero volume)")
 
@@ -99,9 +107,11 @@ class FVMMesh:
         c2f = order[pos]
 
         # ---- orientation signs and left/right ordering (fvm_mesh.py:74-109) ----
-        nh64 = n64 / np.linalg.norm(n64, axis=1, keepdims=True)
-        disp64 = mid64[c2f] - cen64[:, None, :]
-        signs = np.sign(np.einsum("nkd,nkd->nk", nh64[c2f], disp64)).astype(np.int64)
+        if gpu is not None:
+            disp64, signs = gpu.disp_signs(c2f)
+        else:
+            disp64 = mid64[c2f] - cen64[:, None, :]
+            signs = np.sign(np.einsum("nkd,nkd->nk", nh64[c2f], disp64)).astype(np.int64)
         flat_f = c2f.reshape(-1)
         counts = np.bincount(flat_f, minlength=F)
         if np.any(counts == 0) or np.any(counts > 2):
@@ -127,13 +137,16 @@ class FVMMesh:
         other = np.where(lf == np.arange(N)[:, None], rf, lf)
         is_bc = bcm[c2f]
         combined = np.where(is_bc, bc_local[c2f] + N, other)
-        npts = np.where(is_bc[..., None], mid64[c2f], cen64[np.maximum(other, 0)])
-        dv = npts - cen64[:, None, :]
-        w = np.linalg.norm(dv, axis=2) ** -0.25
-        w[is_bc] *= 7.5
-        w2 = w * w
-        A = np.einsum("nk,nki,nkj->nij", w2, dv, dv)
-        G64 = np.einsum("nij,nkj,nk->nik", np.linalg.inv(A), dv, w2)      # [N, d, nl]
+        if gpu is not None:
+            G64 = gpu.lsq(c2f, other, is_bc)
+        else:
+            npts = np.where(is_bc[..., None], mid64[c2f], cen64[np.maximum(other, 0)])
+            dv = npts - cen64[:, None, :]
+            w = np.linalg.norm(dv, axis=2) ** -0.25
+            w[is_bc] *= 7.5
+            w2 = w * w
+            A = np.einsum("nk,nki,nkj->nij", w2, dv, dv)
+            G64 = np.einsum("nij,nkj,nk->nik", np.linalg.inv(A), dv, w2)      # [N, d, nl]
 
         self.geo64 = dict(normals=n64, normals_hat=nh64, midpoints=mid64, volumes=vol64, centroids=cen64,
                           cent_to_facet_disp=disp64, G=G64, left=left, right=right, bc_local=bc_local,
@@ -178,6 +191,64 @@ class FVMMesh:
                 G_mats.append(torch.sparse_coo_tensor(torch.stack([r, c]), v, (N, N + self.n_bc_facet)))
             self._grad_stuff = (cell_disps, dist, G_mats, comb)
         return self._grad_stuff
+
+
+def _on_gpu(device) -> bool:
+    try:
+        return torch.device(device).type == "cuda" and torch.cuda.is_available()
+    except (RuntimeError, TypeError):
+        return False
+
+
+class _GpuGeometry:
+    """The mesh's fp64 geometry computed by the library's kernels (csrc/geom.hip); the
+    results come back as numpy arrays for the host-side layout builder."""
+
+    def __init__(self, V: np.ndarray, dim: int, device):
+        from fvm_solver_amd import _native as nat
+        self.nat, self.dim, self.dev = nat, dim, torch.device(device)
+        self.V = torch.from_numpy(np.ascontiguousarray(V, np.float64)).to(self.dev)
+
+    def _t(self, a, dt=torch.int64):
+        return torch.from_numpy(np.ascontiguousarray(a)).to(self.dev, dt)
+
+    def facets(self, Fv):
+        nat, F, d = self.nat, Fv.shape[0], self.dim
+        fv = self._t(Fv)
+        n, nh, mid = (torch.empty(F, d, dtype=torch.float64, device=self.dev) for _ in range(3))
+        nat.check(nat.lib().fvm_geom_facets_f64(nat.ptr(self.V), d, nat.ptr(fv), F, nat.ptr(n), nat.ptr(nh),
+                                                nat.ptr(mid), nat.stream_ptr(self.dev)), "fvm_geom_facets_f64")
+        self.mid, self.nh = mid, nh
+        return n.cpu().numpy(), nh.cpu().numpy(), mid.cpu().numpy()
+
+    def cells(self, Cv):
+        nat, N, d = self.nat, Cv.shape[0], self.dim
+        cv = self._t(Cv)
+        vol = torch.empty(N, dtype=torch.float64, device=self.dev)
+        cen = torch.empty(N, d, dtype=torch.float64, device=self.dev)
+        nat.check(nat.lib().fvm_geom_cells_f64(nat.ptr(self.V), d, nat.ptr(cv), N, nat.ptr(vol), nat.ptr(cen),
+                                               nat.stream_ptr(self.dev)), "fvm_geom_cells_f64")
+        self.cen = cen
+        return vol.cpu().numpy(), cen.cpu().numpy()
+
+    def disp_signs(self, c2f):
+        nat, (N, nl), d = self.nat, c2f.shape, self.dim
+        self.c2f = self._t(c2f)
+        disp = torch.empty(N, nl, d, dtype=torch.float64, device=self.dev)
+        sg = torch.empty(N, nl, dtype=torch.int64, device=self.dev)
+        nat.check(nat.lib().fvm_geom_disp_signs_f64(nat.ptr(self.cen), nat.ptr(self.mid), nat.ptr(self.nh), d,
+                                                    nat.ptr(self.c2f), N, nat.ptr(disp), nat.ptr(sg),
+                                                    nat.stream_ptr(self.dev)), "fvm_geom_disp_signs_f64")
+        return disp.cpu().numpy(), sg.cpu().numpy()
+
+    def lsq(self, c2f, other, is_bc):
+        nat, (N, nl), d = self.nat, c2f.shape, self.dim
+        G = torch.empty(N, d, nl, dtype=torch.float64, device=self.dev)
+        ot, bc = self._t(other), self._t(is_bc.astype(np.uint8), torch.uint8)
+        nat.check(nat.lib().fvm_geom_lsq_f64(nat.ptr(self.cen), nat.ptr(self.mid), d, nat.ptr(self.c2f),
+                                             nat.ptr(ot), nat.ptr(bc), N, nat.ptr(G), nat.stream_ptr(self.dev)),
+                  "fvm_geom_lsq_f64")
+        return G.cpu().numpy()
 
 
 class FVMMesh2D(FVMMesh):

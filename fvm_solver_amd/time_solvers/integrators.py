@@ -112,7 +112,7 @@ class _FusedButcher(TSolver):
                 for q, (j, c) in enumerate(terms):
                     st.inc_idx[q] = j
                     st.inc_coef[q] = c
-            E.stage(st)
+            E.stage(st, prim=cur)
             self._flip = 1 - self._flip
             cur = nxt
         if self.adaptive:
@@ -179,6 +179,7 @@ class _AdaptiveHost:
 class RK3_SSP4(TSolver, _AdaptiveHost):
     """Hand-coded 4-stage SSP RK3 with the (U_b - U_0) error proxy (integrators.py:65-94).
     The RHS is the fused HIP residual; the stage combinations are GPU tensor ops."""
+    adaptive = True
 
     def __init__(self, cells, equation, cfg):
         super().__init__(cells, equation, cfg)
@@ -197,6 +198,7 @@ class RK3_SSP4(TSolver, _AdaptiveHost):
 class Adams3PC(TSolver, _AdaptiveHost):
     """Adams-Bashforth-Moulton 3 predictor-corrector (integrators.py:97-154) with the
     intended forward(prim) calls."""
+    adaptive = True
 
     def __init__(self, cells, equation, cfg):
         super().__init__(cells, equation, cfg)
@@ -275,7 +277,7 @@ class AB3(_FusedButcher):
                           f_m1=self._f[self._hist[0]].data_ptr() if len(self._hist) >= 1 else None,
                           f_m2=self._f[self._hist[1]].data_ptr() if len(self._hist) >= 2 else None,
                           f_out=self._f[free].data_ptr())
-        E.stage(st)
+        E.stage(st, prim=cur)
         self._flip = 1 - self._flip
         self._hist = [free] + self._hist[:1]
         self._last_out = out

@@ -217,6 +217,32 @@ int fvm_cfl_dt(const float *prim, const double *h, int64_t N, int32_t dim, doubl
                double *out, double *partial /* [256] */, void *stream);
 
 /* ---------------------------------------------------------------------------------
+ * FP64 geometric precompute (FVMMesh2D/3D, fvm_mesh.py:131-261, 263-344, 367-582).
+ * Vertices V [nv][dim] fp64; cells [N][dim+1] and facets [F][dim] int64 vertex ids.
+ * --------------------------------------------------------------------------------- */
+
+/* normals (2-D (dy, -dx) of v1 - v0; 3-D (v1 - v0) x (v2 - v0), |n| = 2 x area),
+ * unit normals and midpoints, [F][dim] each. */
+int fvm_geom_facets_f64(const double *V, int32_t dim, const int64_t *facets, int64_t F, double *normals,
+                        double *normals_hat, double *midpoints, void *stream);
+
+/* cell volumes [N] and centroids [N][dim] (vertex mean). */
+int fvm_geom_cells_f64(const double *V, int32_t dim, const int64_t *cells, int64_t N, double *volumes,
+                       double *centroids, void *stream);
+
+/* disp[c][k] = midpoint(f) - centroid(c), signs[c][k] = sign(n_hat(f) . disp) for
+ * f = cell_to_facet[c][k] ([N][dim+1]). */
+int fvm_geom_disp_signs_f64(const double *centroids, const double *midpoints, const double *normals_hat,
+                            int32_t dim, const int64_t *cell_to_facet, int64_t N, double *disp, int64_t *signs,
+                            void *stream);
+
+/* Weighted LSQ gradient matrices G [N][dim][dim+1]: offsets to the neighbour centroid
+ * (other[c][k]) or, where is_bc[c][k], to the facet midpoint; weights |d|^(-1/4), x 7.5
+ * on boundary facets; G = (sum w^2 d d^T)^(-1) [w^2 d_k]. */
+int fvm_geom_lsq_f64(const double *centroids, const double *midpoints, int32_t dim, const int64_t *cell_to_facet,
+                     const int64_t *other, const uint8_t *is_bc, int64_t N, double *G, void *stream);
+
+/* ---------------------------------------------------------------------------------
  * Learned corrector on the mesh graph (no reference counterpart: the reference has no
  * neural network).  A message-passing network predicts the state a window of k solver
  * steps ahead; these are the kernels of its forward pass, backward pass and Adam update.

@@ -657,6 +657,7 @@ class OracleFVM:
         Ufa = torch.empty((F, 2, C), dtype=dt_)
         Ufa[self.cell_to_facet, self.slots.long()] = Us_face
         Ufa[self.bc_locations, self.bc_facet_side.long()] = U_bc
+        self.last_face = Ufa                 # the reference's E_props.*_facet (Saver input)
         Vs = Ufa[:, :, :dim].contiguous()
         rho = Ufa[:, :, dim].unsqueeze(-1).contiguous()
         T = Ufa[:, :, dim + 1].unsqueeze(-1).contiguous()

@@ -28,9 +28,11 @@ def _parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--config", choices=("channel2d", "channel3d"), default="channel2d",
+    ap.add_argument("--config", choices=("channel2d", "channel3d", "batch"), default="channel2d",
                     help="channel2d: the 1 M-cell 2-D channel (the headline); channel3d: the 3-D channel "
-                         "(tetrahedra, C = 5; default 4 M cells)")
+                         "(tetrahedra, C = 5; default 4 M cells); batch: 64 random 2-D channel meshes of "
+                         "5k-200k cells in one block-diagonal layout, RK4 steps replayed as a HIP graph")
+    ap.add_argument("--meshes", type=int, default=64, help="meshes of the batch configuration")
     ap.add_argument("--cells", type=int, default=None, help="cells (default 1 M in 2-D, 4 M in 3-D)")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--cpu-steps", type=int, default=30)
@@ -285,6 +287,86 @@ def main():
     print(json.dumps(out), flush=True)
 
 
+def batch_main(a):
+    """Dataset-generation throughput: a batch of independent channel meshes stepped
+    together (one launch per kernel for the whole batch), RK4 steps captured as a HIP
+    graph; per-kernel durations from part-wise launches of the same stages."""
+    from fvm_solver_amd import _native as nat
+    from fvm_solver_amd.batch import BatchedSolver, random_channel_batch
+    dev = torch.device("cuda:0")
+    t0 = time.time()
+    problems, params = random_channel_batch(a.meshes, seed=a.seed, n_min=5000, n_max=200_000, device=dev)
+    s = BatchedSolver(problems, device=dev)
+    dts = s.cfl_dt(0.5)
+    s.set_dt(dts)
+    N, F, Fb = s.dm.N, s.dm.F, s.dm.Fb
+    log(f"batch: {a.meshes} meshes, {N} cells, {F} facets ({Fb} boundary), set-up {time.time() - t0:.1f}s")
+    for _ in range(a.warmup):
+        s.step()
+    n_graph = 10
+    g = s.capture_graph(n_graph)
+    steps = max(n_graph, (a.steps // n_graph) * n_graph)
+    g.replay()
+    torch.cuda.synchronize()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record()
+    for _ in range(steps // n_graph):
+        g.replay()
+    ev1.record()
+    torch.cuda.synchronize()
+    gpu_ms = ev0.elapsed_time(ev1)
+    ms_per_step = gpu_ms / steps
+    value = N * steps / (gpu_ms / 1e3)
+    finite = bool(torch.isfinite(s.state).all())
+    log(f"{steps} steps (graph of {n_graph}): {gpu_ms:.2f} ms, {ms_per_step * 1e3:.1f} us/step, finite={finite}")
+    # per-kernel durations of the same stages, launched part by part
+    E = s.E
+    parts = {1: "k_bc_mean", 2: "k_grad", 4: "k_facet_flux", 8: "k_div"}
+    dur = {k: [] for k in parts.values()}
+    s._ensure_prim()
+    for _ in range(3):
+        U0 = s._U[s._cur]
+        out = s._U[1 - s._cur]
+        cur = s._prim[s._flip]
+        for i in range(s.stages):
+            nxt = s._prim[1 - s._flip]
+            st = nat.FvmStage(prim=cur.data_ptr(), U0=U0.data_ptr(), acc=s._acc.data_ptr(), acc2=None, kbuf=None,
+                              prim_next=nxt.data_ptr(), U_out=out.data_ptr(), dt=s.dt.data_ptr(), stage=i,
+                              last=int(i == s.stages - 1), store_k=0, b=float(s.b[i]), b2=float(s.b2[i]))
+            if i < s.stages - 1 and s.A[i + 1][i] != 0:
+                st.n_inc, st.inc_idx[0], st.inc_coef[0] = 1, i, float(s.A[i + 1][i])
+            for mask, name in parts.items():
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record()
+                E.stage(st, parts=mask)
+                e1.record()
+                dur[name].append((e0, e1))
+            s._flip = 1 - s._flip
+            cur = nxt
+        s._cur = 1 - s._cur
+    torch.cuda.synchronize()
+    avg_us = {k: 1e3 * float(np.mean([x.elapsed_time(y) for x, y in v])) for k, v in dur.items()}
+    kbytes = {"k_grad": float(grad_kernel_bytes(N, Fb, 2)), "k_facet_flux": float(facet_kernel_bytes(N, F, Fb, 2)),
+              "k_div": float(np.mean([div_kernel_bytes(N, F, 2, i) for i in range(4)])),
+              "k_bc_mean": float(max(E.mesh.n_slots, 1) * 256 * 4 * 2)}
+    kgbs = {k: kbytes[k] / (avg_us[k] * 1e-6) / 1e9 for k in avg_us}
+    dom = max(avg_us, key=avg_us.get)
+    log("kernel averages (us): " + ", ".join(f"{k}={v:.1f}" for k, v in avg_us.items()))
+    out = {
+        "metric": "cell-updates/s (batched 2-D NS channels, HIP-graph RK4)",
+        "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": steps, "warmup": a.warmup,
+        "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic (seeded random channel meshes, freestream + Gaussian bump)",
+        "config": {"workload": f"{a.meshes} independent 2-D channel meshes (cells log-uniform 5k-200k, obstacle "
+                               "x ~ U(0.3, 0.8), r ~ U(0.05, 0.15), Mach ~ U(0.1, 0.6)), RK4 at CFL 0.5",
+                   "cells": N, "facets": F, "boundary_facets": Fb, "finite": finite, "graph_steps": n_graph,
+                   "kernel_avg_us": avg_us, "kernel_gbs": kgbs},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": kgbs[dom], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": kgbs[dom] / HBM_PEAK_GBS, "traffic": None},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def _oracle_phys(cfg):
     sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
     from oracle import fvm_oracle as O
@@ -402,5 +484,10 @@ if __name__ == "__main__":
             print("bench.py runs on one GPU only", file=sys.stderr)
             sys.exit(2)
         learned_main(_a)
+    elif _a.config == "batch":
+        if _a.gpus != 1:
+            print("bench.py runs on one GPU only", file=sys.stderr)
+            sys.exit(2)
+        batch_main(_a)
     else:
         main()

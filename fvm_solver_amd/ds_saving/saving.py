@@ -46,7 +46,6 @@ class Saver:
         self.D, self.C = E_props.dim, E_props.n_comp
         h = dm.host
         bcell = h["bcell"]
-        side = h["side"]                              # the free slot; the cell holds 1 - side
         fid = np.flatnonzero(bcm)
         c2f = mesh.cell_to_facet.numpy()
         k = np.argmax(c2f[bcell] == fid[:, None], axis=1)
@@ -58,7 +57,6 @@ class Saver:
         self._worker = threading.Thread(target=self._write_loop, daemon=True)
         self._worker.start()
         self._errors: list = []
-        del side
 
     # ------------------------------------------------------------------ device side
     def bc_values(self, E_props, prim):

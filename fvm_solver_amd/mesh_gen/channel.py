@@ -269,7 +269,14 @@ def channel_mesh_3d(n_cells: int, seed: int = 1, spec: ChannelSpec | None = None
     if sampler == "lattice":
         interior = _jittered_lattice(rng, (0, 0, 0), L, h, inside, fixed)
     else:
-        interior = _poisson_disk(rng, (0, 0, 0), L, h, inside, fixed)
+        # Face points are only h apart at the least, so an interior point h from all of
+        # them can still sit almost on the face and span a sliver with three of them:
+        # keep interior points half a spacing off the walls and the sphere.
+        def inside_off_walls(p):
+            return (np.all(p > 0.5 * h, 1) & np.all(p < L - 0.5 * h, 1)
+                    & (np.sum((p - c) ** 2, 1) > (R + 0.5 * h) ** 2))
+
+        interior = _poisson_disk(rng, (0, 0, 0), L, h, inside_off_walls, fixed)
     pts = np.concatenate([fixed, interior])
     # The box is the convex hull; joggled input keeps every coplanar face point a vertex.
     tri = Delaunay(pts, qhull_options="QJ Qbb")

@@ -87,6 +87,22 @@ def div_kernel_bytes(N, F, D, stage):
     return N * 4 * (NL + 1 + 1 + C + (C if stage > 0 else 0) + 2 * C) + F * 4 * C
 
 
+def fused_kernel_bytes(N, F, Fb, D, stage, tiles):
+    """k_fused (tile-fused stage), every array it needs read or written once: per cell
+    prim, LSQ values, displacements, eps, info, the local neighbour row and tile-local
+    facet row (4 x u16 each), 1/vol and the stage epilogue's traffic (as k_div); per
+    facet the compact geometry (n, n_hat, |n|, 1/(n.d), X) and the descriptor; per
+    boundary facet its BC data, boundary value written and the interior cell's gradient
+    record read and written; the tile table.  Halo re-reads (tile + H1 + H2 staging)
+    are not algorithmic: they show in the measured traffic."""
+    C, NL = D + 2, D + 1
+    per_cell = C + (NL + 1) * D + NL * D + 1 + 1 + 2 + 2 + 1
+    dyn = C + (C if stage > 0 else 0) + 2 * C
+    per_facet = (3 * D + 2) + 1
+    per_bc = 1 + C + D + 2 * C + (2 * D + 3) + 2 + C + 2 * (D + 1) * C
+    return 4 * (N * (per_cell + dyn) + F * per_facet + Fb * per_bc + tiles * 12)
+
+
 def grad_kernel_bytes(N, Fb, D):
     """k_grad: prim, neighbour list, info, LSQ values ((NL+1) D), displacements, eps;
     writes the gradient record; boundary values read once."""
@@ -151,9 +167,10 @@ def main():
     tuned = None
     if not a.no_autotune:
         from fvm_solver_amd import autotune
-        tuned = autotune.tune(lambda: ts._solve_step(None), eq.E_props.mesh, N, D)
-        log(f"autotune: persist_blocks={tuned['persist_blocks']} "
-            + ", ".join(f"{k}:{v:.0f}us" for k, v in tuned["us_per_step"].items()))
+        tuned = autotune.tune(lambda: ts._solve_step(None), eq.E_props, N, D)
+        log(f"autotune: {tuned['choice']} " + ", ".join(f"{k}:{v:.0f}us" for k, v in tuned["us_per_step"].items()))
+    elif eq.E_props.mesh.tiles is None:
+        log(f"tile-fused stage unavailable: {eq.E_props.mesh.tiles_reason}")
     torch.cuda.synchronize()
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     wall0 = time.perf_counter()
@@ -180,9 +197,13 @@ def main():
 
     # ---- per-kernel durations: the same stages, launched part by part between events ----
     E = eq.E_props
-    parts = {1: "k_bc_mean", 2: "k_grad", 4: "k_facet_flux", 8: "k_div"}
+    if E.fused:
+        parts = {1: "k_bc_mean", 16: "k_fused"}
+    else:
+        parts = {1: "k_bc_mean", 2: "k_grad", 4: "k_facet_flux", 8: "k_div"}
     dur = {k: [] for k in parts.values()}
-    div_bytes = []
+    div_bytes, fused_bytes = [], []
+    n_tiles = (N + 255) // 256
     U = eq.cells.state
     st_prim = ts._stage_prim0(U)
     s = ts.stages
@@ -206,6 +227,7 @@ def main():
                 e1.record()
                 dur[name].append((e0, e1))
             div_bytes.append(div_kernel_bytes(N, F, D, i))
+            fused_bytes.append(fused_kernel_bytes(N, F, Fb, D, i, n_tiles))
             ts._flip = 1 - ts._flip
             cur = nxt
         ts._last_out = out
@@ -216,12 +238,13 @@ def main():
     avg_us = {k: 1e3 * float(np.mean([e0.elapsed_time(e1) for e0, e1 in v])) for k, v in dur.items()}
     log("kernel averages (us): " + ", ".join(f"{k}={v:.1f}" for k, v in avg_us.items()))
     kbytes = {"k_grad": float(grad_kernel_bytes(N, Fb, D)), "k_facet_flux": float(facet_kernel_bytes(N, F, Fb, D)),
-              "k_div": float(np.mean(div_bytes)), "k_bc_mean": float(max(E.mesh.n_slots, 1) * 256 * 4 * 2)}
+              "k_div": float(np.mean(div_bytes)), "k_fused": float(np.mean(fused_bytes)),
+              "k_bc_mean": float(max(E.mesh.n_slots, 1) * 256 * 4 * 2)}
     kgbs = {k: kbytes[k] / (avg_us[k] * 1e-6) / 1e9 for k in avg_us}
     dom = max(avg_us, key=avg_us.get)
     dom_bytes = kbytes[dom]
     achieved = kgbs[dom]
-    stage_bytes = kbytes["k_grad"] + kbytes["k_facet_flux"] + kbytes["k_div"]
+    stage_bytes = kbytes["k_fused"] if E.fused else kbytes["k_grad"] + kbytes["k_facet_flux"] + kbytes["k_div"]
     log(f"dominant {dom}: {dom_bytes / 1e6:.1f} MB/launch -> {achieved:.0f} GB/s; per kernel GB/s: "
         + ", ".join(f"{k}={v:.0f}" for k, v in kgbs.items()) + f"; {stage_bytes / N:.0f} B/cell/stage")
 
@@ -241,12 +264,15 @@ def main():
         "config": {"workload": ("3-D compressible NS channel with spherical obstacle (tetrahedra)" if D == 3 else
                                 "2-D compressible NS channel with circular obstacle") + ", RK4 at CFL 0.5",
                    "cells": N, "facets": F, "boundary_facets": Fb, "integrator": "Butcher RK4",
-                   "dt": dt, "finite": finite, "kernel_avg_us": avg_us,
-                   "kernel_algorithmic_bytes": kbytes, "kernel_gbs": kgbs},
+                   "dt": dt, "finite": finite, "pipeline": "tile-fused stage" if E.fused else "three kernels",
+                   "kernel_avg_us": avg_us,
+                   "kernel_algorithmic_bytes": {k: kbytes[k] for k in avg_us}, "kernel_gbs": kgbs},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None},
     }
     out["config"]["persist_blocks"] = tuned["persist_blocks"] if tuned else 0
+    if tuned:
+        out["config"]["autotune_us_per_step"] = tuned["us_per_step"]
     out["config"]["algorithmic_bytes_per_launch"] = dom_bytes
     if a.traffic_file and os.path.exists(a.traffic_file):
         tr = json.load(open(a.traffic_file))
@@ -303,6 +329,10 @@ def batch_main(a):
     log(f"batch: {a.meshes} meshes, {N} cells, {F} facets ({Fb} boundary), set-up {time.time() - t0:.1f}s")
     for _ in range(a.warmup):
         s.step()
+    if not a.no_autotune:
+        from fvm_solver_amd import autotune
+        tuned = autotune.tune(s.step, s.E, N, 2)
+        log(f"autotune: {tuned['choice']} " + ", ".join(f"{k}:{v:.0f}us" for k, v in tuned["us_per_step"].items()))
     n_graph = 10
     g = s.capture_graph(n_graph)
     steps = max(n_graph, (a.steps // n_graph) * n_graph)
@@ -321,7 +351,10 @@ def batch_main(a):
     log(f"{steps} steps (graph of {n_graph}): {gpu_ms:.2f} ms, {ms_per_step * 1e3:.1f} us/step, finite={finite}")
     # per-kernel durations of the same stages, launched part by part
     E = s.E
-    parts = {1: "k_bc_mean", 2: "k_grad", 4: "k_facet_flux", 8: "k_div"}
+    if E.fused:
+        parts = {1: "k_bc_mean", 16: "k_fused"}
+    else:
+        parts = {1: "k_bc_mean", 2: "k_grad", 4: "k_facet_flux", 8: "k_div"}
     dur = {k: [] for k in parts.values()}
     s._ensure_prim()
     for _ in range(3):
@@ -348,6 +381,7 @@ def batch_main(a):
     avg_us = {k: 1e3 * float(np.mean([x.elapsed_time(y) for x, y in v])) for k, v in dur.items()}
     kbytes = {"k_grad": float(grad_kernel_bytes(N, Fb, 2)), "k_facet_flux": float(facet_kernel_bytes(N, F, Fb, 2)),
               "k_div": float(np.mean([div_kernel_bytes(N, F, 2, i) for i in range(4)])),
+              "k_fused": float(np.mean([fused_kernel_bytes(N, F, Fb, 2, i, (N + 255) // 256) for i in range(4)])),
               "k_bc_mean": float(max(E.mesh.n_slots, 1) * 256 * 4 * 2)}
     kgbs = {k: kbytes[k] / (avg_us[k] * 1e-6) / 1e9 for k in avg_us}
     dom = max(avg_us, key=avg_us.get)
@@ -360,6 +394,7 @@ def batch_main(a):
         "config": {"workload": f"{a.meshes} independent 2-D channel meshes (cells log-uniform 5k-200k, obstacle "
                                "x ~ U(0.3, 0.8), r ~ U(0.05, 0.15), Mach ~ U(0.1, 0.6)), RK4 at CFL 0.5",
                    "cells": N, "facets": F, "boundary_facets": Fb, "finite": finite, "graph_steps": n_graph,
+                   "pipeline": "tile-fused stage" if E.fused else "three kernels",
                    "kernel_avg_us": avg_us, "kernel_gbs": kgbs},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": kgbs[dom], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": kgbs[dom] / HBM_PEAK_GBS, "traffic": None},

@@ -12,7 +12,7 @@ import torch
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libfvm_hip.so")
-ABI_VERSION = 5
+ABI_VERSION = 6
 
 FVM_ERR_ROW_OVERFLOW = 2
 
@@ -31,7 +31,11 @@ class FvmMeshDev(ctypes.Structure):
                 ("n_slots", i32), ("n_meshes", i32), ("blk_slot", c_void_p), ("slot_blk", c_void_p),
                 ("slot_g0", c_void_p), ("slot_ng", c_void_p), ("group_s0", c_void_p), ("group_s1", c_void_p),
                 ("group_count", c_void_p), ("cell_mesh", c_void_p), ("mesh_phys", c_void_p),
-                ("persist_blocks", i32), ("pad1", i32)]
+                ("persist_blocks", i32), ("pad1", i32),
+                ("fz_tile", c_void_p), ("fz_halo", c_void_p), ("fz_hrec", c_void_p), ("fz_nbr", c_void_p),
+                ("fz_fdesc", c_void_p), ("fz_ffac", c_void_p), ("fz_fdesc_x", c_void_p), ("fz_geo", c_void_p),
+                ("fz_geo_x", c_void_p), ("fz_cfac", c_void_p), ("fz_ncl", i32), ("fz_ngl", i32), ("fz_nfl", i32),
+                ("fz_pad", i32)]
 
 
 class FvmPhys(ctypes.Structure):
@@ -41,7 +45,7 @@ class FvmPhys(ctypes.Structure):
 
 class FvmWork(ctypes.Structure):
     _fields_ = [("ubc", c_void_p), ("gp", c_void_p), ("bc_part", c_void_p), ("red", c_void_p),
-                ("fflux", c_void_p)]
+                ("fflux", c_void_p), ("gp_prev", c_void_p)]
 
 
 class FvmStage(ctypes.Structure):
@@ -66,6 +70,8 @@ _PROTOS = {
                                         ctypes.POINTER(FvmWork), ctypes.POINTER(FvmStage), c_void_p]),
     "fvm_rk_stage_part_f32": (ctypes.c_int, [ctypes.POINTER(FvmMeshDev), ctypes.POINTER(FvmPhys),
                                              ctypes.POINTER(FvmWork), ctypes.POINTER(FvmStage), i32, c_void_p]),
+    "fvm_grad_f32": (ctypes.c_int, [ctypes.POINTER(FvmMeshDev), ctypes.POINTER(FvmPhys), ctypes.POINTER(FvmWork),
+                                    c_void_p, i32, c_void_p]),
     "fvm_update_stepsize": (ctypes.c_int, [ctypes.POINTER(FvmWork), i32, i32, f32, f32, f32, f32, f32, f32,
                                            c_void_p, c_void_p]),
     "fvm_state_to_prim_f32": (ctypes.c_int, [c_void_p, c_void_p, i64, i32, f32, c_void_p]),

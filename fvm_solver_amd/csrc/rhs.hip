@@ -221,7 +221,7 @@ __device__ __forceinline__ void bc_value(const FvmMeshDev &m, const FvmPhys &p, 
 // the kernel boundary orders the writes before the reads.
 template <int D>
 __device__ void bc_mean_block(const FvmMeshDev &m, const FvmPhys &p, const FvmWork &w, int64_t c, bool live,
-                              float T, int slot) {
+                              float T, int slot, float *red) {
     constexpr int NL = D + 1;
     const int64_t N = m.n_cells;
     const int g0 = m.slot_g0[slot], ng = m.slot_ng[slot];
@@ -238,7 +238,6 @@ __device__ void bc_mean_block(const FvmMeshDev &m, const FvmPhys &p, const FvmWo
             }
         }
     }
-    __shared__ float red[kThreads];
     for (int q = 0; q < ng; ++q) {
         float v = cs[0];
 #pragma unroll
@@ -254,13 +253,12 @@ __device__ void bc_mean_block(const FvmMeshDev &m, const FvmPhys &p, const FvmWo
     }
 }
 
-// Group means of the tile's slot (block-uniform call): mean[q] for groups slot_g0 + q,
-// each the sum of its mesh's slot partials in a fixed strided-then-tree order (the same
-// bits in every block) divided by the group's facet count (facet_boundary.py:427).
-__device__ void block_group_means(const FvmMeshDev &m, const FvmWork &w, int slot, float *gmean) {
-    __shared__ float red[kThreads];
+// Group means (block-uniform call): mean[q] for groups g0 + q, q < ng, each the sum of
+// its mesh's slot partials in a fixed strided-then-tree order (the same bits in every
+// block) divided by the group's facet count (facet_boundary.py:427).
+__device__ void block_group_means_g(const FvmMeshDev &m, const FvmWork &w, int g0, int ng, float *gmean,
+                                    float *red) {
     __syncthreads();                       // the previous tile's readers are done
-    const int g0 = m.slot_g0[slot], ng = m.slot_ng[slot];
     for (int q = 0; q < ng; ++q) {
         const int g = g0 + q;
         float v = 0.0f;
@@ -277,6 +275,11 @@ __device__ void block_group_means(const FvmMeshDev &m, const FvmWork &w, int slo
     }
 }
 
+// Group means of a tile's boundary-mean slot: the groups of the meshes it touches.
+__device__ void block_group_means(const FvmMeshDev &m, const FvmWork &w, int slot, float *gmean, float *red) {
+    block_group_means_g(m, w, m.slot_g0[slot], m.slot_ng[slot], gmean, red);
+}
+
 // Pre-pass: boundary means of `prim` when no previous stage produced them.
 template <int D>
 __global__ void __launch_bounds__(kThreads) k_bc_mean(FvmMeshDev m, FvmPhys p, FvmWork w,
@@ -284,12 +287,62 @@ __global__ void __launch_bounds__(kThreads) k_bc_mean(FvmMeshDev m, FvmPhys p, F
     const int64_t c = static_cast<int64_t>(m.slot_blk[blockIdx.x]) * kThreads + threadIdx.x;
     const bool live = c < m.n_cells;
     const float T = live ? prim[c * (D + 2) + D + 1] : 0.0f;
-    bc_mean_block<D>(m, p, w, c, live, T, blockIdx.x);
+    __shared__ float red[kThreads];
+    bc_mean_block<D>(m, p, w, c, live, T, blockIdx.x, red);
 }
 
 // ------------------------------------------------------------------------------------
 // LSQ gradient + slope limiter
 // ------------------------------------------------------------------------------------
+// Gradient G.[U; U_bc] of one cell (G_mats row, columns in ascending order; bits
+// 16+3j of `info` name the source of column j: neighbour slot k, or 7 = the cell) and
+// the limiter factor phi = min over the cell's facets (facet_process.py:387-418,
+// limiter.py).  P: the cell's primitives; Un: the values behind its facets (neighbour
+// primitives, or boundary values).
+template <int D, int LP>
+__device__ __forceinline__ void cell_gradient(const float (&P)[D + 2], const float (&Un)[D + 1][D + 2], int info,
+                                              const float (&gv)[D + 2][D], const float (&disp)[D + 1][D], float eps,
+                                              float (&g)[D][D + 2], float (&phi)[D + 2]) {
+    constexpr int C = D + 2, NL = D + 1;
+#pragma unroll
+    for (int jj = 0; jj <= NL; ++jj) {
+        const int sel = (info >> (16 + 3 * jj)) & 7;
+        float U[C];
+#pragma unroll
+        for (int j = 0; j < C; ++j) {
+            float v = P[j];
+#pragma unroll
+            for (int k = 0; k < NL; ++k) v = (sel == k) ? Un[k][j] : v;
+            U[j] = v;
+        }
+#pragma unroll
+        for (int i = 0; i < D; ++i) {
+#pragma unroll
+            for (int j = 0; j < C; ++j) g[i][j] = (jj == 0) ? gv[jj][i] * U[j] : g[i][j] + gv[jj][i] * U[j];
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < C; ++j) {
+        float mx = P[j], mn = P[j];
+#pragma unroll
+        for (int k = 0; k < NL; ++k) {
+            mx = fmaxf(mx, Un[k][j]);
+            mn = fminf(mn, Un[k][j]);
+        }
+        const float upper = mx - P[j], lower = mn - P[j];
+        float ph = 0.0f;
+#pragma unroll
+        for (int k = 0; k < NL; ++k) {
+            float dU = g[0][j] * disp[k][0];
+#pragma unroll
+            for (int i = 1; i < D; ++i) dU = dU + g[i][j] * disp[k][i];
+            const float lk = lim_phi<LP>(dU > 0.0f ? upper : lower, dU, eps);
+            ph = (k == 0) ? lk : fminf(ph, lk);
+        }
+        phi[j] = ph;
+    }
+}
+
 template <int D, int LP>
 __global__ void __launch_bounds__(kThreads)
 k_grad(FvmMeshDev m, FvmPhys p, FvmWork w, const float *__restrict__ prim, int has_prev) {
@@ -298,12 +351,16 @@ k_grad(FvmMeshDev m, FvmPhys p, FvmWork w, const float *__restrict__ prim, int h
     const int64_t N = m.n_cells;
     const int64_t ntiles = (N + kThreads - 1) / kThreads;
     float *__restrict__ gp = w.gp;
+    // the previous call's records: a separate buffer, or (in place) gp itself -- each
+    // cell reads its own record before overwriting it
+    const float *gprev = w.gp_prev ? w.gp_prev : w.gp;
     __shared__ float gmean[kMaxGroups];
+    __shared__ float red[kThreads];
     for (int64_t v = blockIdx.x; v < ntiles; v += gridDim.x) {
     const int64_t tile = xcd_tile(v, ntiles);
     const int64_t c = tile * kThreads + threadIdx.x;
     const int slot = m.n_groups > 0 ? m.blk_slot[tile] : -1;
-    if (slot >= 0) block_group_means(m, w, slot, gmean);
+    if (slot >= 0) block_group_means(m, w, slot, gmean, red);
     const int g0 = slot >= 0 ? m.slot_g0[slot] : 0;
     if (c >= N) continue;
     // every per-cell load is issued before the neighbour gathers that depend on them,
@@ -335,49 +392,12 @@ k_grad(FvmMeshDev m, FvmPhys p, FvmWork w, const float *__restrict__ prim, int h
             // this cell is the boundary facet's only cell: it computes the facet's values
             // (reading its own previous gradient record before overwriting it below)
             const int64_t b = nbk[k] - N;
-            bc_value<D>(m, p, b, P, gp + c * Dm::REC, has_prev, gmean, g0, Un[k]);
+            bc_value<D>(m, p, b, P, gprev + c * Dm::REC, has_prev, gmean, g0, Un[k]);
             store_vec<C>(w.ubc + b * C, Un[k]);
         }
     }
-    float g[D][C];
-#pragma unroll
-    for (int jj = 0; jj <= NL; ++jj) {
-        const int sel = (info >> (16 + 3 * jj)) & 7;
-        float U[C];
-#pragma unroll
-        for (int j = 0; j < C; ++j) {
-            float v = P[j];
-#pragma unroll
-            for (int k = 0; k < NL; ++k) v = (sel == k) ? Un[k][j] : v;
-            U[j] = v;
-        }
-#pragma unroll
-        for (int i = 0; i < D; ++i) {
-#pragma unroll
-            for (int j = 0; j < C; ++j) g[i][j] = (jj == 0) ? gv[jj][i] * U[j] : g[i][j] + gv[jj][i] * U[j];
-        }
-    }
-    float phi[C];
-#pragma unroll
-    for (int j = 0; j < C; ++j) {
-        float mx = P[j], mn = P[j];
-#pragma unroll
-        for (int k = 0; k < NL; ++k) {
-            mx = fmaxf(mx, Un[k][j]);
-            mn = fminf(mn, Un[k][j]);
-        }
-        const float upper = mx - P[j], lower = mn - P[j];
-        float ph = 0.0f;
-#pragma unroll
-        for (int k = 0; k < NL; ++k) {
-            float dU = g[0][j] * disp[k][0];
-#pragma unroll
-            for (int i = 1; i < D; ++i) dU = dU + g[i][j] * disp[k][i];
-            const float lk = lim_phi<LP>(dU > 0.0f ? upper : lower, dU, eps);
-            ph = (k == 0) ? lk : fminf(ph, lk);
-        }
-        phi[j] = ph;
-    }
+    float g[D][C], phi[C];
+    cell_gradient<D, LP>(P, Un, info, gv, disp, eps, g, phi);
     float *o = gp + c * Dm::REC;
 #pragma unroll
     for (int i = 0; i < D; ++i)
@@ -395,7 +415,7 @@ template <int D>
 __device__ __forceinline__ void facet_flux(const FvmPhys &p, float mu0, float mub0, float kappa,
                                            const float (&U0)[D + 2], const float (&U1)[D + 2],
                                            const float (&gl0)[D][D + 1], const float (&gl1)[D][D + 1],
-                                           const float (&gdn)[D + 1], const float *__restrict__ geo,
+                                           const float (&gdn)[D + 1], const float (&geo)[3 * D + 2],
                                            float (&F)[D + 2]) {
     float n[D], nh[D], X[D];
 #pragma unroll
@@ -619,8 +639,11 @@ k_facet_flux(FvmMeshDev m, FvmPhys p, FvmWork w, const float *__restrict__ prim)
                 gdn[jg] = m.bc_ga[b * C + j] * P0[j] + m.bc_gb[b * C + j];
             }
         }
+        float gr[3 * D + 2];
+#pragma unroll
+        for (int k = 0; k < 3 * D + 2; ++k) gr[k] = geo[k];
         float Fx[C];
-        facet_flux<D>(p, mu0, mub0, kappa, U0s, U1s, g0s, g1s, gdn, geo, Fx);
+        facet_flux<D>(p, mu0, mub0, kappa, U0s, U1s, g0s, g1s, gdn, gr, Fx);
         float *out = w.fflux + f * C;
         if constexpr (C == 4) {
             *reinterpret_cast<float4 *>(out) = make_float4(Fx[0], Fx[1], Fx[2], Fx[3]);
@@ -633,30 +656,13 @@ k_facet_flux(FvmMeshDev m, FvmPhys p, FvmWork w, const float *__restrict__ prim)
 
 enum { EPI_DUDT = 0, EPI_STAGE = 1 };
 
-// Per cell: the divergence (flux_mat row: +-1/vol on its facets in ascending reference
-// facet order) of the facet fluxes, then the epilogue -- dU/dt, or an explicit stage
-// update with the next stage's primitives and their boundary means.
+// The stage epilogue's per-cell inputs: U0 and the RK accumulators, or (Adams-Bashforth)
+// the two previous derivatives.  Issued before the divergence so that they are in
+// flight together with the flux gathers.
 template <int D, int EPI>
-__global__ void __launch_bounds__(kThreads)
-k_div(FvmMeshDev m, FvmPhys p, FvmWork w, float *__restrict__ dudt, FvmStage st) {
-    double *__restrict__ red = w.red;
-    using Dm = Dims<D>;
-    constexpr int C = Dm::C, NL = Dm::NL;
-    const int64_t N = m.n_cells;
-    const int64_t ntiles = (N + kThreads - 1) / kThreads;
-    for (int64_t v = blockIdx.x; v < ntiles; v += gridDim.x) {
-    const int64_t tile = xcd_tile(v, ntiles);
-    const int64_t c = tile * kThreads + threadIdx.x;
-    const bool live = c < N;
-    const int mesh = (m.cell_mesh && live) ? m.cell_mesh[c] : 0;
-    float acc[C];
-#pragma unroll
-    for (int j = 0; j < C; ++j) acc[j] = 0.0f;
-    // the stage epilogue's per-cell inputs are loaded first, so they are in flight
-    // together with the facet lists and the flux gathers
-    const bool first = st.stage == 0;
-    const bool has2 = st.acc2 != nullptr;
-    float U0v[C], ap[C], a2p[C];
+__device__ __forceinline__ void stage_inputs(const FvmStage &st, int64_t c, bool live, float (&U0v)[D + 2],
+                                             float (&ap)[D + 2], float (&a2p)[D + 2]) {
+    constexpr int C = D + 2;
     if (EPI == EPI_STAGE && live) {
         const int64_t o = c * C;
         load_vec<C>(st.U0 + o, U0v);
@@ -664,28 +670,30 @@ k_div(FvmMeshDev m, FvmPhys p, FvmWork w, float *__restrict__ dudt, FvmStage st)
             if (st.n_hist >= 1) load_vec<C>(st.f_m1 + o, ap);
             if (st.n_hist >= 2) load_vec<C>(st.f_m2 + o, a2p);
         } else {
-            if (!first) load_vec<C>(st.acc + o, ap);
-            if (!first && has2) load_vec<C>(st.acc2 + o, a2p);
+            if (st.stage != 0) load_vec<C>(st.acc + o, ap);
+            if (st.stage != 0 && st.acc2 != nullptr) load_vec<C>(st.acc2 + o, a2p);
         }
     }
-    if (live) {
-        const int info = m.cell_info[c];
-        const float inv_vol = m.cell_inv_vol[c];
-        float Fk[NL][C];
-#pragma unroll
-        for (int k = 0; k < NL; ++k) load_vec<C>(w.fflux + static_cast<int64_t>(m.cell_facet[k * N + c]) * C, Fk[k]);
-#pragma unroll
-        for (int k = 0; k < NL; ++k) {
-            const float vk = ((info >> k) & 1) ? inv_vol : -inv_vol;
-#pragma unroll
-            for (int j = 0; j < C; ++j) acc[j] = acc[j] + vk * Fk[k][j];
-        }
-    }
+}
+
+// Epilogue of cell c with dU/dt = acc.  Block-uniform call (every thread of the block,
+// live or not): writes dU/dt, or applies the explicit stage update and emits the next
+// stage's primitives, their boundary-mean partials (slot of 256-cell tile `tile`) and,
+// on the last stage of an embedded tableau, the fp64 column-norm partials of `tile`.
+template <int D, int EPI>
+__device__ __forceinline__ void stage_epilogue(const FvmMeshDev &m, const FvmPhys &p, const FvmWork &w,
+                                               float *__restrict__ dudt, const FvmStage &st, int64_t tile,
+                                               int64_t c, bool live, int mesh, const float (&acc)[D + 2],
+                                               const float (&U0v)[D + 2], const float (&ap)[D + 2],
+                                               const float (&a2p)[D + 2], float *red, double *sred) {
+    constexpr int C = D + 2;
+    const int64_t N = m.n_cells;
     if (EPI == EPI_DUDT) {
         if (live) store_vec<C>(dudt + c * C, acc);
-        continue;
+        return;
     }
-    // ---------------- explicit RK stage epilogue ----------------
+    const bool first = st.stage == 0;
+    const bool has2 = st.acc2 != nullptr;
     // Stage flags are uniform; every per-cell load is issued before the first use so
     // that a cell's memory round trips overlap.
     double sq[3 * C];
@@ -769,23 +777,391 @@ k_div(FvmMeshDev m, FvmPhys p, FvmWork w, float *__restrict__ dudt, FvmStage st)
     // boundary means of the next stage's primitives (block-uniform branch)
     if (m.n_groups > 0) {
         const int slot = m.blk_slot[tile];
-        if (slot >= 0) bc_mean_block<D>(m, p, w, c, live, Pn[D + 1], slot);
+        if (slot >= 0) bc_mean_block<D>(m, p, w, c, live, Pn[D + 1], slot, red);
     }
     if (st.last && st.acc2) {
         // fp64 block partials of the three column norms (Adaptive.update_stepsize)
-        __shared__ double sred[kThreads];
         for (int q = 0; q < 3 * C; ++q) {
             sred[threadIdx.x] = sq[q];
             __syncthreads();
-            for (int w = kThreads / 2; w > 0; w >>= 1) {
-                if (threadIdx.x < w) sred[threadIdx.x] += sred[threadIdx.x + w];
+            for (int h = kThreads / 2; h > 0; h >>= 1) {
+                if (threadIdx.x < h) sred[threadIdx.x] += sred[threadIdx.x + h];
                 __syncthreads();
             }
-            if (threadIdx.x == 0) red[tile * 3 * C + q] = sred[0];
+            if (threadIdx.x == 0) w.red[tile * 3 * C + q] = sred[0];
             __syncthreads();
         }
     }
+}
+
+// Per cell: the divergence (flux_mat row: +-1/vol on its facets in ascending reference
+// facet order) of the facet fluxes, then the epilogue -- dU/dt, or an explicit stage
+// update with the next stage's primitives and their boundary means.
+template <int D, int EPI>
+__global__ void __launch_bounds__(kThreads)
+k_div(FvmMeshDev m, FvmPhys p, FvmWork w, float *__restrict__ dudt, FvmStage st) {
+    using Dm = Dims<D>;
+    constexpr int C = Dm::C, NL = Dm::NL;
+    const int64_t N = m.n_cells;
+    const int64_t ntiles = (N + kThreads - 1) / kThreads;
+    __shared__ float red[kThreads];
+    __shared__ double sred[kThreads];
+    for (int64_t v = blockIdx.x; v < ntiles; v += gridDim.x) {
+        const int64_t tile = xcd_tile(v, ntiles);
+        const int64_t c = tile * kThreads + threadIdx.x;
+        const bool live = c < N;
+        const int mesh = (m.cell_mesh && live) ? m.cell_mesh[c] : 0;
+        float U0v[C], ap[C], a2p[C];
+        stage_inputs<D, EPI>(st, c, live, U0v, ap, a2p);
+        float acc[C];
+#pragma unroll
+        for (int j = 0; j < C; ++j) acc[j] = 0.0f;
+        if (live) {
+            const int info = m.cell_info[c];
+            const float inv_vol = m.cell_inv_vol[c];
+            float Fk[NL][C];
+#pragma unroll
+            for (int k = 0; k < NL; ++k)
+                load_vec<C>(w.fflux + static_cast<int64_t>(m.cell_facet[k * N + c]) * C, Fk[k]);
+#pragma unroll
+            for (int k = 0; k < NL; ++k) {
+                const float vk = ((info >> k) & 1) ? inv_vol : -inv_vol;
+#pragma unroll
+                for (int j = 0; j < C; ++j) acc[j] = acc[j] + vk * Fk[k][j];
+            }
+        }
+        stage_epilogue<D, EPI>(m, p, w, dudt, st, tile, c, live, mesh, acc, U0v, ap, a2p, red, sred);
     }
+}
+
+// ------------------------------------------------------------------------------------
+// Tile-fused stage: one 256-cell tile per workgroup
+// ------------------------------------------------------------------------------------
+// The workgroup stages in LDS the primitives of its tile, of the tile's one-ring halo H1
+// (cells outside the tile sharing a facet with it) and of H1's other neighbours H2;
+// computes the boundary values, gradient and limiter of tile + H1 cells, keeping their
+// limited face values and limited gradients in LDS; evaluates the fluxes of the tile's
+// owned and foreign facets into LDS; then every tile cell sums its facets' fluxes and
+// runs the stage epilogue.  The arithmetic is that of k_grad, k_facet_flux and k_div,
+// through the same device functions in the same order, so the results are bitwise
+// those of the three-kernel pipeline; only the traffic differs (gradient records and
+// facet fluxes never leave the CU).  Gradient records are written back only for cells
+// with a boundary facet (the next call's Neumann correction reads them) into w.gp,
+// which must not alias w.gp_prev: other tiles still read gp_prev for their halo cells.
+constexpr int kFzTab = 12;
+constexpr unsigned kFzNone = 0xFFFFu;
+
+template <int D>
+__host__ __device__ constexpr int fz_geo_n() { return D == 2 ? 8 : 12; }
+template <int D>
+__host__ __device__ constexpr int fz_hrec_n() { return D == 2 ? 16 : 32; }
+
+// LDS layout of k_fused in floats: Pl [ncl][C], FV [ngl][NL][C], GL [ngl][D][D+1]
+// (padded to 4 floats), FL [nfl][C].
+template <int D>
+__host__ __device__ inline int64_t fz_fl_offset(int ncl, int ngl) {
+    constexpr int C = D + 2, NL = D + 1;
+    return static_cast<int64_t>(ncl) * C + static_cast<int64_t>(ngl) * NL * C +
+           ((static_cast<int64_t>(ngl) * D * (D + 1) + 3) & ~int64_t(3));
+}
+
+// One gradient cell of the fused kernel (tile cell or H1 cell, local id l, global gc):
+// boundary values, gradient and limiter; the limited face values and the limited V, T
+// gradients go to LDS.  Tile cells also write their boundary values and, when they have
+// a boundary facet, their gradient record (the next call's Neumann correction).
+template <int D, int LP>
+__device__ __forceinline__ void fused_grad_cell(const FvmMeshDev &m, const FvmPhys &p, const FvmWork &w,
+                                                const float *Pl, float *FV, float *GL, int l, int64_t gc, bool own,
+                                                const float (&gv)[D + 2][D], const float (&disp)[D + 1][D],
+                                                float eps, int info, const ushort4 nrow, int has_prev,
+                                                const float *gmean, int g0) {
+    using Dm = Dims<D>;
+    constexpr int C = Dm::C, NL = Dm::NL, GLN = D * (D + 1);
+    const int64_t N = m.n_cells;
+    const unsigned nl[4] = {nrow.x, nrow.y, nrow.z, nrow.w};
+    float P[C];
+    load_vec<C>(Pl + l * C, P);
+    float Un[NL][C];
+#pragma unroll
+    for (int k = 0; k < NL; ++k) load_vec<C>(Pl + (nl[k] != kFzNone ? nl[k] : l) * C, Un[k]);
+    bool bnd = false;
+#pragma unroll
+    for (int k = 0; k < NL; ++k) {
+        if (nl[k] == kFzNone) {
+            bnd = true;
+            const int64_t b = static_cast<int64_t>(m.cell_nbr[k * N + gc]) - N;
+            bc_value<D>(m, p, b, P, w.gp_prev + gc * Dm::REC, has_prev, gmean, g0, Un[k]);
+            if (own) store_vec<C>(w.ubc + b * C, Un[k]);
+        }
+    }
+    float g[D][C], phi[C];
+    cell_gradient<D, LP>(P, Un, info, gv, disp, eps, g, phi);
+    if (own && bnd) {
+        float *o = w.gp + gc * Dm::REC;
+#pragma unroll
+        for (int ii = 0; ii < D; ++ii)
+#pragma unroll
+            for (int j = 0; j < C; ++j) o[ii * C + j] = g[ii][j];
+#pragma unroll
+        for (int j = 0; j < C; ++j) o[D * C + j] = phi[j];
+    }
+    // limited face values and limited gradients (side_values' operation order)
+#pragma unroll
+    for (int k = 0; k < NL; ++k) {
+        float U[C];
+#pragma unroll
+        for (int j = 0; j < C; ++j) {
+            float dU = g[0][j] * disp[k][0];
+#pragma unroll
+            for (int ii = 1; ii < D; ++ii) dU = dU + g[ii][j] * disp[k][ii];
+            U[j] = P[j] + dU * phi[j];
+        }
+        store_vec<C>(FV + (l * NL + k) * C, U);
+    }
+#pragma unroll
+    for (int ii = 0; ii < D; ++ii) {
+#pragma unroll
+        for (int j = 0; j < D; ++j) GL[l * GLN + ii * (D + 1) + j] = g[ii][j] * phi[j];
+        GL[l * GLN + ii * (D + 1) + D] = g[ii][D + 1] * phi[D + 1];
+    }
+}
+
+// Facet q of the tile's facet list (descriptor d, compact geometry geo): both sides from
+// LDS, the normal gradient, the flux into FL[q].
+template <int D>
+__device__ __forceinline__ void fused_facet(const FvmMeshDev &m, const FvmPhys &p, const FvmWork &w,
+                                            const float *Pl, const float *FV, const float *GL, float *FL, int q,
+                                            int64_t f, unsigned d, const float (&geo)[3 * D + 2], int64_t c0,
+                                            int halo_off, int has_prev, const float *gmean, int g0) {
+    using Dm = Dims<D>;
+    constexpr int C = Dm::C, NL = Dm::NL, GLN = D * (D + 1);
+    const int64_t N = m.n_cells, NF = m.n_facets;
+    const unsigned l0 = d & 0xFFFu, k0 = (d >> 12) & 3u, l1 = (d >> 14) & 0xFFFu, k1 = (d >> 26) & 3u;
+    const bool left_first = (d >> 28) & 1u, bnd = (d >> 29) & 1u, outer0 = (d >> 30) & 1u;
+    float mu0 = p.mu0, mub0 = p.mub0, kappa = p.kappa;
+    if (m.mesh_phys) {
+        const unsigned lo = outer0 ? l1 : l0;
+        const int64_t gown = lo < kThreads ? c0 + lo : static_cast<int64_t>(m.fz_halo[halo_off + lo - kThreads]);
+        const int mesh = m.cell_mesh[gown];
+        mu0 = m.mesh_phys[4 * mesh];
+        mub0 = m.mesh_phys[4 * mesh + 1];
+        kappa = m.mesh_phys[4 * mesh + 2];
+    }
+    float U0s[C], U1s[C], g0s[D][D + 1], g1s[D][D + 1], gdn[D + 1];
+    if (!bnd) {
+        float P0[C], P1[C];
+        load_vec<C>(FV + (l0 * NL + k0) * C, U0s);
+        load_vec<C>(FV + (l1 * NL + k1) * C, U1s);
+        load_vec<C>(Pl + l0 * C, P0);
+        load_vec<C>(Pl + l1 * C, P1);
+#pragma unroll
+        for (int ii = 0; ii < D; ++ii)
+#pragma unroll
+            for (int j = 0; j <= D; ++j) {
+                g0s[ii][j] = GL[l0 * GLN + ii * (D + 1) + j];
+                g1s[ii][j] = GL[l1 * GLN + ii * (D + 1) + j];
+            }
+        // A_face_grad row: (-1/d, +1/d) on (slot-0, slot-1) cells, ascending columns
+        const float inv_d = geo[2 * D + 1];
+#pragma unroll
+        for (int jg = 0; jg <= D; ++jg) {
+            const int j = jg < D ? jg : D + 1;
+            const float tl = (-inv_d) * P0[j], tr = inv_d * P1[j];
+            gdn[jg] = (left_first ? tl + tr : tr + tl) + 0.0f;
+        }
+    } else {
+        // boundary facet: its interior cell is a tile cell (the facet's lowest cell)
+        const unsigned li = outer0 ? l1 : l0, ki = outer0 ? k1 : k0;
+        float Uo[C], Pi[C], Ux[C];
+        load_vec<C>(FV + (li * NL + ki) * C, Uo);
+        load_vec<C>(Pl + li * C, Pi);
+        const int64_t b = static_cast<int64_t>(m.fac_cell[outer0 ? f : NF + f]) - N;
+        bc_value<D>(m, p, b, Pi, w.gp_prev + (c0 + li) * Dm::REC, has_prev, gmean, g0, Ux);
+#pragma unroll
+        for (int j = 0; j < C; ++j) {
+            U0s[j] = outer0 ? Ux[j] : Uo[j];
+            U1s[j] = outer0 ? Uo[j] : Ux[j];
+        }
+#pragma unroll
+        for (int ii = 0; ii < D; ++ii)
+#pragma unroll
+            for (int j = 0; j <= D; ++j) {
+                g0s[ii][j] = GL[li * GLN + ii * (D + 1) + j];
+                g1s[ii][j] = g0s[ii][j];
+            }
+#pragma unroll
+        for (int jg = 0; jg <= D; ++jg) {
+            const int j = jg < D ? jg : D + 1;
+            gdn[jg] = m.bc_ga[b * C + j] * Pi[j] + m.bc_gb[b * C + j];
+        }
+    }
+    float Fx[C];
+    facet_flux<D>(p, mu0, mub0, kappa, U0s, U1s, g0s, g1s, gdn, geo, Fx);
+    store_vec<C>(FL + q * C, Fx);
+}
+
+// Descriptor and compact geometry of facet q of the tile's list (owned run, then the
+// foreign list).
+template <int D>
+__device__ __forceinline__ void fused_facet_load(const FvmMeshDev &m, int q, int fo0, int nfo, int foff, int64_t &f,
+                                                 unsigned &d, float (&geo)[3 * D + 2]) {
+    constexpr int FG = fz_geo_n<D>();
+    const bool owned = q < nfo;
+    f = owned ? static_cast<int64_t>(fo0) + q : static_cast<int64_t>(m.fz_ffac[foff + q - nfo]);
+    d = owned ? m.fz_fdesc[f] : m.fz_fdesc_x[foff + q - nfo];
+    const float *src = owned ? m.fz_geo + f * FG : m.fz_geo_x + static_cast<int64_t>(foff + q - nfo) * FG;
+    if constexpr (D == 2) {
+        const float4 a = reinterpret_cast<const float4 *>(src)[0];
+        const float4 b = reinterpret_cast<const float4 *>(src)[1];
+        geo[0] = a.x; geo[1] = a.y; geo[2] = a.z; geo[3] = a.w;
+        geo[4] = b.x; geo[5] = b.y; geo[6] = b.z; geo[7] = b.w;
+    } else {
+#pragma unroll
+        for (int k = 0; k < 3 * D + 2; ++k) geo[k] = src[k];
+    }
+}
+
+// Each phase's global loads are issued before the previous phase's work (they land in
+// registers while it computes), so a tile costs about one memory round trip per phase
+// boundary rather than one per dependent load.  The host guarantees at most 256 H1
+// cells and 512 facets per tile (two facets per thread).
+template <int D, int LP, int EPI>
+__global__ void __launch_bounds__(kThreads)
+k_fused(FvmMeshDev m, FvmPhys p, FvmWork w, const float *__restrict__ prim, float *__restrict__ dudt, FvmStage st,
+        int has_prev) {
+    using Dm = Dims<D>;
+    constexpr int C = Dm::C, NL = Dm::NL, FVN = NL * C, GLN = D * (D + 1);
+    constexpr int HREC = fz_hrec_n<D>();
+    extern __shared__ float4 fz_lds[];     // 16-byte aligned: 2-D records move as ds_*_b128
+    float *lds = reinterpret_cast<float *>(fz_lds);
+    const int64_t N = m.n_cells;
+    const int64_t ntiles = (N + kThreads - 1) / kThreads;
+    const int64_t tile = xcd_tile(blockIdx.x, ntiles);
+    const int *tb = m.fz_tile + tile * kFzTab;
+    const int halo_off = tb[0], n1 = tb[1], n2 = tb[2], nbr_off = tb[3], fo0 = tb[4], nfo = tb[5];
+    const int foff = tb[6], nfor = tb[7], g0 = tb[8], ng = tb[9], h1_off = tb[10];
+    float *Pl = lds;
+    float *FV = Pl + static_cast<int64_t>(m.fz_ncl) * C;
+    float *GL = FV + static_cast<int64_t>(m.fz_ngl) * FVN;
+    float *FL = lds + fz_fl_offset<D>(m.fz_ncl, m.fz_ngl);
+    float *red = FV;                                   // reductions reuse the face-value rows
+    double *sred = reinterpret_cast<double *>(FV);     // (before and after the two phases using them)
+    const int i = threadIdx.x;
+    const int64_t c0 = tile * kThreads;
+    const int nown = static_cast<int>(N - c0 < kThreads ? N - c0 : kThreads);
+    const int nft = nfo + nfor;
+    const bool has_own = i < nown, has_h1 = i < n1;
+    __shared__ float gmean[kMaxGroups];
+    if (ng > 0) block_group_means_g(m, w, g0, ng, gmean, red);
+
+    // ---- phase 0: primitives of tile, H1 and H2 to LDS; the gradient inputs of this
+    // thread's tile cell and H1 cell to registers ----
+    float ogv[NL + 1][D], odisp[NL][D], oeps = 0.0f;
+    int oinfo = 0;
+    ushort4 onr = make_ushort4(0, 0, 0, 0);
+    if (has_own) {
+        const int64_t c = c0 + i;
+        float u[C];
+        load_prim<D>(prim + c * C, u);
+        store_vec<C>(Pl + i * C, u);
+#pragma unroll
+        for (int jj = 0; jj <= NL; ++jj)
+#pragma unroll
+            for (int ii = 0; ii < D; ++ii) ogv[jj][ii] = m.cell_gval[(jj * D + ii) * N + c];
+#pragma unroll
+        for (int k = 0; k < NL; ++k)
+#pragma unroll
+            for (int ii = 0; ii < D; ++ii) odisp[k][ii] = m.cell_disp[(k * D + ii) * N + c];
+        oeps = m.cell_eps[c];
+        oinfo = m.cell_info[c];
+        onr = *reinterpret_cast<const ushort4 *>(m.fz_nbr + (static_cast<int64_t>(nbr_off) + i) * 4);
+    }
+    float hgv[NL + 1][D], hdisp[NL][D], heps = 0.0f;
+    int hinfo = 0;
+    int64_t hgc = 0;
+    ushort4 hnr = make_ushort4(0, 0, 0, 0);
+    if (has_h1) {
+        hgc = m.fz_halo[halo_off + i];
+        const float4 *r = reinterpret_cast<const float4 *>(m.fz_hrec + static_cast<int64_t>(h1_off + i) * HREC);
+        float rv[HREC];
+#pragma unroll
+        for (int q = 0; q < HREC / 4; ++q) {
+            const float4 t = r[q];
+            rv[4 * q] = t.x;
+            rv[4 * q + 1] = t.y;
+            rv[4 * q + 2] = t.z;
+            rv[4 * q + 3] = t.w;
+        }
+#pragma unroll
+        for (int jj = 0; jj <= NL; ++jj)
+#pragma unroll
+            for (int ii = 0; ii < D; ++ii) hgv[jj][ii] = rv[jj * D + ii];
+#pragma unroll
+        for (int k = 0; k < NL; ++k)
+#pragma unroll
+            for (int ii = 0; ii < D; ++ii) hdisp[k][ii] = rv[(NL + 1) * D + k * D + ii];
+        heps = rv[(NL + 1) * D + NL * D];
+        hinfo = __float_as_int(rv[(NL + 1) * D + NL * D + 1]);
+        hnr = *reinterpret_cast<const ushort4 *>(m.fz_nbr + (static_cast<int64_t>(nbr_off) + kThreads + i) * 4);
+    }
+    for (int q = i; q < n1 + n2; q += kThreads) {
+        float u[C];
+        load_prim<D>(prim + static_cast<int64_t>(m.fz_halo[halo_off + q]) * C, u);
+        store_vec<C>(Pl + (kThreads + q) * C, u);
+    }
+    __syncthreads();
+
+    // ---- phase 1: gradients (the first facet's inputs in flight meanwhile) ----
+    int64_t fA = 0;
+    unsigned dA = 0;
+    float geoA[3 * D + 2];
+    if (i < nft) fused_facet_load<D>(m, i, fo0, nfo, foff, fA, dA, geoA);
+    if (has_own)
+        fused_grad_cell<D, LP>(m, p, w, Pl, FV, GL, i, c0 + i, true, ogv, odisp, oeps, oinfo, onr, has_prev, gmean,
+                               g0);
+    if (has_h1)
+        fused_grad_cell<D, LP>(m, p, w, Pl, FV, GL, kThreads + i, hgc, false, hgv, hdisp, heps, hinfo, hnr,
+                               has_prev, gmean, g0);
+    __syncthreads();
+
+    // ---- phase 2: facet fluxes (the second facet's and the epilogue's inputs in flight) ----
+    int64_t fB = 0;
+    unsigned dB = 0;
+    float geoB[3 * D + 2];
+    const bool twoB = i + kThreads < nft;
+    if (twoB) fused_facet_load<D>(m, i + kThreads, fo0, nfo, foff, fB, dB, geoB);
+    const int64_t c = c0 + i;
+    const int mesh = (m.cell_mesh && has_own) ? m.cell_mesh[c] : 0;
+    float U0v[C], ap[C], a2p[C];
+    stage_inputs<D, EPI>(st, c, has_own, U0v, ap, a2p);
+    float inv_vol = 0.0f;
+    ushort4 cf = make_ushort4(0, 0, 0, 0);
+    if (has_own) {
+        inv_vol = m.cell_inv_vol[c];
+        cf = *reinterpret_cast<const ushort4 *>(m.fz_cfac + c * 4);
+    }
+    if (i < nft) fused_facet<D>(m, p, w, Pl, FV, GL, FL, i, fA, dA, geoA, c0, halo_off, has_prev, gmean, g0);
+    if (twoB)
+        fused_facet<D>(m, p, w, Pl, FV, GL, FL, i + kThreads, fB, dB, geoB, c0, halo_off, has_prev, gmean, g0);
+    __syncthreads();
+
+    // ---- phase 3: divergence and stage epilogue of the tile's cells ----
+    float acc[C];
+#pragma unroll
+    for (int j = 0; j < C; ++j) acc[j] = 0.0f;
+    if (has_own) {
+        const unsigned tf[4] = {cf.x, cf.y, cf.z, cf.w};
+        float Fk[NL][C];
+#pragma unroll
+        for (int k = 0; k < NL; ++k) load_vec<C>(FL + tf[k] * C, Fk[k]);
+#pragma unroll
+        for (int k = 0; k < NL; ++k) {
+            const float vk = ((oinfo >> k) & 1) ? inv_vol : -inv_vol;
+#pragma unroll
+            for (int j = 0; j < C; ++j) acc[j] = acc[j] + vk * Fk[k][j];
+        }
+    }
+    stage_epilogue<D, EPI>(m, p, w, dudt, st, tile, c, has_own, mesh, acc, U0v, ap, a2p, red, sred);
 }
 
 __global__ void k_update_stepsize(const double *__restrict__ red, int C, int nblk, float atol, float rtol, float mtol,
@@ -851,14 +1227,62 @@ static int check_mesh(const FvmMeshDev *m, const FvmPhys *p, const FvmWork *w, c
     return FVM_OK;
 }
 
+// Dynamic LDS of k_fused in bytes.
+template <int D>
+static size_t fused_lds_bytes(const FvmMeshDev &m) {
+    return static_cast<size_t>(fz_fl_offset<D>(m.fz_ncl, m.fz_ngl) + static_cast<int64_t>(m.fz_nfl) * (D + 2)) *
+           sizeof(float);
+}
+
+constexpr size_t kFusedLdsMax = 64 * 1024;
+
+static int check_fused(const FvmMeshDev &m, const FvmWork &w, const char *who) {
+    if (m.dim != 2 || !m.fz_tile || !m.fz_halo || !m.fz_hrec || !m.fz_nbr || !m.fz_fdesc || !m.fz_ffac ||
+        !m.fz_fdesc_x || !m.fz_geo || !m.fz_geo_x || !m.fz_cfac || m.fz_ncl < kThreads || m.fz_ngl < kThreads ||
+        m.fz_ngl > m.fz_ncl || m.fz_ncl > 0xFFF || m.fz_nfl < 1 || fused_lds_bytes<2>(m) > kFusedLdsMax) {
+        set_error("%s: invalid tile tables for the tile-fused stage", who);
+        return FVM_ERR_ARG;
+    }
+    if (!w.gp_prev || w.gp_prev == w.gp) {
+        set_error("%s: the tile-fused stage needs work->gp_prev as a buffer separate from work->gp", who);
+        return FVM_ERR_ARG;
+    }
+    return FVM_OK;
+}
+
+template <int D, int EPI>
+static void launch_fused(const FvmMeshDev &m, const FvmPhys &p, const FvmWork &w, const float *prim, float *dudt,
+                         const FvmStage &st, hipStream_t s, int has_prev) {
+    const dim3 grid(blocks_for(m.n_cells, kThreads));
+    const size_t lds = fused_lds_bytes<D>(m);
+    switch (p.lim_p) {
+        case 1: hipLaunchKernelGGL((k_fused<D, 1, EPI>), grid, dim3(kThreads), lds, s, m, p, w, prim, dudt, st, has_prev); break;
+        case 2: hipLaunchKernelGGL((k_fused<D, 2, EPI>), grid, dim3(kThreads), lds, s, m, p, w, prim, dudt, st, has_prev); break;
+        case 3: hipLaunchKernelGGL((k_fused<D, 3, EPI>), grid, dim3(kThreads), lds, s, m, p, w, prim, dudt, st, has_prev); break;
+        case 5: hipLaunchKernelGGL((k_fused<D, 5, EPI>), grid, dim3(kThreads), lds, s, m, p, w, prim, dudt, st, has_prev); break;
+        default: hipLaunchKernelGGL((k_fused<D, 4, EPI>), grid, dim3(kThreads), lds, s, m, p, w, prim, dudt, st, has_prev); break;
+    }
+}
+
 // parts: 1 = boundary-mean pre-pass, 2 = gradient + limiter (+ boundary values),
-// 4 = facet fluxes, 8 = divergence (+ stage update).  The pre-pass runs only when
-// `need_mean`.
+// 4 = facet fluxes, 8 = divergence (+ stage update), 16 = tile-fused stage (2-D; the
+// caller has checked the tile tables).  The pre-pass runs only when `need_mean`.
 template <int D>
 static int launch_rhs(const FvmMeshDev &m, const FvmPhys &p, const FvmWork &w, const float *prim, float *dudt,
                       const FvmStage *st, hipStream_t s, int has_prev, int need_mean, int parts = 15) {
     if ((parts & 1) && need_mean && m.n_groups > 0)
         hipLaunchKernelGGL(k_bc_mean<D>, dim3(m.n_slots), dim3(kThreads), 0, s, m, p, w, prim);
+    if constexpr (D == 2) {
+        if (parts & 16) {
+            if (st) {
+                launch_fused<D, EPI_STAGE>(m, p, w, prim, nullptr, *st, s, has_prev);
+            } else {
+                FvmStage none{};
+                launch_fused<D, EPI_DUDT>(m, p, w, prim, dudt, none, s, has_prev);
+            }
+            return check_launch(st ? "fvm_rk_stage_f32" : "fvm_rhs_f32");
+        }
+    }
     const unsigned int ntiles = blocks_for(m.n_cells, kThreads);
     const dim3 grid(m.persist_blocks > 0 && static_cast<unsigned int>(m.persist_blocks) < ntiles
                         ? static_cast<unsigned int>(m.persist_blocks) : ntiles);
@@ -943,29 +1367,49 @@ int fvm_rhs_f32(const FvmMeshDev *m, const FvmPhys *p, const FvmWork *w, const f
         return FVM_ERR_ARG;
     }
     e = check_align(m, w, nullptr, dudt, prim, "fvm_rhs_f32");
+    if (!e && m->fz_tile) e = check_fused(*m, *w, "fvm_rhs_f32");
     if (e) return e;
+    if (m->fz_tile) return launch_rhs<2>(*m, *p, *w, prim, dudt, nullptr, as_stream(stream), has_prev, 1, 1 | 16);
     return m->dim == 2 ? launch_rhs<2>(*m, *p, *w, prim, dudt, nullptr, as_stream(stream), has_prev, 1)
                        : launch_rhs<3>(*m, *p, *w, prim, dudt, nullptr, as_stream(stream), has_prev, 1);
+}
+
+int fvm_grad_f32(const FvmMeshDev *m, const FvmPhys *p, const FvmWork *w, const float *prim, int32_t has_prev,
+                 void *stream) {
+    int e = check_mesh(m, p, w, "fvm_grad_f32");
+    if (e) return e;
+    if (!prim || (w->gp_prev && w->gp_prev == w->gp)) {
+        set_error("fvm_grad_f32: null prim, or gp_prev given equal to gp");
+        return FVM_ERR_ARG;
+    }
+    e = check_align(m, w, nullptr, nullptr, prim, "fvm_grad_f32");
+    if (e) return e;
+    return m->dim == 2 ? launch_rhs<2>(*m, *p, *w, prim, nullptr, nullptr, as_stream(stream), has_prev, 1, 1 | 2)
+                       : launch_rhs<3>(*m, *p, *w, prim, nullptr, nullptr, as_stream(stream), has_prev, 1, 1 | 2);
 }
 
 int fvm_rk_stage_f32(const FvmMeshDev *m, const FvmPhys *p, const FvmWork *w, const FvmStage *st, void *stream) {
     int e = check_mesh(m, p, w, "fvm_rk_stage_f32");
     if (!e) e = check_stage(w, st, "fvm_rk_stage_f32");
     if (!e) e = check_align(m, w, st, nullptr, st->prim, "fvm_rk_stage_f32");
+    if (!e && m->fz_tile) e = check_fused(*m, *w, "fvm_rk_stage_f32");
     if (e) return e;
+    if (m->fz_tile)
+        return launch_rhs<2>(*m, *p, *w, st->prim, nullptr, st, as_stream(stream), st->has_prev, !st->bc_ready, 1 | 16);
     return m->dim == 2 ? launch_rhs<2>(*m, *p, *w, st->prim, nullptr, st, as_stream(stream), st->has_prev, !st->bc_ready)
                        : launch_rhs<3>(*m, *p, *w, st->prim, nullptr, st, as_stream(stream), st->has_prev, !st->bc_ready);
 }
 
 int fvm_rk_stage_part_f32(const FvmMeshDev *m, const FvmPhys *p, const FvmWork *w, const FvmStage *st,
                           int32_t parts, void *stream) {
-    if (parts < 1 || parts > 15) {
-        set_error("fvm_rk_stage_part_f32: parts must be a nonzero mask of 1|2|4|8");
+    if (parts < 1 || parts > 31 || ((parts & 16) && (parts & 14))) {
+        set_error("fvm_rk_stage_part_f32: parts must be a nonzero mask of 1|2|4|8, or 1|16");
         return FVM_ERR_ARG;
     }
     int e = check_mesh(m, p, w, "fvm_rk_stage_part_f32");
     if (!e) e = check_stage(w, st, "fvm_rk_stage_part_f32");
     if (!e) e = check_align(m, w, st, nullptr, st->prim, "fvm_rk_stage_part_f32");
+    if (!e && (parts & 16)) e = check_fused(*m, *w, "fvm_rk_stage_part_f32");
     if (e) return e;
     return m->dim == 2
                ? launch_rhs<2>(*m, *p, *w, st->prim, nullptr, st, as_stream(stream), st->has_prev, !st->bc_ready, parts)

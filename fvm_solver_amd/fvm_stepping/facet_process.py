@@ -24,6 +24,7 @@ import torch
 
 from fvm_solver_amd import _native as nat
 from fvm_solver_amd.config_fvm import BCMode
+from fvm_solver_amd.fvm_stepping.fused_tiles import TilesUnavailable, build_tiles
 
 _K_BC_PART = 64
 
@@ -221,17 +222,17 @@ class DeviceMesh:
     the boundary-mean slot tables list, per tile, the groups of every mesh the tile
     touches (at most 4)."""
 
-    def __init__(self, mesh, n_comp: int, bc_tags, cfg, device, persist_blocks: int = 0):
-        self._build([host_layout(mesh, n_comp, bc_tags, cfg)], device, persist_blocks)
+    def __init__(self, mesh, n_comp: int, bc_tags, cfg, device, persist_blocks: int = 0, tiles: bool = True):
+        self._build([host_layout(mesh, n_comp, bc_tags, cfg)], device, persist_blocks, tiles)
 
     @classmethod
-    def batch(cls, items, device, persist_blocks: int = 0):
+    def batch(cls, items, device, persist_blocks: int = 0, tiles: bool = True):
         """items: iterable of (mesh, n_comp, bc_tags, cfg)."""
         obj = cls.__new__(cls)
-        obj._build([host_layout(*it) for it in items], device, persist_blocks)
+        obj._build([host_layout(*it) for it in items], device, persist_blocks, tiles)
         return obj
 
-    def _build(self, L, device, persist_blocks):
+    def _build(self, L, device, persist_blocks, want_tiles=True):
         D, C = L[0]["D"], L[0]["C"]
         if any(l["D"] != D for l in L):
             raise ValueError("all meshes of a batch must have the same dimension")
@@ -299,6 +300,18 @@ class DeviceMesh:
         fpos[fperm] = np.arange(F)
         h["fac_cell"], h["fperm"], h["fpos"] = fac_cell, fperm, fpos
 
+        # tables of the tile-fused stage kernel (2-D)
+        self.tiles, self.tiles_reason = None, "not built" if want_tiles else "disabled"
+        if want_tiles and D == 2:
+            try:
+                self.tiles = build_tiles(N, D, nbr_s, fpos[c2f_s], fac_cell[:, fperm], h["gval"], h["disp_s"],
+                                         h["eps"], h["info"], h["geo"][fperm], bcell, group, cell_mesh, go)
+                self.tiles_reason = None
+            except TilesUnavailable as e:
+                self.tiles_reason = str(e)
+        elif want_tiles:
+            self.tiles_reason = "3-D meshes use the three-kernel pipeline"
+
         dev = torch.device(device)
 
         def up(a, dt):
@@ -322,23 +335,38 @@ class DeviceMesh:
         if nm > 1:
             self.t["cell_mesh"] = up(cell_mesh, i32)
             self.t["mesh_phys"] = up(self.mesh_phys, f32)
+        self.bc_facet_side = h["side"]
+        self.bc_cell_np = bcell
         if dev.type != "cuda":      # host-only layout (inspection and tests)
-            self.t, self.struct = None, None
+            self.t, self.struct, self.struct3 = None, None, None
             return
         self.volumes64 = up(cat("volumes64"), torch.float64)
         self.h64 = up(cat("h64"), torch.float64)
-        self.struct = nat.FvmMeshDev(
-            dim=D, n_groups=self.n_groups, n_cells=N, n_facets=F, n_bc=Fb, n_slots=self.n_slots, n_meshes=nm,
-            persist_blocks=int(persist_blocks), **{k: v.data_ptr() for k, v in self.t.items()})
-        self.bc_facet_side = h["side"]
-        self.bc_cell_np = bcell
+        base = dict(dim=D, n_groups=self.n_groups, n_cells=N, n_facets=F, n_bc=Fb, n_slots=self.n_slots, n_meshes=nm,
+                    persist_blocks=int(persist_blocks), **{k: v.data_ptr() for k, v in self.t.items()})
+        # struct3: the three-kernel pipeline; struct: the tile-fused stage when tiles exist
+        self.struct3 = nat.FvmMeshDev(**base)
+        if self.tiles is None:
+            self.struct = self.struct3
+            return
+        tl = self.tiles
+        self.tz = dict(
+            fz_tile=up(tl["tab"], i32), fz_halo=up_nz(tl["halo"], i32), fz_hrec=up_nz(tl["hrec"], f32),
+            fz_nbr=up(tl["nbr"].view(np.int16), torch.int16), fz_fdesc=up(tl["fdesc"].view(np.int32), i32),
+            fz_ffac=up_nz(tl["ffac"], i32), fz_fdesc_x=up_nz(tl["fdesc_x"].view(np.int32), i32),
+            fz_geo=up(tl["geo"], f32), fz_geo_x=up_nz(tl["geo_x"], f32),
+            fz_cfac=up(tl["cfac"].view(np.int16), torch.int16))
+        self.struct = nat.FvmMeshDev(**base, fz_ncl=tl["ncl"], fz_ngl=tl["ngl"], fz_nfl=tl["nfl"],
+                                     **{k: v.data_ptr() for k, v in self.tz.items()})
 
     def set_persist_blocks(self, n: int):
-        """Workgroups of the per-cell kernels (0 = one per 256-cell tile); must be a
-        multiple of 8 so the XCD-aware tile mapping stays a bijection."""
+        """Workgroups of the per-cell kernels of the three-kernel pipeline (0 = one per
+        256-cell tile); must be a multiple of 8 so the XCD-aware tile mapping stays a
+        bijection.  The tile-fused stage always runs one workgroup per tile."""
         if n < 0 or n % 8:
             raise ValueError("persist_blocks must be a non-negative multiple of 8")
         self.struct.persist_blocks = int(n)
+        self.struct3.persist_blocks = int(n)
 
 
 def phys_struct(cfg) -> "nat.FvmPhys":
@@ -367,17 +395,19 @@ class FacetCalc:
         self.mesh.bc_facet_mask = mesh_setup.bc_facet_mask
 
     @classmethod
-    def from_device_mesh(cls, dm: "DeviceMesh", cfg, device="cuda"):
-        """A FacetCalc over an already built (e.g. batched) device layout."""
+    def from_device_mesh(cls, dm: "DeviceMesh", cfg, device="cuda", fused: bool = True):
+        """A FacetCalc over an already built (e.g. batched) device layout.  fused=False
+        keeps the three-kernel pipeline even where tile tables exist (callers that read
+        every cell's gradient record after each call)."""
         obj = cls.__new__(cls)
         nat.load_library()
         obj.phy_setup = None
         obj.fvm_mesh = None
         obj.bc_type_str = []
-        obj._init(dm, cfg, torch.device(device))
+        obj._init(dm, cfg, torch.device(device), fused)
         return obj
 
-    def _init(self, dm, cfg, device):
+    def _init(self, dm, cfg, device, fused=True):
         self.device = device
         self.cfg = cfg
         self.mesh = dm
@@ -392,15 +422,52 @@ class FacetCalc:
         dev = self.device
         self.nblk = (N + 255) // 256
         self.ubc = torch.zeros((max(Fb, 1), C), dtype=torch.float32, device=dev)
-        self.gp = torch.zeros((N, (D + 1) * C), dtype=torch.float32, device=dev)
+        # gradient records: updated in place by the three-kernel pipeline; the fused
+        # stage reads the previous call's records of boundary cells while writing the
+        # new ones, so with tile tables there is a second buffer and the two swap
+        self._gpbuf = [torch.zeros((N, (D + 1) * C), dtype=torch.float32, device=dev)
+                       for _ in range(2 if dm.tiles is not None else 1)]
+        self._latest = 0             # buffer holding the last call's records
+        self._gfull = None
         self.bc_part = torch.zeros(4 * max(self.mesh.n_slots, 1), dtype=torch.float32, device=dev)
         self.red = torch.zeros(self.nblk * 3 * C, dtype=torch.float64, device=dev)
         self.fflux = torch.zeros((self.n_facets, C), dtype=torch.float32, device=dev)
-        self.work = nat.FvmWork(ubc=self.ubc.data_ptr(), gp=self.gp.data_ptr(), bc_part=self.bc_part.data_ptr(),
-                                red=self.red.data_ptr(), fflux=self.fflux.data_ptr())
+        self.set_fused(fused)
+        self.work = nat.FvmWork(ubc=self.ubc.data_ptr(), gp=self._gpbuf[0].data_ptr(), bc_part=self.bc_part.data_ptr(),
+                                red=self.red.data_ptr(), fflux=self.fflux.data_ptr(), gp_prev=None)
         self.has_prev = False
+        self._last_has_prev = False
+        self._last_fused = False
         self.bc_ready_ptr = None     # prim buffer whose boundary sums bc_part holds
         self.last_prim = None        # primitives of the last evaluation, if known (saving)
+
+    @property
+    def gp(self):
+        """Gradient records [N, (D+1) C] of the last call (unlimited gradient, then phi);
+        with the fused stage only the rows of cells with a boundary facet are current."""
+        return self._gpbuf[self._latest]
+
+    def set_fused(self, on: bool):
+        """Select the tile-fused stage (one launch per RHS; needs the mesh's tile tables)
+        or the three-kernel pipeline.  Either may follow the other: both carry the
+        boundary cells' records in the latest buffer."""
+        self.fused = bool(on) and self.mesh.tiles is not None
+        self.mesh_struct = self.mesh.struct if self.fused else self.mesh.struct3
+
+    def _before_grad(self):
+        """Point work.gp / gp_prev at the buffers of the next gradient launch."""
+        self._last_has_prev = self.has_prev
+        if self.fused:
+            self.work.gp = self._gpbuf[1 - self._latest].data_ptr()
+            self.work.gp_prev = self._gpbuf[self._latest].data_ptr()
+        else:
+            self.work.gp = self._gpbuf[self._latest].data_ptr()
+            self.work.gp_prev = None
+
+    def _after_grad(self):
+        self._last_fused = self.fused
+        if self.fused:
+            self._latest = 1 - self._latest
 
     # The reference keeps the previous call's limited gradients in ``cell_grads``.
     @property
@@ -408,8 +475,28 @@ class FacetCalc:
         if not self.has_prev:
             return None
         D, C = self.dim, self.n_comp
-        g = self.gp.view(self.n_cells, D + 1, C)
+        g = self._full_records().view(self.n_cells, D + 1, C)
         return g[:, :D, :] * g[:, D:D + 1, :]
+
+    def _full_records(self):
+        """Every cell's gradient record of the last call: the record buffer after the
+        three-kernel pipeline; after the fused stage (which stores boundary cells'
+        records only) recomputed from the last call's primitives and the records it
+        read."""
+        if not self._last_fused:
+            return self.gp
+        if self.last_prim is None:
+            raise RuntimeError("cell_grads needs the primitives of the last evaluation (pass prim to stage())")
+        if self._gfull is None:
+            self._gfull = torch.empty_like(self._gpbuf[0])
+        w = nat.FvmWork(ubc=self.ubc.data_ptr(), gp=self._gfull.data_ptr(), bc_part=self.bc_part.data_ptr(),
+                        red=self.red.data_ptr(), fflux=self.fflux.data_ptr(),
+                        gp_prev=self._gpbuf[1 - self._latest].data_ptr())
+        rc = nat.lib().fvm_grad_f32(self.mesh_struct, self.phys, w, nat.ptr(self.last_prim.contiguous()),
+                                    int(self._last_has_prev), nat.stream_ptr(self.device))
+        nat.check(rc, "fvm_grad_f32")
+        self.bc_ready_ptr = None     # the pre-pass overwrote the boundary sums
+        return self._gfull
 
     def reset_state(self):
         """Forget the previous call's gradients (the reference's ``cell_grads = None``)."""
@@ -422,9 +509,11 @@ class FacetCalc:
             raise ValueError(f"primatives must be float32 [{self.n_cells}, {self.n_comp}]")
         prim = prim.contiguous()
         dudt = out if out is not None else torch.empty_like(prim)
-        rc = nat.lib().fvm_rhs_f32(self.mesh.struct, self.phys, self.work, nat.ptr(prim), nat.ptr(dudt),
+        self._before_grad()
+        rc = nat.lib().fvm_rhs_f32(self.mesh_struct, self.phys, self.work, nat.ptr(prim), nat.ptr(dudt),
                                    int(self.has_prev), nat.stream_ptr(self.device))
         nat.check(rc, "fvm_rhs_f32")
+        self._after_grad()
         self.last_prim = prim
         self.has_prev = True
         self.bc_ready_ptr = None
@@ -437,15 +526,21 @@ class FacetCalc:
         self.last_prim = prim
         st.has_prev = int(self.has_prev)
         st.bc_ready = int(self.bc_ready_ptr is not None and self.bc_ready_ptr == st.prim)
+        grad = parts is None or bool(parts & (2 | 16))
+        if grad:
+            self._before_grad()
         if parts is None:
-            rc = nat.lib().fvm_rk_stage_f32(self.mesh.struct, self.phys, self.work, st, nat.stream_ptr(self.device))
+            rc = nat.lib().fvm_rk_stage_f32(self.mesh_struct, self.phys, self.work, st, nat.stream_ptr(self.device))
         else:
-            rc = nat.lib().fvm_rk_stage_part_f32(self.mesh.struct, self.phys, self.work, st, parts,
+            if not self.fused:
+                parts &= ~16
+            rc = nat.lib().fvm_rk_stage_part_f32(self.mesh_struct, self.phys, self.work, st, parts,
                                                  nat.stream_ptr(self.device))
         nat.check(rc, "fvm_rk_stage_f32")
-        if parts is None or parts & 2:
+        if grad:
+            self._after_grad()
             self.has_prev = True
-        if parts is None or parts & 8:
+        if parts is None or parts & (8 | 16):
             self.bc_ready_ptr = st.prim_next
 
     def invalidate_boundary_means(self):

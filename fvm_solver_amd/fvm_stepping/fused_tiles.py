@@ -1,0 +1,290 @@
+"""Tile tables of the tile-fused RK stage kernel (``k_fused`` in csrc/rhs.hip).
+
+The fused kernel gives each workgroup one tile of 256 consecutive cells and evaluates,
+in LDS, everything a stage needs for those cells: the gradient and limiter of the tile's
+cells and of their one-ring halo (H1, the cells outside the tile that share a facet with
+it), the fluxes of every facet of the tile's cells, and the divergence + stage update.
+H1 gradients need the primitives of H1's own neighbours (H2), so the tile stages
+primitives of tile + H1 + H2.  No reference counterpart: the reference evaluates the
+same operators as separate SpMV/SpMM and elementwise passes over the whole mesh
+(facet_process.py:332-450, fvm_equation.py:406-435).
+
+Local cell numbering of tile t (cells c0 = 256 t ...):
+    0 .. 255           the tile's own cells (c0 + l; a short last tile leaves a gap)
+    256 .. 256+n1-1    H1, ascending global id
+    256+n1 .. +n2-1    H2, ascending global id
+
+Facets of tile t: its *owned* facets -- those whose lowest adjacent cell is in the tile,
+a contiguous run [fo0, fo0 + nfo) of the device facet order (sorted by lowest adjacent
+cell) -- then its *foreign* facets, interior facets of a tile cell owned by an earlier
+tile (evaluated by both tiles, each keeping its own cell's share).
+
+Facet descriptor (uint32): bits 0-11 local id of the slot-0 cell, 12-13 that cell's slot
+index k (cell_facet[k] == f), 14-25 / 26-27 the same for the slot-1 cell, bit 28 the
+slot-0 cell has the lower global id (A_face_grad's ascending column order), bit 29
+boundary facet, bit 30 the outer (boundary) side is slot 0.  For a boundary facet the
+outer side's local id is 0xFFF.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+TILE = 256
+TAB = 12          # ints per tile in the tile table
+NONE16 = 0xFFFF   # local neighbour id of a boundary slot
+LOCAL_MAX = 0xFFF
+
+
+class TilesUnavailable(ValueError):
+    pass
+
+
+def _lookup(keys: np.ndarray, q: np.ndarray) -> np.ndarray:
+    """Positions of q in the sorted unique array keys; -1 where absent."""
+    pos = np.searchsorted(keys, q)
+    pos = np.minimum(pos, max(len(keys) - 1, 0))
+    ok = (len(keys) > 0) & (keys[pos] == q) if len(keys) else np.zeros(len(q), bool)
+    return np.where(ok, pos, -1)
+
+
+def build_tiles(N: int, D: int, nbr: np.ndarray, cell_facet: np.ndarray, fac_cell: np.ndarray,
+                gval: np.ndarray, disp: np.ndarray, eps: np.ndarray, info: np.ndarray, geo: np.ndarray,
+                bcell: np.ndarray, group: np.ndarray, cell_mesh: np.ndarray, group_off: np.ndarray,
+                lds_limit: int = 64 * 1024) -> dict:
+    """Tile tables for the fused kernel.
+
+    nbr [N, NL] neighbour per slot (>= N: N + boundary index); cell_facet [N, NL] device
+    facet positions; fac_cell [2, F] device order; gval [N, D, NL+1]; disp [N, NL, D];
+    eps, info [N]; geo [F, G] device order (first 3D+2 floats: n, n_hat, |n|, 1/(n.d),
+    X); bcell/group [Fb]; cell_mesh [N]; group_off = first group of each mesh (+ total).
+    Raises TilesUnavailable when a tile's working set does not fit ``lds_limit`` bytes
+    of LDS or the local ids overflow (a badly ordered mesh)."""
+    NL = D + 1
+    C = D + 2
+    nbr = np.asarray(nbr, np.int64)
+    cell_facet = np.asarray(cell_facet, np.int64)
+    fac_cell = np.asarray(fac_cell, np.int64)
+    F = fac_cell.shape[1]
+    T = (N + TILE - 1) // TILE
+    tile_of = np.arange(N, dtype=np.int64) // TILE
+
+    # ---- H1: (tile, cell) pairs across tile borders ----
+    src = np.repeat(np.arange(N, dtype=np.int64), NL)
+    dst = nbr.reshape(-1)
+    inner = dst < N
+    a, b = src[inner], dst[inner]
+    cross = tile_of[a] != tile_of[b]
+    K1 = np.unique(tile_of[a[cross]] * N + b[cross])
+    t1 = K1 // N
+    n1 = np.bincount(t1, minlength=T).astype(np.int64)
+    s1 = np.concatenate([[0], np.cumsum(n1)])
+    # ---- H2: neighbours of H1 outside tile and H1 ----
+    h1c = K1 % N
+    cand_t = np.repeat(t1, NL)
+    cand_c = nbr[h1c].reshape(-1)
+    ok = cand_c < N
+    cand_t, cand_c = cand_t[ok], cand_c[ok]
+    ok = tile_of[cand_c] != cand_t
+    ck = np.unique(cand_t[ok] * N + cand_c[ok])
+    K2 = ck[_lookup(K1, ck) < 0]
+    t2 = K2 // N
+    n2 = np.bincount(t2, minlength=T).astype(np.int64)
+    s2 = np.concatenate([[0], np.cumsum(n2)])
+
+    if T and int((n1 + n2).max(initial=0)) + TILE > LOCAL_MAX:
+        raise TilesUnavailable("tile halo too large for 12-bit local ids (cells not ordered along a curve?)")
+
+    def local(t, c):
+        """Local ids of global cells c in tiles t (vectorised); -1 if not staged."""
+        t = np.asarray(t, np.int64)
+        c = np.asarray(c, np.int64)
+        out = np.full(c.shape, -1, np.int64)
+        own = tile_of[np.minimum(c, N - 1)] == t
+        own &= c < N
+        out[own] = c[own] - t[own] * TILE
+        rest = ~own & (c < N)
+        key = t[rest] * N + c[rest]
+        p1 = _lookup(K1, key)
+        p2 = _lookup(K2, key)
+        tr = t[rest]
+        loc = np.where(p1 >= 0, TILE + p1 - s1[tr], np.where(p2 >= 0, TILE + n1[tr] + p2 - s2[tr], -1))
+        out[rest] = loc
+        return out
+
+    # ---- local neighbour rows of own + H1 cells ----
+    nrow = TILE + n1
+    nbr_off = np.concatenate([[0], np.cumsum(nrow)])
+    rows = np.full((int(nbr_off[-1]), 4), NONE16, np.uint16)
+    # own cells (rows of a short last tile beyond N stay NONE16)
+    own_row = nbr_off[tile_of] + (np.arange(N) - tile_of * TILE)
+    ln = local(np.repeat(tile_of, NL), nbr.reshape(-1)).reshape(N, NL)
+    if ((ln < 0) & (nbr < N)).any():
+        raise AssertionError("own neighbour not staged")
+    rows[own_row, :NL] = np.where(nbr < N, ln, NONE16).astype(np.uint16)
+    # H1 cells
+    h1_row = nbr_off[t1] + TILE + (np.arange(len(K1)) - s1[t1])
+    lh = local(np.repeat(t1, NL), nbr[h1c].reshape(-1)).reshape(-1, NL)
+    if ((lh < 0) & (nbr[h1c] < N)).any():
+        raise AssertionError("H1 neighbour not staged")
+    rows[h1_row, :NL] = np.where(nbr[h1c] < N, lh, NONE16).astype(np.uint16)
+
+    # ---- H1 static records: LSQ values [NL+1][D], displacements [NL][D], eps, info ----
+    HREC = 16 if D == 2 else 32
+    hrec = np.zeros((len(K1), HREC), np.float32)
+    hrec[:, :(NL + 1) * D] = np.asarray(gval, np.float32)[h1c].transpose(0, 2, 1).reshape(len(K1), -1)
+    o = (NL + 1) * D
+    hrec[:, o:o + NL * D] = np.asarray(disp, np.float32)[h1c].reshape(len(K1), -1)
+    hrec[:, o + NL * D] = np.asarray(eps, np.float32)[h1c]
+    hrec[:, o + NL * D + 1] = np.asarray(info, np.int32)[h1c].view(np.float32)
+
+    # ---- facets ----
+    lo = fac_cell.min(0)
+    hi = fac_cell.max(0)
+    if np.any(np.diff(lo) < 0):
+        raise ValueError("facets are not in device order (ascending lowest cell)")
+    own_t = lo // TILE
+    fo0 = np.searchsorted(own_t, np.arange(T), side="left")
+    fo1 = np.searchsorted(own_t, np.arange(T), side="right")
+    nfo = fo1 - fo0
+    interior = hi < N
+    fx = np.flatnonzero(interior & (tile_of[np.minimum(hi, N - 1)] != own_t))
+    ft = tile_of[hi[fx]]
+    order = np.lexsort((fx, ft))
+    fx, ft = fx[order], ft[order]
+    nfor = np.bincount(ft, minlength=T).astype(np.int64)
+    foff = np.concatenate([[0], np.cumsum(nfor)])
+
+    # slot index of each facet side
+    side_slot = np.full((2, F), 0, np.int64)
+    cidx = np.repeat(np.arange(N, dtype=np.int64), NL)
+    kidx = np.tile(np.arange(NL, dtype=np.int64), N)
+    fl = cell_facet.reshape(-1)
+    sv = np.where(fac_cell[0, fl] == cidx, 0, 1)
+    if np.any(fac_cell[sv, fl] != cidx):
+        raise AssertionError("cell_facet and fac_cell disagree")
+    side_slot[sv, fl] = kidx
+
+    def descriptors(t, f):
+        c0_, c1_ = fac_cell[0, f], fac_cell[1, f]
+        bnd = (c0_ >= N) | (c1_ >= N)
+        outer0 = c0_ >= N
+        l0 = np.where(c0_ < N, local(t, np.minimum(c0_, N - 1)), LOCAL_MAX)
+        l1 = np.where(c1_ < N, local(t, np.minimum(c1_, N - 1)), LOCAL_MAX)
+        if np.any(l0 < 0) or np.any(l1 < 0):
+            raise AssertionError("facet side not staged")
+        k0 = np.where(c0_ < N, side_slot[0, f], 0)
+        k1 = np.where(c1_ < N, side_slot[1, f], 0)
+        lf = (c0_ < c1_).astype(np.int64)
+        d = (l0 | (k0 << 12) | (l1 << 14) | (k1 << 26) | (lf << 28) | (bnd.astype(np.int64) << 29)
+             | (outer0.astype(np.int64) << 30))
+        return d.astype(np.uint32)
+
+    fdesc = descriptors(own_t, np.arange(F))
+    fdesc_x = descriptors(ft, fx)
+    GF = 3 * D + 2 if D == 2 else 12       # n, n_hat, |n|, 1/(n.d), X  (3-D padded to 12)
+    geo = np.asarray(geo, np.float32)
+    geo_c = np.zeros((F, 8 if D == 2 else 12), np.float32)
+    geo_c[:, :3 * D + 2] = geo[:, :3 * D + 2]
+    geo_x = geo_c[fx]
+
+    # ---- tile-local facet index of each cell slot ----
+    cf_t = np.repeat(tile_of, NL)
+    cf_f = cell_facet.reshape(-1)
+    tl = np.where(own_t[cf_f] == cf_t, cf_f - fo0[cf_t], -1)
+    xk = ft * F + fx                                   # sorted (ft, fx)
+    need = tl < 0
+    px = _lookup(xk, cf_t[need] * F + cf_f[need])
+    if np.any(px < 0):
+        raise AssertionError("cell facet neither owned nor foreign")
+    tl[need] = nfo[cf_t[need]] + px - foff[cf_t[need]]
+    cfac = np.full((N, 4), 0, np.uint16)
+    cfac[:, :NL] = tl.reshape(N, NL).astype(np.uint16)
+
+    # ---- characteristic-group means a tile needs (groups of the meshes it touches) ----
+    gcell = np.asarray(bcell, np.int64)[np.asarray(group) >= 0]
+    needs = np.zeros(T, bool)
+    if len(gcell):
+        has = np.zeros(N, bool)
+        has[gcell] = True
+        needs[tile_of[gcell]] = True
+        needs[t1[has[h1c]]] = True
+    cm = np.asarray(cell_mesh, np.int64)
+    m_lo = cm[np.arange(T) * TILE]
+    m_hi = cm[np.minimum(np.arange(T) * TILE + TILE - 1, N - 1)]
+    g0 = np.asarray(group_off, np.int64)[m_lo]
+    ng = np.where(needs, np.asarray(group_off, np.int64)[m_hi + 1] - g0, 0)
+    if np.any(ng > 4):
+        raise TilesUnavailable("a tile touches more than 4 boundary groups")
+
+    tab = np.zeros((T, TAB), np.int32)
+    halo_off = np.concatenate([[0], np.cumsum(n1 + n2)])
+    tab[:, 0] = halo_off[:-1]
+    tab[:, 1] = n1
+    tab[:, 2] = n2
+    tab[:, 3] = nbr_off[:-1]
+    tab[:, 4] = fo0
+    tab[:, 5] = nfo
+    tab[:, 6] = foff[:-1]
+    tab[:, 7] = nfor
+    tab[:, 8] = g0
+    tab[:, 9] = ng
+    tab[:, 10] = s1[:-1]
+    halo = np.zeros(int(halo_off[-1]), np.int32)
+    # H1 then H2 of each tile
+    halo[halo_off[t1] + (np.arange(len(K1)) - s1[t1])] = h1c
+    halo[halo_off[t2] + n1[t2] + (np.arange(len(K2)) - s2[t2])] = K2 % N
+
+    if int(n1.max(initial=0)) > TILE or int((nfo + nfor).max(initial=0)) > 2 * TILE:
+        raise TilesUnavailable("a tile has more than 256 halo cells or 512 facets")
+    ncl = TILE + int((n1 + n2).max(initial=0))
+    ngl = TILE + int(n1.max(initial=0))
+    nfl = int((nfo + nfor).max(initial=1))
+    GLN = D * (D + 1)
+    lds = 4 * (ncl * C + ngl * (NL * C + GLN) + nfl * C)
+    if lds > lds_limit:
+        raise TilesUnavailable(f"tile working set {lds} B exceeds {lds_limit} B of LDS")
+    return dict(tab=tab, halo=halo, hrec=hrec, nbr=rows, fdesc=fdesc, ffac=fx.astype(np.int32), fdesc_x=fdesc_x,
+                geo=geo_c, geo_x=geo_x, cfac=cfac, ncl=ncl, ngl=ngl, nfl=nfl, lds_bytes=lds, n_tiles=T,
+                halo_cells=(int(n1.sum()), int(n2.sum())), foreign=int(nfor.sum()))
+
+
+def check_tiles(t: dict, N: int, D: int, nbr, cell_facet, fac_cell) -> None:
+    """Rebuild, from the tile tables alone, every global relation the fused kernel relies
+    on and compare it with the mesh (raises AssertionError on the first mismatch)."""
+    NL = D + 1
+    nbr = np.asarray(nbr, np.int64)
+    cell_facet = np.asarray(cell_facet, np.int64)
+    fac_cell = np.asarray(fac_cell, np.int64)
+    tab, halo = t["tab"], t["halo"]
+    for ti in range(t["n_tiles"]):
+        ho, n1, n2, no, fo0, nfo, foff, nfor = (int(v) for v in tab[ti, :8])
+        c0 = ti * TILE
+        nown = min(TILE, N - c0)
+        glob = np.full(TILE + n1 + n2, -1, np.int64)
+        glob[:nown] = c0 + np.arange(nown)
+        glob[TILE:] = halo[ho:ho + n1 + n2]
+        rows = t["nbr"][no:no + TILE + n1].astype(np.int64)
+        for l in list(range(nown)) + list(range(TILE, TILE + n1)):
+            g = glob[l]
+            for k in range(NL):
+                r = rows[l, k]
+                want = nbr[g, k]
+                assert (r == NONE16) == (want >= N), (ti, l, k)
+                if r != NONE16:
+                    assert glob[r] == want, (ti, l, k, glob[r], want)
+        fl = np.concatenate([np.arange(fo0, fo0 + nfo), t["ffac"][foff:foff + nfor]])
+        dl = np.concatenate([t["fdesc"][fo0:fo0 + nfo], t["fdesc_x"][foff:foff + nfor]]).astype(np.int64)
+        for f, d in zip(fl, dl):
+            for s, (sh_l, sh_k) in enumerate(((0, 12), (14, 26))):
+                l = (d >> sh_l) & LOCAL_MAX
+                k = (d >> sh_k) & 3
+                c = fac_cell[s, f]
+                if c >= N:
+                    assert l == LOCAL_MAX and ((d >> 29) & 1) and ((d >> 30) & 1) == (s == 0)
+                else:
+                    assert glob[l] == c and cell_facet[c, k] == f, (ti, f, s)
+            assert ((d >> 28) & 1) == (fac_cell[0, f] < fac_cell[1, f])
+        for i in range(nown):
+            c = c0 + i
+            for k in range(NL):
+                assert fl[t["cfac"][c, k]] == cell_facet[c, k], (ti, c, k)

@@ -60,7 +60,7 @@ class LearnedCorrector:
         self.M = solver.n_meshes
         # a private RHS evaluator on the same device mesh: the solver's own carried
         # gradient state is not touched by the corrector's evaluations
-        self.E = FacetCalc.from_device_mesh(dm, solver.E.cfg, self.device)
+        self.E = FacetCalc.from_device_mesh(dm, solver.E.cfg, self.device, fused=False)
         self.nbr = dm.t["cell_nbr"]                                   # int32 [NL, N]
         cm = dm.host["cell_mesh"].astype(np.int32)
         self.cell_mesh = torch.from_numpy(cm).to(self.device)

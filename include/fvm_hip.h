@@ -122,6 +122,32 @@ typedef struct FvmMeshDev {
     int32_t persist_blocks;     /* workgroups of the per-cell kernels (0 = one per
                                    256-cell tile); tiles are walked grid-stride       */
     int32_t pad1;
+    /* tile-fused stage (2-D; fz_tile NULL = the three-kernel pipeline).  Tile t holds
+       cells 256t..256t+255 (local ids 0..255), its one-ring halo H1 (local ids 256..,
+       ascending global id) and H1's other neighbours H2 (after H1).  Its facets are
+       the owned run [fo0, fo0+nfo) of the device facet order (lowest adjacent cell in
+       the tile), then nfor foreign facets (interior facets of a tile cell whose lowest
+       cell is in an earlier tile).  Facet descriptor bits: 0-11 local id of the slot-0
+       cell, 12-13 its slot index k, 14-25 / 26-27 the same for the slot-1 cell, 28
+       slot-0 cell has the lower global id, 29 boundary facet, 30 the outer side is
+       slot 0 (outer local id 0xFFF).                                                 */
+    const int32_t *fz_tile;     /* [T][12] halo_off, n1, n2, nbr_off, fo0, nfo, foff,
+                                   nfor, g0, ng (group means the tile needs), h1_off  */
+    const int32_t *fz_halo;     /* H1 then H2 global ids of each tile, at halo_off    */
+    const float *fz_hrec;       /* [sum n1][16] H1 records at h1_off: LSQ values
+                                   [NL+1][d], displacements [NL][d], eps, info bits   */
+    const uint16_t *fz_nbr;     /* [rows][4] local neighbour ids (0xFFFF = boundary)
+                                   of the tile's 256 + n1 cells, rows at nbr_off      */
+    const uint32_t *fz_fdesc;   /* [F] descriptor of each facet in its owner tile     */
+    const int32_t *fz_ffac;     /* foreign facet ids of each tile, at foff            */
+    const uint32_t *fz_fdesc_x; /* their descriptors                                  */
+    const float *fz_geo;        /* [F][8] n, n_hat, |n|, 1/(n.d), X (device order)    */
+    const float *fz_geo_x;      /* [sum nfor][8] the same for the foreign lists       */
+    const uint16_t *fz_cfac;    /* [N][4] tile-local facet index of each cell slot    */
+    int32_t fz_ncl;             /* LDS rows: max staged cells (256 + n1 + n2)         */
+    int32_t fz_ngl;             /*           max gradient cells (256 + n1)            */
+    int32_t fz_nfl;             /*           max facets (nfo + nfor)                  */
+    int32_t fz_pad;
 } FvmMeshDev;
 
 typedef struct FvmPhys {      /* scalars of FluidConstitution / SlopeLimiter / KT      */
@@ -143,13 +169,18 @@ typedef struct FvmWork {      /* device scratch owned by the caller             
                          primitives the next gradient launch reads                    */
     double *red;      /* [ceil(N/256)][3C] fp64 partial column norms (adaptive dt)    */
     float *fflux;     /* [F][C] facet fluxes of the current evaluation (device order) */
+    const float *gp_prev; /* records of the previous call, read by the Neumann
+                         correction: gp itself (NULL) for the three-kernel pipeline;
+                         a second buffer for the tile-fused stage, which writes the
+                         records of boundary cells only, into gp (callers swap the two
+                         buffers between calls)                                        */
 } FvmWork;
 
 /* One RHS evaluation dU/dt = FVMEquation.forward(prim) (fvm_equation.py:406-435).
  * has_prev != 0 applies the Neumann non-orthogonal correction with the gradients the
- * previous call left in work->gp (facet_boundary.py:135-136).  Up to four launches:
- * boundary-mean pre-pass, gradient + limiter (+ boundary values), facet fluxes,
- * divergence. */
+ * previous call left (work->gp_prev, or work->gp) (facet_boundary.py:135-136).  With
+ * tile tables (m->fz_tile) one tile-fused launch, else up to four: boundary-mean
+ * pre-pass, gradient + limiter (+ boundary values), facet fluxes, divergence. */
 int fvm_rhs_f32(const FvmMeshDev *m, const FvmPhys *p, const FvmWork *w, const float *prim,
                 float *dudt, int32_t has_prev, void *stream);
 
@@ -190,11 +221,19 @@ int fvm_rk_stage_f32(const FvmMeshDev *m, const FvmPhys *p, const FvmWork *w, co
 
 /* The same stage split into its launches, for per-kernel timing with stream events:
  * parts is a mask of 1 = boundary-mean pre-pass (only if !bc_ready), 2 = gradient +
- * limiter + boundary values, 4 = facet fluxes, 8 = divergence + stage update.  Launching
- * the masks 1, 2, 4, 8 in turn equals fvm_rk_stage_f32.  No reference counterpart
- * (measurement hook). */
+ * limiter + boundary values, 4 = facet fluxes, 8 = divergence + stage update (the
+ * three-kernel pipeline, also on a mesh with tile tables), or 16 = the tile-fused stage
+ * (needs m->fz_tile).  Launching the masks 1, 2, 4, 8 in turn, or 1, 16, equals
+ * fvm_rk_stage_f32.  No reference counterpart (measurement hook). */
 int fvm_rk_stage_part_f32(const FvmMeshDev *m, const FvmPhys *p, const FvmWork *w, const FvmStage *st,
                           int32_t parts, void *stream);
+
+/* Boundary values, gradient and limiter of every cell of prim into work->gp (and
+ * work->ubc), reading the previous records from work->gp_prev (or gp): the full
+ * cell_grads state the reference keeps (facet_process.py:387-418), for callers of the
+ * tile-fused stage, which keeps only boundary cells' records. */
+int fvm_grad_f32(const FvmMeshDev *m, const FvmPhys *p, const FvmWork *w, const float *prim, int32_t has_prev,
+                 void *stream);
 
 /* Adaptive.update_stepsize (integrators.py:42-62) from the partial norms the last
  * adaptive stage left in work->red: dt *= alpha + (1 - alpha) 0.9 E^(-1/order). */

@@ -1,0 +1,122 @@
+"""Tile tables of the tile-fused stage kernel, checked on the host (no GPU).
+
+Every relation the kernel reads from the tables (local neighbour ids, facet sides and
+slots, tile-local facet positions) is rebuilt from the tables alone and compared with
+the mesh's global arrays; the values the fused kernel takes from other arrays than the
+three-kernel pipeline (compact facet geometry, halo records, the displacements of each
+facet side) must be the very same floats, so both pipelines compute bitwise alike.
+"""
+import numpy as np
+import pytest
+
+from fvm_solver_amd.config_fvm import ConfigChannel
+from fvm_solver_amd.fvm_stepping.facet_process import DeviceMesh
+from fvm_solver_amd.fvm_stepping.fused_tiles import NONE16, TILE, TilesUnavailable, build_tiles, check_tiles
+from fvm_solver_amd.mesh_gen.channel import channel_mesh_2d
+from fvm_solver_amd.mesh_utils.fvm_mesh import mesh_from_tuple
+from fvm_solver_amd.mesh_utils.mesh_store import channel_bc_tags
+from tests.golden_cases import load, mesh_tuple
+
+
+def _dm(pts, cells, fi, fb, tags, **kw):
+    mesh = mesh_from_tuple(pts, cells, fi, fb, device="cpu")
+    return DeviceMesh(mesh, 4, channel_bc_tags(tags, 2), ConfigChannel(), "cpu", **kw)
+
+
+def _check(dm):
+    h = dm.host
+    t = dm.tiles
+    assert t is not None, dm.tiles_reason
+    cf = h["fpos"][h["c2f_s"]]
+    fc = h["fac_cell"][:, h["fperm"]]
+    check_tiles(t, dm.N, 2, h["nbr_s"], cf, fc)
+    return t, h, cf, fc
+
+
+@pytest.fixture(scope="module", params=["golden", "channel20k"])
+def dm(request):
+    if request.param == "golden":
+        pts, cells, fi, fb, tags = mesh_tuple(load("ref_2d.npz"))
+    else:
+        pts, cells, (fi, fb), tags = channel_mesh_2d(20000, seed=3)
+    return _dm(pts, cells, fi, fb, tags)
+
+
+def test_tables_rebuild_the_mesh(dm):
+    t, h, cf, fc = _check(dm)
+    N = dm.N
+    # every cell of the mesh appears in exactly one tile as an own cell, and every
+    # facet is owned by exactly one tile
+    tab = t["tab"]
+    assert tab[:, 5].sum() == dm.F
+    assert np.array_equal(tab[:, 4], np.concatenate([[0], np.cumsum(tab[:-1, 5])]))
+    # halo lists: H1 cells are outside their tile and adjacent to it
+    for ti in range(t["n_tiles"]):
+        ho, n1, n2 = (int(v) for v in tab[ti, :3])
+        h1 = t["halo"][ho:ho + n1]
+        assert np.all(h1 // TILE != ti) and np.all(np.diff(h1) > 0)
+        assert np.all(t["halo"][ho + n1:ho + n1 + n2] // TILE != ti)
+
+
+def test_fused_inputs_are_the_same_floats(dm):
+    t, h, cf, fc = _check(dm)
+    NL = 3
+    geo = h["geo"][h["fperm"]]
+    # compact facet geometry = the first 3d+2 floats of the facet record
+    assert np.array_equal(t["geo"], geo[:, :8])
+    assert np.array_equal(t["geo_x"], geo[t["ffac"], :8])
+    # the three-kernel facet kernel takes each side's displacement from the facet
+    # record; the fused kernel from the cell's own displacement list at the side's slot
+    F = dm.F
+    for s in range(2):
+        c = fc[s]
+        inner = c < dm.N
+        f = np.flatnonzero(inner)
+        k = (t["fdesc"][f].astype(np.int64) >> (12 if s == 0 else 26)) & 3
+        assert np.array_equal(h["disp_s"][c[inner], k].astype(np.float32), geo[f, 8 + 2 * s:10 + 2 * s])
+    # halo records carry the halo cells' own LSQ values, displacements, eps and info
+    tab = t["tab"]
+    for ti in range(0, t["n_tiles"], max(1, t["n_tiles"] // 7)):
+        ho, n1, h1o = int(tab[ti, 0]), int(tab[ti, 1]), int(tab[ti, 10])
+        cells = t["halo"][ho:ho + n1]
+        rec = t["hrec"][h1o:h1o + n1]
+        assert np.array_equal(rec[:, :8], h["gval"][cells].astype(np.float32).transpose(0, 2, 1).reshape(n1, -1))
+        assert np.array_equal(rec[:, 8:14], h["disp_s"][cells].astype(np.float32).reshape(n1, -1))
+        assert np.array_equal(rec[:, 14], h["eps"][cells].astype(np.float32))
+        assert np.array_equal(rec[:, 15].view(np.int32), h["info"][cells].astype(np.int32))
+    # boundary slots are marked, interior ones point at staged cells
+    own_rows = t["nbr"][np.repeat(tab[:, 3], TILE) + np.tile(np.arange(TILE), t["n_tiles"])][:dm.N, :NL]
+    assert np.array_equal(own_rows == NONE16, h["nbr_s"] >= dm.N)
+    assert t["lds_bytes"] <= 64 * 1024
+
+
+def test_batch_tables():
+    items = []
+    for i, n in enumerate((3000, 700, 5000)):
+        pts, cells, (fi, fb), tags = channel_mesh_2d(n, seed=10 + i)
+        items.append((mesh_from_tuple(pts, cells, fi, fb, device="cpu"), 4, channel_bc_tags(tags, 2),
+                      ConfigChannel()))
+    dm = DeviceMesh.batch(items, "cpu")
+    t, h, cf, fc = _check(dm)
+    # group means: a tile needing them lists the groups of every mesh it touches
+    tab = t["tab"]
+    assert np.all(tab[:, 9] <= 4) and tab[:, 9].max() > 0
+
+
+def test_badly_ordered_mesh_falls_back():
+    pts, cells, (fi, fb), tags = channel_mesh_2d(30000, seed=5)
+    rng = np.random.default_rng(0)
+    dm = _dm(pts, cells[rng.permutation(len(cells))], fi, fb, tags)
+    assert dm.tiles is None and dm.tiles_reason not in (None, "disabled")
+    off = _dm(pts, cells, fi, fb, tags, tiles=False)
+    assert off.tiles is None
+
+
+def test_lds_limit_is_enforced():
+    pts, cells, (fi, fb), tags = channel_mesh_2d(3000, seed=2)
+    dm = _dm(pts, cells, fi, fb, tags)
+    h = dm.host
+    with pytest.raises(TilesUnavailable):
+        build_tiles(dm.N, 2, h["nbr_s"], h["fpos"][h["c2f_s"]], h["fac_cell"][:, h["fperm"]], h["gval"], h["disp_s"],
+                    h["eps"], h["info"], h["geo"][h["fperm"]], h["bcell"], h["group"], h["cell_mesh"],
+                    dm.group_offsets, lds_limit=1024)

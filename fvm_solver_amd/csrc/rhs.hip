@@ -28,8 +28,9 @@
 // separately, in the order the reference's ATen op sequence performs them, so that the
 // branchy limiter sees the same inputs as the CPU restatement.  Two places trade exact
 // rounding for VALU work (the stage is VALU-bound): the facet flux uses explicit fused
-// multiply-adds and takes (T/T0)^1.5 as x sqrt(x), and the limiter factors divide
-// through the hardware reciprocal.
+// multiply-adds, hardware square roots and reciprocals and takes (T/T0)^1.5 as
+// x sqrt(x), and the limiter factors divide through the hardware reciprocal.  Each
+// costs about one ulp per operation; the 1000-step parity bar is unaffected.
 #include "fvm_common.hpp"
 
 namespace fvm {
@@ -48,8 +49,14 @@ struct Dims {
 
 // Quotient through the hardware reciprocal (1 ulp) and one product: the limiter factors
 // are smooth in their inputs (continuous across their branch), and an IEEE division
-// costs ~10 VALU operations; ~12 of them per cell dominated the limiter.
+// costs ~10 VALU operations; ~12 of them per cell dominated the limiter.  The default
+// limiter (p = 4) also fuses its multiply-adds.
 __device__ __forceinline__ float quot(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+
+// Square root by the hardware instruction (1 ulp; the correctly rounded sequence costs
+// ~14 VALU operations).  Arguments here are temperatures and squared speeds: normal
+// floats, never denormal.
+__device__ __forceinline__ float sqrt_hw(float x) { return __builtin_amdgcn_sqrtf(x); }
 
 template <int lp>
 __device__ __forceinline__ float lim_phi(float delta, float dU, float eps) {
@@ -57,10 +64,10 @@ __device__ __forceinline__ float lim_phi(float delta, float dU, float eps) {
         const float a = fabsf(delta), b = fabsf(dU);
         const float b2 = 2.0f * b;
         const float a2 = a * a;
-        const float a_eps = a2 * a2 + eps;
-        const float S = b2 * (a2 + b2 * (b2 - a));
+        const float a_eps = fmaf(a2, a2, eps);
+        const float S = b2 * fmaf(b2, b2 - a, a2);
         const float d3 = (delta * delta) * delta;
-        const float phi = quot(a_eps + a * S, a_eps + b * (d3 + S));
+        const float phi = quot(fmaf(a, S, a_eps), fmaf(b, d3 + S, a_eps));
         return a < b2 ? phi : 1.0f;
     }
     if constexpr (lp == 1) {
@@ -132,17 +139,21 @@ __device__ __forceinline__ void load_prim(const float *__restrict__ src, float (
     load_vec<D + 2>(src, u);
 }
 
+// FluidConstitution.state_to_primative (fvm_equation.py:35-44) with one IEEE division
+// (1/rho, then products: one extra rounding per component against the reference's
+// quotients).
 template <int D>
 __device__ __forceinline__ void state_to_prim(const float (&U)[D + 2], float inv_cv, float (&P)[D + 2]) {
     const float rho = U[D];
+    const float ir = 1.0f / rho;
     float sq = 0.0f;
 #pragma unroll
     for (int i = 0; i < D; ++i) {
-        P[i] = U[i] / rho;
+        P[i] = U[i] * ir;
         sq = (i == 0) ? P[i] * P[i] : sq + P[i] * P[i];
     }
     P[D] = rho;
-    P[D + 1] = inv_cv * (U[D + 1] / rho - 0.5f * sq);
+    P[D + 1] = inv_cv * (U[D + 1] * ir - 0.5f * sq);
 }
 
 // ------------------------------------------------------------------------------------
@@ -489,9 +500,9 @@ __device__ __forceinline__ void facet_flux(const FvmPhys &p, float mu0, float mu
         for (int j = 0; j < D; ++j) Dm_[i][j] = 0.5f * (gV[i][j] + gV[j][i]);
     const float I1 = Dm_[0][0] + Dm_[1][1];
     // Sutherland: (T/T0)^1.5 as x sqrt(x) (a general powf costs ~50 operations)
-    const float x = Tm / p.T0;
-    const float mu = ((mu0 * (x * sqrtf(x))) * p.T0pS) / (Tm + p.S);
-    const float mub = (mub0 * (Tm * Tm)) / p.T0sq;
+    const float x = quot(Tm, p.T0);
+    const float mu = quot((mu0 * (x * sqrt_hw(x))) * p.T0pS, Tm + p.S);
+    const float mub = quot(mub0 * (Tm * Tm), p.T0sq);
     const float a0 = (-mub) * I1;
     const float a1 = -2.0f * mu;
     float tau[D][D];
@@ -501,7 +512,7 @@ __device__ __forceinline__ void facet_flux(const FvmPhys &p, float mu0, float mu
         for (int j = 0; j < D; ++j) tau[i][j] = i == j ? fmaf(a1, Dm_[i][j], a0) : a1 * Dm_[i][j];
 
     const float P0 = (p.R * r0) * T0, P1 = (p.R * r1) * T1;
-    const float c0 = sqrtf(p.gR * T0), c1 = sqrtf(p.gR * T1);
+    const float c0 = sqrt_hw(p.gR * T0), c1 = sqrt_hw(p.gR * T1);
 
     // Advection (fvm_equation.py:210-231) and pressure (:309-321)
     const float Pm = (P0 + P1) * 0.5f;
@@ -532,10 +543,10 @@ __device__ __forceinline__ void facet_flux(const FvmPhys &p, float mu0, float mu
     heat = fmaf((-kappa) * gTn, fs, heat);
     F[D + 1] = F[D + 1] + heat;
     // Kurganov-Tadmor dissipation (:337-368)
-    const float Vn0 = sqrtf(sq0), Vn1 = sqrtf(sq1);
+    const float Vn0 = sqrt_hw(sq0), Vn1 = sqrt_hw(sq1);
     const float Vmax = fmaxf(Vn0, Vn1);
     const float cm = fmaxf(c0, c1);
-    const float M = fabsf(Vmax / (cm + 1e-8f));
+    const float M = fabsf(quot(Vmax, cm + 1e-8f));
     const float vf = fminf(fmaxf(M, p.v_factor), 1.0f);
     const float a_vel = fmaf(vf, cm, Vmax);
     const float a_oth = cm + Vmax;
@@ -871,8 +882,10 @@ __host__ __device__ constexpr int fz_geo_n() { return D == 2 ? 8 : 12; }
 template <int D>
 __host__ __device__ constexpr int fz_hrec_n() { return D == 2 ? 16 : 32; }
 
-// LDS layout of k_fused in floats: Pl [ncl][C], FV [ngl][NL][C], GL [ngl][D][D+1]
-// (padded to 4 floats), FL [nfl][C].
+// LDS layout of k_fused in floats: Pl [ncl][C], FV [NL][ngl][C] (slot-major), GL
+// [D][D+1][ngl] (component-major, padded to 4 floats), FL [nfl][C].  Slot- and
+// component-major rows make the gradient phase's writes (one cell per lane) conflict
+// free.
 template <int D>
 __host__ __device__ inline int64_t fz_fl_offset(int ncl, int ngl) {
     constexpr int C = D + 2, NL = D + 1;
@@ -886,12 +899,13 @@ __host__ __device__ inline int64_t fz_fl_offset(int ncl, int ngl) {
 // a boundary facet, their gradient record (the next call's Neumann correction).
 template <int D, int LP>
 __device__ __forceinline__ void fused_grad_cell(const FvmMeshDev &m, const FvmPhys &p, const FvmWork &w,
-                                                const float *Pl, float *FV, float *GL, int l, int64_t gc, bool own,
+                                                const float *Pl, float *FV, float *GL, int ngl, int l, int64_t gc,
+                                                bool own,
                                                 const float (&gv)[D + 2][D], const float (&disp)[D + 1][D],
                                                 float eps, int info, const ushort4 nrow, int has_prev,
                                                 const float *gmean, int g0) {
     using Dm = Dims<D>;
-    constexpr int C = Dm::C, NL = Dm::NL, GLN = D * (D + 1);
+    constexpr int C = Dm::C, NL = Dm::NL;
     const int64_t N = m.n_cells;
     const unsigned nl[4] = {nrow.x, nrow.y, nrow.z, nrow.w};
     float P[C];
@@ -931,13 +945,13 @@ __device__ __forceinline__ void fused_grad_cell(const FvmMeshDev &m, const FvmPh
             for (int ii = 1; ii < D; ++ii) dU = dU + g[ii][j] * disp[k][ii];
             U[j] = P[j] + dU * phi[j];
         }
-        store_vec<C>(FV + (l * NL + k) * C, U);
+        store_vec<C>(FV + (k * ngl + l) * C, U);
     }
 #pragma unroll
     for (int ii = 0; ii < D; ++ii) {
 #pragma unroll
-        for (int j = 0; j < D; ++j) GL[l * GLN + ii * (D + 1) + j] = g[ii][j] * phi[j];
-        GL[l * GLN + ii * (D + 1) + D] = g[ii][D + 1] * phi[D + 1];
+        for (int j = 0; j < D; ++j) GL[(ii * (D + 1) + j) * ngl + l] = g[ii][j] * phi[j];
+        GL[(ii * (D + 1) + D) * ngl + l] = g[ii][D + 1] * phi[D + 1];
     }
 }
 
@@ -945,11 +959,12 @@ __device__ __forceinline__ void fused_grad_cell(const FvmMeshDev &m, const FvmPh
 // LDS, the normal gradient, the flux into FL[q].
 template <int D>
 __device__ __forceinline__ void fused_facet(const FvmMeshDev &m, const FvmPhys &p, const FvmWork &w,
-                                            const float *Pl, const float *FV, const float *GL, float *FL, int q,
+                                            const float *Pl, const float *FV, const float *GL, int ngl, float *FL,
+                                            int q,
                                             int64_t f, unsigned d, const float (&geo)[3 * D + 2], int64_t c0,
                                             int halo_off, int has_prev, const float *gmean, int g0) {
     using Dm = Dims<D>;
-    constexpr int C = Dm::C, NL = Dm::NL, GLN = D * (D + 1);
+    constexpr int C = Dm::C, NL = Dm::NL;
     const int64_t N = m.n_cells, NF = m.n_facets;
     const unsigned l0 = d & 0xFFFu, k0 = (d >> 12) & 3u, l1 = (d >> 14) & 0xFFFu, k1 = (d >> 26) & 3u;
     const bool left_first = (d >> 28) & 1u, bnd = (d >> 29) & 1u, outer0 = (d >> 30) & 1u;
@@ -965,16 +980,16 @@ __device__ __forceinline__ void fused_facet(const FvmMeshDev &m, const FvmPhys &
     float U0s[C], U1s[C], g0s[D][D + 1], g1s[D][D + 1], gdn[D + 1];
     if (!bnd) {
         float P0[C], P1[C];
-        load_vec<C>(FV + (l0 * NL + k0) * C, U0s);
-        load_vec<C>(FV + (l1 * NL + k1) * C, U1s);
+        load_vec<C>(FV + (k0 * ngl + l0) * C, U0s);
+        load_vec<C>(FV + (k1 * ngl + l1) * C, U1s);
         load_vec<C>(Pl + l0 * C, P0);
         load_vec<C>(Pl + l1 * C, P1);
 #pragma unroll
         for (int ii = 0; ii < D; ++ii)
 #pragma unroll
             for (int j = 0; j <= D; ++j) {
-                g0s[ii][j] = GL[l0 * GLN + ii * (D + 1) + j];
-                g1s[ii][j] = GL[l1 * GLN + ii * (D + 1) + j];
+                g0s[ii][j] = GL[(ii * (D + 1) + j) * ngl + l0];
+                g1s[ii][j] = GL[(ii * (D + 1) + j) * ngl + l1];
             }
         // A_face_grad row: (-1/d, +1/d) on (slot-0, slot-1) cells, ascending columns
         const float inv_d = geo[2 * D + 1];
@@ -988,7 +1003,7 @@ __device__ __forceinline__ void fused_facet(const FvmMeshDev &m, const FvmPhys &
         // boundary facet: its interior cell is a tile cell (the facet's lowest cell)
         const unsigned li = outer0 ? l1 : l0, ki = outer0 ? k1 : k0;
         float Uo[C], Pi[C], Ux[C];
-        load_vec<C>(FV + (li * NL + ki) * C, Uo);
+        load_vec<C>(FV + (ki * ngl + li) * C, Uo);
         load_vec<C>(Pl + li * C, Pi);
         const int64_t b = static_cast<int64_t>(m.fac_cell[outer0 ? f : NF + f]) - N;
         bc_value<D>(m, p, b, Pi, w.gp_prev + (c0 + li) * Dm::REC, has_prev, gmean, g0, Ux);
@@ -1001,7 +1016,7 @@ __device__ __forceinline__ void fused_facet(const FvmMeshDev &m, const FvmPhys &
         for (int ii = 0; ii < D; ++ii)
 #pragma unroll
             for (int j = 0; j <= D; ++j) {
-                g0s[ii][j] = GL[li * GLN + ii * (D + 1) + j];
+                g0s[ii][j] = GL[(ii * (D + 1) + j) * ngl + li];
                 g1s[ii][j] = g0s[ii][j];
             }
 #pragma unroll
@@ -1045,7 +1060,7 @@ __global__ void __launch_bounds__(kThreads)
 k_fused(FvmMeshDev m, FvmPhys p, FvmWork w, const float *__restrict__ prim, float *__restrict__ dudt, FvmStage st,
         int has_prev) {
     using Dm = Dims<D>;
-    constexpr int C = Dm::C, NL = Dm::NL, FVN = NL * C, GLN = D * (D + 1);
+    constexpr int C = Dm::C, NL = Dm::NL, FVN = NL * C;
     constexpr int HREC = fz_hrec_n<D>();
     extern __shared__ float4 fz_lds[];     // 16-byte aligned: 2-D records move as ds_*_b128
     float *lds = reinterpret_cast<float *>(fz_lds);
@@ -1132,11 +1147,11 @@ k_fused(FvmMeshDev m, FvmPhys p, FvmWork w, const float *__restrict__ prim, floa
     float geoA[3 * D + 2];
     if (i < nft) fused_facet_load<D>(m, i, fo0, nfo, foff, fA, dA, geoA);
     if (has_own)
-        fused_grad_cell<D, LP>(m, p, w, Pl, FV, GL, i, c0 + i, true, ogv, odisp, oeps, oinfo, onr, has_prev, gmean,
-                               g0);
-    if (has_h1)
-        fused_grad_cell<D, LP>(m, p, w, Pl, FV, GL, kThreads + i, hgc, false, hgv, hdisp, heps, hinfo, hnr,
+        fused_grad_cell<D, LP>(m, p, w, Pl, FV, GL, m.fz_ngl, i, c0 + i, true, ogv, odisp, oeps, oinfo, onr,
                                has_prev, gmean, g0);
+    if (has_h1)
+        fused_grad_cell<D, LP>(m, p, w, Pl, FV, GL, m.fz_ngl, kThreads + i, hgc, false, hgv, hdisp, heps, hinfo,
+                               hnr, has_prev, gmean, g0);
     __syncthreads();
 
     // ---- phase 2: facet fluxes (the second facet's and the epilogue's inputs in flight) ----
@@ -1155,9 +1170,11 @@ k_fused(FvmMeshDev m, FvmPhys p, FvmWork w, const float *__restrict__ prim, floa
         inv_vol = m.cell_inv_vol[c];
         cf = *reinterpret_cast<const ushort4 *>(m.fz_cfac + c * 4);
     }
-    if (i < nft) fused_facet<D>(m, p, w, Pl, FV, GL, FL, i, fA, dA, geoA, c0, halo_off, has_prev, gmean, g0);
+    if (i < nft)
+        fused_facet<D>(m, p, w, Pl, FV, GL, m.fz_ngl, FL, i, fA, dA, geoA, c0, halo_off, has_prev, gmean, g0);
     if (twoB)
-        fused_facet<D>(m, p, w, Pl, FV, GL, FL, i + kThreads, fB, dB, geoB, c0, halo_off, has_prev, gmean, g0);
+        fused_facet<D>(m, p, w, Pl, FV, GL, m.fz_ngl, FL, i + kThreads, fB, dB, geoB, c0, halo_off, has_prev, gmean,
+                       g0);
     __syncthreads();
 
     // ---- phase 3: divergence and stage epilogue of the tile's cells ----

@@ -432,7 +432,7 @@ def learned_main(a):
     train_s = time.time() - t1
     held = lc.eval_loss(data[-4:])
     zero = float(np.mean([float((T.double() ** 2).mean()) for _, T in data[-4:]]))
-    log(f"trained {a.train_iters} iters in {train_s:.1f}s; held-out loss {held:.4f} (Euler window {zero:.4f})")
+    log(f"trained {a.train_iters} iters in {train_s:.1f}s; held-out loss {held:.4f} (base predictor alone {zero:.4f})")
     stats, weights = lc.stats, lc.net.state_dict()
     del data, lc, ts
     torch.cuda.empty_cache()
@@ -452,18 +452,41 @@ def learned_main(a):
         g.replay()
     torch.cuda.synchronize()
     U0 = plain.state.clone()
+    # the corrector's history (f at the midpoint and end of a window) comes from one plain
+    # window; the plain run takes the same window, so both timed runs start from one state
     learn.set_state(U0)
+    lc.plain_window()
+    g.replay()
     torch.cuda.synchronize()
+    U_start = plain.state.clone()
     tp0 = time.perf_counter()
     for _ in range(a.eval_windows):
         g.replay()
     torch.cuda.synchronize()
     plain_s = time.perf_counter() - tp0
     learn_s, acc = lc.run(a.eval_windows)
+    U_learn = learn.state.clone()
+    # the method without the network: the same windows with the base predictor alone
+    learn.set_state(U0)
+    lc.plain_window()
+    base_s, acc_base = lc.run(a.eval_windows, use_net="base")
+    U_base = learn.state.clone()
     sim_t = a.eval_windows * a.window * dts          # per mesh
     N = plain.dm.N
     log(f"eval: {a.eval_meshes} meshes, {N} cells, {a.eval_windows} windows of {a.window} steps after "
-        f"{a.eval_spinup} spin-up windows: plain {plain_s:.3f}s, learned {learn_s:.3f}s, accepted {acc}/{a.eval_windows}")
+        f"{a.eval_spinup + 1} spin-up windows: plain {plain_s:.3f}s, learned {learn_s:.3f}s (accepted {acc}), "
+        f"base only {base_s:.3f}s (accepted {acc_base})")
+
+    def mesh_rel(X, R):
+        out = []
+        for i in range(a.eval_meshes):
+            lo, hi = int(plain.offsets[i]), int(plain.offsets[i + 1])
+            out.append(float((X[lo:hi] - R[lo:hi]).double().norm() / R[lo:hi].double().norm()))
+        return np.asarray(out)
+
+    vs_plain = {"learned": mesh_rel(U_learn, plain.state), "base": mesh_rel(U_base, plain.state)}
+    log(f"vs plain fp32 RK4 (all meshes): learned max {vs_plain['learned'].max():.3e} "
+        f"median {np.median(vs_plain['learned']):.3e}; base max {vs_plain['base'].max():.3e}")
 
     # CPU leg (the checker): the FP64 CPU oracle re-runs the smallest evaluation meshes
     errs = {}
@@ -475,17 +498,19 @@ def learned_main(a):
         cfg = ep[i][2]
         _, ofvm = O.from_mesh_tuple(pts, cells, fi, fb, tags, _oracle_phys(cfg), dtype=torch.float64)
         st = O.OracleStepper(ofvm, "Butcher", "RK4", dt=float(dts[i]))
-        U = U0[int(plain.offsets[i]):int(plain.offsets[i + 1])].cpu().double()
-        for _ in range(a.eval_windows * a.window):
+        lo, hi = int(plain.offsets[i]), int(plain.offsets[i + 1])
+        U = U0[lo:hi].cpu().double()
+        for _ in range((a.eval_windows + 1) * a.window):
             U = st.step(U)
         ref = U.numpy()
         rel = lambda x: (np.linalg.norm(x.cpu().double().numpy() - ref, axis=0) / np.linalg.norm(ref, axis=0)).tolist()
         full = lambda x: float(np.linalg.norm(x.cpu().double().numpy() - ref) / np.linalg.norm(ref))
-        errs[int(i)] = {"cells": int(eparams[i]["cells"]), "learned": full(learn.state_of(i)),
-                        "plain": full(plain.state_of(i)), "learned_per_variable": rel(learn.state_of(i)),
-                        "plain_per_variable": rel(plain.state_of(i))}
+        errs[int(i)] = {"cells": int(eparams[i]["cells"]), "learned": full(U_learn[lo:hi]),
+                        "base": full(U_base[lo:hi]), "plain": full(plain.state[lo:hi]),
+                        "learned_per_variable": rel(U_learn[lo:hi]), "plain_per_variable": rel(plain.state[lo:hi])}
         log(f"mesh {i} ({eparams[i]['cells']} cells): state rel-L2 vs FP64 learned {errs[int(i)]['learned']:.3e} "
-            f"(per variable {errs[int(i)]['learned_per_variable']}), plain {errs[int(i)]['plain']:.3e}")
+            f"(per variable {errs[int(i)]['learned_per_variable']}), base {errs[int(i)]['base']:.3e}, "
+            f"plain {errs[int(i)]['plain']:.3e}")
     worst = max(e["learned"] for e in errs.values()) if errs else None
     out = {
         "metric": "simulated time per wall second, learned window corrector vs plain RK4 (batched 2-D channel)",
@@ -497,6 +522,10 @@ def learned_main(a):
         "plain_value": float(np.mean(sim_t) / plain_s),
         "accepted_windows": int(acc),
         "windows": a.eval_windows,
+        "base_only": {"value": float(np.mean(sim_t) / base_s), "accepted_windows": int(acc_base),
+                      "vs_plain": plain_s / base_s},
+        "max_state_rel_l2_vs_plain_fp32": {"learned": float(vs_plain["learned"].max()),
+                                           "base": float(vs_plain["base"].max())},
         "max_state_rel_l2_vs_fp64": worst,
         "error_norm": "relative L2 over each mesh's whole conserved state (per-variable values in config)",
         "tol": a.tol,
@@ -507,7 +536,7 @@ def learned_main(a):
                    "train_pairs": a.train_meshes * a.train_windows, "train_iters": a.train_iters,
                    "eval_meshes": a.eval_meshes, "eval_cells": N, "cells_range": [a.n_min, a.n_max],
                    "eval_spinup_windows": a.eval_spinup,
-                   "held_out_loss": held, "euler_window_loss": zero, "train_seconds": train_s,
+                   "held_out_loss": held, "base_window_loss": zero, "train_seconds": train_s,
                    "loss_history": hist[-3:], "errors_vs_fp64": errs},
     }
     print(json.dumps(out), flush=True)

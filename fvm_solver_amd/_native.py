@@ -97,10 +97,14 @@ _PROTOS = {
     "fvm_adam_f32": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, i64, f32, f32, f32, f32, i32,
                                     c_void_p]),
     "fvm_cast_f32_bf16": (ctypes.c_int, [c_void_p, c_void_p, i64, c_void_p]),
-    "fvm_mp_features": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p, i32, c_void_p, c_void_p, i32, i64,
-                                       i32, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, i32, c_void_p]),
-    "fvm_mp_apply": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p, i64, c_void_p, c_void_p, i64, i32,
-                                    c_void_p, c_void_p]),
+    "fvm_mp_encode": (ctypes.c_int, [c_void_p, i64, i32, c_void_p, c_void_p, i32, c_void_p, c_void_p]),
+    "fvm_mp_layer_inf": (ctypes.c_int, [c_void_p, i32, c_void_p, i64, i32, c_void_p, c_void_p, c_void_p, c_void_p,
+                                        c_void_p, i32, c_void_p, c_void_p]),
+    "fvm_mp_window_features": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                              c_void_p, i32, c_void_p, i64, i32, c_void_p, c_void_p, c_void_p,
+                                              c_void_p, c_void_p, c_void_p, i32, c_void_p]),
+    "fvm_mp_window_apply": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, f32, f32, c_void_p, c_void_p, i64,
+                                           c_void_p, c_void_p, i64, i32, c_void_p, c_void_p]),
     "fvm_mesh_rms": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, i32, i32, f32, c_void_p, c_void_p, c_void_p]),
     "fvm_mp_defect3": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                       i32, i32, c_void_p, c_void_p, c_void_p]),

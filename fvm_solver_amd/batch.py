@@ -57,6 +57,7 @@ class BatchedSolver:
         self._cur = 0
         self._flip = 0
         self._prim_valid = False
+        self.version = 0             # bumped by every step and set_state
         self.inv_cv = 1.0 / cfg0.C_v
         self.gR = cfg0.gamma * cfg0.C_v * (cfg0.gamma - 1)
 
@@ -71,6 +72,7 @@ class BatchedSolver:
     def set_state(self, U: torch.Tensor):
         self._U[self._cur].copy_(U)
         self._prim_valid = False
+        self.version += 1
 
     def primitives(self) -> torch.Tensor:
         self._ensure_prim()
@@ -124,6 +126,7 @@ class BatchedSolver:
             self._flip = 1 - self._flip
             cur = nxt
         self._cur = 1 - self._cur
+        self.version += 1
 
     def capture_graph(self, n_steps: int = 2):
         """Capture n_steps batched steps as one HIP graph (n_steps and n_steps * stages even)."""

@@ -254,37 +254,76 @@ __global__ void __launch_bounds__(256) k_cast_f32_bf16(const float *__restrict__
     if (t < n) y[t] = __bf16(x[t]);
 }
 
-template <int D, int G, int KX>
+// Network input of one window of a 2-D batch (C = 4, KX = 32), per cell c:
+//   [0, 4)   (prim - mu_p) / sd_p
+//   [4, 8)   W f0 / s1                 W = window length of the cell's mesh
+//   [8, 12)  W (f0 - fm) / s2          fm = f half a window earlier
+//   [12, 14) geometry features
+//   [14, 22) h d/dx prim / sd_p, then h d/dy prim / sd_p
+//   [22, 30) h W d/dx f0 / s1, then h W d/dy f0 / s1
+//   [30, 32) 0
+// s1, s2: [n_meshes][C] scales of the cell's mesh.  The gradients are least-squares
+// weighted sums over the interior face neighbours, sum_k wg[k][i][c] (v_nb(k) - v_c)
+// (a boundary face contributes nothing); wg is [NL][2][N].
+template <int NL>
 __global__ void __launch_bounds__(256)
-k_mp_features(const float *__restrict__ prim, const float *__restrict__ dudt, const float *__restrict__ win,
-              const float *__restrict__ geo, const float *__restrict__ grad, const float *__restrict__ hc, int64_t N,
-              const float *__restrict__ mu_p, const float *__restrict__ sd_p, const float *__restrict__ sd_d,
-              const int32_t *__restrict__ cell_mesh, __bf16 *__restrict__ X) {
-    constexpr int C = D + 2;
+k_mp_window_features(const float4 *__restrict__ prim, const float4 *__restrict__ f0, const float4 *__restrict__ fm,
+                     const float *__restrict__ win, const float *__restrict__ geo, const float *__restrict__ hc,
+                     const int32_t *__restrict__ nbr, const float *__restrict__ wg, int64_t N,
+                     const float *__restrict__ mu_p, const float *__restrict__ sd_p, const float *__restrict__ s1,
+                     const float *__restrict__ s2, const int32_t *__restrict__ cell_mesh, __bf16 *__restrict__ X) {
+    constexpr int C = 4, KX = 32;
     const int64_t c = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (c >= N) return;
+    int32_t nb[NL];
+    float wx[NL], wy[NL];
+#pragma unroll
+    for (int k = 0; k < NL; ++k) {
+        nb[k] = nbr[k * N + c];
+        wx[k] = wg[(2 * k) * N + c];
+        wy[k] = wg[(2 * k + 1) * N + c];
+    }
+    const float4 pc = prim[c], fc = f0[c], gc = fm[c];
+    float4 pn[NL], fn[NL];
+#pragma unroll
+    for (int k = 0; k < NL; ++k) {
+        const bool in = nb[k] < N;           // boundary faces: the cell's own values (zero difference)
+        pn[k] = in ? prim[nb[k]] : pc;
+        fn[k] = in ? f0[nb[k]] : fc;
+    }
+    const float p[C] = {pc.x, pc.y, pc.z, pc.w}, f[C] = {fc.x, fc.y, fc.z, fc.w}, g[C] = {gc.x, gc.y, gc.z, gc.w};
+    float dp[2][C], df[2][C];
+#pragma unroll
+    for (int j = 0; j < C; ++j) dp[0][j] = dp[1][j] = df[0][j] = df[1][j] = 0.0f;
+#pragma unroll
+    for (int k = 0; k < NL; ++k) {
+        const float q[C] = {pn[k].x, pn[k].y, pn[k].z, pn[k].w}, r[C] = {fn[k].x, fn[k].y, fn[k].z, fn[k].w};
+#pragma unroll
+        for (int j = 0; j < C; ++j) {
+            dp[0][j] += wx[k] * (q[j] - p[j]);
+            dp[1][j] += wy[k] * (q[j] - p[j]);
+            df[0][j] += wx[k] * (r[j] - f[j]);
+            df[1][j] += wy[k] * (r[j] - f[j]);
+        }
+    }
+    const int m = cell_mesh ? cell_mesh[c] : 0;
+    const float w = win[c], h = hc[c];
     float x[KX];
 #pragma unroll
-    for (int j = 0; j < KX; ++j) x[j] = 0.0f;
-    const float w = win[c];
-    const float *sd = sd_d + (cell_mesh ? cell_mesh[c] : 0) * C;
-#pragma unroll
     for (int j = 0; j < C; ++j) {
-        x[j] = (prim[c * C + j] - mu_p[j]) / sd_p[j];
-        x[C + j] = w * dudt[c * C + j] / sd[j];
+        const float isp = 1.0f / sd_p[j], i1 = w / s1[m * C + j], i2 = w / s2[m * C + j];
+        x[j] = (p[j] - mu_p[j]) * isp;
+        x[C + j] = f[j] * i1;
+        x[2 * C + j] = (f[j] - g[j]) * i2;
+        x[14 + j] = h * dp[0][j] * isp;
+        x[14 + C + j] = h * dp[1][j] * isp;
+        x[22 + j] = h * df[0][j] * i1;
+        x[22 + C + j] = h * df[1][j] * i1;
     }
-#pragma unroll
-    for (int g = 0; g < G; ++g) x[2 * C + g] = geo[c * G + g];
-    if (grad) {
-        // limited gradient (unlimited gradient times limiter factor) times the cell size,
-        // in units of the primitive's spread: the local variation across one cell
-        const float *r = grad + c * (D + 1) * C;
-        const float h = hc[c];
-#pragma unroll
-        for (int i = 0; i < D; ++i)
-#pragma unroll
-            for (int j = 0; j < C; ++j) x[2 * C + G + i * C + j] = h * r[i * C + j] * r[D * C + j] / sd_p[j];
-    }
+    x[12] = geo[2 * c];
+    x[13] = geo[2 * c + 1];
+    x[30] = 0.0f;
+    x[31] = 0.0f;
     // the row leaves as 16-byte vectors
 #pragma unroll
     for (int q = 0; q < KX; q += 8) {
@@ -295,16 +334,21 @@ k_mp_features(const float *__restrict__ prim, const float *__restrict__ dudt, co
     }
 }
 
+// U_out = U + win (a f0 + b fm) + scale[m(c)] y   (fm and y optional)
 __global__ void __launch_bounds__(256)
-k_mp_apply(const float *__restrict__ U, const float *__restrict__ dudt, const float *__restrict__ win,
-           const float *__restrict__ y, int64_t ld_y, const float *__restrict__ scale,
-           const int32_t *__restrict__ cell_mesh, int64_t N, int C, float *__restrict__ U_out) {
+k_mp_window_apply(const float *__restrict__ U, const float *__restrict__ f0, const float *__restrict__ fm, float a,
+                  float b, const float *__restrict__ win, const float *__restrict__ y, int64_t ld_y,
+                  const float *__restrict__ scale, const int32_t *__restrict__ cell_mesh, int64_t N, int C,
+                  float *__restrict__ U_out) {
     const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (t >= N * C) return;
     const int64_t c = t / C;
     const int j = static_cast<int>(t % C);
-    const float sc = scale[(cell_mesh ? cell_mesh[c] : 0) * C + j];
-    U_out[t] = (U[t] + win[c] * dudt[t]) + sc * y[c * ld_y + j];
+    float inc = a * f0[t];
+    if (fm) inc += b * fm[t];
+    float v = U[t] + win[c] * inc;
+    if (y) v += scale[(cell_mesh ? cell_mesh[c] : 0) * C + j] * y[c * ld_y + j];
+    U_out[t] = v;
 }
 
 // grid (64, n_meshes): per mesh q, fp64 partial sums of (w x_j)^2 over its cells
@@ -514,6 +558,211 @@ k_mp_layer(const __bf16 *__restrict__ h, const int32_t *__restrict__ nbr, int64_
     }
 }
 
+// Inference encoder: h0 = relu(X W_in + b_in) for KX = 32 inputs and H = 128 features.
+// One wave per 16-cell tile (grid-stride over tiles): the X rows are the B operand (lane
+// g|col holds inputs 8g..8g+7 of cell col), W_in^T the A operand, kept in registers with
+// the lane's bias values for the whole kernel; eight v_mfma_f32_16x16x32_bf16 give
+// features x cells, stored as 8-byte bf16 chunks.
+__global__ void __launch_bounds__(256)
+k_mp_encode(const __bf16 *__restrict__ X, int64_t N, const __bf16 *__restrict__ WT, const float *__restrict__ bias,
+            __bf16 *__restrict__ h0) {
+    constexpr int H = 128, KX = 32;
+    const int lane = threadIdx.x & 63, g = lane >> 4, col = lane & 15;
+    bf16x8_t wf[8];
+    float bb[8][4];
+#pragma unroll
+    for (int cb = 0; cb < 8; ++cb) {
+        wf[cb] = *reinterpret_cast<const bf16x8_t *>(WT + (16 * cb + col) * KX + 8 * g);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) bb[cb][r] = bias[16 * cb + 4 * g + r];
+    }
+    const int64_t ntiles = (N + 15) / 16;
+    const int64_t nw = static_cast<int64_t>(gridDim.x) * (blockDim.x >> 6);
+    for (int64_t t = static_cast<int64_t>(blockIdx.x) * (blockDim.x >> 6) + (threadIdx.x >> 6); t < ntiles; t += nw) {
+        const int64_t row = t * 16 + col;
+        bf16x8_t xv;
+        if (row < N) {
+            xv = *reinterpret_cast<const bf16x8_t *>(X + row * KX + 8 * g);
+        } else {
+#pragma unroll
+            for (int e = 0; e < 8; ++e) xv[e] = __bf16(0.0f);
+        }
+#pragma unroll
+        for (int cb = 0; cb < 8; ++cb) {
+            const f32x4_t acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf[cb], xv, f32x4_t{0.0f, 0.0f, 0.0f, 0.0f},
+                                                                          0, 0, 0);
+            if (row < N) {
+                typedef __bf16 bf16x4_t __attribute__((ext_vector_type(4)));
+                bf16x4_t o;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) o[r] = __bf16(fmaxf(acc[r] + bb[cb][r], 0.0f));
+                *reinterpret_cast<bf16x4_t *>(h0 + row * H + 16 * cb + 4 * g) = o;
+            }
+        }
+    }
+}
+
+// Inference message layer: h_out = h + relu([h | mean_nb h] W + b) as k_mp_layer (same
+// operand layout and K-step order), arranged for registers and latency:
+//  * the rows of W^T are staged in LDS in the order p(16cb + 4g + r) = 32(cb/2) + 8g +
+//    4(cb%2) + r, so the output features a lane holds (rows 4g..4g+3 of each 16-row
+//    block cb) are features it already holds as its own B operand (32i + 8g + e): the
+//    residual comes from registers and no row is loaded twice;
+//  * the neighbour ids of the next tile are fetched while the current tile computes;
+//  * the bias sits in LDS.
+// LAST: the decoder is applied in the epilogue instead of storing h_out,
+//   y[c][j] = b_out[j] + sum_f bf16(h_out[c][f]) W_out[f][j]     (j < 8, fp32),
+// each lane summing its 32 features and the four lanes of a cell reducing by shuffles.
+template <int NLT, bool LAST>
+__global__ void __launch_bounds__(kLayerThreads) __attribute__((amdgpu_waves_per_eu(4)))
+k_mp_layer_inf(const __bf16 *__restrict__ h, const int32_t *__restrict__ nbr, int64_t N,
+               const __bf16 *__restrict__ WT, const float *__restrict__ bias, __bf16 *__restrict__ h_out,
+               const __bf16 *__restrict__ WoT, const float *__restrict__ b_out, float *__restrict__ y) {
+    constexpr int H = 128, K = 2 * H, KP = K + 8, NO = 8;
+    __shared__ __attribute__((aligned(16))) __bf16 Ws[H][KP];
+    __shared__ __attribute__((aligned(16))) float bs[H];
+    __shared__ __attribute__((aligned(16))) float wo[LAST ? H : 1][NO];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    // LDS row m = 16cb + 4g + r holds output feature p(m)
+    auto perm = [](int m) { const int cb = m >> 4, gg = (m >> 2) & 3, r = m & 3; return 32 * (cb >> 1) + 8 * gg + 4 * (cb & 1) + r; };
+    for (int t = tid; t < H * (K / 8); t += kLayerThreads) {
+        const int j = t / (K / 8), kc = (t % (K / 8)) * 8;
+        *reinterpret_cast<bf16x8_t *>(&Ws[j][kc]) = *reinterpret_cast<const bf16x8_t *>(WT + perm(j) * K + kc);
+    }
+    for (int t = tid; t < H; t += kLayerThreads) bs[t] = bias[perm(t)];
+    if constexpr (LAST) {
+        for (int t = tid; t < H * NO; t += kLayerThreads) wo[t % H][t / H] = static_cast<float>(WoT[t]);
+    }
+    __syncthreads();
+    constexpr int nwaves = kLayerThreads / 64;
+    const int64_t ntiles = (N + 15) / 16;
+    const int g = lane >> 4, kq = 8 * g, col = lane & 15, g4 = 4 * g;
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * nwaves;
+    int64_t t = static_cast<int64_t>(blockIdx.x) * nwaves + wave;
+    int32_t nbn[NLT];
+    {
+        const int64_t row = t * 16 + col;
+#pragma unroll
+        for (int k = 0; k < NLT; ++k) nbn[k] = (t < ntiles && row < N) ? nbr[k * N + row] : static_cast<int32_t>(N);
+    }
+    for (; t < ntiles; t += stride) {
+        const int64_t row = t * 16 + col;
+        const bool rlive = row < N;
+        const int64_t rs = rlive ? row : 0;
+        // the LDS fragments do not depend on the tile; an opaque per-tile offset keeps the
+        // compiler from hoisting them out of the loop (they would not fit in registers)
+        int wofs = 0;
+        asm volatile("" : "+v"(wofs));
+        const __bf16 *wsb = &Ws[col][kq] + wofs;
+        const float *wob = &wo[0][0] + wofs;
+        int64_t nb[NLT];
+        bool val[NLT];
+        int cnt = 0;
+#pragma unroll
+        for (int k = 0; k < NLT; ++k) {
+            val[k] = nbn[k] < N;
+            cnt += val[k];
+            nb[k] = val[k] ? static_cast<int64_t>(nbn[k]) : rs;
+        }
+        typedef __bf16 bf16x4_t __attribute__((ext_vector_type(4)));
+        bf16x8_t own[4], nv[NLT][4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            own[i] = *reinterpret_cast<const bf16x8_t *>(h + rs * H + 32 * i + kq);
+#pragma unroll
+            for (int k = 0; k < NLT; ++k) nv[k][i] = *reinterpret_cast<const bf16x8_t *>(h + nb[k] * H + 32 * i + kq);
+        }
+        {
+            const int64_t tn = t + stride, rn = tn * 16 + col;
+#pragma unroll
+            for (int k = 0; k < NLT; ++k) nbn[k] = (tn < ntiles && rn < N) ? nbr[k * N + rn] : static_cast<int32_t>(N);
+        }
+        const float inv = cnt ? 1.0f / static_cast<float>(cnt) : 0.0f;
+        f32x4_t acc[8];
+#pragma unroll
+        for (int cb = 0; cb < 8; ++cb) acc[cb] = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            if (!rlive) {
+#pragma unroll
+                for (int e = 0; e < 8; ++e) own[i][e] = __bf16(0.0f);
+            }
+#pragma unroll
+            for (int cb = 0; cb < 8; ++cb) {
+                const bf16x8_t wf = *reinterpret_cast<const bf16x8_t *>(wsb + 16 * cb * KP + 32 * i);
+                acc[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf, own[i], acc[cb], 0, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            float sacc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+            for (int k = 0; k < NLT; ++k) {
+                if (val[k]) {
+#pragma unroll
+                    for (int e = 0; e < 8; ++e) sacc[e] += static_cast<float>(nv[k][i][e]);
+                }
+            }
+            bf16x8_t mv;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) mv[e] = __bf16(sacc[e] * inv);
+#pragma unroll
+            for (int cb = 0; cb < 8; ++cb) {
+                const bf16x8_t wf = *reinterpret_cast<const bf16x8_t *>(wsb + 16 * cb * KP + H + 32 * i);
+                acc[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf, mv, acc[cb], 0, 0, 0);
+            }
+        }
+        // this lane holds LDS rows 16cb + 4g .. +3 of cell `row`: features f0 .. f0 + 3,
+        // f0 = 32(cb/2) + 8g + 4(cb%2), whose inputs are own[cb/2][4(cb%2) .. +3]
+        float yp[NO];
+        if constexpr (LAST) {
+#pragma unroll
+            for (int j = 0; j < NO; ++j) yp[j] = 0.0f;
+        }
+#pragma unroll
+        for (int cb = 0; cb < 8; ++cb) {
+            const int j0 = 32 * (cb >> 1) + kq + 4 * (cb & 1);
+            const float4 bb = *reinterpret_cast<const float4 *>(&bs[16 * cb + g4]);
+            const float z[4] = {acc[cb][0] + bb.x, acc[cb][1] + bb.y, acc[cb][2] + bb.z, acc[cb][3] + bb.w};
+            bf16x4_t o;
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                o[r] = __bf16(static_cast<float>(own[cb >> 1][4 * (cb & 1) + r]) + fmaxf(z[r], 0.0f));
+            if constexpr (LAST) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const float hv = static_cast<float>(o[r]);
+                    const float4 w0 = *reinterpret_cast<const float4 *>(wob + (j0 + r) * NO);
+                    const float4 w1 = *reinterpret_cast<const float4 *>(wob + (j0 + r) * NO + 4);
+                    yp[0] += hv * w0.x;
+                    yp[1] += hv * w0.y;
+                    yp[2] += hv * w0.z;
+                    yp[3] += hv * w0.w;
+                    yp[4] += hv * w1.x;
+                    yp[5] += hv * w1.y;
+                    yp[6] += hv * w1.z;
+                    yp[7] += hv * w1.w;
+                }
+            } else {
+                if (rlive) *reinterpret_cast<bf16x4_t *>(h_out + row * H + j0) = o;
+            }
+        }
+        if constexpr (LAST) {
+#pragma unroll
+            for (int j = 0; j < NO; ++j) {
+                yp[j] += __shfl_xor(yp[j], 16);
+                yp[j] += __shfl_xor(yp[j], 32);
+            }
+            if (rlive && g == 0) {
+                *reinterpret_cast<float4 *>(y + row * NO) =
+                    make_float4(yp[0] + b_out[0], yp[1] + b_out[1], yp[2] + b_out[2], yp[3] + b_out[3]);
+                *reinterpret_cast<float4 *>(y + row * NO + 4) =
+                    make_float4(yp[4] + b_out[4], yp[5] + b_out[5], yp[6] + b_out[6], yp[7] + b_out[7]);
+            }
+        }
+    }
+}
+
 __global__ void __launch_bounds__(256)
 k_transpose_bf16(const float *__restrict__ W, int64_t nrow, int64_t ncol, __bf16 *__restrict__ WT) {
     const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -693,36 +942,37 @@ int fvm_cast_f32_bf16(const float *x, void *y, int64_t n, void *stream) {
     return check_launch("fvm_cast_f32_bf16");
 }
 
-int fvm_mp_features(const float *prim, const float *dudt, const float *win, const float *geo, int32_t G,
-                    const float *grad, const float *hc, int32_t D, int64_t N, int32_t C, const float *mu_p,
-                    const float *sd_p, const float *sd_d, const int32_t *cell_mesh, void *X, int32_t KX,
-                    void *stream) {
-    if (!prim || !dudt || !win || !geo || G != 2 || (grad && !hc) || (D != 2 && D != 3) || C != D + 2 || !mu_p ||
-        !sd_p || !sd_d || !X || N < 0 || KX != 32 || (reinterpret_cast<uintptr_t>(X) & 15u)) {
-        set_error("fvm_mp_features: invalid arguments (this build takes G = 2, KX = 32, C = D + 2)");
+int fvm_mp_window_features(const float *prim, const float *f0, const float *fm, const float *win, const float *geo,
+                           const float *hc, const int32_t *nbr, int32_t NL, const float *wg, int64_t N, int32_t C,
+                           const float *mu_p, const float *sd_p, const float *s1, const float *s2,
+                           const int32_t *cell_mesh, void *X, int32_t KX, void *stream) {
+    if (!prim || !f0 || !fm || !win || !geo || !hc || !nbr || !wg || NL != 3 || C != 4 || !mu_p || !sd_p || !s1 ||
+        !s2 || !X || N < 0 || KX != 32 || (reinterpret_cast<uintptr_t>(X) & 15u) ||
+        ((reinterpret_cast<uintptr_t>(prim) | reinterpret_cast<uintptr_t>(f0) | reinterpret_cast<uintptr_t>(fm)) &
+         15u)) {
+        set_error("fvm_mp_window_features: invalid arguments (this build takes 2-D meshes: NL = 3, C = 4, KX = 32, "
+                  "16-byte aligned rows)");
         return FVM_ERR_ARG;
     }
     if (N == 0) return FVM_OK;
-    __bf16 *x = static_cast<__bf16 *>(X);
-    if (D == 2)
-        hipLaunchKernelGGL((k_mp_features<2, 2, 32>), dim3(blocks_for(N, 256)), dim3(256), 0, as_stream(stream), prim,
-                           dudt, win, geo, grad, hc, N, mu_p, sd_p, sd_d, cell_mesh, x);
-    else
-        hipLaunchKernelGGL((k_mp_features<3, 2, 32>), dim3(blocks_for(N, 256)), dim3(256), 0, as_stream(stream), prim,
-                           dudt, win, geo, grad, hc, N, mu_p, sd_p, sd_d, cell_mesh, x);
-    return check_launch("fvm_mp_features");
+    hipLaunchKernelGGL((k_mp_window_features<3>), dim3(blocks_for(N, 256)), dim3(256), 0, as_stream(stream),
+                       reinterpret_cast<const float4 *>(prim), reinterpret_cast<const float4 *>(f0),
+                       reinterpret_cast<const float4 *>(fm), win, geo, hc, nbr, wg, N, mu_p, sd_p, s1, s2, cell_mesh,
+                       static_cast<__bf16 *>(X));
+    return check_launch("fvm_mp_window_features");
 }
 
-int fvm_mp_apply(const float *U, const float *dudt, const float *win, const float *y, int64_t ld_y,
-                 const float *scale, const int32_t *cell_mesh, int64_t N, int32_t C, float *U_out, void *stream) {
-    if (!U || !dudt || !win || !y || !scale || !U_out || N < 0 || C < 1 || ld_y < C) {
-        set_error("fvm_mp_apply: invalid arguments");
+int fvm_mp_window_apply(const float *U, const float *f0, const float *fm, float a, float b, const float *win,
+                        const float *y, int64_t ld_y, const float *scale, const int32_t *cell_mesh, int64_t N,
+                        int32_t C, float *U_out, void *stream) {
+    if (!U || !f0 || !win || (y && (!scale || ld_y < C)) || !U_out || N < 0 || C < 1) {
+        set_error("fvm_mp_window_apply: invalid arguments");
         return FVM_ERR_ARG;
     }
     if (N == 0) return FVM_OK;
-    hipLaunchKernelGGL(k_mp_apply, dim3(blocks_for(N * C, 256)), dim3(256), 0, as_stream(stream), U, dudt, win, y,
-                       ld_y, scale, cell_mesh, N, C, U_out);
-    return check_launch("fvm_mp_apply");
+    hipLaunchKernelGGL(k_mp_window_apply, dim3(blocks_for(N * C, 256)), dim3(256), 0, as_stream(stream), U, f0, fm, a,
+                       b, win, y, ld_y, scale, cell_mesh, N, C, U_out);
+    return check_launch("fvm_mp_window_apply");
 }
 
 int fvm_mesh_rms(const float *X, const float *win, const int64_t *off, int32_t n_meshes, int32_t C, float floor_abs,
@@ -783,6 +1033,76 @@ int fvm_mp_layer(const void *h_in, int32_t H, const int32_t *nbr, int64_t N, int
         hipLaunchKernelGGL(k_mp_layer<4>, grid, dim3(kLayerThreads), 0, as_stream(stream), hi, nbr, N, wt, bias, ho, Z,
                            ao);
     return check_launch("fvm_mp_layer");
+}
+
+static int layer_grid_cap() {
+    static int n_cu = 0;
+    if (n_cu == 0) {
+        int dev = 0, v = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0)
+            n_cu = v;
+        else
+            n_cu = 256;
+    }
+    return n_cu;
+}
+
+int fvm_mp_encode(const void *X, int64_t N, int32_t KX, const void *WT, const float *bias, int32_t H, void *h0,
+                  void *stream) {
+    if (!X || !WT || !bias || !h0 || KX != 32 || H != 128 || N < 0 || (reinterpret_cast<uintptr_t>(X) & 15u) ||
+        (reinterpret_cast<uintptr_t>(WT) & 15u) || (reinterpret_cast<uintptr_t>(h0) & 7u)) {
+        set_error("fvm_mp_encode: invalid arguments (KX must be 32, H 128, aligned rows)");
+        return FVM_ERR_ARG;
+    }
+    if (N == 0) return FVM_OK;
+    const int64_t tiles = (N + 15) / 16;
+    const int64_t blocks = (tiles + 3) / 4;
+    const int64_t cap = 8LL * layer_grid_cap();
+    hipLaunchKernelGGL(k_mp_encode, dim3(static_cast<unsigned int>(blocks < cap ? blocks : cap)), dim3(256), 0,
+                       as_stream(stream), static_cast<const __bf16 *>(X), N, static_cast<const __bf16 *>(WT), bias,
+                       static_cast<__bf16 *>(h0));
+    return check_launch("fvm_mp_encode");
+}
+
+int fvm_mp_layer_inf(const void *h_in, int32_t H, const int32_t *nbr, int64_t N, int32_t NL, const void *WT,
+                     const float *bias, void *h_out, const void *WoT, const float *b_out, int32_t n_out, float *y,
+                     void *stream) {
+    const bool last = y != nullptr;
+    if (!h_in || !nbr || !WT || !bias || H != 128 || N < 0 || (NL != 3 && NL != 4) ||
+        (!last && (!h_out || h_out == h_in)) || (last && (!WoT || !b_out || n_out != 8)) ||
+        (reinterpret_cast<uintptr_t>(h_in) & 15u) || (reinterpret_cast<uintptr_t>(WT) & 15u) ||
+        (y && (reinterpret_cast<uintptr_t>(y) & 15u))) {
+        set_error("fvm_mp_layer_inf: invalid arguments (H must be 128, NL 3 or 4; with y: WoT, b_out and n_out = 8; "
+                  "without: h_out distinct from h_in)");
+        return FVM_ERR_ARG;
+    }
+    if (N == 0) return FVM_OK;
+    const int64_t wave_tiles = (N + 15) / 16;
+    const int64_t blocks = (wave_tiles + (kLayerThreads / 64) - 1) / (kLayerThreads / 64);
+    const int64_t cap = 2LL * layer_grid_cap();
+    const dim3 grid(static_cast<unsigned int>(blocks < cap ? blocks : cap));
+    const __bf16 *hi = static_cast<const __bf16 *>(h_in);
+    const __bf16 *wt = static_cast<const __bf16 *>(WT);
+    const __bf16 *wo = static_cast<const __bf16 *>(WoT);
+    __bf16 *ho = static_cast<__bf16 *>(h_out);
+    hipStream_t s = as_stream(stream);
+    if (NL == 3) {
+        if (last)
+            hipLaunchKernelGGL((k_mp_layer_inf<3, true>), grid, dim3(kLayerThreads), 0, s, hi, nbr, N, wt, bias, ho,
+                               wo, b_out, y);
+        else
+            hipLaunchKernelGGL((k_mp_layer_inf<3, false>), grid, dim3(kLayerThreads), 0, s, hi, nbr, N, wt, bias, ho,
+                               wo, b_out, y);
+    } else {
+        if (last)
+            hipLaunchKernelGGL((k_mp_layer_inf<4, true>), grid, dim3(kLayerThreads), 0, s, hi, nbr, N, wt, bias, ho,
+                               wo, b_out, y);
+        else
+            hipLaunchKernelGGL((k_mp_layer_inf<4, false>), grid, dim3(kLayerThreads), 0, s, hi, nbr, N, wt, bias, ho,
+                               wo, b_out, y);
+    }
+    return check_launch("fvm_mp_layer_inf");
 }
 
 int fvm_transpose_bf16(const float *W, int64_t nrow, int64_t ncol, void *WT, void *stream) {

@@ -1,33 +1,36 @@
 """Learned window predictor with the plain Runge-Kutta solver as fallback and judge.
 
-No reference counterpart.  Over a window of k solver steps of a (batched) channel
-problem the corrector predicts the state at the window's midpoint and end,
+No reference counterpart.  Over a window of k solver steps of a (batched) 2-D channel
+problem the corrector predicts the state at the window's midpoint and end from the
+state U, the RHS f0 = f(U) and the RHS fm half a window earlier:
 
-    U(t + W/2) = U + (W/2) f(U) + s * y_mid,     U(t + W) = U + W f(U) + s * y_end,
+    U(t + W/2) = U + W (3/4 f0 - 1/4 fm) + s * y_mid,
+    U(t + W)   = U + W (2 f0 - fm)       + s * y_end,
 
-with W = k dt of the cell's mesh and y = MPNN(features): explicit Euler steps plus a
-learned correction, where s is, per mesh and conserved variable, the RMS of the Euler
-increment W f(U) over the mesh's cells.  The network therefore learns the window's
-residual relative to the size of the window's change, which is what varies least
-between an impulsive start and a developed flow.  Inputs per cell: the normalised
-primitives, the Euler increment divided by s, two geometry features (log cell size,
-share of boundary facets) and the limited gradient of the RHS evaluation times the cell
-size; the graph is the mesh's face adjacency.
+with W = k dt of the cell's mesh and y = MPNN(features).  The first terms integrate the
+straight line through fm and f0 (an Adams-Bashforth extrapolation over the window, the
+"base"); the network learns what the base misses, in units of s, the RMS over the mesh's
+cells of W (f0 - fm) per conserved variable.  The network sees, per cell, the normalised
+primitives, W f0 and W (f0 - fm) in scaled units, two geometry features (log cell size,
+share of boundary facets) and least-squares gradients of the primitives and of f0 times
+the cell size: the direction in which the state is carried, which a neighbour mean alone
+cannot see.
 
-Every predicted window is checked before it is accepted, with two more RHS evaluations
-(at the predicted midpoint and end): the Simpson defect
+The history costs no extra RHS evaluation: every predicted window is checked with two
+RHS evaluations, at the predicted midpoint and end,
 
-    d = U1 - U - W/6 (f(U) + 4 f(Um) + f(U1))
+    d = U1 - U - W/6 (f(U) + 4 f(Um) + f(U1)),     (Simpson)
+    e = Um - (U + U1)/2 - W/8 (f(U) - f(U1)),       (Hermite midpoint)
 
-and the Hermite-midpoint defect e = Um - (U + U1)/2 - W/8 (f(U) - f(U1)).  Both vanish to
-fourth order in W for a smooth exact trajectory.  A window is accepted only if, on every
-mesh of the batch, ||d|| / ||U1|| and ||e|| / ||U1|| over the mesh's whole conserved
-state are at most tol.  Otherwise it is discarded and re-run with the plain fp32 RK4
-solver.
+both of which vanish to fourth order in W on a smooth exact trajectory, and when the
+window is accepted f(Um) and f(U1) are the next window's fm and f0.  A window is
+accepted only if, on every mesh of the batch, ||d|| / ||U1|| and ||e|| / ||U1|| over the
+mesh's whole conserved state are at most tol.  Otherwise it is discarded and re-run with
+the plain fp32 RK4 solver, which also evaluates f at the window's midpoint and end to
+renew the history.
 
-Training data are the batched solver's own trajectories: (U_t, U_{t+k/2}, U_{t+k})
-triples of every mesh of the batch, stored as the network's inputs and the scaled
-residuals of the two Euler steps.
+Training data are the batched solver's own trajectories: per window the network input
+and the scaled residuals of the base at the true midpoint and end states.
 """
 from __future__ import annotations
 
@@ -43,6 +46,30 @@ from fvm_solver_amd.learned.mpnn import MPNN
 
 _BF = torch.bfloat16
 
+# base predictor: U + W (a f0 + b fm) at the window's midpoint and end
+BASE_MID = (0.75, -0.25)
+BASE_END = (2.0, -1.0)
+
+
+def lsq_neighbour_weights(dm) -> np.ndarray:
+    """[NL][2][N] fp32 weights of the cells' interior face neighbours in the LSQ gradient
+    rows (G_mats, slot order of ``cell_nbr``): grad v_c ~ sum_k w[k][:, c] (v_nb(k) - v_c),
+    boundary faces left out (their weight is dropped)."""
+    h = dm.host
+    gval, src, nbr = h["gval"], h["src"], h["nbr_s"]
+    N, NL = nbr.shape
+    D = gval.shape[1]
+    if D != 2:
+        raise NotImplementedError("the learned corrector takes 2-D meshes")
+    w = np.zeros((NL, D, N), np.float32)
+    rows = np.arange(N)
+    for j in range(NL + 1):
+        k = src[:, j].astype(np.int64)
+        m = k < NL
+        w[k[m], :, rows[m]] = gval[rows[m], :, j]
+    w[np.broadcast_to((nbr >= N).T[:, None, :], w.shape)] = 0.0
+    return w
+
 
 class LearnedCorrector:
     """Corrector for one BatchedSolver (whose meshes, physics and dt it uses)."""
@@ -54,14 +81,19 @@ class LearnedCorrector:
         self.k = k
         self.tol = tol
         dm = solver.dm
+        if dm.dim != 2:
+            raise NotImplementedError("the learned corrector takes 2-D meshes")
+        if k % 2:
+            raise ValueError("the window must hold an even number of steps (its midpoint is predicted)")
         self.device = solver.device
         self.C = solver.C
         self.N = dm.N
         self.M = solver.n_meshes
-        # a private RHS evaluator on the same device mesh: the solver's own carried
-        # gradient state is not touched by the corrector's evaluations
-        self.E = FacetCalc.from_device_mesh(dm, solver.E.cfg, self.device, fused=False)
+        # a private RHS evaluator on the same device mesh (tile-fused when the tables
+        # exist): the solver's own carried gradient state is not touched
+        self.E = FacetCalc.from_device_mesh(dm, solver.E.cfg, self.device, fused=True)
         self.nbr = dm.t["cell_nbr"]                                   # int32 [NL, N]
+        self.wg = torch.from_numpy(lsq_neighbour_weights(dm)).to(self.device)
         cm = dm.host["cell_mesh"].astype(np.int32)
         self.cell_mesh = torch.from_numpy(cm).to(self.device)
         self._cm_long = self.cell_mesh.long()
@@ -71,24 +103,28 @@ class LearnedCorrector:
         logh = np.log(dm.h64.cpu().numpy())
         self.geo_raw = np.stack([logh, bfrac], 1)
         self._h = dm.h64.float().contiguous()
-        if k % 2:
-            raise ValueError("the window must hold an even number of steps (its midpoint is predicted)")
         self.net = MPNN(self.KX, 2 * self.C, hidden, layers, device=self.device, seed=seed)
         self.stats = None
         z = lambda *sh: torch.empty(*sh, device=self.device)
-        self._prim = z(self.N, self.C)
-        self._prim1 = z(self.N, self.C)
-        self._f0 = z(self.N, self.C)
-        self._fm = z(self.N, self.C)
-        self._f1 = z(self.N, self.C)
-        self._Um = z(self.N, self.C)
-        self._U1 = z(self.N, self.C)
-        self._sc = z(self.M, self.C)
-        self._X = torch.empty(self.N, self.KX, dtype=_BF, device=self.device)
-        self._def = torch.empty(self.M, 3 * self.C, dtype=torch.float64, device=self.device)
-        self._part = torch.empty(self.M * 64 * 3 * self.C, dtype=torch.float64, device=self.device)
+        N, C = self.N, self.C
+        self._prim0 = z(N, C)          # primitives of the window start
+        self._prim1 = z(N, C)
+        self._f0 = z(N, C)             # f at the window start
+        self._fh = z(N, C)             # f half a window earlier (history)
+        self._fm = z(N, C)             # f at the predicted midpoint
+        self._f1 = z(N, C)             # f at the predicted end
+        self._d = z(N, C)
+        self._Um = z(N, C)
+        self._U1 = z(N, C)
+        self._s1 = z(self.M, C)
+        self._s2 = z(self.M, C)
+        self._X = torch.empty(N, self.KX, dtype=_BF, device=self.device)
+        self._def = torch.empty(self.M, 3 * C, dtype=torch.float64, device=self.device)
+        self._part = torch.empty(self.M * 64 * 3 * C, dtype=torch.float64, device=self.device)
+        self._hist_version = None      # solver version the history belongs to
         self.n_accept = 0
         self.n_reject = 0
+        self.n_plain = 0               # plain windows that (re)started the history
         self.last_defect = None
 
     # ------------------------------------------------------------------ pieces
@@ -103,12 +139,31 @@ class LearnedCorrector:
         self._state_to_prim(U, prim)
         self.E.rhs(prim, out=f)
 
-    def _inc_scale(self, f0, W, out):
-        """Per mesh and variable, the RMS of the Euler increment W f over the mesh."""
-        nat.check(nat.lib().fvm_mesh_rms(nat.ptr(f0), nat.ptr(W), nat.ptr(self.off), self.M, self.C, 1e-30,
+    def _rms(self, X, W, out):
+        """Per mesh and variable, the RMS of W X over the mesh's cells."""
+        nat.check(nat.lib().fvm_mesh_rms(nat.ptr(X), nat.ptr(W), nat.ptr(self.off), self.M, self.C, 1e-30,
                                          nat.ptr(out), nat.ptr(self._part), nat.stream_ptr(self.device)),
                   "fvm_mesh_rms")
         return out
+
+    def _scales(self, f0, fh, W, s1, s2):
+        self._rms(f0, W, s1)
+        torch.sub(f0, fh, out=self._d)
+        self._rms(self._d, W, s2)
+
+    def history_valid(self) -> bool:
+        return self._hist_version == self.s.version
+
+    def plain_window(self):
+        """k plain RK4 steps, evaluating f at the window's midpoint and end (the history
+        of the next prediction)."""
+        for _ in range(self.k // 2):
+            self.s.step()
+        self._rhs(self.s.state, self._prim1, self._fh)
+        for _ in range(self.k // 2):
+            self.s.step()
+        self._rhs(self.s.state, self._prim0, self._f0)
+        self._hist_version = self.s.version
 
     @staticmethod
     def _moments(chunks, floor_rel=1e-3, floor_abs=1e-8):
@@ -135,8 +190,8 @@ class LearnedCorrector:
 
     def set_stats(self, U):
         """Normalisation from a single representative state."""
-        self._rhs(U, self._prim, self._f0)
-        self.set_stats_from([self._prim])
+        self._state_to_prim(U, self._prim1)
+        self.set_stats_from([self._prim1])
 
     def _apply_stats(self):
         st = self.stats
@@ -145,51 +200,58 @@ class LearnedCorrector:
         g = (torch.from_numpy(self.geo_raw).float() - st["mu_g"]) / st["sd_g"]
         self._geo = g.contiguous().to(self.device)
 
-    def _features_from(self, prim, f0, grad, W, sc, X):
-        nat.check(nat.lib().fvm_mp_features(nat.ptr(prim), nat.ptr(f0), nat.ptr(W), nat.ptr(self._geo), 2,
-                                            nat.ptr(grad), nat.ptr(self._h), self.C - 2, self.N, self.C,
-                                            nat.ptr(self._mu_p), nat.ptr(self._sd_p), nat.ptr(sc),
-                                            nat.ptr(self.cell_mesh), nat.ptr(X), self.KX,
-                                            nat.stream_ptr(self.device)), "fvm_mp_features")
+    def _features_from(self, prim, f0, fh, W, s1, s2, X):
+        nat.check(nat.lib().fvm_mp_window_features(nat.ptr(prim), nat.ptr(f0), nat.ptr(fh), nat.ptr(W),
+                                                   nat.ptr(self._geo), nat.ptr(self._h), nat.ptr(self.nbr),
+                                                   self.nbr.shape[0], nat.ptr(self.wg), self.N, self.C,
+                                                   nat.ptr(self._mu_p), nat.ptr(self._sd_p), nat.ptr(s1), nat.ptr(s2),
+                                                   nat.ptr(self.cell_mesh), nat.ptr(X), self.KX,
+                                                   nat.stream_ptr(self.device)), "fvm_mp_window_features")
         return X
 
-    def features(self, U):
-        """prim, f(U), the increment scale and the network input of state U (into the
-        corrector's buffers)."""
-        self._rhs(U, self._prim, self._f0)
-        W = self.window()
-        self._inc_scale(self._f0, W, self._sc)
-        self._features_from(self._prim, self._f0, self.E.gp, W, self._sc, self._X)
-        return self._X, self._f0, W, self._sc
+    def _apply(self, U, f0, fh, W, coef, y, col, scale, out):
+        a, b = coef
+        nat.check(nat.lib().fvm_mp_window_apply(nat.ptr(U), nat.ptr(f0), nat.ptr(fh), a, b, nat.ptr(W),
+                                                nat.ptr(None if y is None else y[:, col:]), 2 * self.C,
+                                                nat.ptr(scale), nat.ptr(self.cell_mesh), self.N, self.C, nat.ptr(out),
+                                                nat.stream_ptr(self.device)), "fvm_mp_window_apply")
+        return out
 
     # ------------------------------------------------------------------ data
     def generate(self, n_windows: int, log=None):
-        """Run the solver for n_windows windows and keep, per window, the network input
-        and the scaled residuals of the half- and full-window Euler steps; the solver's
-        state advances.  Returns the dataset list of (X, target [N, 2C])."""
+        """Run the solver for n_windows windows (after one plain window that starts the
+        history) and keep, per window, the network input and the scaled residuals of
+        the base at the true midpoint and end; the solver's state advances.  Returns the
+        dataset list of (X, target [N, 2C])."""
         W = self.window()
+        if not self.history_valid():
+            self.plain_window()
         raw = []
         for i in range(n_windows):
             U0 = self.s.state.clone()
-            self._rhs(U0, self._prim, self._f0)
-            prim, f0, grad = self._prim.clone(), self._f0.clone(), self.E.gp.clone()
-            sc = self._inc_scale(f0, W, torch.empty(self.M, self.C, device=self.device))
+            prim, f0, fh = self._prim0.clone(), self._f0.clone(), self._fh.clone()
+            s1, s2 = torch.empty_like(self._s1), torch.empty_like(self._s2)
+            self._scales(f0, fh, W, s1, s2)
             for _ in range(self.k // 2):
                 self.s.step()
-            rm = (self.s.state - U0 - 0.5 * W[:, None] * f0).float()
+            rm = self.s.state - self._apply(U0, f0, fh, W, BASE_MID, None, 0, None, self._Um)
+            self._rhs(self.s.state, self._prim1, self._fh)
             for _ in range(self.k // 2):
                 self.s.step()
-            r1 = (self.s.state - U0 - W[:, None] * f0).float()
-            raw.append((prim, f0, grad, sc, torch.cat([rm, r1], 1)))
+            r1 = self.s.state - self._apply(U0, f0, fh, W, BASE_END, None, 0, None, self._U1)
+            self._rhs(self.s.state, self._prim0, self._f0)
+            self._hist_version = self.s.version
+            raw.append((prim, f0, fh, s1, s2, torch.cat([rm, r1], 1)))
+            del U0
             if log and (i + 1) % max(1, n_windows // 5) == 0:
                 log(f"generated {i + 1}/{n_windows} windows")
         self.set_stats_from([r[0] for r in raw])
         data = []
         while raw:
-            prim, f0, grad, sc, r = raw.pop(0)
+            prim, f0, fh, s1, s2, r = raw.pop(0)
             X = torch.empty(self.N, self.KX, dtype=_BF, device=self.device)
-            sc2 = sc[self._cm_long].repeat(1, 2)
-            data.append((self._features_from(prim, f0, grad, W, sc, X), (r / sc2).contiguous()))
+            sc = s2[self._cm_long].repeat(1, 2)
+            data.append((self._features_from(prim, f0, fh, W, s1, s2, X), (r / sc).contiguous()))
         return data
 
     # ------------------------------------------------------------------ training
@@ -216,55 +278,69 @@ class LearnedCorrector:
         return tot / len(data)
 
     # ------------------------------------------------------------------ inference
-    def _apply(self, U, f0, W, y, col, out):
-        nat.check(nat.lib().fvm_mp_apply(nat.ptr(U), nat.ptr(f0), nat.ptr(W), nat.ptr(y[:, col:]), 2 * self.C,
-                                         nat.ptr(self._sc), nat.ptr(self.cell_mesh), self.N, self.C, nat.ptr(out),
-                                         nat.stream_ptr(self.device)), "fvm_mp_apply")
+    def predict(self, use_net: bool = True):
+        """Predicted midpoint and end states of the window from the solver's state (into
+        the corrector's buffers), with the window lengths.  use_net=False gives the base
+        alone.  Needs a valid history."""
+        if not self.history_valid():
+            raise RuntimeError("the history does not belong to the solver's state: run plain_window() first")
+        U0 = self.s.state
+        W = self.window()
+        self._scales(self._f0, self._fh, W, self._s1, self._s2)
+        y = None
+        if use_net:
+            X = self._features_from(self._prim0, self._f0, self._fh, W, self._s1, self._s2, self._X)
+            y = self.net.forward(X, self.nbr)
+        self._apply(U0, self._f0, self._fh, W, BASE_MID, y, 0, self._s2, self._Um)
+        self._apply(U0, self._f0, self._fh, W, BASE_END, y, self.C, self._s2, self._U1)
+        return self._Um, self._U1, W
 
-    def predict(self, U, out_mid=None, out_end=None):
-        """Predicted midpoint and end states of the window from U (into the corrector's
-        buffers unless given), with f(U) and the window lengths."""
-        X, f0, W, sc = self.features(U)
-        y = self.net.forward(X, self.nbr)
-        Um = self._Um if out_mid is None else out_mid
-        U1 = self._U1 if out_end is None else out_end
-        self._apply(U, f0, 0.5 * W, y, 0, Um)
-        self._apply(U, f0, W, y, self.C, U1)
-        return Um, U1, f0, W
-
-    def defect(self, U0, Um, U1, f0, W) -> np.ndarray:
+    def defect(self, U0, Um, U1, W) -> np.ndarray:
         """Per mesh, max(||d||, ||e||) / ||U1|| of the Simpson and Hermite-midpoint
-        defects over the mesh's whole state (host array); per-variable ratios are kept
-        in ``last_defect``."""
+        defects over the mesh's whole state (host array), with f(U0) = the history's f0;
+        f(Um) and f(U1) are left in the corrector's buffers.  Per-variable ratios are
+        kept in ``last_defect``."""
         self._rhs(Um, self._prim1, self._fm)
         self._rhs(U1, self._prim1, self._f1)
-        nat.check(nat.lib().fvm_mp_defect3(nat.ptr(U0), nat.ptr(Um), nat.ptr(U1), nat.ptr(f0), nat.ptr(self._fm),
-                                           nat.ptr(self._f1), nat.ptr(W), nat.ptr(self.off), self.M, self.C,
-                                           nat.ptr(self._def), nat.ptr(self._part), nat.stream_ptr(self.device)),
-                  "fvm_mp_defect3")
+        nat.check(nat.lib().fvm_mp_defect3(nat.ptr(U0), nat.ptr(Um), nat.ptr(U1), nat.ptr(self._f0),
+                                           nat.ptr(self._fm), nat.ptr(self._f1), nat.ptr(W), nat.ptr(self.off),
+                                           self.M, self.C, nat.ptr(self._def), nat.ptr(self._part),
+                                           nat.stream_ptr(self.device)), "fvm_mp_defect3")
         d = self._def.cpu().numpy()
         C = self.C
         u2 = np.maximum(d[:, 2 * C:].sum(1), 1e-300)
         self.last_defect = np.sqrt(d[:, :C] / np.maximum(d[:, 2 * C:], 1e-300))
         return np.sqrt(np.maximum(d[:, :C].sum(1), d[:, C:2 * C].sum(1)) / u2)
 
-    def advance(self, use_net: bool = True) -> bool:
-        """One window: predict and check, or (if the check fails or use_net is False)
-        k plain RK4 steps.  Returns True when the prediction was accepted."""
-        if use_net:
+    def advance(self, use_net: bool | str = True) -> bool:
+        """One window.  use_net True: predict with the network and check; "base": the
+        same with the base predictor alone; False: plain RK4.  A rejected prediction is
+        re-run with plain RK4, and so is a window without a valid history.  Returns True
+        when a prediction was accepted."""
+        if use_net is False:
+            for _ in range(self.k):
+                self.s.step()
+            return False
+        if self.history_valid():
             U0 = self.s.state
-            Um, U1, f0, W = self.predict(U0)
-            rel = self.defect(U0, Um, U1, f0, W)
+            Um, U1, W = self.predict(use_net is True)
+            rel = self.defect(U0, Um, U1, W)
             if np.all(rel <= self.tol):
                 self.s.set_state(U1)
+                # f at the accepted midpoint and end are the next window's history
+                self._fh, self._fm = self._fm, self._fh
+                self._f0, self._f1 = self._f1, self._f0
+                self._prim0, self._prim1 = self._prim1, self._prim0
+                self._hist_version = self.s.version
                 self.n_accept += 1
                 return True
             self.n_reject += 1
-        for _ in range(self.k):
-            self.s.step()
+        else:
+            self.n_plain += 1
+        self.plain_window()
         return False
 
-    def run(self, n_windows: int, use_net: bool = True):
+    def run(self, n_windows: int, use_net: bool | str = True):
         """n_windows windows; returns (seconds, accepted windows)."""
         torch.cuda.synchronize()
         t0 = time.perf_counter()

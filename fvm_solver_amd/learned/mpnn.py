@@ -150,6 +150,8 @@ class MPNN:
         ws = self._work(M, train)
         lib, sp = nat.lib(), nat.stream_ptr(self.device)
         Hs = ws["Hs"]
+        if not train and H == 128 and self.kx == 32 and self.n_out == 8 and L >= 1:
+            return self._forward_inference(X, nbr, ws)
         gemm(X, self.kx, 1, self.wt["W_in"], 1, self.kx, M, H, self.kx, bias=self.w["b_in"],
              Cf=ws["Z"][0] if train else None, Cb=Hs[0], relu=True)
         for l in range(L):
@@ -168,6 +170,25 @@ class MPNN:
                  Cf=ws["Z"][l + 1] if train else None, Cb=h_out, relu=True, res=h_in)
         h_last = Hs[L] if train else Hs[L % 2]
         gemm(h_last, H, 1, self.wt["W_out"], 1, H, M, self.n_out, H, bias=self.w["b_out"], Cf=ws["Y"])
+        self._last = (X, nbr, ws)
+        return ws["Y"]
+
+    def _forward_inference(self, X, nbr, ws):
+        """Inference forward with the fused kernels: encoder, L - 1 message layers, and
+        the last layer with the decoder in its epilogue (h_L is never stored)."""
+        M, H, L = X.shape[0], self.H, self.L
+        lib, sp = nat.lib(), nat.stream_ptr(self.device)
+        Hs = ws["Hs"]
+        nat.check(lib.fvm_mp_encode(nat.ptr(X), M, self.kx, nat.ptr(self.wt["W_in"]), nat.ptr(self.w["b_in"]), H,
+                                    nat.ptr(Hs[0]), sp), "fvm_mp_encode")
+        for l in range(L):
+            last = l == L - 1
+            nat.check(lib.fvm_mp_layer_inf(nat.ptr(Hs[l % 2]), H, nat.ptr(nbr), M, nbr.shape[0],
+                                           nat.ptr(self.wt[f"W_{l}"]), nat.ptr(self.w[f"b_{l}"]),
+                                           None if last else nat.ptr(Hs[(l + 1) % 2]),
+                                           nat.ptr(self.wt["W_out"]) if last else None,
+                                           nat.ptr(self.w["b_out"]) if last else None, self.n_out if last else 0,
+                                           nat.ptr(ws["Y"]) if last else None, sp), "fvm_mp_layer_inf")
         self._last = (X, nbr, ws)
         return ws["Y"]
 

@@ -339,21 +339,40 @@ int fvm_adam_f32(float *w, const float *g, float *m, float *v, void *wb, int64_t
 
 int fvm_cast_f32_bf16(const float *x, void *y, int64_t n, void *stream);
 
-/* Node features of the corrector: X[c] (bf16, row stride KX) =
- * [(prim[c] - mu_p) / sd_p, win[c] * dudt[c] / sd_d[m(c)], geo[c][0..G),
- *  hc[c] * g[c][i][j] * phi[c][j] / sd_p[j] (i < D, j < C; only if grad), 0 ...]
- * with win[c] the window length (k dt of the cell's mesh), m(c) = cell_mesh[c] (0 when
- * cell_mesh is NULL), sd_d [n_meshes][C], grad the gradient record of FvmWork.gp
- * ([N][(D+1)C]: g[D][C] then phi[C]) and hc a cell size. */
-int fvm_mp_features(const float *prim, const float *dudt, const float *win, const float *geo, int32_t G,
-                    const float *grad, const float *hc, int32_t D, int64_t N, int32_t C, const float *mu_p,
-                    const float *sd_p, const float *sd_d, const int32_t *cell_mesh, void *X, int32_t KX,
-                    void *stream);
+/* Inference encoder of the corrector network: h0 = relu(X W_in + b_in), X bf16 [N][KX],
+ * WT = W_in^T bf16 [H][KX], h0 bf16 [N][H]; KX = 32, H = 128.  No reference counterpart. */
+int fvm_mp_encode(const void *X, int64_t N, int32_t KX, const void *WT, const float *bias, int32_t H, void *h0,
+                  void *stream);
 
-/* Window update: U_out = U + win * dudt + scale[m(c)] * y[:, :C] (y fp32, row stride
- * ld_y; scale [n_meshes][C]). */
-int fvm_mp_apply(const float *U, const float *dudt, const float *win, const float *y, int64_t ld_y,
-                 const float *scale, const int32_t *cell_mesh, int64_t N, int32_t C, float *U_out, void *stream);
+/* Inference message layer h_out = h_in + relu([h_in | mean_nb h_in] W + b) (as
+ * fvm_mp_layer, without the training outputs).  With y non-NULL the layer is the last one
+ * and its decoder is fused: y[c][j] = b_out[j] + sum_f bf16(h_out[c][f]) W_out[f][j]
+ * (fp32 [N][n_out], n_out = 8, WoT = W_out^T bf16 [n_out][H]) and h_out is not written.
+ * No reference counterpart. */
+int fvm_mp_layer_inf(const void *h_in, int32_t H, const int32_t *nbr, int64_t N, int32_t NL, const void *WT,
+                     const float *bias, void *h_out, const void *WoT, const float *b_out, int32_t n_out, float *y,
+                     void *stream);
+
+/* Network input of one window of the learned corrector (2-D batches: NL = 3, C = 4,
+ * KX = 32), per cell c, bf16 row X[c]:
+ *   [(prim - mu_p) / sd_p, W f0 / s1, W (f0 - fm) / s2, geo[c][0..2),
+ *    h d/dx prim / sd_p, h d/dy prim / sd_p, h W d/dx f0 / s1, h W d/dy f0 / s1, 0, 0]
+ * with W = win[c] (the window length of the cell's mesh), h = hc[c] (cell size), f0 the
+ * RHS at the window start, fm the RHS half a window earlier, s1/s2 [n_meshes][C] scales
+ * of mesh cell_mesh[c] (0 when cell_mesh is NULL), and gradients
+ * sum_k wg[k][i][c] (v[nbr[k][c]] - v[c]) over the interior face neighbours (nbr >= N:
+ * boundary, no term; wg [NL][2][N]).  prim, f0, fm and X rows must be 16-byte aligned.
+ * No reference counterpart (the reference has no learned path). */
+int fvm_mp_window_features(const float *prim, const float *f0, const float *fm, const float *win, const float *geo,
+                           const float *hc, const int32_t *nbr, int32_t NL, const float *wg, int64_t N, int32_t C,
+                           const float *mu_p, const float *sd_p, const float *s1, const float *s2,
+                           const int32_t *cell_mesh, void *X, int32_t KX, void *stream);
+
+/* Window update: U_out = U + win * (a f0 + b fm) + scale[m(c)] * y[:, :C] (fm and y may
+ * be NULL; y fp32 with row stride ld_y, scale [n_meshes][C]). */
+int fvm_mp_window_apply(const float *U, const float *f0, const float *fm, float a, float b, const float *win,
+                        const float *y, int64_t ld_y, const float *scale, const int32_t *cell_mesh, int64_t N,
+                        int32_t C, float *U_out, void *stream);
 
 /* Per mesh and column, out[mesh][j] = max(floor_abs, sqrt(mean over the mesh's cells of
  * (win[c] X[c][j])^2)) (win NULL = 1), fp64 sums; part is [n_meshes][64][C] scratch. */

@@ -186,3 +186,44 @@ def test_fused_layer_matches_aggregate_then_gemm():
     diff = (h1.float() - h2.float()).abs()
     assert float((diff > 0).float().mean()) < 1e-3
     assert float(diff.max()) <= float(h2.float().abs().max()) * 2 ** -7
+
+
+def test_inference_forward_matches_training_forward():
+    """The inference kernels (dedicated encoder, layers with permuted W^T rows, decoder
+    fused into the last layer) against the training forward (generic GEMMs and the
+    training layer) on the same network and inputs: hidden layers to one bf16 rounding
+    step, the output to fp32 accumulation order."""
+    nbr = _graph(3000, 12)
+    NL, N = nbr.shape
+    g = torch.Generator().manual_seed(8)
+    x = torch.randn(N, 32, generator=g).to(torch.bfloat16).to(DEV)
+    net = MPNN(32, 8, 128, device=DEV, seed=3)
+    for k in net.names:                       # non-zero biases exercise the permuted bias rows
+        if k.startswith("b"):
+            net.w[k].copy_(torch.randn(net.w[k].shape, generator=g).to(DEV) * 0.1)
+    nb = torch.from_numpy(nbr.astype(np.int32)).to(DEV)
+    y_tr = net.forward(x, nb, train=True).clone()
+    y_inf = net.forward(x, nb, train=False).clone()
+    torch.cuda.synchronize()
+    e = _rel(y_inf, y_tr)
+    print(f"[gpu-test]   inference vs training forward: rel err {e:.2e}, bitwise equal {torch.equal(y_inf, y_tr)}")
+    assert e < 1e-3
+    # the encoder alone: equal to the training path's h0 (same MFMA products, same rounding)
+    ws = net._work(N, True)
+    h0 = torch.empty(N, 128, dtype=torch.bfloat16, device=DEV)
+    lib, sp = nat.lib(), nat.stream_ptr(DEV)
+    nat.check(lib.fvm_mp_encode(nat.ptr(x), N, 32, nat.ptr(net.wt["W_in"]), nat.ptr(net.w["b_in"]), 128, nat.ptr(h0),
+                                sp), "mp_encode")
+    net.forward(x, nb, train=True)
+    torch.cuda.synchronize()
+    d0 = (h0.float() - ws["Hs"][0].float()).abs()
+    assert float(d0.max()) <= float(ws["Hs"][0].float().abs().max()) * 2 ** -7
+    # one inference layer against the training layer
+    h1 = torch.empty_like(h0)
+    nat.check(lib.fvm_mp_layer_inf(nat.ptr(ws["Hs"][0]), 128, nat.ptr(nb), N, NL, nat.ptr(net.wt["W_0"]),
+                                   nat.ptr(net.w["b_0"]), nat.ptr(h1), None, None, 0, None, sp), "mp_layer_inf")
+    torch.cuda.synchronize()
+    d1 = (h1.float() - ws["Hs"][1].float()).abs()
+    print(f"[gpu-test]   layer: differing entries {float((d1 > 0).float().mean()):.2e}")
+    assert float((d1 > 0).float().mean()) < 1e-3
+    assert float(d1.max()) <= float(ws["Hs"][1].float().abs().max()) * 2 ** -7

@@ -32,6 +32,8 @@ def _parse():
                     help="channel2d: the 1 M-cell 2-D channel (the headline); channel3d: the 3-D channel "
                          "(tetrahedra, C = 5; default 4 M cells); batch: 64 random 2-D channel meshes of "
                          "5k-200k cells in one block-diagonal layout, RK4 steps replayed as a HIP graph")
+    ap.add_argument("--integrator", choices=("RK4", "AB3"), default="RK4",
+                    help="fixed-dt classical RK4 (4 fused stages per step) or Adams-Bashforth 3 (one fused stage)")
     ap.add_argument("--meshes", type=int, default=64, help="meshes of the batch configuration")
     ap.add_argument("--cells", type=int, default=None, help="cells (default 1 M in 2-D, 4 M in 3-D)")
     ap.add_argument("--seed", type=int, default=1)
@@ -87,7 +89,7 @@ def div_kernel_bytes(N, F, D, stage):
     return N * 4 * (NL + 1 + 1 + C + (C if stage > 0 else 0) + 2 * C) + F * 4 * C
 
 
-def fused_kernel_bytes(N, F, Fb, D, stage, tiles):
+def fused_kernel_bytes(N, F, Fb, D, stage, tiles, ab3=False):
     """k_fused (tile-fused stage), every array it needs read or written once: per cell
     prim, LSQ values, displacements, eps, info, the local neighbour row and tile-local
     facet row (4 x u16 each), 1/vol and the stage epilogue's traffic (as k_div); per
@@ -97,7 +99,10 @@ def fused_kernel_bytes(N, F, Fb, D, stage, tiles):
     are not algorithmic: they show in the measured traffic."""
     C, NL = D + 2, D + 1
     per_cell = C + (NL + 1) * D + NL * D + 1 + 1 + 2 + 2 + 1
-    dyn = C + (C if stage > 0 else 0) + 2 * C
+    # RK stage: U0, the accumulator after stage 0, accumulator (or state) and next-stage
+    # primitives written; Adams-Bashforth 3: U0 and two past RHS read, state, RHS and
+    # primitives written
+    dyn = 6 * C if ab3 else C + (C if stage > 0 else 0) + 2 * C
     per_facet = (3 * D + 2) + 1
     per_bc = 1 + C + D + 2 * C + (2 * D + 3) + 2 + C + 2 * (D + 1) * C
     return 4 * (N * (per_cell + dyn) + F * per_facet + Fb * per_bc + tiles * 12)
@@ -142,7 +147,9 @@ def main():
             np.savez(a.mesh_cache, pts=pts, cells=cells, fi=fi, fb=fb, tags=np.array(tags))
     log(f"mesh: {len(cells)} cells, {len(fi) + len(fb)} facets ({len(fb)} boundary) in {time.time() - t0:.1f}s")
     Cfg = ConfigChannel3D if D == 3 else ConfigChannel
-    cfg = Cfg(device="cuda:0", n_cells=a.cells, solver_name="Butcher", solver_extra="RK4")
+    ab3 = a.integrator == "AB3"
+    cfg = Cfg(device="cuda:0", n_cells=a.cells, solver_name="AB3" if ab3 else "Butcher",
+              solver_extra="" if ab3 else "RK4")
     t0 = time.time()
     mesh = mesh_from_tuple(pts, cells, fi, fb, device=dev)
     phy = FluidConstitution3D(cfg) if D == 3 else FluidConstitution2D(cfg)
@@ -207,7 +214,7 @@ def main():
     U = eq.cells.state
     st_prim = ts._stage_prim0(U)
     s = ts.stages
-    for _ in range(a.kernel_timing_steps):
+    for _ in range(0 if ab3 else a.kernel_timing_steps):
         cur = st_prim
         out = ts._out[1] if U.data_ptr() == ts._out[0].data_ptr() else ts._out[0]
         for i in range(s):
@@ -234,22 +241,35 @@ def main():
         eq.cells.update_cells(out)
         U = out
         st_prim = cur
+    if ab3:
+        # one fused stage per step (the boundary means come from the previous epilogue):
+        # the step is the kernel launch
+        name = "k_fused" if E.fused else "stage"
+        dur = {name: []}
+        for _ in range(a.kernel_timing_steps):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            ts._solve_step(None)
+            e1.record()
+            dur[name].append((e0, e1))
+            fused_bytes.append(fused_kernel_bytes(N, F, Fb, D, 0, n_tiles, ab3=True))
     torch.cuda.synchronize()
     avg_us = {k: 1e3 * float(np.mean([e0.elapsed_time(e1) for e0, e1 in v])) for k, v in dur.items()}
     log("kernel averages (us): " + ", ".join(f"{k}={v:.1f}" for k, v in avg_us.items()))
     kbytes = {"k_grad": float(grad_kernel_bytes(N, Fb, D)), "k_facet_flux": float(facet_kernel_bytes(N, F, Fb, D)),
-              "k_div": float(np.mean(div_bytes)), "k_fused": float(np.mean(fused_bytes)),
+              "k_div": float(np.mean(div_bytes)) if div_bytes else 0.0, "k_fused": float(np.mean(fused_bytes)),
+              "stage": float(np.mean(fused_bytes)) if ab3 else 0.0,
               "k_bc_mean": float(max(E.mesh.n_slots, 1) * 256 * 4 * 2)}
     kgbs = {k: kbytes[k] / (avg_us[k] * 1e-6) / 1e9 for k in avg_us}
     dom = max(avg_us, key=avg_us.get)
     dom_bytes = kbytes[dom]
     achieved = kgbs[dom]
-    stage_bytes = kbytes["k_fused"] if E.fused else kbytes["k_grad"] + kbytes["k_facet_flux"] + kbytes["k_div"]
+    stage_bytes = kbytes["k_fused"] if (E.fused or ab3) else kbytes["k_grad"] + kbytes["k_facet_flux"] + kbytes["k_div"]
     log(f"dominant {dom}: {dom_bytes / 1e6:.1f} MB/launch -> {achieved:.0f} GB/s; per kernel GB/s: "
         + ", ".join(f"{k}={v:.0f}" for k, v in kgbs.items()) + f"; {stage_bytes / N:.0f} B/cell/stage")
 
     out = {
-        "metric": f"cell-updates/s (fused HIP RK4 step, {D}-D NS channel)",
+        "metric": f"cell-updates/s (fused HIP {a.integrator} step, {D}-D NS channel)",
         "value": value,
         "unit": "cell-updates/s",
         "n_gpus": 1,
@@ -262,8 +282,9 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded Poisson-disk Delaunay mesh, freestream + Gaussian bump)",
         "config": {"workload": ("3-D compressible NS channel with spherical obstacle (tetrahedra)" if D == 3 else
-                                "2-D compressible NS channel with circular obstacle") + ", RK4 at CFL 0.5",
-                   "cells": N, "facets": F, "boundary_facets": Fb, "integrator": "Butcher RK4",
+                                "2-D compressible NS channel with circular obstacle") + f", {a.integrator} at CFL 0.5",
+                   "cells": N, "facets": F, "boundary_facets": Fb,
+                   "integrator": "AB3 (fixed dt)" if ab3 else "Butcher RK4",
                    "dt": dt, "finite": finite, "pipeline": "tile-fused stage" if E.fused else "three kernels",
                    "kernel_avg_us": avg_us,
                    "kernel_algorithmic_bytes": {k: kbytes[k] for k in avg_us}, "kernel_gbs": kgbs},
@@ -274,7 +295,7 @@ def main():
     if tuned:
         out["config"]["autotune_us_per_step"] = tuned["us_per_step"]
     out["config"]["algorithmic_bytes_per_launch"] = dom_bytes
-    if a.traffic_file and os.path.exists(a.traffic_file):
+    if a.traffic_file and os.path.exists(a.traffic_file) and not ab3:
         tr = json.load(open(a.traffic_file))
         k = tr.get("kernels", {}).get(dom)
         if k and tr.get("cells") == N:
@@ -298,7 +319,9 @@ def main():
         _, ofvm = O.from_mesh_tuple(pts, cells, fi, fb, tags, ph)
         if D == 3:
             a.cpu_steps = max(1, a.cpu_steps // 6)
-        stepper = O.OracleStepper(ofvm, "Butcher", "RK4", dt=dt)
+        stepper = O.OracleStepper(ofvm, "AB3" if ab3 else "Butcher", "" if ab3 else "RK4", dt=dt)
+        if ab3:
+            a.cpu_steps *= 4             # one RHS per step
         Uc = U0.clone()
         Uc = stepper.step(Uc)            # warm-up
         t0 = time.perf_counter()
@@ -307,7 +330,7 @@ def main():
         cpu_s = time.perf_counter() - t0
         out["cpu_baseline"] = {"value": N * a.cpu_steps / cpu_s, "unit": "cell-updates/s", "cores": threads,
                                "kind": "port",
-                               "sample": f"{a.cpu_steps} RK4 steps of the same {N}-cell mesh (after 1 warm-up step), "
+                               "sample": f"{a.cpu_steps} {a.integrator} steps of the same {N}-cell mesh (after 1 warm-up step), "
                                          f"fp32 torch-CPU restatement of the reference, {cpu_s:.1f}s"}
         log(f"cpu baseline: {cpu_s:.1f}s for {a.cpu_steps} steps on {threads} threads")
     print(json.dumps(out), flush=True)
@@ -464,18 +487,26 @@ def learned_main(a):
         g.replay()
     torch.cuda.synchronize()
     plain_s = time.perf_counter() - tp0
+    def counts():
+        return np.array([lc.n_accept, lc.n_partial, lc.n_reject, lc.mesh_accepted, lc.mesh_windows])
+
+    c0 = counts()
     learn_s, acc = lc.run(a.eval_windows)
+    c_learn = counts() - c0
     U_learn = learn.state.clone()
     # the method without the network: the same windows with the base predictor alone
     learn.set_state(U0)
     lc.plain_window()
+    c0 = counts()
     base_s, acc_base = lc.run(a.eval_windows, use_net="base")
+    c_base = counts() - c0
     U_base = learn.state.clone()
     sim_t = a.eval_windows * a.window * dts          # per mesh
     N = plain.dm.N
     log(f"eval: {a.eval_meshes} meshes, {N} cells, {a.eval_windows} windows of {a.window} steps after "
-        f"{a.eval_spinup + 1} spin-up windows: plain {plain_s:.3f}s, learned {learn_s:.3f}s (accepted {acc}), "
-        f"base only {base_s:.3f}s (accepted {acc_base})")
+        f"{a.eval_spinup + 1} spin-up windows: plain {plain_s:.3f}s, learned {learn_s:.3f}s (windows accepted/"
+        f"partial/rejected {c_learn[:3]}, mesh predictions accepted {c_learn[3]}/{c_learn[4]}), base only "
+        f"{base_s:.3f}s ({c_base[:3]}, {c_base[3]}/{c_base[4]})")
 
     def mesh_rel(X, R):
         out = []
@@ -521,8 +552,13 @@ def learned_main(a):
         "vs_plain": plain_s / learn_s,
         "plain_value": float(np.mean(sim_t) / plain_s),
         "accepted_windows": int(acc),
+        "partial_windows": int(c_learn[1]),
+        "rejected_windows": int(c_learn[2]),
+        "mesh_predictions_accepted": f"{int(c_learn[3])}/{int(c_learn[4])}",
         "windows": a.eval_windows,
         "base_only": {"value": float(np.mean(sim_t) / base_s), "accepted_windows": int(acc_base),
+                      "partial_windows": int(c_base[1]),
+                      "mesh_predictions_accepted": f"{int(c_base[3])}/{int(c_base[4])}",
                       "vs_plain": plain_s / base_s},
         "max_state_rel_l2_vs_plain_fp32": {"learned": float(vs_plain["learned"].max()),
                                            "base": float(vs_plain["base"].max())},

@@ -12,7 +12,7 @@ import torch
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libfvm_hip.so")
-ABI_VERSION = 6
+ABI_VERSION = 7
 
 FVM_ERR_ROW_OVERFLOW = 2
 
@@ -53,7 +53,8 @@ class FvmStage(ctypes.Structure):
                 ("prim_next", c_void_p), ("U_out", c_void_p), ("dt", c_void_p),
                 ("stage", i32), ("last", i32), ("store_k", i32), ("has_prev", i32), ("bc_ready", i32),
                 ("b", f32), ("b2", f32), ("n_inc", i32), ("inc_idx", i32 * 10), ("inc_coef", f32 * 10),
-                ("mode", i32), ("n_hist", i32), ("f_m1", c_void_p), ("f_m2", c_void_p), ("f_out", c_void_p)]
+                ("mode", i32), ("n_hist", i32), ("f_m1", c_void_p), ("f_m2", c_void_p), ("f_out", c_void_p),
+                ("tile_active", c_void_p)]
 
 
 _PROTOS = {
@@ -105,6 +106,10 @@ _PROTOS = {
                                               c_void_p, c_void_p, c_void_p, i32, c_void_p]),
     "fvm_mp_window_apply": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, f32, f32, c_void_p, c_void_p, i64,
                                            c_void_p, c_void_p, i64, i32, c_void_p, c_void_p]),
+    "fvm_mp_window_predict": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, f32, f32, f32, f32, c_void_p, c_void_p,
+                                             i64, c_void_p, c_void_p, i64, i32, c_void_p, c_void_p, c_void_p]),
+    "fvm_mesh_rms2": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p, i32, i32, f32, c_void_p, c_void_p,
+                                     c_void_p, c_void_p]),
     "fvm_mesh_rms": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, i32, i32, f32, c_void_p, c_void_p, c_void_p]),
     "fvm_mp_defect3": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                       i32, i32, c_void_p, c_void_p, c_void_p]),

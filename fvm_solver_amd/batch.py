@@ -60,6 +60,11 @@ class BatchedSolver:
         self.version = 0             # bumped by every step and set_state
         self.inv_cv = 1.0 / cfg0.C_v
         self.gR = cfg0.gamma * cfg0.C_v * (cfg0.gamma - 1)
+        # first and last mesh of every 256-cell tile of the tile-fused stage
+        cm = dm.host["cell_mesh"]
+        t0 = np.arange(0, dm.N, 256)
+        self._tile_mesh = (cm[t0], cm[np.minimum(t0 + 255, dm.N - 1)])
+        self.cell_mesh = torch.from_numpy(cm.astype(np.int64)).to(self.device)
 
     # ------------------------------------------------------------------ state access
     @property
@@ -106,8 +111,19 @@ class BatchedSolver:
     def set_dt(self, dts):
         self.dt.copy_(torch.as_tensor(np.asarray(dts, np.float32)))
 
-    def step(self):
-        """One fused RK step of every mesh of the batch."""
+    def tile_mask(self, meshes: np.ndarray) -> torch.Tensor:
+        """uint8 flags of the 256-cell tiles holding a cell of any mesh with meshes[i]
+        true (the tiles a masked step() computes)."""
+        if not self.E.fused:
+            raise RuntimeError("masked steps need the tile-fused stage")
+        cs = np.concatenate([[0], np.cumsum(np.asarray(meshes, bool))])
+        lo, hi = self._tile_mesh
+        return torch.from_numpy((cs[hi + 1] - cs[lo] > 0).astype(np.uint8)).to(self.device)
+
+    def step(self, tiles: torch.Tensor | None = None):
+        """One fused RK step of every mesh of the batch, or, with a tile mask from
+        tile_mask(), of the meshes it was made for (the other meshes' cells are left
+        stale or computed from stale inputs: overwrite them afterwards)."""
         self._ensure_prim()
         U0 = self._U[self._cur]
         out = self._U[1 - self._cur]
@@ -117,7 +133,8 @@ class BatchedSolver:
             st = nat.FvmStage(prim=cur.data_ptr(), U0=U0.data_ptr(), acc=self._acc.data_ptr(), acc2=None,
                               kbuf=None, prim_next=nxt.data_ptr(), U_out=out.data_ptr(), dt=self.dt.data_ptr(),
                               stage=i, last=int(i == self.stages - 1), store_k=0,
-                              b=float(self.b[i]), b2=float(self.b2[i]))
+                              b=float(self.b[i]), b2=float(self.b2[i]),
+                              tile_active=None if tiles is None else tiles.data_ptr())
             if i < self.stages - 1 and self.A[i + 1][i] != 0:
                 st.n_inc = 1
                 st.inc_idx[0] = i

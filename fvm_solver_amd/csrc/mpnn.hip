@@ -351,6 +351,74 @@ k_mp_window_apply(const float *__restrict__ U, const float *__restrict__ f0, con
     U_out[t] = v;
 }
 
+// Both predicted states of a window in one pass:
+//   Um = U + win (am f0 + bm fm) + scale y[:, 0:C],  U1 = U + win (ae f0 + be fm) + scale y[:, C:2C]
+// (y fp32 [N][ld_y], optional).
+__global__ void __launch_bounds__(256)
+k_mp_window_predict(const float *__restrict__ U, const float *__restrict__ f0, const float *__restrict__ fm,
+                    float am, float bm, float ae, float be, const float *__restrict__ win, const float *__restrict__ y,
+                    int64_t ld_y, const float *__restrict__ scale, const int32_t *__restrict__ cell_mesh, int64_t N,
+                    int C, float *__restrict__ Um, float *__restrict__ U1) {
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= N * C) return;
+    const int64_t c = t / C;
+    const int j = static_cast<int>(t % C);
+    const float u = U[t], a = f0[t], b = fm[t], w = win[c];
+    float vm = u + w * (am * a + bm * b), v1 = u + w * (ae * a + be * b);
+    if (y) {
+        const float sc = scale[(cell_mesh ? cell_mesh[c] : 0) * C + j];
+        vm += sc * y[c * ld_y + j];
+        v1 += sc * y[c * ld_y + C + j];
+    }
+    Um[t] = vm;
+    U1[t] = v1;
+}
+
+// grid (64, n_meshes): per mesh q, fp64 partial sums of (w a_j)^2 and (w (a_j - b_j))^2
+__global__ void __launch_bounds__(256)
+k_mesh_sumsq2_partial(const float *__restrict__ A, const float *__restrict__ B, const float *__restrict__ win,
+                      const int64_t *__restrict__ off, int C, double *__restrict__ part) {
+    __shared__ double red[256];
+    const int q = blockIdx.y;
+    const int64_t a = off[q], b = off[q + 1];
+    double s[16];
+    for (int j = 0; j < 2 * C; ++j) s[j] = 0.0;
+    for (int64_t c = a + static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; c < b;
+         c += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const float w = win[c];
+        for (int j = 0; j < C; ++j) {
+            const float x = A[c * C + j];
+            const float v = w * x, d = w * (x - B[c * C + j]);
+            s[j] += static_cast<double>(v) * v;
+            s[C + j] += static_cast<double>(d) * d;
+        }
+    }
+    for (int j = 0; j < 2 * C; ++j) {
+        red[threadIdx.x] = s[j];
+        __syncthreads();
+        for (int h = 128; h > 0; h >>= 1) {
+            if (threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) part[(static_cast<int64_t>(q) * gridDim.x + blockIdx.x) * 2 * C + j] = red[0];
+        __syncthreads();
+    }
+}
+
+// out1[q][j] = max(floor, sqrt(sum_b part[q][b][j] / n_q)), out2 from columns C + j
+__global__ void k_mesh_rms2_final(const double *__restrict__ part, int nblk, int n_meshes, int C,
+                                  const int64_t *__restrict__ off, float floor_abs, float *__restrict__ out1,
+                                  float *__restrict__ out2) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_meshes * 2 * C) return;
+    const int q = t / (2 * C), j = t % (2 * C);
+    double s = 0.0;
+    for (int b = 0; b < nblk; ++b) s += part[(static_cast<int64_t>(q) * nblk + b) * 2 * C + j];
+    const double n = static_cast<double>(off[q + 1] - off[q]);
+    const float r = static_cast<float>(sqrt(s / (n > 0 ? n : 1.0)));
+    (j < C ? out1 : out2)[q * C + (j % C)] = r > floor_abs ? r : floor_abs;
+}
+
 // grid (64, n_meshes): per mesh q, fp64 partial sums of (w x_j)^2 over its cells
 __global__ void __launch_bounds__(256)
 k_mesh_sumsq_partial(const float *__restrict__ X, const float *__restrict__ win, const int64_t *__restrict__ off,
@@ -559,7 +627,7 @@ k_mp_layer(const __bf16 *__restrict__ h, const int32_t *__restrict__ nbr, int64_
 }
 
 // Inference encoder: h0 = relu(X W_in + b_in) for KX = 32 inputs and H = 128 features.
-// One wave per 16-cell tile (grid-stride over tiles): the X rows are the B operand (lane
+// One wave per 16-cell tile (grid-stride over tiles, four tiles per pass): the X rows are the B operand (lane
 // g|col holds inputs 8g..8g+7 of cell col), W_in^T the A operand, kept in registers with
 // the lane's bias values for the whole kernel; eight v_mfma_f32_16x16x32_bf16 give
 // features x cells, stored as 8-byte bf16 chunks.
@@ -578,25 +646,36 @@ k_mp_encode(const __bf16 *__restrict__ X, int64_t N, const __bf16 *__restrict__ 
     }
     const int64_t ntiles = (N + 15) / 16;
     const int64_t nw = static_cast<int64_t>(gridDim.x) * (blockDim.x >> 6);
-    for (int64_t t = static_cast<int64_t>(blockIdx.x) * (blockDim.x >> 6) + (threadIdx.x >> 6); t < ntiles; t += nw) {
-        const int64_t row = t * 16 + col;
-        bf16x8_t xv;
-        if (row < N) {
-            xv = *reinterpret_cast<const bf16x8_t *>(X + row * KX + 8 * g);
-        } else {
+    // four tiles per pass: their X rows are loaded together before the MFMAs
+    constexpr int U = 4;
+    for (int64_t t0 = static_cast<int64_t>(blockIdx.x) * (blockDim.x >> 6) + (threadIdx.x >> 6); t0 < ntiles;
+         t0 += U * nw) {
+        bf16x8_t xv[U];
 #pragma unroll
-            for (int e = 0; e < 8; ++e) xv[e] = __bf16(0.0f);
+        for (int u = 0; u < U; ++u) {
+            const int64_t row = (t0 + u * nw) * 16 + col;
+            if (t0 + u * nw < ntiles && row < N) {
+                xv[u] = *reinterpret_cast<const bf16x8_t *>(X + row * KX + 8 * g);
+            } else {
+#pragma unroll
+                for (int e = 0; e < 8; ++e) xv[u][e] = __bf16(0.0f);
+            }
         }
 #pragma unroll
-        for (int cb = 0; cb < 8; ++cb) {
-            const f32x4_t acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf[cb], xv, f32x4_t{0.0f, 0.0f, 0.0f, 0.0f},
-                                                                          0, 0, 0);
-            if (row < N) {
-                typedef __bf16 bf16x4_t __attribute__((ext_vector_type(4)));
-                bf16x4_t o;
+        for (int u = 0; u < U; ++u) {
+            const int64_t row = (t0 + u * nw) * 16 + col;
+            const bool live = t0 + u * nw < ntiles && row < N;
 #pragma unroll
-                for (int r = 0; r < 4; ++r) o[r] = __bf16(fmaxf(acc[r] + bb[cb][r], 0.0f));
-                *reinterpret_cast<bf16x4_t *>(h0 + row * H + 16 * cb + 4 * g) = o;
+            for (int cb = 0; cb < 8; ++cb) {
+                const f32x4_t acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf[cb], xv[u],
+                                                                              f32x4_t{0.0f, 0.0f, 0.0f, 0.0f}, 0, 0, 0);
+                if (live) {
+                    typedef __bf16 bf16x4_t __attribute__((ext_vector_type(4)));
+                    bf16x4_t o;
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) o[r] = __bf16(fmaxf(acc[r] + bb[cb][r], 0.0f));
+                    *reinterpret_cast<bf16x4_t *>(h0 + row * H + 16 * cb + 4 * g) = o;
+                }
             }
         }
     }
@@ -973,6 +1052,32 @@ int fvm_mp_window_apply(const float *U, const float *f0, const float *fm, float 
     hipLaunchKernelGGL(k_mp_window_apply, dim3(blocks_for(N * C, 256)), dim3(256), 0, as_stream(stream), U, f0, fm, a,
                        b, win, y, ld_y, scale, cell_mesh, N, C, U_out);
     return check_launch("fvm_mp_window_apply");
+}
+
+int fvm_mp_window_predict(const float *U, const float *f0, const float *fm, float am, float bm, float ae, float be,
+                          const float *win, const float *y, int64_t ld_y, const float *scale,
+                          const int32_t *cell_mesh, int64_t N, int32_t C, float *Um, float *U1, void *stream) {
+    if (!U || !f0 || !fm || !win || (y && (!scale || ld_y < 2 * C)) || !Um || !U1 || N < 0 || C < 1) {
+        set_error("fvm_mp_window_predict: invalid arguments");
+        return FVM_ERR_ARG;
+    }
+    if (N == 0) return FVM_OK;
+    hipLaunchKernelGGL(k_mp_window_predict, dim3(blocks_for(N * C, 256)), dim3(256), 0, as_stream(stream), U, f0, fm,
+                       am, bm, ae, be, win, y, ld_y, scale, cell_mesh, N, C, Um, U1);
+    return check_launch("fvm_mp_window_predict");
+}
+
+int fvm_mesh_rms2(const float *A, const float *B, const float *win, const int64_t *off, int32_t n_meshes, int32_t C,
+                  float floor_abs, float *out1, float *out2, double *part, void *stream) {
+    if (!A || !B || !win || !off || !out1 || !out2 || !part || n_meshes < 1 || n_meshes > 65535 || C < 1 || C > 8) {
+        set_error("fvm_mesh_rms2: invalid arguments");
+        return FVM_ERR_ARG;
+    }
+    hipStream_t s = as_stream(stream);
+    hipLaunchKernelGGL(k_mesh_sumsq2_partial, dim3(64, n_meshes), dim3(256), 0, s, A, B, win, off, C, part);
+    hipLaunchKernelGGL(k_mesh_rms2_final, dim3(blocks_for(static_cast<int64_t>(n_meshes) * 2 * C, 256)), dim3(256), 0,
+                       s, part, 64, n_meshes, C, off, floor_abs, out1, out2);
+    return check_launch("fvm_mesh_rms2");
 }
 
 int fvm_mesh_rms(const float *X, const float *win, const int64_t *off, int32_t n_meshes, int32_t C, float floor_abs,

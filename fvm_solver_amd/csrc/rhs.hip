@@ -322,7 +322,8 @@ __global__ void __launch_bounds__(kThreads) k_bc_mean(FvmMeshDev m, FvmPhys p, F
 template <int D, int LP>
 __device__ __forceinline__ void cell_gradient(const float (&P)[D + 2], const float (&Un)[D + 1][D + 2], int info,
                                               const float (&gv)[D + 2][D], const float (&disp)[D + 1][D], float eps,
-                                              float (&g)[D][D + 2], float (&phi)[D + 2]) {
+                                              float (&g)[D][D + 2], float (&phi)[D + 2],
+                                              float (&dUk)[D + 1][D + 2]) {
     constexpr int C = D + 2, NL = D + 1;
 #pragma unroll
     for (int jj = 0; jj <= NL; ++jj) {
@@ -356,6 +357,7 @@ __device__ __forceinline__ void cell_gradient(const float (&P)[D + 2], const flo
             float dU = g[0][j] * disp[k][0];
 #pragma unroll
             for (int i = 1; i < D; ++i) dU = dU + g[i][j] * disp[k][i];
+            dUk[k][j] = dU;       // the unlimited reconstruction, reused for the face values
             const float lk = lim_phi<LP>(dU > 0.0f ? upper : lower, dU, eps);
             ph = (k == 0) ? lk : fminf(ph, lk);
         }
@@ -416,8 +418,8 @@ k_grad(FvmMeshDev m, FvmPhys p, FvmWork w, const float *__restrict__ prim, int h
             store_vec<C>(w.ubc + b * C, Un[k]);
         }
     }
-    float g[D][C], phi[C];
-    cell_gradient<D, LP>(P, Un, info, gv, disp, eps, g, phi);
+    float g[D][C], phi[C], dUk[NL][C];
+    cell_gradient<D, LP>(P, Un, info, gv, disp, eps, g, phi, dUk);
     float *o = gp + c * Dm::REC;
 #pragma unroll
     for (int i = 0; i < D; ++i)
@@ -923,8 +925,8 @@ __device__ __forceinline__ void fused_grad_cell(const FvmMeshDev &m, const FvmPh
             if (own) store_vec<C>(w.ubc + b * C, Un[k]);
         }
     }
-    float g[D][C], phi[C];
-    cell_gradient<D, LP>(P, Un, info, gv, disp, eps, g, phi);
+    float g[D][C], phi[C], dUk[NL][C];
+    cell_gradient<D, LP>(P, Un, info, gv, disp, eps, g, phi, dUk);
     if (own && bnd) {
         float *o = w.gp + gc * Dm::REC;
 #pragma unroll
@@ -939,12 +941,7 @@ __device__ __forceinline__ void fused_grad_cell(const FvmMeshDev &m, const FvmPh
     for (int k = 0; k < NL; ++k) {
         float U[C];
 #pragma unroll
-        for (int j = 0; j < C; ++j) {
-            float dU = g[0][j] * disp[k][0];
-#pragma unroll
-            for (int ii = 1; ii < D; ++ii) dU = dU + g[ii][j] * disp[k][ii];
-            U[j] = P[j] + dU * phi[j];
-        }
+        for (int j = 0; j < C; ++j) U[j] = P[j] + dUk[k][j] * phi[j];
         store_vec<C>(FV + (k * ngl + l) * C, U);
     }
 #pragma unroll
@@ -1067,6 +1064,9 @@ k_fused(FvmMeshDev m, FvmPhys p, FvmWork w, const float *__restrict__ prim, floa
     const int64_t N = m.n_cells;
     const int64_t ntiles = (N + kThreads - 1) / kThreads;
     const int64_t tile = xcd_tile(blockIdx.x, ntiles);
+    if constexpr (EPI == EPI_STAGE) {
+        if (st.tile_active && !st.tile_active[tile]) return;     // a tile the caller does not step
+    }
     const int *tb = m.fz_tile + tile * kFzTab;
     const int halo_off = tb[0], n1 = tb[1], n2 = tb[2], nbr_off = tb[3], fo0 = tb[4], nfo = tb[5];
     const int foff = tb[6], nfor = tb[7], g0 = tb[8], ng = tb[9], h1_off = tb[10];
@@ -1080,7 +1080,12 @@ k_fused(FvmMeshDev m, FvmPhys p, FvmWork w, const float *__restrict__ prim, floa
     const int64_t c0 = tile * kThreads;
     const int nown = static_cast<int>(N - c0 < kThreads ? N - c0 : kThreads);
     const int nft = nfo + nfor;
-    const bool has_own = i < nown, has_h1 = i < n1;
+    // The H1 cells and the second facets are extra work for some threads; the thread
+    // that takes H1 cell / second facet q is rotated by 64 (tile % 4) lanes so that the
+    // doubly loaded waves differ from tile to tile (a workgroup's wave w tends to run on
+    // the same SIMD every time)
+    const int hq = (i - 64 * static_cast<int>(tile & 3)) & (kThreads - 1);
+    const bool has_own = i < nown, has_h1 = hq < n1;
     __shared__ float gmean[kMaxGroups];
     if (ng > 0) block_group_means_g(m, w, g0, ng, gmean, red);
 
@@ -1111,8 +1116,8 @@ k_fused(FvmMeshDev m, FvmPhys p, FvmWork w, const float *__restrict__ prim, floa
     int64_t hgc = 0;
     ushort4 hnr = make_ushort4(0, 0, 0, 0);
     if (has_h1) {
-        hgc = m.fz_halo[halo_off + i];
-        const float4 *r = reinterpret_cast<const float4 *>(m.fz_hrec + static_cast<int64_t>(h1_off + i) * HREC);
+        hgc = m.fz_halo[halo_off + hq];
+        const float4 *r = reinterpret_cast<const float4 *>(m.fz_hrec + static_cast<int64_t>(h1_off + hq) * HREC);
         float rv[HREC];
 #pragma unroll
         for (int q = 0; q < HREC / 4; ++q) {
@@ -1132,7 +1137,7 @@ k_fused(FvmMeshDev m, FvmPhys p, FvmWork w, const float *__restrict__ prim, floa
             for (int ii = 0; ii < D; ++ii) hdisp[k][ii] = rv[(NL + 1) * D + k * D + ii];
         heps = rv[(NL + 1) * D + NL * D];
         hinfo = __float_as_int(rv[(NL + 1) * D + NL * D + 1]);
-        hnr = *reinterpret_cast<const ushort4 *>(m.fz_nbr + (static_cast<int64_t>(nbr_off) + kThreads + i) * 4);
+        hnr = *reinterpret_cast<const ushort4 *>(m.fz_nbr + (static_cast<int64_t>(nbr_off) + kThreads + hq) * 4);
     }
     for (int q = i; q < n1 + n2; q += kThreads) {
         float u[C];
@@ -1150,7 +1155,7 @@ k_fused(FvmMeshDev m, FvmPhys p, FvmWork w, const float *__restrict__ prim, floa
         fused_grad_cell<D, LP>(m, p, w, Pl, FV, GL, m.fz_ngl, i, c0 + i, true, ogv, odisp, oeps, oinfo, onr,
                                has_prev, gmean, g0);
     if (has_h1)
-        fused_grad_cell<D, LP>(m, p, w, Pl, FV, GL, m.fz_ngl, kThreads + i, hgc, false, hgv, hdisp, heps, hinfo,
+        fused_grad_cell<D, LP>(m, p, w, Pl, FV, GL, m.fz_ngl, kThreads + hq, hgc, false, hgv, hdisp, heps, hinfo,
                                hnr, has_prev, gmean, g0);
     __syncthreads();
 
@@ -1158,8 +1163,8 @@ k_fused(FvmMeshDev m, FvmPhys p, FvmWork w, const float *__restrict__ prim, floa
     int64_t fB = 0;
     unsigned dB = 0;
     float geoB[3 * D + 2];
-    const bool twoB = i + kThreads < nft;
-    if (twoB) fused_facet_load<D>(m, i + kThreads, fo0, nfo, foff, fB, dB, geoB);
+    const bool twoB = hq + kThreads < nft;
+    if (twoB) fused_facet_load<D>(m, hq + kThreads, fo0, nfo, foff, fB, dB, geoB);
     const int64_t c = c0 + i;
     const int mesh = (m.cell_mesh && has_own) ? m.cell_mesh[c] : 0;
     float U0v[C], ap[C], a2p[C];
@@ -1173,7 +1178,7 @@ k_fused(FvmMeshDev m, FvmPhys p, FvmWork w, const float *__restrict__ prim, floa
     if (i < nft)
         fused_facet<D>(m, p, w, Pl, FV, GL, m.fz_ngl, FL, i, fA, dA, geoA, c0, halo_off, has_prev, gmean, g0);
     if (twoB)
-        fused_facet<D>(m, p, w, Pl, FV, GL, m.fz_ngl, FL, i + kThreads, fB, dB, geoB, c0, halo_off, has_prev, gmean,
+        fused_facet<D>(m, p, w, Pl, FV, GL, m.fz_ngl, FL, hq + kThreads, fB, dB, geoB, c0, halo_off, has_prev, gmean,
                        g0);
     __syncthreads();
 
@@ -1362,7 +1367,11 @@ static int check_align(const FvmMeshDev *m, const FvmWork *w, const FvmStage *st
     return FVM_OK;
 }
 
-static int check_stage(const FvmWork *w, const FvmStage *st, const char *who) {
+static int check_stage(const FvmWork *w, const FvmStage *st, const char *who, bool m_has_tiles = true) {
+    if (st && st->tile_active && !m_has_tiles) {
+        set_error("%s: a tile mask needs the tile-fused stage (2-D mesh with tile tables)", who);
+        return FVM_ERR_ARG;
+    }
     if (st && st->mode == 1) {
         if (!st->prim || !st->U0 || !st->prim_next || !st->dt || !st->U_out || !st->f_out || st->n_hist < 0 ||
             (st->n_hist >= 1 && !st->f_m1) || (st->n_hist >= 2 && !st->f_m2) || st->acc2) {
@@ -1422,7 +1431,7 @@ int fvm_grad_f32(const FvmMeshDev *m, const FvmPhys *p, const FvmWork *w, const 
 
 int fvm_rk_stage_f32(const FvmMeshDev *m, const FvmPhys *p, const FvmWork *w, const FvmStage *st, void *stream) {
     int e = check_mesh(m, p, w, "fvm_rk_stage_f32");
-    if (!e) e = check_stage(w, st, "fvm_rk_stage_f32");
+    if (!e) e = check_stage(w, st, "fvm_rk_stage_f32", m->fz_tile != nullptr);
     if (!e) e = check_align(m, w, st, nullptr, st->prim, "fvm_rk_stage_f32");
     if (!e && m->fz_tile) e = check_fused(*m, *w, "fvm_rk_stage_f32");
     if (e) return e;
@@ -1439,7 +1448,7 @@ int fvm_rk_stage_part_f32(const FvmMeshDev *m, const FvmPhys *p, const FvmWork *
         return FVM_ERR_ARG;
     }
     int e = check_mesh(m, p, w, "fvm_rk_stage_part_f32");
-    if (!e) e = check_stage(w, st, "fvm_rk_stage_part_f32");
+    if (!e) e = check_stage(w, st, "fvm_rk_stage_part_f32", m->fz_tile != nullptr && !(parts & 14));
     if (!e) e = check_align(m, w, st, nullptr, st->prim, "fvm_rk_stage_part_f32");
     if (!e && (parts & 16)) e = check_fused(*m, *w, "fvm_rk_stage_part_f32");
     if (e) return e;

@@ -24,10 +24,11 @@ RHS evaluations, at the predicted midpoint and end,
 
 both of which vanish to fourth order in W on a smooth exact trajectory, and when the
 window is accepted f(Um) and f(U1) are the next window's fm and f0.  A window is
-accepted only if, on every mesh of the batch, ||d|| / ||U1|| and ||e|| / ||U1|| over the
-mesh's whole conserved state are at most tol.  Otherwise it is discarded and re-run with
-the plain fp32 RK4 solver, which also evaluates f at the window's midpoint and end to
-renew the history.
+accepted for a mesh only if ||d|| / ||U1|| and ||e|| / ||U1|| over the mesh's whole
+conserved state are at most tol.  The meshes that fail are re-run with the plain fp32 RK4
+solver (a masked step computes only the 256-cell tiles holding their cells), the others
+keep their predictions, and f is evaluated at the merged midpoint and end to renew the
+history.
 
 Training data are the batched solver's own trajectories: per window the network input
 and the scaled residuals of the base at the true midpoint and end states.
@@ -113,7 +114,6 @@ class LearnedCorrector:
         self._fh = z(N, C)             # f half a window earlier (history)
         self._fm = z(N, C)             # f at the predicted midpoint
         self._f1 = z(N, C)             # f at the predicted end
-        self._d = z(N, C)
         self._Um = z(N, C)
         self._U1 = z(N, C)
         self._s1 = z(self.M, C)
@@ -125,6 +125,9 @@ class LearnedCorrector:
         self.n_accept = 0
         self.n_reject = 0
         self.n_plain = 0               # plain windows that (re)started the history
+        self.n_partial = 0             # windows with some meshes re-run
+        self.mesh_accepted = 0         # accepted (mesh, window) predictions
+        self.mesh_windows = 0          # checked (mesh, window) predictions
         self.last_defect = None
 
     # ------------------------------------------------------------------ pieces
@@ -147,9 +150,10 @@ class LearnedCorrector:
         return out
 
     def _scales(self, f0, fh, W, s1, s2):
-        self._rms(f0, W, s1)
-        torch.sub(f0, fh, out=self._d)
-        self._rms(self._d, W, s2)
+        """Per mesh and variable: s1 = RMS of W f0, s2 = RMS of W (f0 - fh)."""
+        nat.check(nat.lib().fvm_mesh_rms2(nat.ptr(f0), nat.ptr(fh), nat.ptr(W), nat.ptr(self.off), self.M, self.C,
+                                          1e-30, nat.ptr(s1), nat.ptr(s2), nat.ptr(self._part),
+                                          nat.stream_ptr(self.device)), "fvm_mesh_rms2")
 
     def history_valid(self) -> bool:
         return self._hist_version == self.s.version
@@ -291,8 +295,12 @@ class LearnedCorrector:
         if use_net:
             X = self._features_from(self._prim0, self._f0, self._fh, W, self._s1, self._s2, self._X)
             y = self.net.forward(X, self.nbr)
-        self._apply(U0, self._f0, self._fh, W, BASE_MID, y, 0, self._s2, self._Um)
-        self._apply(U0, self._f0, self._fh, W, BASE_END, y, self.C, self._s2, self._U1)
+        (am, bm), (ae, be) = BASE_MID, BASE_END
+        nat.check(nat.lib().fvm_mp_window_predict(nat.ptr(U0), nat.ptr(self._f0), nat.ptr(self._fh), am, bm, ae, be,
+                                                  nat.ptr(W), nat.ptr(y), 2 * self.C, nat.ptr(self._s2),
+                                                  nat.ptr(self.cell_mesh), self.N, self.C, nat.ptr(self._Um),
+                                                  nat.ptr(self._U1), nat.stream_ptr(self.device)),
+                  "fvm_mp_window_predict")
         return self._Um, self._U1, W
 
     def defect(self, U0, Um, U1, W) -> np.ndarray:
@@ -314,31 +322,55 @@ class LearnedCorrector:
 
     def advance(self, use_net: bool | str = True) -> bool:
         """One window.  use_net True: predict with the network and check; "base": the
-        same with the base predictor alone; False: plain RK4.  A rejected prediction is
-        re-run with plain RK4, and so is a window without a valid history.  Returns True
-        when a prediction was accepted."""
+        same with the base predictor alone; False: plain RK4.  Meshes whose prediction
+        fails the check are re-run with plain RK4 (only the tiles holding their cells are
+        stepped, the other meshes keep their accepted predictions); a window without a
+        valid history is run plainly.  Returns True when every mesh's prediction was
+        accepted."""
         if use_net is False:
             for _ in range(self.k):
                 self.s.step()
             return False
-        if self.history_valid():
-            U0 = self.s.state
-            Um, U1, W = self.predict(use_net is True)
-            rel = self.defect(U0, Um, U1, W)
-            if np.all(rel <= self.tol):
-                self.s.set_state(U1)
-                # f at the accepted midpoint and end are the next window's history
-                self._fh, self._fm = self._fm, self._fh
-                self._f0, self._f1 = self._f1, self._f0
-                self._prim0, self._prim1 = self._prim1, self._prim0
-                self._hist_version = self.s.version
-                self.n_accept += 1
-                return True
-            self.n_reject += 1
-        else:
+        if not self.history_valid():
             self.n_plain += 1
-        self.plain_window()
+            self.plain_window()
+            return False
+        U0 = self.s.state
+        Um, U1, W = self.predict(use_net is True)
+        ok = self.defect(U0, Um, U1, W) <= self.tol
+        self.mesh_accepted += int(ok.sum())
+        self.mesh_windows += self.M
+        if ok.all():
+            self.s.set_state(U1)
+            # f at the accepted midpoint and end are the next window's history
+            self._fh, self._fm = self._fm, self._fh
+            self._f0, self._f1 = self._f1, self._f0
+            self._prim0, self._prim1 = self._prim1, self._prim0
+            self._hist_version = self.s.version
+            self.n_accept += 1
+            return True
+        if not ok.any() or not self.s.E.fused:
+            self.n_reject += 1
+            self.plain_window()
+            return False
+        self.n_partial += 1
+        self._partial_window(ok)
         return False
+
+    def _partial_window(self, ok: np.ndarray):
+        """Plain RK4 for the meshes with ok false, predictions for the others; the
+        history is evaluated on the merged midpoint and end states."""
+        tiles = self.s.tile_mask(~ok)
+        redo = torch.from_numpy(~ok).to(self.device)[self.s.cell_mesh][:, None]
+        for _ in range(self.k // 2):
+            self.s.step(tiles)
+        mid = torch.where(redo, self.s.state, self._Um)
+        self._rhs(mid, self._prim1, self._fh)
+        for _ in range(self.k // 2):
+            self.s.step(tiles)
+        self.s.set_state(torch.where(redo, self.s.state, self._U1))
+        self._rhs(self.s.state, self._prim0, self._f0)
+        self._hist_version = self.s.version
 
     def run(self, n_windows: int, use_net: bool | str = True):
         """n_windows windows; returns (seconds, accepted windows)."""

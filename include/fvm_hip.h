@@ -214,6 +214,12 @@ typedef struct FvmStage {
     const float *f_m1;     /* [N][C]                                                  */
     const float *f_m2;     /* [N][C]                                                  */
     float *f_out;          /* [N][C]                                                  */
+    /* [ceil(N / 256)] or NULL: with the tile-fused stage, 256-cell tiles whose flag is 0
+       are skipped (nothing of them is read or written); a batch re-steps only the meshes
+       whose cells lie in flagged tiles.  Cells of other meshes that share a flagged tile
+       are computed from stale inputs and must be overwritten by the caller.  No
+       reference counterpart. */
+    const uint8_t *tile_active;
 } FvmStage;
 
 int fvm_rk_stage_f32(const FvmMeshDev *m, const FvmPhys *p, const FvmWork *w, const FvmStage *st,
@@ -373,6 +379,19 @@ int fvm_mp_window_features(const float *prim, const float *f0, const float *fm, 
 int fvm_mp_window_apply(const float *U, const float *f0, const float *fm, float a, float b, const float *win,
                         const float *y, int64_t ld_y, const float *scale, const int32_t *cell_mesh, int64_t N,
                         int32_t C, float *U_out, void *stream);
+
+/* Both predicted states of a window: Um = U + win (am f0 + bm fm) + scale[m(c)] y[:, 0:C],
+ * U1 = U + win (ae f0 + be fm) + scale[m(c)] y[:, C:2C] (y optional, fp32, row stride
+ * ld_y >= 2C).  No reference counterpart. */
+int fvm_mp_window_predict(const float *U, const float *f0, const float *fm, float am, float bm, float ae, float be,
+                          const float *win, const float *y, int64_t ld_y, const float *scale,
+                          const int32_t *cell_mesh, int64_t N, int32_t C, float *Um, float *U1, void *stream);
+
+/* Per mesh and column: out1[mesh][j] = RMS over the mesh's cells of win[c] A[c][j] and
+ * out2[mesh][j] = RMS of win[c] (A[c][j] - B[c][j]), each at least floor_abs (fp64 sums;
+ * part is [n_meshes][64][2C] scratch).  No reference counterpart. */
+int fvm_mesh_rms2(const float *A, const float *B, const float *win, const int64_t *off, int32_t n_meshes, int32_t C,
+                  float floor_abs, float *out1, float *out2, double *part, void *stream);
 
 /* Per mesh and column, out[mesh][j] = max(floor_abs, sqrt(mean over the mesh's cells of
  * (win[c] X[c][j])^2)) (win NULL = 1), fp64 sums; part is [n_meshes][64][C] scratch. */

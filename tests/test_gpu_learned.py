@@ -112,3 +112,45 @@ def test_corrupted_network_falls_back_to_plain_rk4():
     torch.cuda.synchronize()
     assert lc.n_plain == 1 and lc.n_reject == 2 and lc.n_accept == 0
     assert torch.equal(s.state, ref.state)
+
+
+def test_masked_step_equals_full_step_on_selected_meshes():
+    s, problems, _ = _solver(41, n=5)
+    ref = BatchedSolver(problems, device=DEV)
+    ref.set_dt(s.dt.cpu().numpy())
+    sel = np.array([True, False, True, False, True])
+    tiles = s.tile_mask(sel)
+    for _ in range(6):
+        s.step(tiles)
+        ref.step()
+    torch.cuda.synchronize()
+    for i in np.flatnonzero(sel):
+        assert torch.equal(s.state_of(i), ref.state_of(i)), i
+
+
+def test_partial_window_merges_predictions_and_plain_steps():
+    """Meshes that fail the check take the plain RK4 window (bit for bit), the others
+    keep their predicted end state."""
+    s, problems, _ = _solver(31, n=4)
+    ref = BatchedSolver(problems, device=DEV)
+    ref.set_dt(s.dt.cpu().numpy())
+    lc = LearnedCorrector(s, k=4, seed=2)
+    lc.set_stats(s.state)
+    lc.plain_window()
+    for _ in range(lc.k):
+        ref.step()
+    assert torch.equal(s.state, ref.state)
+    Um, U1, W = lc.predict(use_net=False)
+    rel = lc.defect(s.state, Um, U1, W)
+    ok = rel <= np.median(rel)
+    assert ok.any() and not ok.all()
+    U1p = U1.clone()
+    lc._partial_window(ok)
+    for _ in range(lc.k):
+        ref.step()
+    torch.cuda.synchronize()
+    assert lc.history_valid()
+    for i in range(4):
+        lo, hi = int(s.offsets[i]), int(s.offsets[i + 1])
+        want = U1p[lo:hi] if ok[i] else ref.state_of(i)
+        assert torch.equal(s.state_of(i), want), i

@@ -322,8 +322,7 @@ __global__ void __launch_bounds__(kThreads) k_bc_mean(FvmMeshDev m, FvmPhys p, F
 template <int D, int LP>
 __device__ __forceinline__ void cell_gradient(const float (&P)[D + 2], const float (&Un)[D + 1][D + 2], int info,
                                               const float (&gv)[D + 2][D], const float (&disp)[D + 1][D], float eps,
-                                              float (&g)[D][D + 2], float (&phi)[D + 2],
-                                              float (&dUk)[D + 1][D + 2]) {
+                                              float (&g)[D][D + 2], float (&phi)[D + 2]) {
     constexpr int C = D + 2, NL = D + 1;
 #pragma unroll
     for (int jj = 0; jj <= NL; ++jj) {
@@ -357,7 +356,6 @@ __device__ __forceinline__ void cell_gradient(const float (&P)[D + 2], const flo
             float dU = g[0][j] * disp[k][0];
 #pragma unroll
             for (int i = 1; i < D; ++i) dU = dU + g[i][j] * disp[k][i];
-            dUk[k][j] = dU;       // the unlimited reconstruction, reused for the face values
             const float lk = lim_phi<LP>(dU > 0.0f ? upper : lower, dU, eps);
             ph = (k == 0) ? lk : fminf(ph, lk);
         }
@@ -418,8 +416,8 @@ k_grad(FvmMeshDev m, FvmPhys p, FvmWork w, const float *__restrict__ prim, int h
             store_vec<C>(w.ubc + b * C, Un[k]);
         }
     }
-    float g[D][C], phi[C], dUk[NL][C];
-    cell_gradient<D, LP>(P, Un, info, gv, disp, eps, g, phi, dUk);
+    float g[D][C], phi[C];
+    cell_gradient<D, LP>(P, Un, info, gv, disp, eps, g, phi);
     float *o = gp + c * Dm::REC;
 #pragma unroll
     for (int i = 0; i < D; ++i)
@@ -925,8 +923,8 @@ __device__ __forceinline__ void fused_grad_cell(const FvmMeshDev &m, const FvmPh
             if (own) store_vec<C>(w.ubc + b * C, Un[k]);
         }
     }
-    float g[D][C], phi[C], dUk[NL][C];
-    cell_gradient<D, LP>(P, Un, info, gv, disp, eps, g, phi, dUk);
+    float g[D][C], phi[C];
+    cell_gradient<D, LP>(P, Un, info, gv, disp, eps, g, phi);
     if (own && bnd) {
         float *o = w.gp + gc * Dm::REC;
 #pragma unroll
@@ -941,7 +939,12 @@ __device__ __forceinline__ void fused_grad_cell(const FvmMeshDev &m, const FvmPh
     for (int k = 0; k < NL; ++k) {
         float U[C];
 #pragma unroll
-        for (int j = 0; j < C; ++j) U[j] = P[j] + dUk[k][j] * phi[j];
+        for (int j = 0; j < C; ++j) {
+            float dU = g[0][j] * disp[k][0];
+#pragma unroll
+            for (int ii = 1; ii < D; ++ii) dU = dU + g[ii][j] * disp[k][ii];
+            U[j] = P[j] + dU * phi[j];
+        }
         store_vec<C>(FV + (k * ngl + l) * C, U);
     }
 #pragma unroll
@@ -1080,12 +1083,7 @@ k_fused(FvmMeshDev m, FvmPhys p, FvmWork w, const float *__restrict__ prim, floa
     const int64_t c0 = tile * kThreads;
     const int nown = static_cast<int>(N - c0 < kThreads ? N - c0 : kThreads);
     const int nft = nfo + nfor;
-    // The H1 cells and the second facets are extra work for some threads; the thread
-    // that takes H1 cell / second facet q is rotated by 64 (tile % 4) lanes so that the
-    // doubly loaded waves differ from tile to tile (a workgroup's wave w tends to run on
-    // the same SIMD every time)
-    const int hq = (i - 64 * static_cast<int>(tile & 3)) & (kThreads - 1);
-    const bool has_own = i < nown, has_h1 = hq < n1;
+    const bool has_own = i < nown, has_h1 = i < n1;
     __shared__ float gmean[kMaxGroups];
     if (ng > 0) block_group_means_g(m, w, g0, ng, gmean, red);
 
@@ -1116,8 +1114,8 @@ k_fused(FvmMeshDev m, FvmPhys p, FvmWork w, const float *__restrict__ prim, floa
     int64_t hgc = 0;
     ushort4 hnr = make_ushort4(0, 0, 0, 0);
     if (has_h1) {
-        hgc = m.fz_halo[halo_off + hq];
-        const float4 *r = reinterpret_cast<const float4 *>(m.fz_hrec + static_cast<int64_t>(h1_off + hq) * HREC);
+        hgc = m.fz_halo[halo_off + i];
+        const float4 *r = reinterpret_cast<const float4 *>(m.fz_hrec + static_cast<int64_t>(h1_off + i) * HREC);
         float rv[HREC];
 #pragma unroll
         for (int q = 0; q < HREC / 4; ++q) {
@@ -1137,7 +1135,7 @@ k_fused(FvmMeshDev m, FvmPhys p, FvmWork w, const float *__restrict__ prim, floa
             for (int ii = 0; ii < D; ++ii) hdisp[k][ii] = rv[(NL + 1) * D + k * D + ii];
         heps = rv[(NL + 1) * D + NL * D];
         hinfo = __float_as_int(rv[(NL + 1) * D + NL * D + 1]);
-        hnr = *reinterpret_cast<const ushort4 *>(m.fz_nbr + (static_cast<int64_t>(nbr_off) + kThreads + hq) * 4);
+        hnr = *reinterpret_cast<const ushort4 *>(m.fz_nbr + (static_cast<int64_t>(nbr_off) + kThreads + i) * 4);
     }
     for (int q = i; q < n1 + n2; q += kThreads) {
         float u[C];
@@ -1155,7 +1153,7 @@ k_fused(FvmMeshDev m, FvmPhys p, FvmWork w, const float *__restrict__ prim, floa
         fused_grad_cell<D, LP>(m, p, w, Pl, FV, GL, m.fz_ngl, i, c0 + i, true, ogv, odisp, oeps, oinfo, onr,
                                has_prev, gmean, g0);
     if (has_h1)
-        fused_grad_cell<D, LP>(m, p, w, Pl, FV, GL, m.fz_ngl, kThreads + hq, hgc, false, hgv, hdisp, heps, hinfo,
+        fused_grad_cell<D, LP>(m, p, w, Pl, FV, GL, m.fz_ngl, kThreads + i, hgc, false, hgv, hdisp, heps, hinfo,
                                hnr, has_prev, gmean, g0);
     __syncthreads();
 
@@ -1163,8 +1161,8 @@ k_fused(FvmMeshDev m, FvmPhys p, FvmWork w, const float *__restrict__ prim, floa
     int64_t fB = 0;
     unsigned dB = 0;
     float geoB[3 * D + 2];
-    const bool twoB = hq + kThreads < nft;
-    if (twoB) fused_facet_load<D>(m, hq + kThreads, fo0, nfo, foff, fB, dB, geoB);
+    const bool twoB = i + kThreads < nft;
+    if (twoB) fused_facet_load<D>(m, i + kThreads, fo0, nfo, foff, fB, dB, geoB);
     const int64_t c = c0 + i;
     const int mesh = (m.cell_mesh && has_own) ? m.cell_mesh[c] : 0;
     float U0v[C], ap[C], a2p[C];
@@ -1178,7 +1176,7 @@ k_fused(FvmMeshDev m, FvmPhys p, FvmWork w, const float *__restrict__ prim, floa
     if (i < nft)
         fused_facet<D>(m, p, w, Pl, FV, GL, m.fz_ngl, FL, i, fA, dA, geoA, c0, halo_off, has_prev, gmean, g0);
     if (twoB)
-        fused_facet<D>(m, p, w, Pl, FV, GL, m.fz_ngl, FL, hq + kThreads, fB, dB, geoB, c0, halo_off, has_prev, gmean,
+        fused_facet<D>(m, p, w, Pl, FV, GL, m.fz_ngl, FL, i + kThreads, fB, dB, geoB, c0, halo_off, has_prev, gmean,
                        g0);
     __syncthreads();
 

@@ -449,6 +449,8 @@ def learned_main(a):
     tp, tparams = random_channel_batch(a.train_meshes, seed=a.seed + 1000, n_min=a.n_min, n_max=a.n_max, device=dev)
     ts = BatchedSolver(tp, device=dev)
     ts.set_dt(ts.cfl_dt(0.5))
+    for _ in range(a.eval_spinup * a.window):         # past the impulsive start, as the evaluation
+        ts.step()
     lc = LearnedCorrector(ts, k=a.window, seed=a.seed, tol=a.tol)
     data = lc.generate(a.train_windows, log=log)
     gen_s = time.time() - t0
@@ -494,6 +496,29 @@ def learned_main(a):
     def counts():
         return np.array([lc.n_accept, lc.n_partial, lc.n_reject, lc.mesh_accepted, lc.mesh_windows])
 
+    # what the network gains per window: predictions from states of the plain trajectory
+    # (history from plain steps) against the plain RK4 window, network and base alone
+    def mesh_rel(X, R):
+        out = []
+        for i in range(a.eval_meshes):
+            lo, hi = int(plain.offsets[i]), int(plain.offsets[i + 1])
+            out.append(float((X[lo:hi] - R[lo:hi]).double().norm() / R[lo:hi].double().norm()))
+        return np.asarray(out)
+
+    learn.set_state(U0)
+    lc.plain_window()
+    werr = {"net": [], "base": []}
+    for _ in range(4):
+        U1n = lc.predict(True)[1].clone()
+        U1b = lc.predict(False)[1].clone()
+        lc.plain_window()
+        werr["net"].append(mesh_rel(U1n, learn.state))
+        werr["base"].append(mesh_rel(U1b, learn.state))
+    werr = {k: np.concatenate(v) for k, v in werr.items()}
+    log(f"one window from the plain trajectory, end-state rel-L2 vs plain RK4 (64 meshes x 4 windows): network "
+        f"median {np.median(werr['net']):.2e} max {werr['net'].max():.2e}; base alone median "
+        f"{np.median(werr['base']):.2e} max {werr['base'].max():.2e}")
+
     c0 = counts()
     learn_s, acc = lc.run(a.eval_windows)
     c_learn = counts() - c0
@@ -511,13 +536,6 @@ def learned_main(a):
         f"{a.eval_spinup + 1} spin-up windows: plain {plain_s:.3f}s, learned {learn_s:.3f}s (windows accepted/"
         f"partial/rejected {c_learn[:3]}, mesh predictions accepted {c_learn[3]}/{c_learn[4]}), base only "
         f"{base_s:.3f}s ({c_base[:3]}, {c_base[3]}/{c_base[4]})")
-
-    def mesh_rel(X, R):
-        out = []
-        for i in range(a.eval_meshes):
-            lo, hi = int(plain.offsets[i]), int(plain.offsets[i + 1])
-            out.append(float((X[lo:hi] - R[lo:hi]).double().norm() / R[lo:hi].double().norm()))
-        return np.asarray(out)
 
     vs_plain = {"learned": mesh_rel(U_learn, plain.state), "base": mesh_rel(U_base, plain.state)}
     log(f"vs plain fp32 RK4 (all meshes): learned max {vs_plain['learned'].max():.3e} "
@@ -564,6 +582,11 @@ def learned_main(a):
                       "partial_windows": int(c_base[1]),
                       "mesh_predictions_accepted": f"{int(c_base[3])}/{int(c_base[4])}",
                       "vs_plain": plain_s / base_s},
+        "window_error_vs_rk4": {"network_median": float(np.median(werr["net"])),
+                                "network_max": float(werr["net"].max()),
+                                "base_median": float(np.median(werr["base"])), "base_max": float(werr["base"].max()),
+                                "note": "end state of one predicted window vs the plain RK4 window from the same "
+                                        "state of the plain trajectory, per mesh, 4 windows"},
         "max_state_rel_l2_vs_plain_fp32": {"learned": float(vs_plain["learned"].max()),
                                            "base": float(vs_plain["base"].max())},
         "max_state_rel_l2_vs_fp64": worst,

@@ -9,8 +9,8 @@ state U, the RHS f0 = f(U) and the RHS fm half a window earlier:
 
 with W = k dt of the cell's mesh and y = MPNN(features).  The first terms integrate the
 straight line through fm and f0 (an Adams-Bashforth extrapolation over the window, the
-"base"); the network learns what the base misses, in units of s, the RMS over the mesh's
-cells of W (f0 - fm) per conserved variable.  The network sees, per cell, the normalised
+"base"); the network learns what the base misses, in units of s, per mesh the RMS over
+its cells and conserved variables of W (f0 - fm).  The network sees, per cell, the normalised
 primitives, W f0 and W (f0 - fm) in scaled units, two geometry features (log cell size,
 share of boundary facets) and least-squares gradients of the primitives and of f0 times
 the cell size: the direction in which the state is carried, which a neighbour mean alone
@@ -118,6 +118,7 @@ class LearnedCorrector:
         self._U1 = z(N, C)
         self._s1 = z(self.M, C)
         self._s2 = z(self.M, C)
+        self._st = z(self.M, C)
         self._X = torch.empty(N, self.KX, dtype=_BF, device=self.device)
         self._def = torch.empty(self.M, 3 * C, dtype=torch.float64, device=self.device)
         self._part = torch.empty(self.M * 64 * 3 * C, dtype=torch.float64, device=self.device)
@@ -149,11 +150,15 @@ class LearnedCorrector:
                   "fvm_mesh_rms")
         return out
 
-    def _scales(self, f0, fh, W, s1, s2):
-        """Per mesh and variable: s1 = RMS of W f0, s2 = RMS of W (f0 - fh)."""
+    def _scales(self, f0, fh, W, s1, s2, st):
+        """Per mesh and variable: s1 = RMS of W f0, s2 = RMS of W (f0 - fh); st = the
+        target scale, per mesh the RMS of s2 over the variables (one scale for the whole
+        state: the check and the error are relative L2 norms of the whole state, so the
+        network's accuracy is spent where those norms are large)."""
         nat.check(nat.lib().fvm_mesh_rms2(nat.ptr(f0), nat.ptr(fh), nat.ptr(W), nat.ptr(self.off), self.M, self.C,
                                           1e-30, nat.ptr(s1), nat.ptr(s2), nat.ptr(self._part),
                                           nat.stream_ptr(self.device)), "fvm_mesh_rms2")
+        st.copy_(torch.sqrt((s2 * s2).mean(1, keepdim=True)).expand(-1, self.C))
 
     def history_valid(self) -> bool:
         return self._hist_version == self.s.version
@@ -234,8 +239,8 @@ class LearnedCorrector:
         for i in range(n_windows):
             U0 = self.s.state.clone()
             prim, f0, fh = self._prim0.clone(), self._f0.clone(), self._fh.clone()
-            s1, s2 = torch.empty_like(self._s1), torch.empty_like(self._s2)
-            self._scales(f0, fh, W, s1, s2)
+            s1, s2, st = torch.empty_like(self._s1), torch.empty_like(self._s2), torch.empty_like(self._st)
+            self._scales(f0, fh, W, s1, s2, st)
             for _ in range(self.k // 2):
                 self.s.step()
             rm = self.s.state - self._apply(U0, f0, fh, W, BASE_MID, None, 0, None, self._Um)
@@ -245,16 +250,16 @@ class LearnedCorrector:
             r1 = self.s.state - self._apply(U0, f0, fh, W, BASE_END, None, 0, None, self._U1)
             self._rhs(self.s.state, self._prim0, self._f0)
             self._hist_version = self.s.version
-            raw.append((prim, f0, fh, s1, s2, torch.cat([rm, r1], 1)))
+            raw.append((prim, f0, fh, s1, s2, st, torch.cat([rm, r1], 1)))
             del U0
             if log and (i + 1) % max(1, n_windows // 5) == 0:
                 log(f"generated {i + 1}/{n_windows} windows")
         self.set_stats_from([r[0] for r in raw])
         data = []
         while raw:
-            prim, f0, fh, s1, s2, r = raw.pop(0)
+            prim, f0, fh, s1, s2, st, r = raw.pop(0)
             X = torch.empty(self.N, self.KX, dtype=_BF, device=self.device)
-            sc = s2[self._cm_long].repeat(1, 2)
+            sc = st[self._cm_long].repeat(1, 2)
             data.append((self._features_from(prim, f0, fh, W, s1, s2, X), (r / sc).contiguous()))
         return data
 
@@ -290,14 +295,14 @@ class LearnedCorrector:
             raise RuntimeError("the history does not belong to the solver's state: run plain_window() first")
         U0 = self.s.state
         W = self.window()
-        self._scales(self._f0, self._fh, W, self._s1, self._s2)
+        self._scales(self._f0, self._fh, W, self._s1, self._s2, self._st)
         y = None
         if use_net:
             X = self._features_from(self._prim0, self._f0, self._fh, W, self._s1, self._s2, self._X)
             y = self.net.forward(X, self.nbr)
         (am, bm), (ae, be) = BASE_MID, BASE_END
         nat.check(nat.lib().fvm_mp_window_predict(nat.ptr(U0), nat.ptr(self._f0), nat.ptr(self._fh), am, bm, ae, be,
-                                                  nat.ptr(W), nat.ptr(y), 2 * self.C, nat.ptr(self._s2),
+                                                  nat.ptr(W), nat.ptr(y), 2 * self.C, nat.ptr(self._st),
                                                   nat.ptr(self.cell_mesh), self.N, self.C, nat.ptr(self._Um),
                                                   nat.ptr(self._U1), nat.stream_ptr(self.device)),
                   "fvm_mp_window_predict")

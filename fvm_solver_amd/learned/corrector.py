@@ -4,8 +4,8 @@ No reference counterpart.  Over a window of k solver steps of a (batched) 2-D ch
 problem the corrector predicts the state at the window's midpoint and end from the
 state U, the RHS f0 = f(U) and the RHS fm half a window earlier:
 
-    U(t + W/2) = U + W (3/4 f0 - 1/4 fm) + s * y_mid,
-    U(t + W)   = U + W (2 f0 - fm)       + s * y_end,
+    U(t + W/2) = U + W (a_m f0 + b_m fm) + s * y_mid,
+    U(t + W)   = U + W (a_e f0 + b_e fm) + s * y_end,
 
 with W = k dt of the cell's mesh and y = MPNN(features).  The first terms integrate the
 straight line through fm and f0 (an Adams-Bashforth extrapolation over the window, the
@@ -30,6 +30,13 @@ solver (a masked step computes only the 256-cell tiles holding their cells), the
 keep their predictions, and f is evaluated at the merged midpoint and end to renew the
 history.
 
+After each window (accepted, partly re-run or plain) the solver takes ``settle`` plain RK4
+steps before the next prediction (default 1): the scheme's own dissipation removes the
+cell-scale roughness a prediction leaves, which the check amplifies and which would
+otherwise build up over consecutive accepted windows.  The history fh is then f at the
+previous window's midpoint, W/2 + settle dt before the new start, and the base integrates
+the line through fh and f0 with that spacing (``base_coefficients``).
+
 Training data are the batched solver's own trajectories: per window the network input
 and the scaled residuals of the base at the true midpoint and end states.
 """
@@ -47,9 +54,14 @@ from fvm_solver_amd.learned.mpnn import MPNN
 
 _BF = torch.bfloat16
 
-# base predictor: U + W (a f0 + b fm) at the window's midpoint and end
-BASE_MID = (0.75, -0.25)
-BASE_END = (2.0, -1.0)
+
+def base_coefficients(k: int, settle: int):
+    """(a, b) of the base U + W (a f0 + b fh) at a window's midpoint and end, for a
+    history fh taken W/2 + settle dt before f0 (W = k dt): the integrals over [0, W/2]
+    and [0, W] of the straight line through fh and f0, divided by W."""
+    rho = 0.5 + settle / k                  # history spacing in units of W
+    return (0.5 + 0.125 / rho, -0.125 / rho), (1.0 + 0.5 / rho, -0.5 / rho)
+
 
 
 def lsq_neighbour_weights(dm) -> np.ndarray:
@@ -77,10 +89,13 @@ class LearnedCorrector:
 
     KX = 32
 
-    def __init__(self, solver, k: int = 8, hidden: int = 128, layers: int = 3, seed: int = 0, tol: float = 1e-3):
+    def __init__(self, solver, k: int = 8, hidden: int = 128, layers: int = 3, seed: int = 0, tol: float = 1e-3,
+                 settle: int = 1):
         self.s = solver
         self.k = k
         self.tol = tol
+        self.settle = settle
+        self.base_mid, self.base_end = base_coefficients(k, settle)
         dm = solver.dm
         if dm.dim != 2:
             raise NotImplementedError("the learned corrector takes 2-D meshes")
@@ -163,16 +178,25 @@ class LearnedCorrector:
     def history_valid(self) -> bool:
         return self._hist_version == self.s.version
 
-    def plain_window(self):
-        """k plain RK4 steps, evaluating f at the window's midpoint and end (the history
-        of the next prediction)."""
+    def _settle_and_start(self):
+        """The settle steps after a window, then f at the next window's start."""
+        for _ in range(self.settle):
+            self.s.step()
+        self._rhs(self.s.state, self._prim0, self._f0)
+        self._hist_version = self.s.version
+
+    def plain_window(self, end_out: torch.Tensor | None = None):
+        """k plain RK4 steps (their end state copied to end_out if given) and the settle
+        steps, evaluating f at the window's midpoint (the history) and at the next
+        window's start."""
         for _ in range(self.k // 2):
             self.s.step()
         self._rhs(self.s.state, self._prim1, self._fh)
         for _ in range(self.k // 2):
             self.s.step()
-        self._rhs(self.s.state, self._prim0, self._f0)
-        self._hist_version = self.s.version
+        if end_out is not None:
+            end_out.copy_(self.s.state)
+        self._settle_and_start()
 
     @staticmethod
     def _moments(chunks, floor_rel=1e-3, floor_abs=1e-8):
@@ -243,13 +267,12 @@ class LearnedCorrector:
             self._scales(f0, fh, W, s1, s2, st)
             for _ in range(self.k // 2):
                 self.s.step()
-            rm = self.s.state - self._apply(U0, f0, fh, W, BASE_MID, None, 0, None, self._Um)
+            rm = self.s.state - self._apply(U0, f0, fh, W, self.base_mid, None, 0, None, self._Um)
             self._rhs(self.s.state, self._prim1, self._fh)
             for _ in range(self.k // 2):
                 self.s.step()
-            r1 = self.s.state - self._apply(U0, f0, fh, W, BASE_END, None, 0, None, self._U1)
-            self._rhs(self.s.state, self._prim0, self._f0)
-            self._hist_version = self.s.version
+            r1 = self.s.state - self._apply(U0, f0, fh, W, self.base_end, None, 0, None, self._U1)
+            self._settle_and_start()
             raw.append((prim, f0, fh, s1, s2, st, torch.cat([rm, r1], 1)))
             del U0
             if log and (i + 1) % max(1, n_windows // 5) == 0:
@@ -300,7 +323,7 @@ class LearnedCorrector:
         if use_net:
             X = self._features_from(self._prim0, self._f0, self._fh, W, self._s1, self._s2, self._X)
             y = self.net.forward(X, self.nbr)
-        (am, bm), (ae, be) = BASE_MID, BASE_END
+        (am, bm), (ae, be) = self.base_mid, self.base_end
         nat.check(nat.lib().fvm_mp_window_predict(nat.ptr(U0), nat.ptr(self._f0), nat.ptr(self._fh), am, bm, ae, be,
                                                   nat.ptr(W), nat.ptr(y), 2 * self.C, nat.ptr(self._st),
                                                   nat.ptr(self.cell_mesh), self.N, self.C, nat.ptr(self._Um),
@@ -333,7 +356,7 @@ class LearnedCorrector:
         valid history is run plainly.  Returns True when every mesh's prediction was
         accepted."""
         if use_net is False:
-            for _ in range(self.k):
+            for _ in range(self.k + self.settle):
                 self.s.step()
             return False
         if not self.history_valid():
@@ -347,11 +370,15 @@ class LearnedCorrector:
         self.mesh_windows += self.M
         if ok.all():
             self.s.set_state(U1)
-            # f at the accepted midpoint and end are the next window's history
+            # f at the accepted midpoint is the next window's history; without settle
+            # steps f at the accepted end is the next start's f
             self._fh, self._fm = self._fm, self._fh
-            self._f0, self._f1 = self._f1, self._f0
-            self._prim0, self._prim1 = self._prim1, self._prim0
-            self._hist_version = self.s.version
+            if self.settle == 0:
+                self._f0, self._f1 = self._f1, self._f0
+                self._prim0, self._prim1 = self._prim1, self._prim0
+                self._hist_version = self.s.version
+            else:
+                self._settle_and_start()
             self.n_accept += 1
             return True
         if not ok.any() or not self.s.E.fused:
@@ -374,8 +401,7 @@ class LearnedCorrector:
         for _ in range(self.k // 2):
             self.s.step(tiles)
         self.s.set_state(torch.where(redo, self.s.state, self._U1))
-        self._rhs(self.s.state, self._prim0, self._f0)
-        self._hist_version = self.s.version
+        self._settle_and_start()
 
     def run(self, n_windows: int, use_net: bool | str = True):
         """n_windows windows; returns (seconds, accepted windows)."""

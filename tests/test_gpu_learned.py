@@ -7,7 +7,7 @@ import pytest
 import torch
 
 from fvm_solver_amd.batch import BatchedSolver, random_channel_batch
-from fvm_solver_amd.learned.corrector import BASE_END, LearnedCorrector, lsq_neighbour_weights
+from fvm_solver_amd.learned.corrector import LearnedCorrector, lsq_neighbour_weights
 
 pytestmark = pytest.mark.gpu
 DEV = "cuda:0"
@@ -69,7 +69,7 @@ def test_prediction_error_vs_rk4(trained):
     U1 = U1.clone()
     _, Ub, _ = lc.predict(use_net=False)
     Ub = Ub.clone()
-    a, b = BASE_END
+    a, b = lc.base_end
     assert torch.allclose(Ub, U0 + W[:, None] * (a * lc._f0 + b * lc._fh), rtol=1e-6, atol=1e-6)
     for _ in range(lc.k):
         s.step()
@@ -91,7 +91,7 @@ def test_accepted_windows_pass_the_check(trained):
     acc = sum(lc.advance() for _ in range(4))
     Ul = s.state.clone()
     s.set_state(U0)
-    for _ in range(4 * lc.k):
+    for _ in range(4 * (lc.k + lc.settle)):
         s.step()
     err = float((Ul - s.state).double().norm() / s.state.double().norm())
     print(f"[gpu-test]   accepted {acc}/4 windows, state rel-L2 vs plain after 4 windows {err:.2e}")
@@ -107,7 +107,7 @@ def test_corrupted_network_falls_back_to_plain_rk4():
     lc.net.w["b_out"].fill_(1e3)               # a prediction far off the solution
     for _ in range(3):
         assert lc.advance() is False
-        for _ in range(lc.k):
+        for _ in range(lc.k + lc.settle):
             ref.step()
     torch.cuda.synchronize()
     assert lc.n_plain == 1 and lc.n_reject == 2 and lc.n_accept == 0
@@ -134,7 +134,7 @@ def test_partial_window_merges_predictions_and_plain_steps():
     s, problems, _ = _solver(31, n=4)
     ref = BatchedSolver(problems, device=DEV)
     ref.set_dt(s.dt.cpu().numpy())
-    lc = LearnedCorrector(s, k=4, seed=2)
+    lc = LearnedCorrector(s, k=4, seed=2, settle=0)
     lc.set_stats(s.state)
     lc.plain_window()
     for _ in range(lc.k):
@@ -154,3 +154,4 @@ def test_partial_window_merges_predictions_and_plain_steps():
         lo, hi = int(s.offsets[i]), int(s.offsets[i + 1])
         want = U1p[lo:hi] if ok[i] else ref.state_of(i)
         assert torch.equal(s.state_of(i), want), i
+

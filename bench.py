@@ -66,6 +66,9 @@ def _parse():
     g.add_argument("--n-max", type=int, default=200_000)
     g.add_argument("--oracle-meshes", type=int, default=2, help="evaluation meshes re-run by the FP64 CPU oracle")
     g.add_argument("--tol", type=float, default=1e-3)
+    g.add_argument("--corrector-timing", action="store_true",
+                   help="time the pieces of one corrector window on the evaluation batch (untrained network: "
+                        "the kernels' cost does not depend on the weights) against a plain window, then exit")
     return ap.parse_args()
 
 
@@ -615,9 +618,65 @@ def learned_main(a):
     print(json.dumps(out), flush=True)
 
 
+def corrector_timing_main(a):
+    """Stream-event medians of the pieces of one corrector window (scales, network input,
+    network forward, predicted states, the two check RHS) and of a plain RK4 window (HIP
+    graph), on the evaluation batch of --learned."""
+    from fvm_solver_amd import _native as nat
+    from fvm_solver_amd.batch import BatchedSolver, random_channel_batch
+    from fvm_solver_amd.learned.corrector import LearnedCorrector
+    dev = torch.device("cuda:0")
+    ep, _ = random_channel_batch(a.eval_meshes, seed=a.seed + 2000, n_min=a.n_min, n_max=a.n_max, device=dev)
+    s = BatchedSolver(ep, device=dev)
+    s.set_dt(s.cfl_dt(0.5))
+    lc = LearnedCorrector(s, k=a.window, tol=a.tol, settle=a.settle)
+    lc.set_stats(s.state)
+    lc.plain_window()
+    torch.cuda.synchronize()
+    W = lc.window()
+    names = ["scales", "features", "network", "predict", "check_rhs"]
+    res = {n: [] for n in names}
+    (am, bm), (ae, be) = lc.base_mid, lc.base_end
+    for it in range(8):
+        e = [torch.cuda.Event(enable_timing=True) for _ in range(6)]
+        e[0].record()
+        lc._scales(lc._f0, lc._fh, W, lc._s1, lc._s2, lc._st)
+        e[1].record()
+        X = lc._features_from(lc._prim0, lc._f0, lc._fh, W, lc._s1, lc._s2, lc._X)
+        e[2].record()
+        y = lc.net.forward(X, lc.nbr)
+        e[3].record()
+        nat.check(nat.lib().fvm_mp_window_predict(nat.ptr(s.state), nat.ptr(lc._f0), nat.ptr(lc._fh), am, bm, ae, be,
+                                                  nat.ptr(W), nat.ptr(y), 2 * lc.C, nat.ptr(lc._st),
+                                                  nat.ptr(lc.cell_mesh), lc.N, lc.C, nat.ptr(lc._Um),
+                                                  nat.ptr(lc._U1), nat.stream_ptr(dev)), "fvm_mp_window_predict")
+        e[4].record()
+        lc._rhs(lc._Um, lc._prim1, lc._fm)
+        lc._rhs(lc._U1, lc._prim1, lc._f1)
+        e[5].record()
+        torch.cuda.synchronize()
+        if it >= 2:
+            for i, n in enumerate(names):
+                res[n].append(e[i].elapsed_time(e[i + 1]))
+    g = s.capture_graph(a.window)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(5):
+        g.replay()
+    e1.record()
+    torch.cuda.synchronize()
+    out = {"cells": int(s.dm.N), "meshes": a.eval_meshes, "ms": {n: float(np.median(v)) for n, v in res.items()},
+           "plain_window_ms": e0.elapsed_time(e1) / 5}
+    out["corrector_window_ms"] = float(sum(out["ms"].values()))
+    print(json.dumps(out), flush=True)
+
+
 if __name__ == "__main__":
     _a = _parse()
-    if _a.learned:
+    if _a.corrector_timing:
+        corrector_timing_main(_a)
+    elif _a.learned:
         if _a.gpus != 1:
             print("bench.py runs on one GPU only", file=sys.stderr)
             sys.exit(2)

@@ -125,8 +125,9 @@ def grad_kernel_bytes(N, Fb, D):
 def main():
     a = _parse()
     if a.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for {a.gpus}); the multi-GPU design is described in "
-              "DESIGN.md and no multi-GPU figure is measured", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for {a.gpus}): batches shard by meshes and a large mesh "
+              "by slabs of cells with a halo exchange per stage, but no multi-GPU figure is measured",
+              file=sys.stderr)
         sys.exit(2)
     from fvm_solver_amd import _native as nat
     from fvm_solver_amd.config_fvm import ConfigChannel, ConfigChannel3D

@@ -1,4 +1,4 @@
-"""Learned window predictor with the plain Runge-Kutta solver as fallback and judge.
+"""Learned window predictor with the plain Runge-Kutta solver as fallback and checker.
 
 No reference counterpart.  Over a window of k solver steps of a (batched) 2-D channel
 problem the corrector predicts the state at the window's midpoint and end from the

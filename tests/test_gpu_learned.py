@@ -155,3 +155,55 @@ def test_partial_window_merges_predictions_and_plain_steps():
         want = U1p[lo:hi] if ok[i] else ref.state_of(i)
         assert torch.equal(s.state_of(i), want), i
 
+
+
+def test_window_kernels_against_torch():
+    """The window's scale, feature and predict kernels against torch restatements of
+    their formulas (fp32; the features to bf16 storage precision)."""
+    from fvm_solver_amd import _native as nat
+    s, problems, _ = _solver(17, n=3)
+    lc = LearnedCorrector(s, k=8, seed=4)
+    lc.set_stats(s.state)
+    lc.plain_window()
+    W = lc.window()
+    M, C, N = lc.M, lc.C, lc.N
+    lc._scales(lc._f0, lc._fh, W, lc._s1, lc._s2, lc._st)
+    cm = lc._cm_long
+
+    def mesh_rms(x):
+        ss = torch.zeros(M, C, dtype=torch.float64, device=DEV).index_add_(0, cm, x.double() ** 2)
+        n = torch.bincount(cm, minlength=M).double()[:, None]
+        return torch.sqrt(ss / n)
+
+    f0, fh, prim = lc._f0, lc._fh, lc._prim0
+    s1 = mesh_rms(W[:, None] * f0)
+    s2 = mesh_rms(W[:, None] * (f0 - fh))
+    assert torch.allclose(lc._s1.double(), s1, rtol=1e-5)
+    assert torch.allclose(lc._s2.double(), s2, rtol=1e-5)
+    assert torch.allclose(lc._st.double(), torch.sqrt((s2 ** 2).mean(1, keepdim=True)).expand(-1, C), rtol=1e-5)
+
+    X = lc._features_from(prim, f0, fh, W, lc._s1, lc._s2, lc._X).float()
+    nbr = lc.nbr.long()
+    h = lc._h[:, None]
+    grad = lambda v: torch.stack([sum(torch.where((nbr[k] < N)[:, None], lc.wg[k, i][:, None] *
+                                                   (v[nbr[k].clamp(max=N - 1)] - v), torch.zeros_like(v))
+                                      for k in range(nbr.shape[0])) for i in range(2)])
+    gp, gf = grad(prim), grad(f0)
+    s1c, s2c = lc._s1[cm], lc._s2[cm]
+    ref = torch.cat([(prim - lc._mu_p) / lc._sd_p, W[:, None] * f0 / s1c, W[:, None] * (f0 - fh) / s2c, lc._geo,
+                     h * gp[0] / lc._sd_p, h * gp[1] / lc._sd_p, h * W[:, None] * gf[0] / s1c,
+                     h * W[:, None] * gf[1] / s1c, torch.zeros(N, 2, device=DEV)], 1)
+    err = (X - ref).abs() / (ref.abs() + 1e-2)
+    print(f"[gpu-test]   window features: max rel err {float(err.max()):.2e}")
+    assert float(err.max()) < 2e-2
+
+    y = torch.randn(N, 2 * C, device=DEV, generator=torch.Generator(device=DEV).manual_seed(3))
+    Um, U1 = torch.empty_like(f0), torch.empty_like(f0)
+    (am, bm), (ae, be) = lc.base_mid, lc.base_end
+    U = s.state
+    nat.check(nat.lib().fvm_mp_window_predict(nat.ptr(U), nat.ptr(f0), nat.ptr(fh), am, bm, ae, be, nat.ptr(W),
+                                              nat.ptr(y), 2 * C, nat.ptr(lc._st), nat.ptr(lc.cell_mesh), N, C,
+                                              nat.ptr(Um), nat.ptr(U1), nat.stream_ptr(DEV)), "predict")
+    st = lc._st[cm]
+    assert torch.allclose(Um, U + W[:, None] * (am * f0 + bm * fh) + st * y[:, :C], rtol=1e-5, atol=1e-5)
+    assert torch.allclose(U1, U + W[:, None] * (ae * f0 + be * fh) + st * y[:, C:], rtol=1e-5, atol=1e-5)

@@ -66,7 +66,7 @@ __device__ __forceinline__ float lim_phi(float delta, float dU, float eps) {
         const float a2 = a * a;
         const float a_eps = fmaf(a2, a2, eps);
         const float S = b2 * fmaf(b2, b2 - a, a2);
-        const float d3 = (delta * delta) * delta;
+        const float d3 = a2 * delta;        // (delta delta) delta: a2 = |delta|^2 is delta^2 exactly
         const float phi = quot(fmaf(a, S, a_eps), fmaf(b, d3 + S, a_eps));
         return a < b2 ? phi : 1.0f;
     }
@@ -319,6 +319,8 @@ __global__ void __launch_bounds__(kThreads) k_bc_mean(FvmMeshDev m, FvmPhys p, F
 // the limiter factor phi = min over the cell's facets (facet_process.py:387-418,
 // limiter.py).  P: the cell's primitives; Un: the values behind its facets (neighbour
 // primitives, or boundary values).
+// LSQ gradient of a cell (G row times [U; U_bc] in the reference's ascending column order,
+// accumulated with fused multiply-adds) and the limiter factor per component.
 template <int D, int LP>
 __device__ __forceinline__ void cell_gradient(const float (&P)[D + 2], const float (&Un)[D + 1][D + 2], int info,
                                               const float (&gv)[D + 2][D], const float (&disp)[D + 1][D], float eps,
@@ -338,7 +340,7 @@ __device__ __forceinline__ void cell_gradient(const float (&P)[D + 2], const flo
 #pragma unroll
         for (int i = 0; i < D; ++i) {
 #pragma unroll
-            for (int j = 0; j < C; ++j) g[i][j] = (jj == 0) ? gv[jj][i] * U[j] : g[i][j] + gv[jj][i] * U[j];
+            for (int j = 0; j < C; ++j) g[i][j] = (jj == 0) ? gv[jj][i] * U[j] : fmaf(gv[jj][i], U[j], g[i][j]);
         }
     }
 #pragma unroll
@@ -355,7 +357,7 @@ __device__ __forceinline__ void cell_gradient(const float (&P)[D + 2], const flo
         for (int k = 0; k < NL; ++k) {
             float dU = g[0][j] * disp[k][0];
 #pragma unroll
-            for (int i = 1; i < D; ++i) dU = dU + g[i][j] * disp[k][i];
+            for (int i = 1; i < D; ++i) dU = fmaf(g[i][j], disp[k][i], dU);
             const float lk = lim_phi<LP>(dU > 0.0f ? upper : lower, dU, eps);
             ph = (k == 0) ? lk : fminf(ph, lk);
         }
@@ -567,8 +569,8 @@ __device__ __forceinline__ void side_values(const float (&P)[D + 2], const float
         const float ph = g[D * C + j];
         float dU = g[j] * disp[0];
 #pragma unroll
-        for (int i = 1; i < D; ++i) dU = dU + g[i * C + j] * disp[i];
-        U[j] = P[j] + dU * ph;
+        for (int i = 1; i < D; ++i) dU = fmaf(g[i * C + j], disp[i], dU);
+        U[j] = fmaf(dU, ph, P[j]);
     }
 #pragma unroll
     for (int i = 0; i < D; ++i) {
@@ -942,8 +944,8 @@ __device__ __forceinline__ void fused_grad_cell(const FvmMeshDev &m, const FvmPh
         for (int j = 0; j < C; ++j) {
             float dU = g[0][j] * disp[k][0];
 #pragma unroll
-            for (int ii = 1; ii < D; ++ii) dU = dU + g[ii][j] * disp[k][ii];
-            U[j] = P[j] + dU * phi[j];
+            for (int ii = 1; ii < D; ++ii) dU = fmaf(g[ii][j], disp[k][ii], dU);
+            U[j] = fmaf(dU, phi[j], P[j]);
         }
         store_vec<C>(FV + (k * ngl + l) * C, U);
     }

@@ -2,7 +2,7 @@
 against 1000 steps of the fp32 CPU oracle from the same state, with the error of both
 against the fp64 oracle reported.
 
-Tolerance (north-star contract): relative L2 <= 1e-4 per conserved variable after 1000
+Tolerance: relative L2 <= 1e-4 per conserved variable after 1000
 steps.  The fp64 run is reported, and bounds the product's error: the product may not
 be further from fp64 than 10x the fp32 oracle's own distance to it (or 1e-5).
 """

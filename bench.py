@@ -48,6 +48,9 @@ def _parse():
     ap.add_argument("--traffic-file", default=os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
                                                            "rk4_channel2d_1M_traffic.json"),
                     help="measured HBM bytes per launch (rocprofv3 FETCH_SIZE/WRITE_SIZE, calibrated)")
+    ap.add_argument("--spmv", action="store_true",
+                    help="also time the drop-in ELL SpMV/SpMM (ell_spmm, ell_spmv) on the reference's operators of "
+                         "the mesh (G_mats interleaved, flux_mat, A_face_grad's pattern) and report their HBM GB/s")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run one 256 MiB device-to-device copy (calibrates HBM byte counters)")
     g = ap.add_argument_group("learned corrector (--learned)")
@@ -113,6 +116,80 @@ def fused_kernel_bytes(N, F, Fb, D, stage, tiles, ab3=False):
     per_facet = (3 * D + 2) + 1
     per_bc = 1 + C + D + 2 * C + (2 * D + 3) + 2 + C + 2 * (D + 1) * C
     return 4 * (N * (per_cell + dyn) + F * per_facet + Fb * per_bc + tiles * 12)
+
+
+def spmv_operators(mesh, C):
+    """The reference's sparse operators of a mesh, as torch CSR on the host: G_mats with
+    rows interleaved (row d cell + dir; sparse.py:281-325, fvm_mesh.py:5-42), flux_mat
+    (value (2 slot - 1) / vol = -sign / vol on each cell's facets; facet_process.py:200-226)
+    and A_face_grad's pattern (rows f C + c; interior facets -1/(n.d), +1/(n.d) on the two
+    cells' component c, boundary facets one entry; facet_process.py:114-198), with the
+    right-hand sides they multiply: [N + F_b, C], [F, C], [C N]."""
+    g = mesh.geo64
+    N, F, Fb, D = mesh.n_cells, mesh.n_facets, mesh.n_bc_facet, mesh.dim
+    _, _, G_mats, _ = mesh.cell_grad_stuff
+    r, c, v = [], [], []
+    for i, G in enumerate(G_mats):
+        G = G.coalesce()
+        r.append(G.indices()[0] * D + i)
+        c.append(G.indices()[1])
+        v.append(G.values())
+    Gi = torch.sparse_coo_tensor(torch.stack([torch.cat(r), torch.cat(c)]), torch.cat(v), (D * N, N + Fb))
+    c2f, signs = torch.from_numpy(np.asarray(g["c2f"], np.int64)), torch.from_numpy(np.asarray(g["signs"], np.int64))
+    vol = torch.from_numpy(np.asarray(g["volumes"])).float()
+    rows = torch.arange(N)[:, None].expand_as(c2f)
+    fm = torch.sparse_coo_tensor(torch.stack([rows.reshape(-1), c2f.reshape(-1)]),
+                                 (-signs.float() / vol[:, None]).reshape(-1), (N, F))
+    left, right = np.asarray(g["left"], np.int64), np.asarray(g["right"], np.int64)
+    bcm = mesh.bc_facet_mask.numpy()
+    fi = np.flatnonzero(~bcm)
+    fb = np.flatnonzero(bcm)
+    cen, nh = g["centroids"], g["normals_hat"]
+    inv_d = 1.0 / np.einsum("nd,nd->n", nh[fi], cen[right[fi]] - cen[left[fi]])
+    comp = np.arange(C)
+    ri = (fi[:, None] * C + comp).reshape(-1)
+    rl = (left[fi][:, None] * C + comp).reshape(-1)
+    rr = (right[fi][:, None] * C + comp).reshape(-1)
+    vi = np.repeat(inv_d, C)
+    rb = (fb[:, None] * C + comp).reshape(-1)
+    cb = (left[fb][:, None] * C + comp).reshape(-1)
+    A = torch.sparse_coo_tensor(torch.from_numpy(np.stack([np.concatenate([ri, ri, rb]), np.concatenate([rl, rr, cb])])),
+                                torch.from_numpy(np.concatenate([-vi, vi, np.ones(len(rb))])).float(), (C * F, C * N))
+    return {"G_mats": (Gi.coalesce().to_sparse_csr(), (N + Fb, C)), "flux_mat": (fm.coalesce().to_sparse_csr(), (F, C)),
+            "A_face_grad": (A.coalesce().to_sparse_csr(), (C * N,))}
+
+
+def spmv_bench(mesh, C, dev, reps=50):
+    """Per operator: ELL packing (csr_to_ell), then reps products timed with stream events;
+    GB/s from the algorithmic bytes nnz (4 B value + 4 B column) + the right-hand side's
+    used columns + the output (the SpMV figure of the reference's operators)."""
+    from fvm_solver_amd.utils.ell_kernel import csr_to_ell, ell_spmm, ell_spmv
+    out = {}
+    g = torch.Generator().manual_seed(7)
+    for name, (A, xshape) in spmv_operators(mesh, C).items():
+        vals, cols = csr_to_ell(A.to(dev))
+        x = torch.randn(*xshape, generator=g).to(dev)
+        fn = ell_spmv if len(xshape) == 1 else ell_spmm
+        y = fn(vals, cols, x)
+        ref = (A.to_dense() @ x.cpu()) if A.shape[0] * A.shape[1] < 2 ** 26 else None
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            y = fn(vals, cols, x)
+        e1.record()
+        torch.cuda.synchronize()
+        us = e0.elapsed_time(e1) * 1e3 / reps
+        B = 1 if len(xshape) == 1 else xshape[1]
+        M = A.shape[0]
+        nnz = int(A.values().numel())
+        ncols = int(torch.unique(A.col_indices()).numel())
+        nbytes = nnz * 8 + ncols * B * 4 + M * B * 4
+        out[name] = {"rows": M, "K": int(vals.shape[1]), "nnz": nnz, "B": B, "us": us,
+                     "algorithmic_bytes": nbytes, "gbs": nbytes / (us * 1e-6) / 1e9}
+        if ref is not None:
+            out[name]["rel_err"] = float((y.cpu() - ref).norm() / ref.norm())
+    return out
 
 
 def grad_kernel_bytes(N, Fb, D):
@@ -310,6 +387,11 @@ def main():
         if k and tr.get("cells") == N:
             out["roofline"]["traffic"] = k["read_bytes"] + k["write_bytes"]
             out["roofline"]["traffic_source"] = os.path.basename(a.traffic_file)
+
+    if a.spmv:
+        sp = spmv_bench(mesh, C, dev)
+        out["config"]["spmv"] = sp
+        log("ELL products: " + ", ".join(f"{k} {v['us']:.1f} us {v['gbs']:.0f} GB/s" for k, v in sp.items()))
 
     if not a.no_cpu_baseline:
         sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))

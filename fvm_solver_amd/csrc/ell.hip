@@ -46,6 +46,79 @@ k_ell_spmm(const float *__restrict__ vals, int64_t vsm, int64_t vsk, const CT *_
         if (j < nb) y[r * ysm + static_cast<int64_t>(b0 + j) * ysb] = bias ? acc[j] + bb : acc[j];
 }
 
+// Fast path for the operators of the solver (K <= 8 slots, slot-major values and 32-bit
+// columns, contiguous right-hand side and output): the K values and columns of the row
+// are loaded first, then the K gathered rows of x, then the products in slot order, so
+// a row costs two memory round trips instead of one per slot; with B = 4 the x rows and
+// the output row move as 16-byte vectors.  Same products and summation order as
+// k_ell_spmm.
+template <int B, int KT>
+__global__ void __launch_bounds__(256)
+k_ell_spmm_fast(const float *__restrict__ vals, const int32_t *__restrict__ cols, int64_t vsk, int64_t M,
+                const float *__restrict__ x, const float *__restrict__ bias, float *__restrict__ y) {
+    const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (r >= M) return;
+    float v[KT];
+    int32_t c[KT];
+#pragma unroll
+    for (int k = 0; k < KT; ++k) {
+        v[k] = vals[r + k * vsk];
+        c[k] = cols[r + k * vsk];
+    }
+    float acc[B];
+#pragma unroll
+    for (int j = 0; j < B; ++j) acc[j] = 0.0f;
+    if constexpr (B == 4) {
+        float4 xr[KT];
+#pragma unroll
+        for (int k = 0; k < KT; ++k) xr[k] = reinterpret_cast<const float4 *>(x)[c[k]];
+#pragma unroll
+        for (int k = 0; k < KT; ++k) {
+            acc[0] += v[k] * xr[k].x;
+            acc[1] += v[k] * xr[k].y;
+            acc[2] += v[k] * xr[k].z;
+            acc[3] += v[k] * xr[k].w;
+        }
+    } else {
+        float xr[KT][B];
+#pragma unroll
+        for (int k = 0; k < KT; ++k)
+#pragma unroll
+            for (int j = 0; j < B; ++j) xr[k][j] = x[static_cast<int64_t>(c[k]) * B + j];
+#pragma unroll
+        for (int k = 0; k < KT; ++k)
+#pragma unroll
+            for (int j = 0; j < B; ++j) acc[j] += v[k] * xr[k][j];
+    }
+    if (bias) {
+        const float bb = bias[r];
+#pragma unroll
+        for (int j = 0; j < B; ++j) acc[j] = acc[j] + bb;
+    }
+    if constexpr (B == 4) {
+        reinterpret_cast<float4 *>(y)[r] = make_float4(acc[0], acc[1], acc[2], acc[3]);
+    } else {
+#pragma unroll
+        for (int j = 0; j < B; ++j) y[r * B + j] = acc[j];
+    }
+}
+
+template <int B>
+static bool launch_spmm_fast(const float *vals, const int32_t *cols, int64_t vsk, int64_t M, int K, const float *x,
+                             const float *bias, float *y, hipStream_t s) {
+    const dim3 grid(blocks_for(M, 256));
+    switch (K) {
+        case 1: hipLaunchKernelGGL((k_ell_spmm_fast<B, 1>), grid, dim3(256), 0, s, vals, cols, vsk, M, x, bias, y); return true;
+        case 2: hipLaunchKernelGGL((k_ell_spmm_fast<B, 2>), grid, dim3(256), 0, s, vals, cols, vsk, M, x, bias, y); return true;
+        case 3: hipLaunchKernelGGL((k_ell_spmm_fast<B, 3>), grid, dim3(256), 0, s, vals, cols, vsk, M, x, bias, y); return true;
+        case 4: hipLaunchKernelGGL((k_ell_spmm_fast<B, 4>), grid, dim3(256), 0, s, vals, cols, vsk, M, x, bias, y); return true;
+        case 5: hipLaunchKernelGGL((k_ell_spmm_fast<B, 5>), grid, dim3(256), 0, s, vals, cols, vsk, M, x, bias, y); return true;
+        case 6: hipLaunchKernelGGL((k_ell_spmm_fast<B, 6>), grid, dim3(256), 0, s, vals, cols, vsk, M, x, bias, y); return true;
+        case 8: hipLaunchKernelGGL((k_ell_spmm_fast<B, 8>), grid, dim3(256), 0, s, vals, cols, vsk, M, x, bias, y); return true;
+        default: return false;
+    }
+}
+
 template <int BT, typename CT>
 static void launch_spmm(const float *vals, int64_t vsm, int64_t vsk, const CT *cols, int64_t csm, int64_t csk,
                         int64_t M, int K, const float *x, int64_t xsn, int64_t xsb, int B, const float *bias,
@@ -118,6 +191,16 @@ int fvm_ell_spmm_f32(const float *vals, int64_t vsm, int64_t vsk, const void *co
     }
     if (M == 0) return FVM_OK;
     hipStream_t s = as_stream(stream);
+    // fast path: slot-major int32 ELL (value k of row r at r + k vsk, the same strides for
+    // the columns), contiguous x [n][B] and y [M][B], B in {1, 4} (16-byte aligned for 4)
+    const bool aligned4 = ((reinterpret_cast<uintptr_t>(x) | reinterpret_cast<uintptr_t>(y)) & 15u) == 0;
+    if (!cols_is64 && K >= 1 && vsm == 1 && csm == 1 && csk == vsk && xsb == 1 && ysb == 1 && xsn == B &&
+        ysm == B && (B == 1 || (B == 4 && aligned4))) {
+        const int32_t *c32 = static_cast<const int32_t *>(cols);
+        const bool done = B == 4 ? launch_spmm_fast<4>(vals, c32, vsk, M, K, x, bias, y, s)
+                                 : launch_spmm_fast<1>(vals, c32, vsk, M, K, x, bias, y, s);
+        if (done) return check_launch("fvm_ell_spmm_f32");
+    }
     if (cols_is64)
         dispatch_spmm(vals, vsm, vsk, static_cast<const int64_t *>(cols), csm, csk, M, K, x, xsn, xsb, B, bias, y,
                       ysm, ysb, s);

@@ -48,9 +48,9 @@ def _parse():
     ap.add_argument("--traffic-file", default=os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
                                                            "rk4_channel2d_1M_traffic.json"),
                     help="measured HBM bytes per launch (rocprofv3 FETCH_SIZE/WRITE_SIZE, calibrated)")
-    ap.add_argument("--spmv", action="store_true",
-                    help="also time the drop-in ELL SpMV/SpMM (ell_spmm, ell_spmv) on the reference's operators of "
-                         "the mesh (G_mats interleaved, flux_mat, A_face_grad's pattern) and report their HBM GB/s")
+    ap.add_argument("--no-spmv", action="store_true",
+                    help="skip timing the drop-in ELL SpMV/SpMM (ell_spmm, ell_spmv) on the reference's operators of "
+                         "the mesh (G_mats interleaved, flux_mat, A_face_grad's pattern), reported as config.spmv")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run one 256 MiB device-to-device copy (calibrates HBM byte counters)")
     g = ap.add_argument_group("learned corrector (--learned)")
@@ -388,7 +388,7 @@ def main():
             out["roofline"]["traffic"] = k["read_bytes"] + k["write_bytes"]
             out["roofline"]["traffic_source"] = os.path.basename(a.traffic_file)
 
-    if a.spmv:
+    if not a.no_spmv:
         sp = spmv_bench(mesh, C, dev)
         out["config"]["spmv"] = sp
         log("ELL products: " + ", ".join(f"{k} {v['us']:.1f} us {v['gbs']:.0f} GB/s" for k, v in sp.items()))

@@ -388,7 +388,7 @@ def main():
             out["roofline"]["traffic"] = k["read_bytes"] + k["write_bytes"]
             out["roofline"]["traffic_source"] = os.path.basename(a.traffic_file)
 
-    if not a.no_spmv:
+    if not a.no_spmv and D == 2:          # (the 3-D operators take minutes to assemble on the host)
         sp = spmv_bench(mesh, C, dev)
         out["config"]["spmv"] = sp
         log("ELL products: " + ", ".join(f"{k} {v['us']:.1f} us {v['gbs']:.0f} GB/s" for k, v in sp.items()))

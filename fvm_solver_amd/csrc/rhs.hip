@@ -905,7 +905,7 @@ __device__ __forceinline__ void fused_grad_cell(const FvmMeshDev &m, const FvmPh
                                                 bool own,
                                                 const float (&gv)[D + 2][D], const float (&disp)[D + 1][D],
                                                 float eps, int info, const ushort4 nrow, int has_prev,
-                                                const float *gmean, int g0) {
+                                                const float *gmean, int g0, const ushort4 cfac, float *FL) {
     using Dm = Dims<D>;
     constexpr int C = Dm::C, NL = Dm::NL;
     const int64_t N = m.n_cells;
@@ -922,7 +922,12 @@ __device__ __forceinline__ void fused_grad_cell(const FvmMeshDev &m, const FvmPh
             bnd = true;
             const int64_t b = static_cast<int64_t>(m.cell_nbr[k * N + gc]) - N;
             bc_value<D>(m, p, b, P, w.gp_prev + gc * Dm::REC, has_prev, gmean, g0, Un[k]);
-            if (own) store_vec<C>(w.ubc + b * C, Un[k]);
+            if (own) {
+                store_vec<C>(w.ubc + b * C, Un[k]);
+                // the boundary facet's outer value, kept in its flux slot for the flux phase
+                const unsigned q = k == 0 ? cfac.x : k == 1 ? cfac.y : k == 2 ? cfac.z : cfac.w;
+                store_vec<C>(FL + q * C, Un[k]);
+            }
         }
     }
     float g[D][C], phi[C];
@@ -1008,7 +1013,9 @@ __device__ __forceinline__ void fused_facet(const FvmMeshDev &m, const FvmPhys &
         load_vec<C>(FV + (ki * ngl + li) * C, Uo);
         load_vec<C>(Pl + li * C, Pi);
         const int64_t b = static_cast<int64_t>(m.fac_cell[outer0 ? f : NF + f]) - N;
-        bc_value<D>(m, p, b, Pi, w.gp_prev + (c0 + li) * Dm::REC, has_prev, gmean, g0, Ux);
+        // the outer value: computed by the interior cell in the gradient phase (the same
+        // bc_value on the same inputs) and left in this facet's flux slot
+        load_vec<C>(FL + q * C, Ux);
 #pragma unroll
         for (int j = 0; j < C; ++j) {
             U0s[j] = outer0 ? Ux[j] : Uo[j];
@@ -1093,7 +1100,7 @@ k_fused(FvmMeshDev m, FvmPhys p, FvmWork w, const float *__restrict__ prim, floa
     // thread's tile cell and H1 cell to registers ----
     float ogv[NL + 1][D], odisp[NL][D], oeps = 0.0f;
     int oinfo = 0;
-    ushort4 onr = make_ushort4(0, 0, 0, 0);
+    ushort4 onr = make_ushort4(0, 0, 0, 0), cf = make_ushort4(0, 0, 0, 0);
     if (has_own) {
         const int64_t c = c0 + i;
         float u[C];
@@ -1110,6 +1117,7 @@ k_fused(FvmMeshDev m, FvmPhys p, FvmWork w, const float *__restrict__ prim, floa
         oeps = m.cell_eps[c];
         oinfo = m.cell_info[c];
         onr = *reinterpret_cast<const ushort4 *>(m.fz_nbr + (static_cast<int64_t>(nbr_off) + i) * 4);
+        cf = *reinterpret_cast<const ushort4 *>(m.fz_cfac + c * 4);
     }
     float hgv[NL + 1][D], hdisp[NL][D], heps = 0.0f;
     int hinfo = 0;
@@ -1153,10 +1161,10 @@ k_fused(FvmMeshDev m, FvmPhys p, FvmWork w, const float *__restrict__ prim, floa
     if (i < nft) fused_facet_load<D>(m, i, fo0, nfo, foff, fA, dA, geoA);
     if (has_own)
         fused_grad_cell<D, LP>(m, p, w, Pl, FV, GL, m.fz_ngl, i, c0 + i, true, ogv, odisp, oeps, oinfo, onr,
-                               has_prev, gmean, g0);
+                               has_prev, gmean, g0, cf, FL);
     if (has_h1)
         fused_grad_cell<D, LP>(m, p, w, Pl, FV, GL, m.fz_ngl, kThreads + i, hgc, false, hgv, hdisp, heps, hinfo,
-                               hnr, has_prev, gmean, g0);
+                               hnr, has_prev, gmean, g0, cf, FL);
     __syncthreads();
 
     // ---- phase 2: facet fluxes (the second facet's and the epilogue's inputs in flight) ----
@@ -1170,11 +1178,7 @@ k_fused(FvmMeshDev m, FvmPhys p, FvmWork w, const float *__restrict__ prim, floa
     float U0v[C], ap[C], a2p[C];
     stage_inputs<D, EPI>(st, c, has_own, U0v, ap, a2p);
     float inv_vol = 0.0f;
-    ushort4 cf = make_ushort4(0, 0, 0, 0);
-    if (has_own) {
-        inv_vol = m.cell_inv_vol[c];
-        cf = *reinterpret_cast<const ushort4 *>(m.fz_cfac + c * 4);
-    }
+    if (has_own) inv_vol = m.cell_inv_vol[c];
     if (i < nft)
         fused_facet<D>(m, p, w, Pl, FV, GL, m.fz_ngl, FL, i, fA, dA, geoA, c0, halo_off, has_prev, gmean, g0);
     if (twoB)

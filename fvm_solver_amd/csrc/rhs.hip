@@ -969,7 +969,8 @@ __device__ __forceinline__ void fused_facet(const FvmMeshDev &m, const FvmPhys &
                                             const float *Pl, const float *FV, const float *GL, int ngl, float *FL,
                                             int q,
                                             int64_t f, unsigned d, const float (&geo)[3 * D + 2], int64_t c0,
-                                            int halo_off, int has_prev, const float *gmean, int g0) {
+                                            int halo_off, int has_prev, const float *gmean, int g0,
+                                            const unsigned short *meshT) {
     using Dm = Dims<D>;
     constexpr int C = Dm::C, NL = Dm::NL;
     const int64_t N = m.n_cells, NF = m.n_facets;
@@ -977,9 +978,11 @@ __device__ __forceinline__ void fused_facet(const FvmMeshDev &m, const FvmPhys &
     const bool left_first = (d >> 28) & 1u, bnd = (d >> 29) & 1u, outer0 = (d >> 30) & 1u;
     float mu0 = p.mu0, mub0 = p.mub0, kappa = p.kappa;
     if (m.mesh_phys) {
-        const unsigned lo = outer0 ? l1 : l0;
-        const int64_t gown = lo < kThreads ? c0 + lo : static_cast<int64_t>(m.fz_halo[halo_off + lo - kThreads]);
-        const int mesh = m.cell_mesh[gown];
+        // one side of every facet of the tile's list is a tile cell (the interior cell of
+        // a boundary facet; the lowest cell of an owned facet; the higher one of a foreign
+        // facet), whose mesh id the staging phase left in LDS
+        const unsigned lt = bnd ? (outer0 ? l1 : l0) : (l0 < kThreads ? l0 : l1);
+        const int mesh = meshT[lt];
         mu0 = m.mesh_phys[4 * mesh];
         mub0 = m.mesh_phys[4 * mesh + 1];
         kappa = m.mesh_phys[4 * mesh + 2];
@@ -1086,6 +1089,8 @@ k_fused(FvmMeshDev m, FvmPhys p, FvmWork w, const float *__restrict__ prim, floa
     float *FV = Pl + static_cast<int64_t>(m.fz_ncl) * C;
     float *GL = FV + static_cast<int64_t>(m.fz_ngl) * FVN;
     float *FL = lds + fz_fl_offset<D>(m.fz_ncl, m.fz_ngl);
+    // batches only: mesh id of each tile cell (after FL)
+    unsigned short *meshT = reinterpret_cast<unsigned short *>(FL + static_cast<int64_t>(m.fz_nfl) * C);
     float *red = FV;                                   // reductions reuse the face-value rows
     double *sred = reinterpret_cast<double *>(FV);     // (before and after the two phases using them)
     const int i = threadIdx.x;
@@ -1118,6 +1123,7 @@ k_fused(FvmMeshDev m, FvmPhys p, FvmWork w, const float *__restrict__ prim, floa
         oinfo = m.cell_info[c];
         onr = *reinterpret_cast<const ushort4 *>(m.fz_nbr + (static_cast<int64_t>(nbr_off) + i) * 4);
         cf = *reinterpret_cast<const ushort4 *>(m.fz_cfac + c * 4);
+        if (m.mesh_phys) meshT[i] = static_cast<unsigned short>(m.cell_mesh[c]);
     }
     float hgv[NL + 1][D], hdisp[NL][D], heps = 0.0f;
     int hinfo = 0;
@@ -1180,10 +1186,10 @@ k_fused(FvmMeshDev m, FvmPhys p, FvmWork w, const float *__restrict__ prim, floa
     float inv_vol = 0.0f;
     if (has_own) inv_vol = m.cell_inv_vol[c];
     if (i < nft)
-        fused_facet<D>(m, p, w, Pl, FV, GL, m.fz_ngl, FL, i, fA, dA, geoA, c0, halo_off, has_prev, gmean, g0);
+        fused_facet<D>(m, p, w, Pl, FV, GL, m.fz_ngl, FL, i, fA, dA, geoA, c0, halo_off, has_prev, gmean, g0, meshT);
     if (twoB)
         fused_facet<D>(m, p, w, Pl, FV, GL, m.fz_ngl, FL, i + kThreads, fB, dB, geoB, c0, halo_off, has_prev, gmean,
-                       g0);
+                       g0, meshT);
     __syncthreads();
 
     // ---- phase 3: divergence and stage epilogue of the tile's cells ----
@@ -1271,8 +1277,10 @@ static int check_mesh(const FvmMeshDev *m, const FvmPhys *p, const FvmWork *w, c
 // Dynamic LDS of k_fused in bytes.
 template <int D>
 static size_t fused_lds_bytes(const FvmMeshDev &m) {
+    // + the 16-bit mesh ids of the tile's cells, for a batch of meshes
     return static_cast<size_t>(fz_fl_offset<D>(m.fz_ncl, m.fz_ngl) + static_cast<int64_t>(m.fz_nfl) * (D + 2)) *
-           sizeof(float);
+               sizeof(float) +
+           (m.mesh_phys ? kThreads * sizeof(unsigned short) : 0);
 }
 
 constexpr size_t kFusedLdsMax = 64 * 1024;
@@ -1280,7 +1288,8 @@ constexpr size_t kFusedLdsMax = 64 * 1024;
 static int check_fused(const FvmMeshDev &m, const FvmWork &w, const char *who) {
     if (m.dim != 2 || !m.fz_tile || !m.fz_halo || !m.fz_hrec || !m.fz_nbr || !m.fz_fdesc || !m.fz_ffac ||
         !m.fz_fdesc_x || !m.fz_geo || !m.fz_geo_x || !m.fz_cfac || m.fz_ncl < kThreads || m.fz_ngl < kThreads ||
-        m.fz_ngl > m.fz_ncl || m.fz_ncl > 0xFFF || m.fz_nfl < 1 || fused_lds_bytes<2>(m) > kFusedLdsMax) {
+        m.fz_ngl > m.fz_ncl || m.fz_ncl > 0xFFF || m.fz_nfl < 1 || fused_lds_bytes<2>(m) > kFusedLdsMax ||
+        (m.mesh_phys && m.n_meshes > 0xFFFF)) {
         set_error("%s: invalid tile tables for the tile-fused stage", who);
         return FVM_ERR_ARG;
     }

@@ -314,33 +314,27 @@ __global__ void __launch_bounds__(kThreads) k_bc_mean(FvmMeshDev m, FvmPhys p, F
 // ------------------------------------------------------------------------------------
 // LSQ gradient + slope limiter
 // ------------------------------------------------------------------------------------
-// Gradient G.[U; U_bc] of one cell (G_mats row, columns in ascending order; bits
-// 16+3j of `info` name the source of column j: neighbour slot k, or 7 = the cell) and
-// the limiter factor phi = min over the cell's facets (facet_process.py:387-418,
+// Gradient G.[U; U_bc] of one cell (its G_mats row, stored in slot order: the value of
+// each facet slot's neighbour or boundary column, then the cell's own column) and the
+// limiter factor phi = min over the cell's facets (facet_process.py:387-418,
 // limiter.py).  P: the cell's primitives; Un: the values behind its facets (neighbour
-// primitives, or boundary values).
-// LSQ gradient of a cell (G row times [U; U_bc] in the reference's ascending column order,
-// accumulated with fused multiply-adds) and the limiter factor per component.
+// primitives, or boundary values).  The row is summed in slot order with fused
+// multiply-adds: fixed register operands, where the reference's ascending column order
+// would select each column's source per cell.
 template <int D, int LP>
-__device__ __forceinline__ void cell_gradient(const float (&P)[D + 2], const float (&Un)[D + 1][D + 2], int info,
+__device__ __forceinline__ void cell_gradient(const float (&P)[D + 2], const float (&Un)[D + 1][D + 2],
                                               const float (&gv)[D + 2][D], const float (&disp)[D + 1][D], float eps,
                                               float (&g)[D][D + 2], float (&phi)[D + 2]) {
     constexpr int C = D + 2, NL = D + 1;
 #pragma unroll
     for (int jj = 0; jj <= NL; ++jj) {
-        const int sel = (info >> (16 + 3 * jj)) & 7;
-        float U[C];
-#pragma unroll
-        for (int j = 0; j < C; ++j) {
-            float v = P[j];
-#pragma unroll
-            for (int k = 0; k < NL; ++k) v = (sel == k) ? Un[k][j] : v;
-            U[j] = v;
-        }
 #pragma unroll
         for (int i = 0; i < D; ++i) {
 #pragma unroll
-            for (int j = 0; j < C; ++j) g[i][j] = (jj == 0) ? gv[jj][i] * U[j] : fmaf(gv[jj][i], U[j], g[i][j]);
+            for (int j = 0; j < C; ++j) {
+                const float u = jj < NL ? Un[jj < NL ? jj : 0][j] : P[j];
+                g[i][j] = (jj == 0) ? gv[jj][i] * u : fmaf(gv[jj][i], u, g[i][j]);
+            }
         }
     }
 #pragma unroll
@@ -393,7 +387,6 @@ k_grad(FvmMeshDev m, FvmPhys p, FvmWork w, const float *__restrict__ prim, int h
     int64_t nbk[NL];
 #pragma unroll
     for (int k = 0; k < NL; ++k) nbk[k] = m.cell_nbr[k * N + c];
-    const int info = m.cell_info[c];
     float gv[NL + 1][D];
 #pragma unroll
     for (int jj = 0; jj <= NL; ++jj)
@@ -419,7 +412,7 @@ k_grad(FvmMeshDev m, FvmPhys p, FvmWork w, const float *__restrict__ prim, int h
         }
     }
     float g[D][C], phi[C];
-    cell_gradient<D, LP>(P, Un, info, gv, disp, eps, g, phi);
+    cell_gradient<D, LP>(P, Un, gv, disp, eps, g, phi);
     float *o = gp + c * Dm::REC;
 #pragma unroll
     for (int i = 0; i < D; ++i)
@@ -904,7 +897,7 @@ __device__ __forceinline__ void fused_grad_cell(const FvmMeshDev &m, const FvmPh
                                                 const float *Pl, float *FV, float *GL, int ngl, int l, int64_t gc,
                                                 bool own,
                                                 const float (&gv)[D + 2][D], const float (&disp)[D + 1][D],
-                                                float eps, int info, const ushort4 nrow, int has_prev,
+                                                float eps, const ushort4 nrow, int has_prev,
                                                 const float *gmean, int g0, const ushort4 cfac, float *FL) {
     using Dm = Dims<D>;
     constexpr int C = Dm::C, NL = Dm::NL;
@@ -931,7 +924,7 @@ __device__ __forceinline__ void fused_grad_cell(const FvmMeshDev &m, const FvmPh
         }
     }
     float g[D][C], phi[C];
-    cell_gradient<D, LP>(P, Un, info, gv, disp, eps, g, phi);
+    cell_gradient<D, LP>(P, Un, gv, disp, eps, g, phi);
     if (own && bnd) {
         float *o = w.gp + gc * Dm::REC;
 #pragma unroll
@@ -1126,7 +1119,6 @@ k_fused(FvmMeshDev m, FvmPhys p, FvmWork w, const float *__restrict__ prim, floa
         if (m.mesh_phys) meshT[i] = static_cast<unsigned short>(m.cell_mesh[c]);
     }
     float hgv[NL + 1][D], hdisp[NL][D], heps = 0.0f;
-    int hinfo = 0;
     int64_t hgc = 0;
     ushort4 hnr = make_ushort4(0, 0, 0, 0);
     if (has_h1) {
@@ -1150,7 +1142,6 @@ k_fused(FvmMeshDev m, FvmPhys p, FvmWork w, const float *__restrict__ prim, floa
 #pragma unroll
             for (int ii = 0; ii < D; ++ii) hdisp[k][ii] = rv[(NL + 1) * D + k * D + ii];
         heps = rv[(NL + 1) * D + NL * D];
-        hinfo = __float_as_int(rv[(NL + 1) * D + NL * D + 1]);
         hnr = *reinterpret_cast<const ushort4 *>(m.fz_nbr + (static_cast<int64_t>(nbr_off) + kThreads + i) * 4);
     }
     for (int q = i; q < n1 + n2; q += kThreads) {
@@ -1166,11 +1157,11 @@ k_fused(FvmMeshDev m, FvmPhys p, FvmWork w, const float *__restrict__ prim, floa
     float geoA[3 * D + 2];
     if (i < nft) fused_facet_load<D>(m, i, fo0, nfo, foff, fA, dA, geoA);
     if (has_own)
-        fused_grad_cell<D, LP>(m, p, w, Pl, FV, GL, m.fz_ngl, i, c0 + i, true, ogv, odisp, oeps, oinfo, onr,
+        fused_grad_cell<D, LP>(m, p, w, Pl, FV, GL, m.fz_ngl, i, c0 + i, true, ogv, odisp, oeps, onr,
                                has_prev, gmean, g0, cf, FL);
     if (has_h1)
-        fused_grad_cell<D, LP>(m, p, w, Pl, FV, GL, m.fz_ngl, kThreads + i, hgc, false, hgv, hdisp, heps, hinfo,
-                               hnr, has_prev, gmean, g0, cf, FL);
+        fused_grad_cell<D, LP>(m, p, w, Pl, FV, GL, m.fz_ngl, kThreads + i, hgc, false, hgv, hdisp, heps, hnr,
+                               has_prev, gmean, g0, cf, FL);
     __syncthreads();
 
     // ---- phase 2: facet fluxes (the second facet's and the epilogue's inputs in flight) ----

@@ -7,7 +7,7 @@ per-cell / per-facet arrays of ``FvmMeshDev`` (include/fvm_hip.h) and one RHS is
 HIP launches (see csrc/rhs.hip).  What the reference assembles as
 
     flux_mat    (facet_process.py:200-226)   -> cell_inv_vol + slot bits, facets ascending
-    G_mats      (fvm_mesh.py:5-42, 263-344)  -> cell_gval in ascending column order
+    G_mats      (fvm_mesh.py:5-42, 263-344)  -> cell_gval in slot order (neighbours, then the cell)
     A_face_grad (facet_process.py:114-198)   -> fac_geo[1/(n.d)] + bc_ga/bc_gb
     A_bc, b_bc  (facet_boundary.py:238-280)  -> bc_flags + bc_val
     A_corr      (facet_boundary.py:170-236)  -> bc_l
@@ -73,7 +73,6 @@ def host_layout(mesh, n_comp: int, bc_tags, cfg) -> dict:
 
     # ---- per-cell lists in ascending facet order (flux_mat ELL order) ----
     perm = np.argsort(c2f, axis=1, kind="stable")
-    invp = np.argsort(perm, axis=1, kind="stable")
     c2f_s = np.take_along_axis(c2f, perm, 1)
     nbr_s = np.take_along_axis(comb, perm, 1)
     slot_s = np.take_along_axis(slots, perm, 1)
@@ -90,23 +89,18 @@ def host_layout(mesh, n_comp: int, bc_tags, cfg) -> dict:
     nbk[pb] = pa % NL
     nbk = nbk.reshape(N, NL)
 
-    # ---- LSQ rows in ascending column order (G_mats ELL order) ----
+    # ---- LSQ rows (G_mats' values) in slot order: the neighbour in each slot, then the
+    # cell itself (the kernels sum the row in this order) ----
     G32 = g["G"].astype(np.float32)                   # [N, D, NL], reference local order
     diag = G32[:, :, 0].copy()
     for k in range(1, NL):
         diag = (diag + G32[:, :, k]).astype(np.float32)
-    cols = np.concatenate([comb, np.arange(N)[:, None]], 1)
-    vals = np.concatenate([G32, -diag[:, :, None]], 2)    # [N, D, NL+1]
-    eo = np.argsort(cols, axis=1, kind="stable")
-    gval = np.take_along_axis(vals, eo[:, None, :], 2)    # [N, D, NL+1]
-    src = np.where(eo == NL, 7, np.take_along_axis(invp, np.minimum(eo, NL - 1), 1))
+    gval = np.concatenate([np.take_along_axis(G32, perm[:, None, :], 2), -diag[:, :, None]], 2)  # [N, D, NL+1]
 
     info = np.zeros(N, np.int64)
     for k in range(NL):
         info |= slot_s[:, k] << k
         info |= nbk[:, k] << (8 + 2 * k)
-    for j in range(NL + 1):
-        info |= src[:, j].astype(np.int64) << (16 + 3 * j)
 
     vol32 = g["volumes"].astype(np.float32)
     inv_vol = (np.float32(1.0) / vol32).astype(np.float32)
@@ -205,7 +199,7 @@ def host_layout(mesh, n_comp: int, bc_tags, cfg) -> dict:
             bc_char[mask, D + 3:] = vinf[None, :] - nn * vn[:, None]
     per = fs64[c2f].sum(1)
     return dict(D=D, C=C, N=N, F=F, Fb=Fb, n_groups=len(modes), c2f_s=c2f_s, nbr_s=nbr_s, slot_s=slot_s, nbk=nbk,
-                gval=gval, src=src, info=info, disp_s=disp_s, inv_vol=inv_vol, eps=eps, geo=geo, bcell=bcell,
+                gval=gval, info=info, disp_s=disp_s, inv_vol=inv_vol, eps=eps, geo=geo, bcell=bcell,
                 group=group, flags=flags, bc_val=bc_val, bc_l=l_vec.astype(np.float32), bc_ga=bc_ga, bc_gb=bc_gb,
                 bc_char=bc_char, side=side, volumes64=g["volumes"], h64=D * g["volumes"] / per,
                 phys=(float(cfg.viscosity), float(cfg.visc_bulk), float(cfg.thermal_cond)))
@@ -279,7 +273,7 @@ class DeviceMesh:
         self.host = dict(c2f_s=c2f_s, nbr_s=nbr_s, bcell=bcell, group=group, cell_mesh=cell_mesh,
                          blk_slot=blk_slot, slot_g0=slot_g0, slot_ng=slot_ng, group_s0=group_s0,
                          group_s1=group_s1, group_count=group_count)
-        for k in ("slot_s", "nbk", "gval", "src", "info", "disp_s", "inv_vol", "eps", "geo", "flags", "bc_val",
+        for k in ("slot_s", "nbk", "gval", "info", "disp_s", "inv_vol", "eps", "geo", "flags", "bc_val",
                   "bc_l", "bc_ga", "bc_gb", "bc_char", "side"):
             self.host[k] = cat(k)
         h = self.host
@@ -305,7 +299,7 @@ class DeviceMesh:
         if want_tiles and D == 2:
             try:
                 self.tiles = build_tiles(N, D, nbr_s, fpos[c2f_s], fac_cell[:, fperm], h["gval"], h["disp_s"],
-                                         h["eps"], h["info"], h["geo"][fperm], bcell, group, cell_mesh, go)
+                                         h["eps"], h["geo"][fperm], bcell, group, cell_mesh, go)
                 self.tiles_reason = None
             except TilesUnavailable as e:
                 self.tiles_reason = str(e)

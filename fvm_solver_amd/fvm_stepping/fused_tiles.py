@@ -48,14 +48,14 @@ def _lookup(keys: np.ndarray, q: np.ndarray) -> np.ndarray:
 
 
 def build_tiles(N: int, D: int, nbr: np.ndarray, cell_facet: np.ndarray, fac_cell: np.ndarray,
-                gval: np.ndarray, disp: np.ndarray, eps: np.ndarray, info: np.ndarray, geo: np.ndarray,
+                gval: np.ndarray, disp: np.ndarray, eps: np.ndarray, geo: np.ndarray,
                 bcell: np.ndarray, group: np.ndarray, cell_mesh: np.ndarray, group_off: np.ndarray,
                 lds_limit: int = 64 * 1024) -> dict:
     """Tile tables for the fused kernel.
 
     nbr [N, NL] neighbour per slot (>= N: N + boundary index); cell_facet [N, NL] device
     facet positions; fac_cell [2, F] device order; gval [N, D, NL+1]; disp [N, NL, D];
-    eps, info [N]; geo [F, G] device order (first 3D+2 floats: n, n_hat, |n|, 1/(n.d),
+    eps [N]; geo [F, G] device order (first 3D+2 floats: n, n_hat, |n|, 1/(n.d),
     X); bcell/group [Fb]; cell_mesh [N]; group_off = first group of each mesh (+ total).
     Raises TilesUnavailable when a tile's working set does not fit ``lds_limit`` bytes
     of LDS or the local ids overflow (a badly ordered mesh)."""
@@ -128,14 +128,14 @@ def build_tiles(N: int, D: int, nbr: np.ndarray, cell_facet: np.ndarray, fac_cel
         raise AssertionError("H1 neighbour not staged")
     rows[h1_row, :NL] = np.where(nbr[h1c] < N, lh, NONE16).astype(np.uint16)
 
-    # ---- H1 static records: LSQ values [NL+1][D], displacements [NL][D], eps, info ----
+    # ---- H1 static records: LSQ values [NL+1][D] (slot order), displacements [NL][D], eps,
+    # zero padding to 16 / 32 floats ----
     HREC = 16 if D == 2 else 32
     hrec = np.zeros((len(K1), HREC), np.float32)
     hrec[:, :(NL + 1) * D] = np.asarray(gval, np.float32)[h1c].transpose(0, 2, 1).reshape(len(K1), -1)
     o = (NL + 1) * D
     hrec[:, o:o + NL * D] = np.asarray(disp, np.float32)[h1c].reshape(len(K1), -1)
     hrec[:, o + NL * D] = np.asarray(eps, np.float32)[h1c]
-    hrec[:, o + NL * D + 1] = np.asarray(info, np.int32)[h1c].view(np.float32)
 
     # ---- facets ----
     lo = fac_cell.min(0)

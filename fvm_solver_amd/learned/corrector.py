@@ -69,17 +69,12 @@ def lsq_neighbour_weights(dm) -> np.ndarray:
     rows (G_mats, slot order of ``cell_nbr``): grad v_c ~ sum_k w[k][:, c] (v_nb(k) - v_c),
     boundary faces left out (their weight is dropped)."""
     h = dm.host
-    gval, src, nbr = h["gval"], h["src"], h["nbr_s"]
+    gval, nbr = h["gval"], h["nbr_s"]              # gval: [N, D, NL+1], slots then the cell
     N, NL = nbr.shape
     D = gval.shape[1]
     if D != 2:
         raise NotImplementedError("the learned corrector takes 2-D meshes")
-    w = np.zeros((NL, D, N), np.float32)
-    rows = np.arange(N)
-    for j in range(NL + 1):
-        k = src[:, j].astype(np.int64)
-        m = k < NL
-        w[k[m], :, rows[m]] = gval[rows[m], :, j]
+    w = np.ascontiguousarray(gval[:, :, :NL].transpose(2, 1, 0)).astype(np.float32)
     w[np.broadcast_to((nbr >= N).T[:, None, :], w.shape)] = 0.0
     return w
 

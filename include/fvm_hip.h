@@ -31,7 +31,7 @@ extern "C" {
 
 /* Description of the last failure on the calling thread ("" if none). */
 const char *fvm_last_error(void);
-/* Library ABI version (bumped on any struct or signature change). */
+/* Library ABI version (bumped on any struct, signature or array-layout change). */
 int fvm_abi_version(void);
 
 /* ---------------------------------------------------------------------------------
@@ -80,11 +80,11 @@ typedef struct FvmMeshDev {
     const int32_t *cell_nbr;    /* [NL][N] neighbour cell, or N + boundary index      */
     const int32_t *cell_info;   /* [N] bit k: this cell's slot in facet k;
                                    bits 8+2k..9+2k: facet k's position in the
-                                   neighbour's list; bits 16+3j..18+3j: source of
-                                   LSQ column j (k = neighbour k, 7 = the cell)       */
+                                   neighbour's list                                   */
     const float *cell_disp;     /* [NL][d][N] centroid -> facet midpoint              */
-    const float *cell_gval;     /* [NL+1][d][N] LSQ gradient row values, ascending
-                                   column order (G_mats rows d*c+i)                   */
+    const float *cell_gval;     /* [NL+1][d][N] LSQ gradient row values (G_mats rows
+                                   d*c+i) in slot order: the column of neighbour k
+                                   (or boundary value) for k < NL, then the cell's    */
     const float *cell_inv_vol;  /* [N] fp32(1/vol): |flux_mat| entries                */
     const float *cell_eps;      /* [N] limiter epsilon (K sqrt(vol))^(p+1)            */
     /* per facet ------------------------------------------------------------------- */

@@ -74,7 +74,7 @@ def test_fused_inputs_are_the_same_floats(dm):
         f = np.flatnonzero(inner)
         k = (t["fdesc"][f].astype(np.int64) >> (12 if s == 0 else 26)) & 3
         assert np.array_equal(h["disp_s"][c[inner], k].astype(np.float32), geo[f, 8 + 2 * s:10 + 2 * s])
-    # halo records carry the halo cells' own LSQ values, displacements, eps and info
+    # halo records carry the halo cells' own LSQ values, displacements and eps
     tab = t["tab"]
     for ti in range(0, t["n_tiles"], max(1, t["n_tiles"] // 7)):
         ho, n1, h1o = int(tab[ti, 0]), int(tab[ti, 1]), int(tab[ti, 10])
@@ -83,7 +83,7 @@ def test_fused_inputs_are_the_same_floats(dm):
         assert np.array_equal(rec[:, :8], h["gval"][cells].astype(np.float32).transpose(0, 2, 1).reshape(n1, -1))
         assert np.array_equal(rec[:, 8:14], h["disp_s"][cells].astype(np.float32).reshape(n1, -1))
         assert np.array_equal(rec[:, 14], h["eps"][cells].astype(np.float32))
-        assert np.array_equal(rec[:, 15].view(np.int32), h["info"][cells].astype(np.int32))
+        assert np.all(rec[:, 15] == 0)
     # boundary slots are marked, interior ones point at staged cells
     own_rows = t["nbr"][np.repeat(tab[:, 3], TILE) + np.tile(np.arange(TILE), t["n_tiles"])][:dm.N, :NL]
     assert np.array_equal(own_rows == NONE16, h["nbr_s"] >= dm.N)
@@ -118,5 +118,5 @@ def test_lds_limit_is_enforced():
     h = dm.host
     with pytest.raises(TilesUnavailable):
         build_tiles(dm.N, 2, h["nbr_s"], h["fpos"][h["c2f_s"]], h["fac_cell"][:, h["fperm"]], h["gval"], h["disp_s"],
-                    h["eps"], h["info"], h["geo"][h["fperm"]], h["bcell"], h["group"], h["cell_mesh"],
+                    h["eps"], h["geo"][h["fperm"]], h["bcell"], h["group"], h["cell_mesh"],
                     dm.group_offsets, lds_limit=1024)

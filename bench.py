@@ -476,6 +476,11 @@ def batch_main(a):
     value = N * steps / (gpu_ms / 1e3)
     finite = bool(torch.isfinite(s.state).all())
     log(f"{steps} steps (graph of {n_graph}): {gpu_ms:.2f} ms, {ms_per_step * 1e3:.1f} us/step, finite={finite}")
+    if a.calib_copy:
+        src = torch.empty(64 * 2 ** 20, dtype=torch.float32, device=dev).fill_(1.0)
+        torch.empty_like(src).copy_(src)
+        torch.cuda.synchronize()
+        log("calibration copy: 256 MiB read + 256 MiB written")
     # per-kernel durations of the same stages, each kernel launched back to back
     E = s.E
     parts = {16: "k_fused"} if E.fused else {2: "k_grad", 4: "k_facet_flux", 8: "k_div"}

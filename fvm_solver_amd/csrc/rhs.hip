@@ -37,6 +37,10 @@ namespace fvm {
 
 constexpr int kThreads = 256;
 constexpr int kMaxGroups = 4;
+// Meshes of a batch whose physics and time step a tile of the fused stage keeps in LDS
+// (cells are ordered by mesh: a 256-cell tile spans its first mesh and the next ones;
+// facets and cells of further meshes read them from global memory).
+constexpr int kTileMeshes = 4;
 
 template <int D>
 struct Dims {
@@ -701,10 +705,11 @@ __device__ __forceinline__ void stage_inputs(const FvmStage &st, int64_t c, bool
 // live or not): writes dU/dt, or applies the explicit stage update and emits the next
 // stage's primitives, their boundary-mean partials (slot of 256-cell tile `tile`) and,
 // on the last stage of an embedded tableau, the fp64 column-norm partials of `tile`.
+// dtm: the time step of the cell's mesh (read by the caller; unused for dU/dt).
 template <int D, int EPI>
 __device__ __forceinline__ void stage_epilogue(const FvmMeshDev &m, const FvmPhys &p, const FvmWork &w,
                                                float *__restrict__ dudt, const FvmStage &st, int64_t tile,
-                                               int64_t c, bool live, int mesh, const float (&acc)[D + 2],
+                                               int64_t c, bool live, float dtm, const float (&acc)[D + 2],
                                                const float (&U0v)[D + 2], const float (&ap)[D + 2],
                                                const float (&a2p)[D + 2], float *red, double *sred) {
     constexpr int C = D + 2;
@@ -728,7 +733,7 @@ __device__ __forceinline__ void stage_epilogue(const FvmMeshDev &m, const FvmPhy
         const int64_t o = c * C;
         const float(&fm1)[C] = ap;
         const float(&fm2)[C] = a2p;
-        const float c12 = st.dt[mesh] / 12.0f;
+        const float c12 = dtm / 12.0f;
         float U1[C];
 #pragma unroll
         for (int j = 0; j < C; ++j) {
@@ -744,7 +749,7 @@ __device__ __forceinline__ void stage_epilogue(const FvmMeshDev &m, const FvmPhy
         store_vec<C>(st.prim_next + o, Pn);
     } else if (live) {
         const int64_t o = c * C;
-        const float dt = st.dt[mesh];
+        const float dt = dtm;
         float kk[C], a[C], a2[C];
 #pragma unroll
         for (int j = 0; j < C; ++j) {
@@ -831,7 +836,7 @@ k_div(FvmMeshDev m, FvmPhys p, FvmWork w, float *__restrict__ dudt, FvmStage st)
         const int64_t tile = xcd_tile(v, ntiles);
         const int64_t c = tile * kThreads + threadIdx.x;
         const bool live = c < N;
-        const int mesh = (m.cell_mesh && live) ? m.cell_mesh[c] : 0;
+        const float dtm = (EPI == EPI_STAGE && live) ? st.dt[m.cell_mesh ? m.cell_mesh[c] : 0] : 0.0f;
         float U0v[C], ap[C], a2p[C];
         stage_inputs<D, EPI>(st, c, live, U0v, ap, a2p);
         float acc[C];
@@ -851,7 +856,7 @@ k_div(FvmMeshDev m, FvmPhys p, FvmWork w, float *__restrict__ dudt, FvmStage st)
                 for (int j = 0; j < C; ++j) acc[j] = acc[j] + vk * Fk[k][j];
             }
         }
-        stage_epilogue<D, EPI>(m, p, w, dudt, st, tile, c, live, mesh, acc, U0v, ap, a2p, red, sred);
+        stage_epilogue<D, EPI>(m, p, w, dudt, st, tile, c, live, dtm, acc, U0v, ap, a2p, red, sred);
     }
 }
 
@@ -963,9 +968,9 @@ __device__ __forceinline__ void fused_facet(const FvmMeshDev &m, const FvmPhys &
                                             int q,
                                             int64_t f, unsigned d, const float (&geo)[3 * D + 2], int64_t c0,
                                             int halo_off, int has_prev, const float *gmean, int g0,
-                                            const unsigned short *meshT) {
+                                            const unsigned short *meshT, int mt0, const float *physT) {
     using Dm = Dims<D>;
-    constexpr int C = Dm::C, NL = Dm::NL;
+    constexpr int C = Dm::C;
     const int64_t N = m.n_cells, NF = m.n_facets;
     const unsigned l0 = d & 0xFFFu, k0 = (d >> 12) & 3u, l1 = (d >> 14) & 0xFFFu, k1 = (d >> 26) & 3u;
     const bool left_first = (d >> 28) & 1u, bnd = (d >> 29) & 1u, outer0 = (d >> 30) & 1u;
@@ -973,12 +978,20 @@ __device__ __forceinline__ void fused_facet(const FvmMeshDev &m, const FvmPhys &
     if (m.mesh_phys) {
         // one side of every facet of the tile's list is a tile cell (the interior cell of
         // a boundary facet; the lowest cell of an owned facet; the higher one of a foreign
-        // facet), whose mesh id the staging phase left in LDS
+        // facet), whose mesh id the staging phase left in LDS, with the physics of the
+        // tile's first kTileMeshes meshes (mt0, mt0 + 1, ...)
         const unsigned lt = bnd ? (outer0 ? l1 : l0) : (l0 < kThreads ? l0 : l1);
         const int mesh = meshT[lt];
-        mu0 = m.mesh_phys[4 * mesh];
-        mub0 = m.mesh_phys[4 * mesh + 1];
-        kappa = m.mesh_phys[4 * mesh + 2];
+        const int rel = mesh - mt0;
+        if (rel >= 0 && rel < kTileMeshes) {
+            mu0 = physT[4 * rel];
+            mub0 = physT[4 * rel + 1];
+            kappa = physT[4 * rel + 2];
+        } else {
+            mu0 = m.mesh_phys[4 * mesh];
+            mub0 = m.mesh_phys[4 * mesh + 1];
+            kappa = m.mesh_phys[4 * mesh + 2];
+        }
     }
     float U0s[C], U1s[C], g0s[D][D + 1], g1s[D][D + 1], gdn[D + 1];
     if (!bnd) {
@@ -1082,8 +1095,10 @@ k_fused(FvmMeshDev m, FvmPhys p, FvmWork w, const float *__restrict__ prim, floa
     float *FV = Pl + static_cast<int64_t>(m.fz_ncl) * C;
     float *GL = FV + static_cast<int64_t>(m.fz_ngl) * FVN;
     float *FL = lds + fz_fl_offset<D>(m.fz_ncl, m.fz_ngl);
-    // batches only: mesh id of each tile cell (after FL)
+    // batches only (after FL): the mesh id of each tile cell, then mu0, mub0, kappa and dt
+    // of the tile's first kTileMeshes meshes (cells are ordered by mesh)
     unsigned short *meshT = reinterpret_cast<unsigned short *>(FL + static_cast<int64_t>(m.fz_nfl) * C);
+    float *physT = reinterpret_cast<float *>(meshT + kThreads);
     float *red = FV;                                   // reductions reuse the face-value rows
     double *sred = reinterpret_cast<double *>(FV);     // (before and after the two phases using them)
     const int i = threadIdx.x;
@@ -1093,6 +1108,17 @@ k_fused(FvmMeshDev m, FvmPhys p, FvmWork w, const float *__restrict__ prim, floa
     const bool has_own = i < nown, has_h1 = i < n1;
     __shared__ float gmean[kMaxGroups];
     if (ng > 0) block_group_means_g(m, w, g0, ng, gmean, red);
+    int mt0 = 0;
+    if (m.mesh_phys) {
+        mt0 = m.cell_mesh[c0];
+        const int mi = mt0 + i;
+        if (i < kTileMeshes && mi < m.n_meshes) {
+            physT[4 * i] = m.mesh_phys[4 * mi];
+            physT[4 * i + 1] = m.mesh_phys[4 * mi + 1];
+            physT[4 * i + 2] = m.mesh_phys[4 * mi + 2];
+            physT[4 * i + 3] = EPI == EPI_STAGE ? st.dt[mi] : 0.0f;
+        }
+    }
 
     // ---- phase 0: primitives of tile, H1 and H2 to LDS; the gradient inputs of this
     // thread's tile cell and H1 cell to registers ----
@@ -1171,16 +1197,25 @@ k_fused(FvmMeshDev m, FvmPhys p, FvmWork w, const float *__restrict__ prim, floa
     const bool twoB = i + kThreads < nft;
     if (twoB) fused_facet_load<D>(m, i + kThreads, fo0, nfo, foff, fB, dB, geoB);
     const int64_t c = c0 + i;
-    const int mesh = (m.cell_mesh && has_own) ? m.cell_mesh[c] : 0;
+    float dtm = 0.0f;
+    if (EPI == EPI_STAGE && has_own) {
+        if (m.mesh_phys) {
+            const int mesh = meshT[i], rel = mesh - mt0;
+            dtm = rel >= 0 && rel < kTileMeshes ? physT[4 * rel + 3] : st.dt[mesh];
+        } else {
+            dtm = st.dt[m.cell_mesh ? m.cell_mesh[c] : 0];
+        }
+    }
     float U0v[C], ap[C], a2p[C];
     stage_inputs<D, EPI>(st, c, has_own, U0v, ap, a2p);
     float inv_vol = 0.0f;
     if (has_own) inv_vol = m.cell_inv_vol[c];
     if (i < nft)
-        fused_facet<D>(m, p, w, Pl, FV, GL, m.fz_ngl, FL, i, fA, dA, geoA, c0, halo_off, has_prev, gmean, g0, meshT);
+        fused_facet<D>(m, p, w, Pl, FV, GL, m.fz_ngl, FL, i, fA, dA, geoA, c0, halo_off, has_prev, gmean, g0, meshT,
+                       mt0, physT);
     if (twoB)
         fused_facet<D>(m, p, w, Pl, FV, GL, m.fz_ngl, FL, i + kThreads, fB, dB, geoB, c0, halo_off, has_prev, gmean,
-                       g0, meshT);
+                       g0, meshT, mt0, physT);
     __syncthreads();
 
     // ---- phase 3: divergence and stage epilogue of the tile's cells ----
@@ -1199,7 +1234,7 @@ k_fused(FvmMeshDev m, FvmPhys p, FvmWork w, const float *__restrict__ prim, floa
             for (int j = 0; j < C; ++j) acc[j] = acc[j] + vk * Fk[k][j];
         }
     }
-    stage_epilogue<D, EPI>(m, p, w, dudt, st, tile, c, has_own, mesh, acc, U0v, ap, a2p, red, sred);
+    stage_epilogue<D, EPI>(m, p, w, dudt, st, tile, c, has_own, dtm, acc, U0v, ap, a2p, red, sred);
 }
 
 __global__ void k_update_stepsize(const double *__restrict__ red, int C, int nblk, float atol, float rtol, float mtol,
@@ -1268,10 +1303,10 @@ static int check_mesh(const FvmMeshDev *m, const FvmPhys *p, const FvmWork *w, c
 // Dynamic LDS of k_fused in bytes.
 template <int D>
 static size_t fused_lds_bytes(const FvmMeshDev &m) {
-    // + the 16-bit mesh ids of the tile's cells, for a batch of meshes
+    // + the 16-bit mesh ids of the tile's cells and the tile's mesh table, for a batch
     return static_cast<size_t>(fz_fl_offset<D>(m.fz_ncl, m.fz_ngl) + static_cast<int64_t>(m.fz_nfl) * (D + 2)) *
                sizeof(float) +
-           (m.mesh_phys ? kThreads * sizeof(unsigned short) : 0);
+           (m.mesh_phys ? kThreads * sizeof(unsigned short) + kTileMeshes * 4 * sizeof(float) : 0);
 }
 
 constexpr size_t kFusedLdsMax = 64 * 1024;

@@ -170,29 +170,37 @@ __device__ __forceinline__ void bc_value(const FvmMeshDev &m, const FvmPhys &p, 
                                          const float *__restrict__ gprev, int has_prev, const float *gmean,
                                          int gbase, float (&u)[D + 2]) {
     using Dm = Dims<D>;
-    constexpr int C = Dm::C;
+    constexpr int C = Dm::C, CH = Dm::CH, REC = Dm::REC;
+    // every row this facet may need is loaded up front, in one round trip: the value and
+    // Neumann rows, the cell's previous gradient record (used under has_prev) and the
+    // characteristic row (used by group facets; every boundary facet has one)
     const int flags = m.bc_flags[b];
+    const int grp = m.bc_group[b];
+    float bv[C], l[D], gr[REC], ch[CH];
 #pragma unroll
-    for (int j = 0; j < C; ++j) {
-        const float v = m.bc_val[b * C + j];
-        u[j] = ((flags >> j) & 1) ? P[j] + v : v;
-    }
+    for (int j = 0; j < C; ++j) bv[j] = m.bc_val[b * C + j];
+#pragma unroll
+    for (int i = 0; i < D; ++i) l[i] = m.bc_l[b * D + i];
+#pragma unroll
+    for (int q = 0; q < REC; ++q) gr[q] = gprev[q];
+#pragma unroll
+    for (int q = 0; q < CH; ++q) ch[q] = m.bc_char[b * CH + q];
+#pragma unroll
+    for (int j = 0; j < C; ++j) u[j] = ((flags >> j) & 1) ? P[j] + bv[j] : bv[j];
     if (has_prev && flags) {
 #pragma unroll
         for (int j = 0; j < C; ++j) {
             if ((flags >> j) & 1) {
-                const float ph = gprev[D * C + j];
-                float s = m.bc_l[b * D] * (gprev[j] * ph);
+                const float ph = gr[D * C + j];
+                float s = l[0] * (gr[j] * ph);
 #pragma unroll
-                for (int i = 1; i < D; ++i) s = s + m.bc_l[b * D + i] * (gprev[i * C + j] * ph);
+                for (int i = 1; i < D; ++i) s = s + l[i] * (gr[i * C + j] * ph);
                 u[j] = u[j] + s;
             }
         }
     }
-    const int grp = m.bc_group[b];
     if (grp < 0) return;
     const float cmean = gmean[grp - gbase];
-    const float *ch = m.bc_char + b * Dm::CH;
     float n[D], vt[D];
     float vn = 0.0f;
 #pragma unroll

@@ -16,7 +16,8 @@ boundary facets follow, grouped by tag.  Boundary tags use the reference's names
 Points are sampled by grid-based parallel dart throwing (a maximal-ish Poisson-disk
 set with minimum spacing ``h``), boundary points are placed at spacing ``h`` on the
 outer box and on the obstacle, and a ring of ghost points outside the box keeps the
-collinear/coplanar boundary points on the Delaunay hull.  Vertices and cells are then
+collinear/coplanar boundary points on the Delaunay hull.  Large 3-D meshes use a
+jittered body-centred cubic lattice instead (sliver-free Delaunay tessellation).  Vertices and cells are then
 sorted along a Morton curve so that neighbouring cells sit close in memory.
 """
 from __future__ import annotations
@@ -128,23 +129,130 @@ def _poisson_disk(rng: np.random.Generator, lo, hi, h: float, inside, fixed: np.
     return pts[~np.isnan(pts[:, 0])]
 
 
-def _jittered_lattice(rng: np.random.Generator, lo, hi, h: float, inside, fixed: np.ndarray,
-                      jitter: float = 0.15) -> np.ndarray:
-    """Points of a cubic lattice with the Poisson-disk sampler's density (spacing
-    1.16 h), each moved by a uniform jitter of +-jitter spacings, kept if admissible and
-    at least 0.6 h from every fixed (boundary) point.  O(n) and vectorised: the interior
-    sampler for large 3-D meshes."""
-    lo = np.asarray(lo, float)
-    hi = np.asarray(hi, float)
-    a = h / 0.64 ** (1.0 / 3.0)
-    axes = [np.arange(l + 0.5 * a, u - 0.25 * a, a) for l, u in zip(lo, hi)]
-    g = np.stack(np.meshgrid(*axes, indexing="ij"), -1).reshape(-1, len(lo))
-    g = g + (rng.random(g.shape) * 2.0 - 1.0) * (jitter * a)
-    g = g[inside(g) & np.all(g > lo + 0.3 * h, 1) & np.all(g < hi - 0.3 * h, 1)]
-    if len(fixed):
-        dmin, _ = cKDTree(fixed).query(g, k=1)
-        g = g[dmin >= 0.6 * h]
-    return g
+def _bcc_lattice(rng: np.random.Generator, L: np.ndarray, a: float, jitter: float = 0.07):
+    """Body-centred cubic points filling the box [0, L] with cube side a (L / a integral):
+    the corner sublattice includes every face, edge and corner of the box, the centre
+    sublattice sits half a cube inside.  The Delaunay tessellation of a BCC lattice is
+    the disphenoid honeycomb (all tetrahedra congruent, circumspheres empty of other
+    points by a margin, no four-point degeneracy), so small jitter keeps it free of
+    slivers; a jittered cubic lattice, whose cubes have eight cospherical corners, is
+    not.  Interior points move by +-jitter cube sides per axis; points on a box face move
+    only within that face (edge points along the edge, corners not at all)."""
+    n = [int(round(Lk / a)) for Lk in L]
+    A = np.stack(np.meshgrid(*[np.arange(k + 1) for k in n], indexing="ij"), -1).reshape(-1, 3)
+    on_face = (A == 0) | (A == np.asarray(n))
+    B = np.stack(np.meshgrid(*[np.arange(k) for k in n], indexing="ij"), -1).reshape(-1, 3) + 0.5
+    jA = (rng.random(A.shape) * 2.0 - 1.0) * jitter
+    jA[on_face] = 0.0
+    jB = (rng.random(B.shape) * 2.0 - 1.0) * jitter
+    return np.concatenate([(A + jA) * a, (B + jB) * a])
+
+
+def _tet_quality(pts: np.ndarray, tet: np.ndarray) -> np.ndarray:
+    """Inscribed-sphere-like size 3 vol / (sum of face areas) of each tetrahedron."""
+    P = pts[tet]
+    a, b, c, d = P[:, 0], P[:, 1], P[:, 2], P[:, 3]
+    vol = np.abs(np.einsum("ij,ij->i", b - a, np.cross(c - a, d - a))) / 6.0
+
+    def area(x, y, z):
+        return 0.5 * np.linalg.norm(np.cross(y - x, z - x), axis=1)
+
+    return 3.0 * vol / (area(a, b, c) + area(a, b, d) + area(a, c, d) + area(b, c, d))
+
+
+def _circumspheres(pts: np.ndarray, tet: np.ndarray):
+    """Circumcentres [n, 3] and circumradii [n] of tetrahedra."""
+    P = pts[tet]
+    a = P[:, 0]
+    M = P[:, 1:] - a[:, None]
+    rhs = 0.5 * np.sum(M * M, axis=2)
+    det = np.linalg.det(M)
+    ok = np.abs(det) > 1e-300
+    x = np.zeros_like(a)
+    x[ok] = np.linalg.solve(M[ok], rhs[ok][..., None])[..., 0]
+    r = np.linalg.norm(x, axis=1)
+    r[~ok] = np.inf
+    return a + x, r
+
+
+def _channel_tets(pts: np.ndarray, n_sph: int, L: np.ndarray):
+    """Delaunay tetrahedra of pts (sphere points first) minus the flat ones of the box
+    faces and those spanned by sphere points only (the obstacle's inside)."""
+    tet = Delaunay(pts, qhull_options="QJ Qbb").simplices
+    flat = np.zeros(len(tet), bool)
+    for ax in range(3):
+        for side in (0.0, L[ax]):
+            flat |= np.all(np.abs(pts[tet][:, :, ax] - side) < 1e-12, axis=1)
+    tet = tet[~flat]
+    return tet[~np.all(tet < n_sph, axis=1)]
+
+
+def _bcc_channel(rng: np.random.Generator, L: np.ndarray, c: np.ndarray, R: float, n_cells: int,
+                 min_quality: float = 0.15, rounds: int = 8):
+    """Points and tetrahedra of the box minus a ball: a BCC lattice with ~n_cells
+    Delaunay tetrahedra, lattice points within 0.4 spacings of the ball removed, the ball's
+    surface sampled by a Fibonacci spiral.  Slivers can only form where the spiral meets
+    the lattice: the lattice vertices of every tetrahedron whose size 3 vol / area is
+    below min_quality of the mean are moved by a random quarter spacing and the ball's
+    neighbourhood B is triangulated again, until none is left.  Moving points near the
+    ball changes the Delaunay tessellation only inside B (tetrahedra whose circumsphere
+    contains an old or new position), so the global tessellation is computed once: its
+    tetrahedra whose circumsphere leaves B are kept, the rest come from the tessellation
+    of the points in B."""
+    vol = float(np.prod(L)) - 4.0 / 3.0 * math.pi * R ** 3
+    # the disphenoid honeycomb has 12 tetrahedra per cube of side a; the box lengths must
+    # be whole numbers of cubes
+    m = max(2, int(round(float(np.min(L)) / (12.0 * vol / n_cells) ** (1.0 / 3.0))))
+    a = float(np.min(L)) / m
+    if not np.allclose(L / a, np.round(L / a)):
+        raise ValueError("the BCC sampler needs box lengths that are whole multiples of the shortest one")
+    h = a * math.sqrt(3.0) / 2.0           # nearest-neighbour distance of the lattice
+    P = _bcc_lattice(rng, L, a)
+    P = P[np.sum((P - c) ** 2, 1) > (R + 0.4 * h) ** 2]
+    ns = max(int(round(4 * math.pi * R * R / (0.9 * h * h))), 12)
+    sph = c + R * _fibonacci_sphere(ns)
+    pts = np.concatenate([sph, P])
+    tet = _channel_tets(pts, ns, L)
+    q = _tet_quality(pts, tet)
+    qmean = float(q.mean())
+    RB = R + 8.0 * h                      # the neighbourhood B of the ball
+    cc, r = _circumspheres(pts, tet)
+    inside_B = np.linalg.norm(cc - c, axis=1) + r < RB
+    if np.any(q[~inside_B] < min_quality * qmean):
+        raise MeshGenerationError("sliver away from the obstacle (lattice jitter too large?)")
+    keep_far = tet[~inside_B]
+    # local points: everything inside B (the tessellation of B's interior depends on them
+    # only); lattice points beyond R + 3 spacings are never moved
+    loc = np.flatnonzero(np.sum((pts - c) ** 2, 1) < RB * RB)
+    loc = loc[np.argsort(loc >= ns, kind="stable")]        # sphere points first
+    lp = pts[loc]
+    for _ in range(rounds):
+        lt = _channel_tets(lp, ns, L)
+        lcc, lr = _circumspheres(lp, lt)
+        lt = lt[np.linalg.norm(lcc - c, axis=1) + lr < RB]
+        bad = _tet_quality(lp, lt) < min_quality * qmean
+        if not bad.any():
+            break
+        # move the lattice vertices of every sliver by a random quarter spacing (a
+        # sliver is four nearly cospherical points: a perturbation breaks it), keeping
+        # them off the ball and inside B
+        mv = np.unique(lt[bad][lt[bad] >= ns])
+        if len(mv) == 0:
+            raise MeshGenerationError("a sliver at the obstacle has no lattice vertex")
+        for _try in range(20):
+            step = rng.standard_normal((len(mv), 3))
+            new = lp[mv] + 0.25 * h * step / np.linalg.norm(step, axis=1, keepdims=True)
+            dist = np.linalg.norm(new - c, axis=1)
+            ok = (dist > R + 0.4 * h) & (dist < R + 3.0 * h) & np.all(new > 0, 1) & np.all(new < L, 1)
+            lp[mv[ok]] = new[ok]
+            mv = mv[~ok]
+            if len(mv) == 0:
+                break
+    else:
+        raise MeshGenerationError("slivers at the obstacle remain")
+    pts = pts.copy()
+    pts[loc] = lp
+    return pts, np.concatenate([keep_far, loc[lt]]), h
 
 
 def _segment_points(a, b, h):
@@ -219,22 +327,25 @@ def channel_mesh_3d(n_cells: int, seed: int = 1, spec: ChannelSpec | None = None
                     sampler: str = "auto"):
     """Tetrahedral mesh of [0,Lx]x[0,Ly]x[0,Lz] minus a ball, with about ``n_cells`` cells.
 
-    sampler: "poisson" (dart throwing, slow above a few 100k cells), "lattice" (jittered
-    cubic lattice inside, Poisson-disk on the faces) or "auto" (lattice above 200k)."""
+    sampler: "poisson" (dart throwing inside, Poisson-disk on the faces; slow above a few
+    100k cells), "bcc" (jittered body-centred cubic lattice aligned with the box, slivers
+    at the ball removed; ``h`` is ignored) or "auto" (bcc above 200k cells)."""
     if sampler == "auto":
-        sampler = "lattice" if n_cells > 200_000 else "poisson"
-    if sampler not in ("poisson", "lattice"):
+        sampler = "bcc" if n_cells > 200_000 else "poisson"
+    if sampler not in ("poisson", "bcc"):
         raise ValueError(f"unknown sampler {sampler!r}")
     spec = spec or ChannelSpec(lengths=(2.0, 1.0, 1.0), centre=(0.5, 0.5, 0.5), radius=0.1)
     L = np.asarray(spec.lengths, float)
     c = np.asarray(spec.centre, float)
     R = spec.radius
+    rng = np.random.default_rng(seed)
+    if sampler == "bcc":
+        pts, tet, h = _bcc_channel(rng, L, c, R, n_cells)
+        return _finalise(pts, tet, spec, h, dim=3)
     if h is None:
         vol = float(np.prod(L)) - 4.0 / 3.0 * math.pi * R ** 3
-        # ~0.64/h^3 points per unit volume for this sampler, ~6.3 tets per point (the
-        # jittered lattice gives ~10 % more tets per point).
-        h = (0.64 * 6.3 * vol / n_cells) ** (1.0 / 3.0) * (1.031 if sampler == "lattice" else 1.0)
-    rng = np.random.default_rng(seed)
+        # ~0.64/h^3 points per unit volume for this sampler, ~6.3 tets per point
+        h = (0.64 * 6.3 * vol / n_cells) ** (1.0 / 3.0)
 
     # Box edges, then face interiors (2-D Poisson disk per face), then the sphere.
     fixed = []
@@ -263,20 +374,14 @@ def channel_mesh_3d(n_cells: int, seed: int = 1, spec: ChannelSpec | None = None
     sph = c + R * _fibonacci_sphere(ns)
     fixed = np.concatenate([fixed] + face_pts + [sph])
 
-    def inside(p):
-        return np.all(p > 0, 1) & np.all(p < L, 1) & (np.sum((p - c) ** 2, 1) > R * R)
+    # Face points are only h apart at the least, so an interior point h from all of them
+    # can still sit almost on the face and span a sliver with three of them: keep
+    # interior points half a spacing off the walls and the sphere.
+    def inside_off_walls(p):
+        return (np.all(p > 0.5 * h, 1) & np.all(p < L - 0.5 * h, 1)
+                & (np.sum((p - c) ** 2, 1) > (R + 0.5 * h) ** 2))
 
-    if sampler == "lattice":
-        interior = _jittered_lattice(rng, (0, 0, 0), L, h, inside, fixed)
-    else:
-        # Face points are only h apart at the least, so an interior point h from all of
-        # them can still sit almost on the face and span a sliver with three of them:
-        # keep interior points half a spacing off the walls and the sphere.
-        def inside_off_walls(p):
-            return (np.all(p > 0.5 * h, 1) & np.all(p < L - 0.5 * h, 1)
-                    & (np.sum((p - c) ** 2, 1) > (R + 0.5 * h) ** 2))
-
-        interior = _poisson_disk(rng, (0, 0, 0), L, h, inside_off_walls, fixed)
+    interior = _poisson_disk(rng, (0, 0, 0), L, h, inside_off_walls, fixed)
     pts = np.concatenate([fixed, interior])
     # The box is the convex hull; joggled input keeps every coplanar face point a vertex.
     tri = Delaunay(pts, qhull_options="QJ Qbb")

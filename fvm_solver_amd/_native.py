@@ -12,7 +12,7 @@ import torch
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libfvm_hip.so")
-ABI_VERSION = 8
+ABI_VERSION = 9
 
 FVM_ERR_ROW_OVERFLOW = 2
 
@@ -45,7 +45,8 @@ class FvmPhys(ctypes.Structure):
 
 class FvmWork(ctypes.Structure):
     _fields_ = [("ubc", c_void_p), ("gp", c_void_p), ("bc_part", c_void_p), ("red", c_void_p),
-                ("fflux", c_void_p), ("gp_prev", c_void_p)]
+                ("fflux", c_void_p), ("gp_prev", c_void_p),
+                ("bc_part_next", c_void_p)]
 
 
 class FvmStage(ctypes.Structure):

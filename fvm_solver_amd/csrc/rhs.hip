@@ -248,11 +248,14 @@ __device__ __forceinline__ void bc_value(const FvmMeshDev &m, const FvmPhys &p, 
 // every thread of the block calls it (uniform control flow).  A slot is a 256-cell tile
 // holding interior cells of group facets; the groups of the meshes its cells belong to
 // are slot_g0 .. slot_g0 + slot_ng - 1 (<= 4).  Thread contributions go through a fixed
-// tree into bc_part[slot][q].  The consumer (k_grad of the next evaluation) sums the
-// partials of a group in a fixed order, so no grid-wide fence or counter is needed and
-// the kernel boundary orders the writes before the reads.
+// tree into part[slot][q].  The consumer (the gradient phase of the next evaluation) sums
+// the partials of a group in a fixed order, so no grid-wide fence or counter is needed
+// and the kernel boundary orders the writes before the reads.  A stage epilogue writes
+// into work->bc_part_next, never into the work->bc_part its own launch reads: tiles of
+// one launch run in no fixed order, and a tile that starts after another tile's epilogue
+// must still see the partials of the stage's input.
 template <int D>
-__device__ void bc_mean_block(const FvmMeshDev &m, const FvmPhys &p, const FvmWork &w, int64_t c, bool live,
+__device__ void bc_mean_block(const FvmMeshDev &m, const FvmPhys &p, float *__restrict__ part, int64_t c, bool live,
                               float T, int slot, float *red) {
     constexpr int NL = D + 1;
     const int64_t N = m.n_cells;
@@ -280,7 +283,7 @@ __device__ void bc_mean_block(const FvmMeshDev &m, const FvmPhys &p, const FvmWo
             if (threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
             __syncthreads();
         }
-        if (threadIdx.x == 0) w.bc_part[slot * kMaxGroups + q] = red[0];
+        if (threadIdx.x == 0) part[slot * kMaxGroups + q] = red[0];
         __syncthreads();
     }
 }
@@ -320,7 +323,7 @@ __global__ void __launch_bounds__(kThreads) k_bc_mean(FvmMeshDev m, FvmPhys p, F
     const bool live = c < m.n_cells;
     const float T = live ? prim[c * (D + 2) + D + 1] : 0.0f;
     __shared__ float red[kThreads];
-    bc_mean_block<D>(m, p, w, c, live, T, blockIdx.x, red);
+    bc_mean_block<D>(m, p, w.bc_part, c, live, T, blockIdx.x, red);
 }
 
 // ------------------------------------------------------------------------------------
@@ -811,7 +814,7 @@ __device__ __forceinline__ void stage_epilogue(const FvmMeshDev &m, const FvmPhy
     // boundary means of the next stage's primitives (block-uniform branch)
     if (m.n_groups > 0) {
         const int slot = m.blk_slot[tile];
-        if (slot >= 0) bc_mean_block<D>(m, p, w, c, live, Pn[D + 1], slot, red);
+        if (slot >= 0) bc_mean_block<D>(m, p, w.bc_part_next, c, live, Pn[D + 1], slot, red);
     }
     if (st.last && st.acc2) {
         // fp64 block partials of the three column norms (Adaptive.update_stepsize)
@@ -1414,7 +1417,12 @@ static int check_align(const FvmMeshDev *m, const FvmWork *w, const FvmStage *st
     return FVM_OK;
 }
 
-static int check_stage(const FvmWork *w, const FvmStage *st, const char *who, bool m_has_tiles = true) {
+static int check_stage(const FvmMeshDev *m, const FvmWork *w, const FvmStage *st, const char *who,
+                       bool m_has_tiles = true) {
+    if (m->n_groups > 0 && (!w->bc_part_next || w->bc_part_next == w->bc_part)) {
+        set_error("%s: a stage needs work->bc_part_next as a buffer separate from work->bc_part", who);
+        return FVM_ERR_ARG;
+    }
     if (st && st->tile_active && !m_has_tiles) {
         set_error("%s: a tile mask needs the tile-fused stage (2-D mesh with tile tables)", who);
         return FVM_ERR_ARG;
@@ -1478,7 +1486,7 @@ int fvm_grad_f32(const FvmMeshDev *m, const FvmPhys *p, const FvmWork *w, const 
 
 int fvm_rk_stage_f32(const FvmMeshDev *m, const FvmPhys *p, const FvmWork *w, const FvmStage *st, void *stream) {
     int e = check_mesh(m, p, w, "fvm_rk_stage_f32");
-    if (!e) e = check_stage(w, st, "fvm_rk_stage_f32", m->fz_tile != nullptr);
+    if (!e) e = check_stage(m, w, st, "fvm_rk_stage_f32", m->fz_tile != nullptr);
     if (!e) e = check_align(m, w, st, nullptr, st->prim, "fvm_rk_stage_f32");
     if (!e && m->fz_tile) e = check_fused(*m, *w, "fvm_rk_stage_f32");
     if (e) return e;
@@ -1495,7 +1503,7 @@ int fvm_rk_stage_part_f32(const FvmMeshDev *m, const FvmPhys *p, const FvmWork *
         return FVM_ERR_ARG;
     }
     int e = check_mesh(m, p, w, "fvm_rk_stage_part_f32");
-    if (!e) e = check_stage(w, st, "fvm_rk_stage_part_f32", m->fz_tile != nullptr && !(parts & 14));
+    if (!e) e = check_stage(m, w, st, "fvm_rk_stage_part_f32", m->fz_tile != nullptr && !(parts & 14));
     if (!e) e = check_align(m, w, st, nullptr, st->prim, "fvm_rk_stage_part_f32");
     if (!e && (parts & 16)) e = check_fused(*m, *w, "fvm_rk_stage_part_f32");
     if (e) return e;

@@ -423,17 +423,32 @@ class FacetCalc:
                        for _ in range(2 if dm.tiles is not None else 1)]
         self._latest = 0             # buffer holding the last call's records
         self._gfull = None
-        self.bc_part = torch.zeros(4 * max(self.mesh.n_slots, 1), dtype=torch.float32, device=dev)
+        # boundary-mean partials: a stage reads one buffer (the sums of its input) and its
+        # epilogue writes the other (the sums of the next stage's input); they swap after
+        # every stage
+        self._bcp = [torch.zeros(4 * max(self.mesh.n_slots, 1), dtype=torch.float32, device=dev) for _ in range(2)]
+        self._bcp_cur = 0
         self.red = torch.zeros(self.nblk * 3 * C, dtype=torch.float64, device=dev)
         self.fflux = torch.zeros((self.n_facets, C), dtype=torch.float32, device=dev)
         self.set_fused(fused)
-        self.work = nat.FvmWork(ubc=self.ubc.data_ptr(), gp=self._gpbuf[0].data_ptr(), bc_part=self.bc_part.data_ptr(),
-                                red=self.red.data_ptr(), fflux=self.fflux.data_ptr(), gp_prev=None)
+        self.work = nat.FvmWork(ubc=self.ubc.data_ptr(), gp=self._gpbuf[0].data_ptr(), bc_part=self._bcp[0].data_ptr(),
+                                red=self.red.data_ptr(), fflux=self.fflux.data_ptr(), gp_prev=None,
+                                bc_part_next=self._bcp[1].data_ptr())
         self.has_prev = False
         self._last_has_prev = False
         self._last_fused = False
         self.bc_ready_ptr = None     # prim buffer whose boundary sums bc_part holds
         self.last_prim = None        # primitives of the last evaluation, if known (saving)
+
+    @property
+    def bc_part(self):
+        """Boundary-mean partials of the next stage's input (read by its gradient phase)."""
+        return self._bcp[self._bcp_cur]
+
+    def _swap_bc_part(self):
+        self._bcp_cur = 1 - self._bcp_cur
+        self.work.bc_part = self._bcp[self._bcp_cur].data_ptr()
+        self.work.bc_part_next = self._bcp[1 - self._bcp_cur].data_ptr()
 
     @property
     def gp(self):
@@ -485,7 +500,8 @@ class FacetCalc:
             self._gfull = torch.empty_like(self._gpbuf[0])
         w = nat.FvmWork(ubc=self.ubc.data_ptr(), gp=self._gfull.data_ptr(), bc_part=self.bc_part.data_ptr(),
                         red=self.red.data_ptr(), fflux=self.fflux.data_ptr(),
-                        gp_prev=self._gpbuf[1 - self._latest].data_ptr())
+                        gp_prev=self._gpbuf[1 - self._latest].data_ptr(),
+                        bc_part_next=self._bcp[1 - self._bcp_cur].data_ptr())
         rc = nat.lib().fvm_grad_f32(self.mesh_struct, self.phys, w, nat.ptr(self.last_prim.contiguous()),
                                     int(self._last_has_prev), nat.stream_ptr(self.device))
         nat.check(rc, "fvm_grad_f32")
@@ -535,6 +551,8 @@ class FacetCalc:
             self._after_grad()
             self.has_prev = True
         if parts is None or parts & (8 | 16):
+            # the epilogue wrote the next stage's partials into bc_part_next
+            self._swap_bc_part()
             self.bc_ready_ptr = st.prim_next
 
     def invalidate_boundary_means(self):

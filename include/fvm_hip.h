@@ -174,6 +174,11 @@ typedef struct FvmWork {      /* device scratch owned by the caller             
                          a second buffer for the tile-fused stage, which writes the
                          records of boundary cells only, into gp (callers swap the two
                          buffers between calls)                                        */
+    float *bc_part_next; /* [n_slots][4] where a stage epilogue writes the partial sums
+                         of c of the next stage's primitives (required by the stage
+                         entry points when n_groups > 0, separate from bc_part: tiles
+                         of one launch still read bc_part; callers swap the two after
+                         each stage)                                                   */
 } FvmWork;
 
 /* One RHS evaluation dU/dt = FVMEquation.forward(prim) (fvm_equation.py:406-435).
@@ -201,7 +206,8 @@ typedef struct FvmStage {
     const float *dt;       /* device time step; [n_meshes] for a batch                */
     int32_t stage, last, store_k, has_prev;
     int32_t bc_ready;      /* work->bc_part already holds the boundary sums of prim
-                              (the previous stage's epilogue wrote them)              */
+                              (the previous stage's epilogue wrote them, into what
+                              was then work->bc_part_next)                            */
     float b, b2;
     int32_t n_inc;
     int32_t inc_idx[10];

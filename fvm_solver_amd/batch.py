@@ -161,12 +161,14 @@ class BatchedSolver:
         return g
 
 
-def random_channel_batch(n_meshes: int, seed: int = 0, n_min: int = 5000, n_max: int = 200_000, device="cuda"):
+def random_channel_batch(n_meshes: int, seed: int = 0, n_min: int = 5000, n_max: int = 200_000, device="cuda",
+                         progress=None):
     """Dataset-generation batch: per mesh, cell count log-uniform in [n_min, n_max],
     obstacle centre x ~ U(0.3, 0.8) and radius ~ U(0.05, 0.15), freestream Mach
     ~ U(0.1, 0.6); state = freestream + Gaussian density bump.  Returns the problems
     list BatchedSolver takes and the per-mesh parameters (with the generated mesh
-    tuple, for checks against other implementations)."""
+    tuple, for checks against other implementations).  progress(i, n_cells), if given,
+    is called after each mesh."""
     from fvm_solver_amd.config_fvm import channel_config
     from fvm_solver_amd.mesh_gen.channel import ChannelSpec, channel_mesh_2d
     from fvm_solver_amd.mesh_utils.fvm_mesh import mesh_from_tuple
@@ -188,4 +190,6 @@ def random_channel_batch(n_meshes: int, seed: int = 0, n_min: int = 5000, n_max:
         problems.append((mesh, channel_bc_tags(tags, 2), cfg, torch.from_numpy(U).float()))
         params.append(dict(cells=mesh.n_cells, centre_x=cx, radius=r, mach=mach,
                            mesh_tuple=(pts, cells, fi, fb, tags)))
+        if progress is not None:
+            progress(i, mesh.n_cells)
     return problems, params

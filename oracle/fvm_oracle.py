@@ -18,9 +18,10 @@ Pinning (see tests/test_oracle_*.py):
     sum_f s_f n_f = 0 per cell, global conservation (sum vol*dU/dt equals minus the
     boundary flux), free-stream preservation, the RK4 stability polynomial, and the
     limiter's special cases.
-  * ``adams3`` and ``ab3`` (no runnable reference: the reference's Adams3PC raises
-    TypeError, integrators.py:111) are pinned by their order of accuracy on y'=λy
-    and by exactness on polynomials; ``cfl_dt`` and ``conservation_totals`` (no
+  * ``Adams3_PC``, ``Adams4_PC``, ``Euler`` and ``AB3`` (no runnable reference: the
+    reference's Adams and Euler solvers raise TypeError, integrators.py:111,199,404) are
+    pinned by the closed-form solution of their linear recurrences on y' = λy (power
+    sums over the roots of the characteristic polynomial) and by their order; ``cfl_dt`` and ``conservation_totals`` (no
     reference counterpart) are pinned by closed forms.
 """
 from __future__ import annotations
@@ -793,6 +794,8 @@ class OracleStepper:
             self.ad = dict(order=4, atol=2e-3, rtol=2e-3, mtol=1e-6, alphas=torch.tensor((0.8, 0.995)).to(dt_))
         if name == "Adams3_PC":
             self.ad = dict(order=4, atol=2e-3, rtol=2e-3, mtol=1e-7, alphas=torch.tensor((0.9, 0.99)).to(dt_))
+        if name == "Adams4_PC":
+            self.ad = dict(order=4, atol=1e-3, rtol=1e-3, mtol=1e-6, alphas=torch.tensor((0.9, 0.99)).to(dt_))
         self.hist = None
 
     def rhs(self, U):
@@ -853,6 +856,24 @@ class OracleStepper:
             self.hist = (f0, fm1)
             self._update_stepsize(dU_high, dU_low, U0)
             return U0 + dU_high
+        if self.name == "Adams4_PC":
+            # Adams4PC._step (integrators.py:157-210): the same predictor as Adams3PC, a
+            # four-term corrector and the three-term corrector as the low-order estimate
+            if self.hist is None:
+                f0 = self.rhs(U0)
+                self.hist = (f0, f0)
+            f0 = self.rhs(U0)
+            fm1, fm2 = self.hist
+            Ua = U0 + dt / 36 * (53 * f0 - 22 * fm1 + 6 * fm2)
+            fa = self.rhs(Ua)
+            dU_high = dt / 24 * (9 * fa + 19 * f0 - 5 * fm1 + fm2)
+            dU_low = dt / 12 * (5 * fa + 8 * f0 - fm1)
+            self.hist = (f0, fm1)
+            self._update_stepsize(dU_high, dU_low, U0)
+            return U0 + dU_high
+        if self.name == "Euler":
+            # Euler._step (integrators.py:396-407): U + dt f(U)
+            return U0 + dt * self.rhs(U0)
         if self.name == "AB3":
             # Build-defined fixed-dt Adams-Bashforth 3, one RHS per step; the history
             # starts as (f0, f0), so step 1 is forward Euler and step 2 is AB2-consistent.

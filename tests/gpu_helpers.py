@@ -1,5 +1,8 @@
 """Shared set-up for the GPU parity tests: the same mesh and state through the product
 (HIP, via the C ABI) and through the CPU oracle."""
+import os
+import time
+
 import numpy as np
 import torch
 
@@ -10,6 +13,22 @@ from fvm_solver_amd.mesh_utils.mesh_store import channel_bc_tags
 from oracle import fvm_oracle as O
 
 DEV = "cuda:0"
+
+_PROGRESS = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "gpurun_out",
+                         "gpu_test_progress.log")
+
+
+def announce(msg):
+    """Print a progress line and append it to gpurun_out/gpu_test_progress.log: pytest
+    holds a test's output until the test ends, and a long GPU test must show progress
+    while it runs."""
+    print(f"[gpu-test] {msg}", flush=True)
+    try:
+        os.makedirs(os.path.dirname(_PROGRESS), exist_ok=True)
+        with open(_PROGRESS, "a") as f:
+            f.write(f"{time.strftime('%H:%M:%S')} {msg}\n")
+    except OSError:
+        pass
 
 
 def oracle_phys(cfg):
@@ -55,3 +74,69 @@ def rel_l2_cols(a, b):
     a = np.asarray(a, np.float64)
     b = np.asarray(b, np.float64)
     return np.linalg.norm(a - b, axis=0) / np.maximum(np.linalg.norm(b, axis=0), 1e-300)
+
+
+class FluxLedger:
+    """Conservation bookkeeping for long runs, per mesh of a device layout and conserved
+    variable: the fp64 totals sum_c vol_c U_c, and the trapezoid integral over the steps
+    of their rate sum_c vol_c f_c(U) -- interior facet fluxes cancel in that sum, so it is
+    the boundary-flux integral -- next to the gross flux scale sum_f |F_f| (what the
+    cancellation has to cancel).  f and F come from a separate three-kernel evaluator on
+    the same device layout (it keeps its own carried gradients), so the solver is
+    untouched.  A step with an interior flux that does not cancel, or a divergence
+    weight off, shows as |total change - integral| of the order of the gross scale."""
+
+    def __init__(self, dm, cfg):
+        from fvm_solver_amd import _native as nat
+        from fvm_solver_amd.fvm_stepping.facet_process import FacetCalc
+        self.nat = nat
+        self.E = FacetCalc.from_device_mesh(dm, cfg, DEV, fused=False)
+        self.dm = dm
+        self.C = dm.C
+        self.M = dm.n_meshes
+        self.vol = dm.volumes64[:, None]
+        # cells of a mesh are contiguous, and so are its facets in the device order
+        # (sorted by lowest adjacent cell): per-mesh sums are sums over ranges
+        self.coff = [int(v) for v in dm.cell_offsets]
+        fc = dm.host["fac_cell"][:, dm.host["fperm"]]
+        lo = fc.min(0)
+        self.foff = [int(v) for v in np.searchsorted(lo, np.asarray(dm.cell_offsets))]
+        self.inv_cv = 1.0 / cfg.C_v
+        self.prim = torch.empty(dm.N, dm.C, device=DEV)
+        self.integral = torch.zeros(self.M, self.C, dtype=torch.float64, device=DEV)
+        self.gross = torch.zeros_like(self.integral)
+        self.last = None
+        self.start = None
+
+    @staticmethod
+    def _segsum(x, off):
+        return torch.stack([x[off[i]:off[i + 1]].sum(0) for i in range(len(off) - 1)])
+
+    def totals(self, U):
+        return self._segsum(self.vol * U.double(), self.coff)
+
+    def _rates(self, U):
+        nat = self.nat
+        nat.check(nat.lib().fvm_state_to_prim_f32(nat.ptr(U), nat.ptr(self.prim), self.dm.N, self.dm.dim,
+                                                  self.inv_cv, nat.stream_ptr(torch.device(DEV))),
+                  "fvm_state_to_prim_f32")
+        f = self.E.rhs(self.prim)
+        return self._segsum(self.vol * f.double(), self.coff), self._segsum(self.E.fflux.abs().double(), self.foff)
+
+    def sample(self, U, dt=None):
+        """Record state U (contiguous [N, C] fp32); dt [M] (host floats): the time since
+        the previous sample, per mesh."""
+        U = U.contiguous()
+        r = self._rates(U)
+        if self.last is None:
+            self.start = self.totals(U)
+        else:
+            w = torch.as_tensor(np.asarray(dt, np.float64), device=DEV)[:, None] * 0.5
+            self.integral += w * (self.last[0] + r[0])
+            self.gross += w * (self.last[1] + r[1])
+        self.last = r
+
+    def report(self, U):
+        """(total change, boundary-flux integral, gross flux integral) as [M, C] numpy."""
+        d = (self.totals(U.contiguous()) - self.start).cpu().numpy()
+        return d, self.integral.cpu().numpy(), self.gross.cpu().numpy()

@@ -192,6 +192,88 @@ def test_integrator_order(name, extra, order):
     assert abs(rate - order) < 0.4
 
 
+def _recurrence(name, z):
+    """Coefficients (a0, a1, a2) of y_{n+1} = a0 y_n + a1 y_{n-1} + a2 y_{n-2} that each
+    fixed-step scheme reduces to on y' = lam y (z = lam dt), expanded by hand from the
+    scheme's stated weights: predictor U_a = y + dt/36 (53 f0 - 22 f-1 + 6 f-2) and
+    correctors dt/12 (5 f_a + 8 f0 - f-1) (Adams3PC, integrators.py:126-141),
+    dt/24 (9 f_a + 19 f0 - 5 f-1 + f-2) (Adams4PC, :186-196); AB3 dt/12 (23, -16, 5);
+    Euler 1 + z (:396-407)."""
+    p0, p1, p2 = 1 + 53 * z / 36, -22 * z / 36, 6 * z / 36          # U_a in y_n, y_n-1, y_n-2
+    if name == "Adams3_PC":
+        return 1 + 8 * z / 12 + 5 * z / 12 * p0, -z / 12 + 5 * z / 12 * p1, 5 * z / 12 * p2
+    if name == "Adams4_PC":
+        return 1 + 19 * z / 24 + 9 * z / 24 * p0, -5 * z / 24 + 9 * z / 24 * p1, z / 24 + 9 * z / 24 * p2
+    if name == "AB3":
+        return 1 + 23 * z / 12, -16 * z / 12, 5 * z / 12
+    if name == "Euler":
+        return 1 + z, 0.0, 0.0
+    raise ValueError(name)
+
+
+@pytest.mark.parametrize("name", ["Adams3_PC", "Adams4_PC", "AB3", "Euler"])
+@pytest.mark.parametrize("lam", [-3.0, 0.7])
+def test_multistep_closed_form(name, lam):
+    """30 fixed-dt steps of the oracle on y' = lam y from given y_-2, y_-1, y_0 equal the
+    closed-form solution y_n = sum_i c_i r_i^n over the roots r_i of the scheme's
+    characteristic polynomial r^3 - a0 r^2 - a1 r - a2 (a different computation from
+    stepping: eigenvalues and a Vandermonde solve).  A wrong predictor or corrector
+    weight, or a swapped history (f-1 / f-2), changes a root; the Adams schemes' step
+    size is held fixed (alpha = 1 in both branches of the update)."""
+    dt, n = 0.05, 30
+    z = lam * dt
+    y_init = np.array([0.9, 1.1, 1.0])            # y_-2, y_-1, y_0
+    a0, a1, a2 = _recurrence(name, z)
+    r = np.roots([1.0, -a0, -a1, -a2]).astype(complex)
+    if name == "Euler":
+        want = y_init[2] * (1 + z) ** n
+    else:
+        V = np.stack([r ** -2, r ** -1, r ** 0])
+        coef = np.linalg.solve(V, y_init.astype(complex))
+        want = float(np.real(np.sum(coef * r ** n)))
+    st = _stepper(name, "", lam, dt)
+    st.dt = torch.tensor(dt, dtype=torch.float64)
+    if name in ("Adams3_PC", "Adams4_PC"):
+        st.ad["alphas"] = torch.tensor((1.0, 1.0), dtype=torch.float64)
+    if name != "Euler":
+        st.hist = (torch.full((1, 1), lam * y_init[1], dtype=torch.float64),
+                   torch.full((1, 1), lam * y_init[0], dtype=torch.float64))
+    y = torch.full((1, 1), y_init[2], dtype=torch.float64)
+    for _ in range(n):
+        y = st.step(y)
+    assert abs(float(y) - want) <= 1e-11 * max(1.0, abs(want)), (float(y), want)
+    assert float(st.dt) == dt
+
+
+def test_adams_pc_step_size_update():
+    """The Adams schemes' adaptive step: after one step dt becomes
+    dt (alpha + (1 - alpha) 0.9 E^(-1/4)) with E from the high/low corrector difference
+    (Adaptive.update_stepsize, integrators.py:42-62), checked on a two-column linear
+    system against the step's own dU_high / dU_low expanded by hand."""
+    lam = torch.tensor([[-2.0, 0.5]], dtype=torch.float64)
+    for name, (atol, rtol, mtol, a_gt, a_le) in (("Adams3_PC", (2e-3, 2e-3, 1e-7, 0.9, 0.99)),
+                                                  ("Adams4_PC", (1e-3, 1e-3, 1e-6, 0.9, 0.99))):
+        dt = 0.1
+        st = _stepper(name, "", lam, dt)
+        st.dt = torch.tensor(dt, dtype=torch.float64)
+        y0 = torch.tensor([[1.0, 2.0], [0.5, -1.0], [3.0, 0.25]], dtype=torch.float64)
+        y1 = st.step(y0)
+        f0 = lam * y0
+        z = lam * dt
+        ya = y0 + z / 36 * (53 - 22 + 6) * y0          # history (f0, f0) on the first step
+        if name == "Adams3_PC":
+            hi, lo = dt / 12 * (5 * lam * ya + 7 * f0), dt / 2 * (lam * ya + f0)
+        else:
+            hi, lo = dt / 24 * (9 * lam * ya + 15 * f0), dt / 12 * (5 * lam * ya + 7 * f0)
+        assert torch.allclose(y1, y0 + hi, rtol=1e-13, atol=1e-15)
+        E = float((torch.linalg.norm(hi - lo, dim=0) / (atol + rtol * torch.linalg.norm(hi, dim=0)
+                                                        + mtol * torch.linalg.norm(y0, dim=0))).mean())
+        # the reference keeps the alphas in a default (float32) tensor
+        alpha = float(np.float32(a_gt if E > 1 else a_le))
+        want = dt * (alpha + (1 - alpha) * 0.9 * E ** -0.25)
+        assert abs(float(st.dt) - want) <= 1e-12 * want, (name, float(st.dt), want)
+
+
 def test_adams3_pc_consistency():
     class Const(_Linear):
         def forward(self, prim):

@@ -78,6 +78,7 @@ _PROTOS = {
                                            c_void_p, c_void_p]),
     "fvm_state_to_prim_f32": (ctypes.c_int, [c_void_p, c_void_p, i64, i32, f32, c_void_p]),
     "fvm_conservation_f64": (ctypes.c_int, [c_void_p, c_void_p, i64, i32, c_void_p, c_void_p, c_void_p]),
+    "fvm_conservation_f64_noalloc": (ctypes.c_int, [c_void_p, c_void_p, i64, i32, c_void_p, c_void_p]),
     "fvm_cfl_dt": (ctypes.c_int, [c_void_p, c_void_p, i64, i32, f64, f64, c_void_p, c_void_p, c_void_p]),
     "fvm_geom_facets_f64": (ctypes.c_int, [c_void_p, i32, c_void_p, i64, c_void_p, c_void_p, c_void_p, c_void_p]),
     "fvm_geom_cells_f64": (ctypes.c_int, [c_void_p, i32, c_void_p, i64, c_void_p, c_void_p, c_void_p]),

@@ -3,10 +3,45 @@
 // per-block partials, and one block combines them in block order.
 #include "fvm_common.hpp"
 
+#include <mutex>
+
 namespace fvm {
 
 constexpr int kRedBlocks = 256;
 constexpr int kRedThreads = 256;
+
+// Scratch of the allocation-free conservation entry point: the per-block partials
+// [kRedBlocks][C] for C <= kScratchMaxC (16 KiB).  Every GPU the library runs on holds
+// its own copy for the life of the process.  Host code never passes the array's name to
+// a kernel (in host code it is a stand-in, not a device address): it passes the address
+// hipGetSymbolAddress returned for the current device.
+constexpr int kScratchMaxC = 8;
+static __device__ __align__(256) double scratch[kRedBlocks * kScratchMaxC];
+
+constexpr int kMaxDevices = 64;
+static double *g_scratch_ptr[kMaxDevices] = {};
+static std::mutex g_scratch_mu;
+
+// The current device's scratch address, looked up once per device (with that device
+// current) on the first call that needs it; nullptr (error set) on failure.
+static double *device_scratch(const char *who) {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+        set_error("%s: no current HIP device (or device id beyond %d)", who, kMaxDevices - 1);
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) {
+            set_error("%s: hipGetSymbolAddress(scratch) failed on device %d: %s", who, dev, hipGetErrorString(e));
+            return nullptr;
+        }
+        g_scratch_ptr[dev] = static_cast<double *>(ptr);
+    }
+    return g_scratch_ptr[dev];
+}
 
 __global__ void __launch_bounds__(kRedThreads)
 k_cons_partial(const float *__restrict__ U, const double *__restrict__ vol, int64_t N, int C, double *__restrict__ part) {
@@ -81,6 +116,25 @@ int fvm_conservation_f64(const float *state, const double *vol, int64_t N, int32
     hipLaunchKernelGGL(k_cons_partial, dim3(kRedBlocks), dim3(kRedThreads), 0, s, state, vol, N, C, partial);
     hipLaunchKernelGGL(k_cons_final, dim3(1), dim3(64), 0, s, partial, kRedBlocks, C, out);
     return check_launch("fvm_conservation_f64");
+}
+
+int fvm_conservation_f64_noalloc(const float *state, const double *vol, int64_t N, int32_t C, double *out,
+                                 void *stream) {
+    if (!state || !vol || !out || N < 1 || C < 1) {
+        set_error("fvm_conservation_f64_noalloc: invalid arguments");
+        return FVM_ERR_ARG;
+    }
+    if (C > kScratchMaxC) {
+        set_error("fvm_conservation_f64_noalloc: C = %d exceeds the scratch limit of %d variables", C, kScratchMaxC);
+        return FVM_ERR_ARG;
+    }
+    double *part = device_scratch("fvm_conservation_f64_noalloc");
+    if (!part) return FVM_ERR_HIP;
+    // every block overwrites all C of its partials, so nothing needs clearing
+    hipStream_t s = as_stream(stream);
+    hipLaunchKernelGGL(k_cons_partial, dim3(kRedBlocks), dim3(kRedThreads), 0, s, state, vol, N, C, part);
+    hipLaunchKernelGGL(k_cons_final, dim3(1), dim3(64), 0, s, part, kRedBlocks, C, out);
+    return check_launch("fvm_conservation_f64_noalloc");
 }
 
 int fvm_cfl_dt(const float *prim, const double *h, int64_t N, int32_t dim, double gR, double cfl, double *out,

@@ -262,6 +262,15 @@ int fvm_state_to_prim_f32(const float *state, float *prim, int64_t N, int32_t di
 int fvm_conservation_f64(const float *state, const double *vol, int64_t N, int32_t C, double *out,
                          double *partial /* [256*C] */, void *stream);
 
+/* fvm_conservation_f64 without the caller's workspace: the per-block partials live in
+ * the library's own device array (16 KiB per GPU, held until the process ends), so the
+ * call allocates nothing -- for callers that cannot allocate device memory (code under
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  No counterpart
+ * in the reference.  C <= 8, else FVM_ERR_ARG.  Every call on one GPU shares that GPU's
+ * array: calls that overlap on one GPU (other streams, other host threads) race. */
+int fvm_conservation_f64_noalloc(const float *state, const double *vol, int64_t N, int32_t C, double *out,
+                                 void *stream);
+
 /* CFL time step (no reference counterpart; the reference takes a fixed cfg.dt):
  * dt = cfl * min_c h_c / (|V_c| + c_c), h_c = d vol_c / sum_f |n_f|.  out (device fp64). */
 int fvm_cfl_dt(const float *prim, const double *h, int64_t N, int32_t dim, double gR, double cfl,

@@ -69,6 +69,9 @@ def _parse():
     g.add_argument("--n-max", type=int, default=200_000)
     g.add_argument("--oracle-meshes", type=int, default=2, help="evaluation meshes re-run by the FP64 CPU oracle")
     g.add_argument("--tol", type=float, default=1e-3)
+    ap.add_argument("--cpu-config1", action="store_true",
+                    help="time only the CPU baseline of the 20k-cell 2-D channel, the full 1000 RK4 steps (the fp32 "
+                         "torch-CPU restatement of the reference), on the usable physical cores and on one thread")
     g.add_argument("--corrector-timing", action="store_true",
                    help="time the pieces of one corrector window on the evaluation batch (untrained network: "
                         "the kernels' cost does not depend on the weights) against a plain window, then exit")
@@ -77,6 +80,27 @@ def _parse():
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def host_cpu():
+    """CPU model, physical cores (sockets x cores per socket, from lscpu) and the CPUs
+    this process may run on."""
+    import subprocess
+    info = {"model": None, "physical_cores": None, "logical_cpus": os.cpu_count()}
+    try:
+        txt = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        kv = dict(line.split(":", 1) for line in txt.splitlines() if ":" in line)
+        kv = {k.strip(): v.strip() for k, v in kv.items()}
+        info["model"] = kv.get("Model name")
+        if kv.get("Core(s) per socket") and kv.get("Socket(s)"):
+            info["physical_cores"] = int(kv["Core(s) per socket"]) * int(kv["Socket(s)"])
+    except (OSError, ValueError, subprocess.SubprocessError):
+        pass
+    try:
+        info["affinity_cpus"] = len(os.sched_getaffinity(0))
+    except AttributeError:
+        info["affinity_cpus"] = os.cpu_count()
+    return info
 
 
 def facet_kernel_bytes(N, F, Fb, D):
@@ -314,27 +338,22 @@ def main():
         torch.cuda.synchronize()
         log("calibration copy: 256 MiB read + 256 MiB written")
 
-    # ---- per-kernel durations: each stage kernel launched back to back between events ----
+    # ---- kernel durations ----
+    # The step of the tile-fused pipeline is exactly `stages` launches of k_fused (the
+    # boundary means of each stage come from the previous stage's epilogue), so the
+    # average launch duration is the timed region divided by its launches: the figure the
+    # roofline uses, and one a kernel trace of the same command must agree with (the
+    # trace excludes the few-us dispatch gaps between launches, so it reads slightly less).
     E = eq.E_props
     parts = {16: "k_fused"} if E.fused else {2: "k_grad", 4: "k_facet_flux", 8: "k_div"}
     n_tiles = (N + 255) // 256
     s = ts.stages
     div_bytes = [div_kernel_bytes(N, F, D, i) for i in range(s)]
-    fused_bytes = [fused_kernel_bytes(N, F, Fb, D, i, n_tiles) for i in range(s)]
-    if ab3:
-        # one fused stage per step (the boundary means come from the previous epilogue):
-        # the step is the kernel launch
-        name = "k_fused" if E.fused else "stage"
-        fused_bytes = [fused_kernel_bytes(N, F, Fb, D, 0, n_tiles, ab3=True)]
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        for _ in range(4):                 # untimed: the queue is full at the first event
-            ts._solve_step(None)
-        e0.record()
-        for _ in range(a.kernel_timing_steps):
-            ts._solve_step(None)
-        e1.record()
-        torch.cuda.synchronize()
-        avg_us = {name: 1e3 * e0.elapsed_time(e1) / a.kernel_timing_steps}
+    fused_bytes = [fused_kernel_bytes(N, F, Fb, D, i, n_tiles, ab3=ab3) for i in range(s)]
+    launches = a.steps * s
+    step_us = 1e3 * gpu_ms / launches          # average duration of one stage launch (fused)
+    if E.fused:
+        avg_us = {"k_fused": step_us}
     else:
         U = eq.cells.state
         cur = ts._stage_prim0(U)
@@ -354,18 +373,24 @@ def main():
             stages.append(st)
             flip = 1 - flip
             cur = nxt
+        # three kernels per stage: per-kernel durations from back-to-back launches of each
+        # (informational); the roofline below is the stage's, from the timed region
         avg_us = block_kernel_us(E, stages, parts, a.kernel_timing_steps)
-    log("kernel averages (us, back-to-back launches): " + ", ".join(f"{k}={v:.1f}" for k, v in avg_us.items()))
+    log("kernel averages (us): " + ", ".join(f"{k}={v:.1f}" for k, v in avg_us.items()))
     kbytes = {"k_grad": float(grad_kernel_bytes(N, Fb, D)), "k_facet_flux": float(facet_kernel_bytes(N, F, Fb, D)),
-              "k_div": float(np.mean(div_bytes)) if div_bytes else 0.0, "k_fused": float(np.mean(fused_bytes)),
-              "stage": float(np.mean(fused_bytes)) if ab3 else 0.0}
+              "k_div": float(np.mean(div_bytes)) if div_bytes else 0.0, "k_fused": float(np.mean(fused_bytes))}
     kgbs = {k: kbytes[k] / (avg_us[k] * 1e-6) / 1e9 for k in avg_us}
-    dom = max(avg_us, key=avg_us.get)
+    # single-pass roofline of the step: a stage's compulsory bytes (every array it needs
+    # read or written once, as the tile-fused stage does) over the measured time of one
+    # stage in the timed region; the three-kernel pipeline moves more (the gradient
+    # records and facet fluxes make a round trip through HBM), which this figure charges
+    # to it
+    stage_bytes = kbytes["k_fused"]
+    dom = "k_fused" if E.fused else max(avg_us, key=avg_us.get)
     dom_bytes = kbytes[dom]
-    achieved = kgbs[dom]
-    stage_bytes = kbytes["k_fused"] if (E.fused or ab3) else kbytes["k_grad"] + kbytes["k_facet_flux"] + kbytes["k_div"]
-    log(f"dominant {dom}: {dom_bytes / 1e6:.1f} MB/launch -> {achieved:.0f} GB/s; per kernel GB/s: "
-        + ", ".join(f"{k}={v:.0f}" for k, v in kgbs.items()) + f"; {stage_bytes / N:.0f} B/cell/stage")
+    achieved = stage_bytes / (step_us * 1e-6) / 1e9
+    log(f"stage: {stage_bytes / 1e6:.1f} MB algorithmic ({stage_bytes / N:.0f} B/cell) in {step_us:.1f} us -> "
+        f"{achieved:.0f} GB/s; per kernel GB/s: " + ", ".join(f"{k}={v:.0f}" for k, v in kgbs.items()))
 
     out = {
         "metric": f"cell-updates/s (fused HIP {a.integrator} step, {D}-D NS channel)",
@@ -385,16 +410,26 @@ def main():
                    "cells": N, "facets": F, "boundary_facets": Fb,
                    "integrator": "AB3 (fixed dt)" if ab3 else "Butcher RK4",
                    "dt": dt, "finite": finite, "pipeline": "tile-fused stage" if E.fused else "three kernels",
-                   "kernel_avg_us": avg_us, "kernel_timing": "one event pair around back-to-back launches of each kernel",
-                   "kernel_algorithmic_bytes": {k: kbytes[k] for k in avg_us}, "kernel_gbs": kgbs},
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+                   "kernel_avg_us": avg_us,
+                   "kernel_timing": ("timed region / launches (a step is `stages` k_fused launches)" if E.fused else
+                                     "one event pair around back-to-back launches of each kernel"),
+                   "kernel_algorithmic_bytes": {k: kbytes[k] for k in avg_us}, "kernel_gbs": kgbs,
+                   "stage_us": step_us, "stage_algorithmic_bytes": stage_bytes},
+        "roofline": {"bound": "hbm", "kernel": dom if E.fused else "stage (k_grad + k_facet_flux + k_div)",
+                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "timing": "timed region: ms_per_step / stages per step",
+                     "bytes_per_launch": stage_bytes},
     }
+    if D == 3:
+        hc = eq.E_props.mesh.h64
+        out["config"]["cell_size"] = {"min_h": float(hc.min()), "mean_h": float(hc.mean()),
+                                      "min_over_mean": float(hc.min() / hc.mean())}
     out["config"]["persist_blocks"] = tuned["persist_blocks"] if tuned else 0
     if tuned:
         out["config"]["autotune_us_per_step"] = tuned["us_per_step"]
     out["config"]["algorithmic_bytes_per_launch"] = dom_bytes
-    if a.traffic_file and os.path.exists(a.traffic_file) and not ab3:
+    if a.traffic_file and os.path.exists(a.traffic_file) and not ab3 and E.fused:
         tr = json.load(open(a.traffic_file))
         k = tr.get("kernels", {}).get(dom)
         if k and tr.get("cells") == N:
@@ -414,29 +449,36 @@ def main():
                     S_const=cfg.S_const, gamma=cfg.gamma, C_v=cfg.C_v, v_factor=cfg.v_factor, lim_p=cfg.lim_p,
                     lim_K=cfg.lim_K, inlet=O.BCSpec(ins.v_n_inf, ins.v_t_inf, ins.rho_inf, ins.T_inf, ins.v_inf),
                     exit=O.BCSpec(ex.v_n_inf, ex.v_t_inf, ex.rho_inf, ex.T_inf, ex.v_inf))
-        threads = os.cpu_count() or 1
-        try:
-            threads = min(threads, len(os.sched_getaffinity(0)))
-        except AttributeError:
-            pass
-        torch.set_num_threads(threads)
+        cpu = host_cpu()
+        # the physical cores this process may use (the box grants a share of the host)
+        threads = min(x for x in (cpu["physical_cores"], cpu["affinity_cpus"], cpu["logical_cpus"]) if x)
         _, ofvm = O.from_mesh_tuple(pts, cells, fi, fb, tags, ph)
         if D == 3:
             a.cpu_steps = max(1, a.cpu_steps // 6)
-        stepper = O.OracleStepper(ofvm, "AB3" if ab3 else "Butcher", "" if ab3 else "RK4", dt=dt)
         if ab3:
             a.cpu_steps *= 4             # one RHS per step
-        Uc = U0.clone()
-        Uc = stepper.step(Uc)            # warm-up
-        t0 = time.perf_counter()
-        for _ in range(a.cpu_steps):
-            Uc = stepper.step(Uc)
-        cpu_s = time.perf_counter() - t0
-        out["cpu_baseline"] = {"value": N * a.cpu_steps / cpu_s, "unit": "cell-updates/s", "cores": threads,
+
+        def cpu_rate(n_threads, n_steps):
+            torch.set_num_threads(n_threads)
+            stepper = O.OracleStepper(ofvm, "AB3" if ab3 else "Butcher", "" if ab3 else "RK4", dt=dt)
+            Uc = stepper.step(U0.clone())            # warm-up
+            t0 = time.perf_counter()
+            for _ in range(n_steps):
+                Uc = stepper.step(Uc)
+            return N * n_steps / (time.perf_counter() - t0), time.perf_counter() - t0
+
+        rate, cpu_s = cpu_rate(threads, a.cpu_steps)
+        n1 = max(1, a.cpu_steps // (threads * 2))
+        rate1, cpu1_s = cpu_rate(1, n1)
+        out["cpu_baseline"] = {"value": rate, "unit": "cell-updates/s", "cores": threads,
                                "kind": "port",
-                               "sample": f"{a.cpu_steps} {a.integrator} steps of the same {N}-cell mesh (after 1 warm-up step), "
-                                         f"fp32 torch-CPU restatement of the reference, {cpu_s:.1f}s"}
-        log(f"cpu baseline: {cpu_s:.1f}s for {a.cpu_steps} steps on {threads} threads")
+                               "sample": f"{a.cpu_steps} {a.integrator} steps of the same {N}-cell mesh (after 1 warm-up "
+                                         f"step), fp32 torch-CPU restatement of the reference, {cpu_s:.1f}s on {threads} "
+                                         f"threads; single thread: {n1} steps, {cpu1_s:.1f}s",
+                               "single_thread_value": rate1, "cpu_model": cpu["model"],
+                               "host_physical_cores": cpu["physical_cores"], "affinity_cpus": cpu["affinity_cpus"]}
+        log(f"cpu baseline ({cpu['model']}, {cpu['physical_cores']} physical cores, {cpu['affinity_cpus']} usable): "
+            f"{cpu_s:.1f}s for {a.cpu_steps} steps on {threads} threads; 1 thread {cpu1_s:.1f}s for {n1} steps")
     print(json.dumps(out), flush=True)
 
 
@@ -481,9 +523,12 @@ def batch_main(a):
         torch.empty_like(src).copy_(src)
         torch.cuda.synchronize()
         log("calibration copy: 256 MiB read + 256 MiB written")
-    # per-kernel durations of the same stages, each kernel launched back to back
+    # kernel durations: a replayed step is `stages` k_fused launches, so the average launch
+    # is the timed region over its launches; the three-kernel pipeline's kernels are
+    # timed by back-to-back launches of each
     E = s.E
     parts = {16: "k_fused"} if E.fused else {2: "k_grad", 4: "k_facet_flux", 8: "k_div"}
+    step_us = 1e3 * gpu_ms / (steps * s.stages)
     s._ensure_prim()
     U0, out, cur, flip = s._U[s._cur], s._U[1 - s._cur], s._prim[s._flip], s._flip
     stages = []
@@ -497,13 +542,14 @@ def batch_main(a):
         stages.append(st)
         flip = 1 - flip
         cur = nxt
-    avg_us = block_kernel_us(E, stages, parts, 10)
+    avg_us = {"k_fused": step_us} if E.fused else block_kernel_us(E, stages, parts, 10)
     kbytes = {"k_grad": float(grad_kernel_bytes(N, Fb, 2)), "k_facet_flux": float(facet_kernel_bytes(N, F, Fb, 2)),
               "k_div": float(np.mean([div_kernel_bytes(N, F, 2, i) for i in range(4)])),
               "k_fused": float(np.mean([fused_kernel_bytes(N, F, Fb, 2, i, (N + 255) // 256) for i in range(4)]))}
     kgbs = {k: kbytes[k] / (avg_us[k] * 1e-6) / 1e9 for k in avg_us}
     dom = max(avg_us, key=avg_us.get)
-    log("kernel averages (us, back-to-back launches): " + ", ".join(f"{k}={v:.1f}" for k, v in avg_us.items()))
+    achieved = kbytes["k_fused"] / (step_us * 1e-6) / 1e9
+    log("kernel averages (us): " + ", ".join(f"{k}={v:.1f}" for k, v in avg_us.items()))
     out = {
         "metric": "cell-updates/s (batched 2-D NS channels, HIP-graph RK4)",
         "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": steps, "warmup": a.warmup,
@@ -513,9 +559,14 @@ def batch_main(a):
                                "x ~ U(0.3, 0.8), r ~ U(0.05, 0.15), Mach ~ U(0.1, 0.6)), RK4 at CFL 0.5",
                    "cells": N, "facets": F, "boundary_facets": Fb, "finite": finite, "graph_steps": n_graph,
                    "pipeline": "tile-fused stage" if E.fused else "three kernels",
-                   "kernel_avg_us": avg_us, "kernel_timing": "one event pair around back-to-back launches of each kernel", "kernel_gbs": kgbs},
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": kgbs[dom], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": kgbs[dom] / HBM_PEAK_GBS, "traffic": None},
+                   "kernel_avg_us": avg_us,
+                   "kernel_timing": ("timed region / launches (a step is 4 k_fused launches)" if E.fused else
+                                     "one event pair around back-to-back launches of each kernel"),
+                   "kernel_gbs": kgbs, "stage_us": step_us},
+        "roofline": {"bound": "hbm", "kernel": "k_fused" if E.fused else "stage (three kernels)",
+                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": None, "timing": "timed region: ms_per_step / stages per step",
+                     "bytes_per_launch": kbytes["k_fused"]},
     }
     print(json.dumps(out), flush=True)
 
@@ -759,9 +810,55 @@ def corrector_timing_main(a):
     print(json.dumps(out), flush=True)
 
 
+def cpu_config1_main(a):
+    """The 20k-cell 2-D channel, 1000 RK4 steps at CFL 0.5, on the CPU restatement of
+    the reference (no GPU is used): the reference CPU path's own configuration."""
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from oracle import fvm_oracle as O
+    from fvm_solver_amd.config_fvm import ConfigChannel
+    from fvm_solver_amd.mesh_gen.channel import channel_mesh_2d
+    n_cells = a.cells or 20_000
+    pts, cells, (fi, fb), tags = channel_mesh_2d(n_cells, seed=a.seed)
+    cfg = ConfigChannel(device="cpu", n_cells=n_cells, solver_name="Butcher", solver_extra="RK4")
+    _, ofvm = O.from_mesh_tuple(pts, cells, fi, fb, tags, _oracle_phys(cfg))
+    N, D = len(cells), 2
+    c = pts[cells].mean(1)
+    prim = torch.zeros(N, 4, dtype=torch.float32)
+    prim[:, 0] = cfg.inlet_cfg.v_n_inf
+    r2 = torch.from_numpy(((c - np.asarray(cfg.bump_centre)[:D]) ** 2).sum(1)).float()
+    prim[:, 2] = 1.0 + cfg.bump_amp * torch.exp(-r2 / (2 * cfg.bump_sigma ** 2))
+    prim[:, 3] = cfg.inlet_cfg.T_inf
+    U0 = torch.cat(ofvm.primatives_to_state(prim[:, :2], prim[:, 2:3], prim[:, 3:]), -1)
+    dt = float(np.float32(ofvm.cfl_dt(prim.double(), cfg.cfl)))
+    cpu = host_cpu()
+    threads = min(x for x in (cpu["physical_cores"], cpu["affinity_cpus"], cpu["logical_cpus"]) if x)
+    res = {}
+    for nt, steps in ((threads, 1000), (1, 100)):
+        torch.set_num_threads(nt)
+        st = O.OracleStepper(ofvm, "Butcher", "RK4", dt=dt)
+        U = st.step(U0.clone())
+        t0 = time.perf_counter()
+        for i in range(steps):
+            U = st.step(U)
+            if i % 200 == 199:
+                log(f"{nt} threads: step {i + 1}")
+        res[nt] = (steps, time.perf_counter() - t0)
+    out = {"metric": "cell-updates/s (CPU restatement of the reference, 2-D NS channel, RK4)",
+           "value": N * res[threads][0] / res[threads][1], "unit": "cell-updates/s", "higher_is_better": True,
+           "config": {"workload": f"2-D channel, {N} cells, RK4 at CFL 0.5, dt {dt:.4e}", "steps": res[threads][0],
+                      "seconds": res[threads][1], "threads": threads,
+                      "single_thread": {"steps": res[1][0], "seconds": res[1][1],
+                                        "value": N * res[1][0] / res[1][1]},
+                      "cpu_model": cpu["model"], "host_physical_cores": cpu["physical_cores"],
+                      "affinity_cpus": cpu["affinity_cpus"]}}
+    print(json.dumps(out), flush=True)
+
+
 if __name__ == "__main__":
     _a = _parse()
-    if _a.corrector_timing:
+    if _a.cpu_config1:
+        cpu_config1_main(_a)
+    elif _a.corrector_timing:
         corrector_timing_main(_a)
     elif _a.learned:
         if _a.gpus != 1:

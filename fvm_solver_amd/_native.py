@@ -12,7 +12,7 @@ import torch
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libfvm_hip.so")
-ABI_VERSION = 9
+ABI_VERSION = 10
 
 FVM_ERR_ROW_OVERFLOW = 2
 
@@ -34,7 +34,8 @@ class FvmMeshDev(ctypes.Structure):
                 ("persist_blocks", i32), ("pad1", i32),
                 ("fz_tile", c_void_p), ("fz_halo", c_void_p), ("fz_hrec", c_void_p), ("fz_nbr", c_void_p),
                 ("fz_fdesc", c_void_p), ("fz_ffac", c_void_p), ("fz_fdesc_x", c_void_p), ("fz_geo", c_void_p),
-                ("fz_geo_x", c_void_p), ("fz_cfac", c_void_p), ("fz_ncl", i32), ("fz_ngl", i32), ("fz_nfl", i32),
+                ("fz_geo_x", c_void_p), ("fz_cfac", c_void_p), ("fz_h1f", c_void_p), ("fz_ncl", i32), ("fz_ngl", i32),
+                ("fz_nfl", i32),
                 ("fz_pad", i32)]
 
 
@@ -74,6 +75,7 @@ _PROTOS = {
                                              ctypes.POINTER(FvmWork), ctypes.POINTER(FvmStage), i32, c_void_p]),
     "fvm_grad_f32": (ctypes.c_int, [ctypes.POINTER(FvmMeshDev), ctypes.POINTER(FvmPhys), ctypes.POINTER(FvmWork),
                                     c_void_p, i32, c_void_p]),
+    "fvm_max_lds_per_workgroup": (ctypes.c_int, [c_void_p]),
     "fvm_update_stepsize": (ctypes.c_int, [ctypes.POINTER(FvmWork), i32, i32, f32, f32, f32, f32, f32, f32,
                                            c_void_p, c_void_p]),
     "fvm_state_to_prim_f32": (ctypes.c_int, [c_void_p, c_void_p, i64, i32, f32, c_void_p]),
@@ -168,3 +170,12 @@ def stream_ptr(device=None) -> int:
 
 def ptr(t: torch.Tensor | None):
     return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def max_lds_per_workgroup(device=None) -> int:
+    """LDS bytes one workgroup may use on `device` (the tile tables of the fused stage are
+    sized to it)."""
+    out = ctypes.c_int32(0)
+    with torch.cuda.device(torch.device(device) if device is not None else torch.cuda.current_device()):
+        check(lib().fvm_max_lds_per_workgroup(ctypes.byref(out)), "fvm_max_lds_per_workgroup")
+    return int(out.value)

@@ -872,53 +872,69 @@ k_div(FvmMeshDev m, FvmPhys p, FvmWork w, float *__restrict__ dudt, FvmStage st)
 }
 
 // ------------------------------------------------------------------------------------
-// Tile-fused stage: one 256-cell tile per workgroup
+// Tile-fused stage: one 256-cell tile per workgroup (2-D and 3-D)
 // ------------------------------------------------------------------------------------
 // The workgroup stages in LDS the primitives of its tile, of the tile's one-ring halo H1
 // (cells outside the tile sharing a facet with it) and of H1's other neighbours H2;
 // computes the boundary values, gradient and limiter of tile + H1 cells, keeping their
-// limited face values and limited gradients in LDS; evaluates the fluxes of the tile's
-// owned and foreign facets into LDS; then every tile cell sums its facets' fluxes and
-// runs the stage epilogue.  The arithmetic is that of k_grad, k_facet_flux and k_div,
-// through the same device functions in the same order, so the results are bitwise
-// those of the three-kernel pipeline; only the traffic differs (gradient records and
-// facet fluxes never leave the CU).  Gradient records are written back only for cells
-// with a boundary facet (the next call's Neumann correction reads them) into w.gp,
-// which must not alias w.gp_prev: other tiles still read gp_prev for their halo cells.
+// limited gradients and their face values in LDS; evaluates the fluxes of the tile's
+// owned and foreign facets; then every tile cell sums its facets' fluxes and runs the
+// stage epilogue.  The arithmetic is that of k_grad, k_facet_flux and k_div, through the
+// same device functions in the same order, so the results are bitwise those of the
+// three-kernel pipeline; only the traffic differs (gradient records and facet fluxes
+// never leave the CU).  Gradient records are written back only for cells with a
+// boundary facet (the next call's Neumann correction reads them) into w.gp, which must
+// not alias w.gp_prev: other tiles still read gp_prev for their halo cells.
+//
+// Face values are kept per facet of the tile's list and side, FF[q][side][C]: each is
+// written once, by the cell on that side (a boundary facet's outer value by its interior
+// cell), and the facet's flux overwrites FF[q][0] in place (the thread evaluating facet
+// q is the only one reading FF[q] in that phase).  Per cell slot storage would hold the
+// face values of H1 cells' facets outside the tile too, and a separate flux array
+// would add nfl C floats: this layout needs 29.5 KB on the 1M-cell 2-D mesh (5
+// workgroups per CU) and ~67 KB for 256 tetrahedra.
 constexpr int kFzTab = 12;
 constexpr unsigned kFzNone = 0xFFFFu;
+constexpr unsigned kFzSide1 = 0x8000u;   // H1 facet table: the H1 cell is the slot-1 side
 
 template <int D>
 __host__ __device__ constexpr int fz_geo_n() { return D == 2 ? 8 : 12; }
 template <int D>
 __host__ __device__ constexpr int fz_hrec_n() { return D == 2 ? 16 : 32; }
-
-// LDS layout of k_fused in floats: Pl [ncl][C], FV [NL][ngl][C] (slot-major), GL
-// [D][D+1][ngl] (component-major, padded to 4 floats), FL [nfl][C].  Slot- and
-// component-major rows make the gradient phase's writes (one cell per lane) conflict
-// free.
 template <int D>
-__host__ __device__ inline int64_t fz_fl_offset(int ncl, int ngl) {
-    constexpr int C = D + 2, NL = D + 1;
-    return static_cast<int64_t>(ncl) * C + static_cast<int64_t>(ngl) * NL * C +
-           ((static_cast<int64_t>(ngl) * D * (D + 1) + 3) & ~int64_t(3));
+__host__ __device__ constexpr int fz_fpt() { return D == 2 ? 2 : 3; }   // facets per thread
+
+__host__ __device__ inline int64_t fz_r4(int64_t x) { return (x + 3) & ~int64_t(3); }
+
+// LDS layout of k_fused in floats, each block 16-byte aligned: Pl [ncl][C], GL
+// [D][D+1][ngl] (component-major: the gradient phase's writes, one cell per lane, are
+// conflict free), FF [nfl][2][C].
+template <int D>
+__host__ __device__ inline int64_t fz_gl_offset(int ncl) {
+    return fz_r4(static_cast<int64_t>(ncl) * (D + 2));
+}
+template <int D>
+__host__ __device__ inline int64_t fz_ff_offset(int ncl, int ngl) {
+    return fz_gl_offset<D>(ncl) + fz_r4(static_cast<int64_t>(ngl) * D * (D + 1));
 }
 
 // One gradient cell of the fused kernel (tile cell or H1 cell, local id l, global gc):
-// boundary values, gradient and limiter; the limited face values and the limited V, T
-// gradients go to LDS.  Tile cells also write their boundary values and, when they have
-// a boundary facet, their gradient record (the next call's Neumann correction).
+// boundary values, gradient and limiter; the limited V, T gradients go to GL, the face
+// value at each facet of the tile's list to FF[q][side] (fq[k] = q | side << 15, or
+// kFzNone for a facet outside the list).  Tile cells also write their boundary values
+// (to w.ubc and, as the facet's outer value, to FF) and, when they have a boundary facet,
+// their gradient record (the next call's Neumann correction).
 template <int D, int LP>
 __device__ __forceinline__ void fused_grad_cell(const FvmMeshDev &m, const FvmPhys &p, const FvmWork &w,
-                                                const float *Pl, float *FV, float *GL, int ngl, int l, int64_t gc,
-                                                bool own,
-                                                const float (&gv)[D + 2][D], const float (&disp)[D + 1][D],
-                                                float eps, const ushort4 nrow, int has_prev,
-                                                const float *gmean, int g0, const ushort4 cfac, float *FL) {
+                                                const float *Pl, float *GL, float *FF, int ngl, int l, int64_t gc,
+                                                bool own, const float (&gv)[D + 2][D], const float (&disp)[D + 1][D],
+                                                float eps, const ushort4 nrow, int has_prev, const float *gmean,
+                                                int g0, const ushort4 fq4) {
     using Dm = Dims<D>;
     constexpr int C = Dm::C, NL = Dm::NL;
     const int64_t N = m.n_cells;
     const unsigned nl[4] = {nrow.x, nrow.y, nrow.z, nrow.w};
+    const unsigned fq[4] = {fq4.x, fq4.y, fq4.z, fq4.w};
     float P[C];
     load_vec<C>(Pl + l * C, P);
     float Un[NL][C];
@@ -933,9 +949,9 @@ __device__ __forceinline__ void fused_grad_cell(const FvmMeshDev &m, const FvmPh
             bc_value<D>(m, p, b, P, w.gp_prev + gc * Dm::REC, has_prev, gmean, g0, Un[k]);
             if (own) {
                 store_vec<C>(w.ubc + b * C, Un[k]);
-                // the boundary facet's outer value, kept in its flux slot for the flux phase
-                const unsigned q = k == 0 ? cfac.x : k == 1 ? cfac.y : k == 2 ? cfac.z : cfac.w;
-                store_vec<C>(FL + q * C, Un[k]);
+                // the boundary facet's outer value: the side opposite this cell's
+                const unsigned q = fq[k] & ~kFzSide1, side = (fq[k] & kFzSide1) ? 1u : 0u;
+                store_vec<C>(FF + (q * 2 + (1u - side)) * C, Un[k]);
             }
         }
     }
@@ -950,9 +966,10 @@ __device__ __forceinline__ void fused_grad_cell(const FvmMeshDev &m, const FvmPh
 #pragma unroll
         for (int j = 0; j < C; ++j) o[D * C + j] = phi[j];
     }
-    // limited face values and limited gradients (side_values' operation order)
+    // limited face values (side_values' operation order) at the facets in the list
 #pragma unroll
     for (int k = 0; k < NL; ++k) {
+        if (fq[k] == kFzNone) continue;
         float U[C];
 #pragma unroll
         for (int j = 0; j < C; ++j) {
@@ -961,7 +978,8 @@ __device__ __forceinline__ void fused_grad_cell(const FvmMeshDev &m, const FvmPh
             for (int ii = 1; ii < D; ++ii) dU = fmaf(g[ii][j], disp[k][ii], dU);
             U[j] = fmaf(dU, phi[j], P[j]);
         }
-        store_vec<C>(FV + (k * ngl + l) * C, U);
+        const unsigned q = fq[k] & ~kFzSide1, side = (fq[k] & kFzSide1) ? 1u : 0u;
+        store_vec<C>(FF + (q * 2 + side) * C, U);
     }
 #pragma unroll
     for (int ii = 0; ii < D; ++ii) {
@@ -971,19 +989,18 @@ __device__ __forceinline__ void fused_grad_cell(const FvmMeshDev &m, const FvmPh
     }
 }
 
-// Facet q of the tile's facet list (descriptor d, compact geometry geo): both sides from
-// LDS, the normal gradient, the flux into FL[q].
+// Facet q of the tile's list (descriptor d, compact geometry geo): both sides' face
+// values from FF[q], limited gradients from GL, the normal gradient, and the flux into
+// FF[q][0].
 template <int D>
-__device__ __forceinline__ void fused_facet(const FvmMeshDev &m, const FvmPhys &p, const FvmWork &w,
-                                            const float *Pl, const float *FV, const float *GL, int ngl, float *FL,
-                                            int q,
-                                            int64_t f, unsigned d, const float (&geo)[3 * D + 2], int64_t c0,
-                                            int halo_off, int has_prev, const float *gmean, int g0,
-                                            const unsigned short *meshT, int mt0, const float *physT) {
+__device__ __forceinline__ void fused_facet(const FvmMeshDev &m, const FvmPhys &p, const float *Pl, const float *GL,
+                                            int ngl, float *FF, int q, int64_t f, unsigned d,
+                                            const float (&geo)[3 * D + 2], const unsigned short *meshT, int mt0,
+                                            const float *physT) {
     using Dm = Dims<D>;
     constexpr int C = Dm::C;
     const int64_t N = m.n_cells, NF = m.n_facets;
-    const unsigned l0 = d & 0xFFFu, k0 = (d >> 12) & 3u, l1 = (d >> 14) & 0xFFFu, k1 = (d >> 26) & 3u;
+    const unsigned l0 = d & 0xFFFu, l1 = (d >> 14) & 0xFFFu;
     const bool left_first = (d >> 28) & 1u, bnd = (d >> 29) & 1u, outer0 = (d >> 30) & 1u;
     float mu0 = p.mu0, mub0 = p.mub0, kappa = p.kappa;
     if (m.mesh_phys) {
@@ -1005,10 +1022,10 @@ __device__ __forceinline__ void fused_facet(const FvmMeshDev &m, const FvmPhys &
         }
     }
     float U0s[C], U1s[C], g0s[D][D + 1], g1s[D][D + 1], gdn[D + 1];
+    load_vec<C>(FF + (2 * q) * C, U0s);
+    load_vec<C>(FF + (2 * q + 1) * C, U1s);
     if (!bnd) {
         float P0[C], P1[C];
-        load_vec<C>(FV + (k0 * ngl + l0) * C, U0s);
-        load_vec<C>(FV + (k1 * ngl + l1) * C, U1s);
         load_vec<C>(Pl + l0 * C, P0);
         load_vec<C>(Pl + l1 * C, P1);
 #pragma unroll
@@ -1027,20 +1044,13 @@ __device__ __forceinline__ void fused_facet(const FvmMeshDev &m, const FvmPhys &
             gdn[jg] = (left_first ? tl + tr : tr + tl) + 0.0f;
         }
     } else {
-        // boundary facet: its interior cell is a tile cell (the facet's lowest cell)
-        const unsigned li = outer0 ? l1 : l0, ki = outer0 ? k1 : k0;
-        float Uo[C], Pi[C], Ux[C];
-        load_vec<C>(FV + (ki * ngl + li) * C, Uo);
+        // boundary facet: its interior cell is a tile cell (the facet's lowest cell); the
+        // outer side's value was computed by that cell in the gradient phase (the same
+        // bc_value on the same inputs as the three-kernel pipeline)
+        const unsigned li = outer0 ? l1 : l0;
+        float Pi[C];
         load_vec<C>(Pl + li * C, Pi);
         const int64_t b = static_cast<int64_t>(m.fac_cell[outer0 ? f : NF + f]) - N;
-        // the outer value: computed by the interior cell in the gradient phase (the same
-        // bc_value on the same inputs) and left in this facet's flux slot
-        load_vec<C>(FL + q * C, Ux);
-#pragma unroll
-        for (int j = 0; j < C; ++j) {
-            U0s[j] = outer0 ? Ux[j] : Uo[j];
-            U1s[j] = outer0 ? Uo[j] : Ux[j];
-        }
 #pragma unroll
         for (int ii = 0; ii < D; ++ii)
 #pragma unroll
@@ -1056,7 +1066,7 @@ __device__ __forceinline__ void fused_facet(const FvmMeshDev &m, const FvmPhys &
     }
     float Fx[C];
     facet_flux<D>(p, mu0, mub0, kappa, U0s, U1s, g0s, g1s, gdn, geo, Fx);
-    store_vec<C>(FL + q * C, Fx);
+    store_vec<C>(FF + (2 * q) * C, Fx);
 }
 
 // Descriptor and compact geometry of facet q of the tile's list (owned run, then the
@@ -1068,29 +1078,33 @@ __device__ __forceinline__ void fused_facet_load(const FvmMeshDev &m, int q, int
     const bool owned = q < nfo;
     f = owned ? static_cast<int64_t>(fo0) + q : static_cast<int64_t>(m.fz_ffac[foff + q - nfo]);
     d = owned ? m.fz_fdesc[f] : m.fz_fdesc_x[foff + q - nfo];
-    const float *src = owned ? m.fz_geo + f * FG : m.fz_geo_x + static_cast<int64_t>(foff + q - nfo) * FG;
-    if constexpr (D == 2) {
-        const float4 a = reinterpret_cast<const float4 *>(src)[0];
-        const float4 b = reinterpret_cast<const float4 *>(src)[1];
-        geo[0] = a.x; geo[1] = a.y; geo[2] = a.z; geo[3] = a.w;
-        geo[4] = b.x; geo[5] = b.y; geo[6] = b.z; geo[7] = b.w;
-    } else {
+    const float4 *src = reinterpret_cast<const float4 *>(
+        owned ? m.fz_geo + f * FG : m.fz_geo_x + static_cast<int64_t>(foff + q - nfo) * FG);
+    float t[FG];
 #pragma unroll
-        for (int k = 0; k < 3 * D + 2; ++k) geo[k] = src[k];
+    for (int r = 0; r < FG / 4; ++r) {
+        const float4 a = src[r];
+        t[4 * r] = a.x;
+        t[4 * r + 1] = a.y;
+        t[4 * r + 2] = a.z;
+        t[4 * r + 3] = a.w;
     }
+#pragma unroll
+    for (int k = 0; k < 3 * D + 2; ++k) geo[k] = t[k];
 }
 
 // Each phase's global loads are issued before the previous phase's work (they land in
 // registers while it computes), so a tile costs about one memory round trip per phase
 // boundary rather than one per dependent load.  The host guarantees at most 256 H1
-// cells and 512 facets per tile (two facets per thread).
+// cells and fz_fpt<D>() x 256 facets per tile.
 template <int D, int LP, int EPI>
 __global__ void __launch_bounds__(kThreads)
 k_fused(FvmMeshDev m, FvmPhys p, FvmWork w, const float *__restrict__ prim, float *__restrict__ dudt, FvmStage st,
         int has_prev) {
     using Dm = Dims<D>;
-    constexpr int C = Dm::C, NL = Dm::NL, FVN = NL * C;
+    constexpr int C = Dm::C, NL = Dm::NL;
     constexpr int HREC = fz_hrec_n<D>();
+    constexpr int FPT = fz_fpt<D>();
     extern __shared__ float4 fz_lds[];     // 16-byte aligned: 2-D records move as ds_*_b128
     float *lds = reinterpret_cast<float *>(fz_lds);
     const int64_t N = m.n_cells;
@@ -1103,15 +1117,14 @@ k_fused(FvmMeshDev m, FvmPhys p, FvmWork w, const float *__restrict__ prim, floa
     const int halo_off = tb[0], n1 = tb[1], n2 = tb[2], nbr_off = tb[3], fo0 = tb[4], nfo = tb[5];
     const int foff = tb[6], nfor = tb[7], g0 = tb[8], ng = tb[9], h1_off = tb[10];
     float *Pl = lds;
-    float *FV = Pl + static_cast<int64_t>(m.fz_ncl) * C;
-    float *GL = FV + static_cast<int64_t>(m.fz_ngl) * FVN;
-    float *FL = lds + fz_fl_offset<D>(m.fz_ncl, m.fz_ngl);
-    // batches only (after FL): the mesh id of each tile cell, then mu0, mub0, kappa and dt
+    float *GL = lds + fz_gl_offset<D>(m.fz_ncl);
+    float *FF = lds + fz_ff_offset<D>(m.fz_ncl, m.fz_ngl);
+    // batches only (after FF): the mesh id of each tile cell, then mu0, mub0, kappa and dt
     // of the tile's first kTileMeshes meshes (cells are ordered by mesh)
-    unsigned short *meshT = reinterpret_cast<unsigned short *>(FL + static_cast<int64_t>(m.fz_nfl) * C);
+    unsigned short *meshT = reinterpret_cast<unsigned short *>(FF + static_cast<int64_t>(m.fz_nfl) * 2 * C);
     float *physT = reinterpret_cast<float *>(meshT + kThreads);
-    float *red = FV;                                   // reductions reuse the face-value rows
-    double *sred = reinterpret_cast<double *>(FV);     // (before and after the two phases using them)
+    float *red = GL;                                   // reductions reuse the gradient rows
+    double *sred = reinterpret_cast<double *>(GL);     // (before and after the phases using them)
     const int i = threadIdx.x;
     const int64_t c0 = tile * kThreads;
     const int nown = static_cast<int>(N - c0 < kThreads ? N - c0 : kThreads);
@@ -1157,13 +1170,14 @@ k_fused(FvmMeshDev m, FvmPhys p, FvmWork w, const float *__restrict__ prim, floa
     }
     float hgv[NL + 1][D], hdisp[NL][D], heps = 0.0f;
     int64_t hgc = 0;
-    ushort4 hnr = make_ushort4(0, 0, 0, 0);
+    ushort4 hnr = make_ushort4(0, 0, 0, 0), hf = make_ushort4(kFzNone, kFzNone, kFzNone, kFzNone);
     if (has_h1) {
         hgc = m.fz_halo[halo_off + i];
         const float4 *r = reinterpret_cast<const float4 *>(m.fz_hrec + static_cast<int64_t>(h1_off + i) * HREC);
-        float rv[HREC];
+        constexpr int HUSE = ((NL + 1) * D + NL * D + 1 + 3) / 4;   // float4s holding data
+        float rv[4 * HUSE];
 #pragma unroll
-        for (int q = 0; q < HREC / 4; ++q) {
+        for (int q = 0; q < HUSE; ++q) {
             const float4 t = r[q];
             rv[4 * q] = t.x;
             rv[4 * q + 1] = t.y;
@@ -1180,12 +1194,17 @@ k_fused(FvmMeshDev m, FvmPhys p, FvmWork w, const float *__restrict__ prim, floa
             for (int ii = 0; ii < D; ++ii) hdisp[k][ii] = rv[(NL + 1) * D + k * D + ii];
         heps = rv[(NL + 1) * D + NL * D];
         hnr = *reinterpret_cast<const ushort4 *>(m.fz_nbr + (static_cast<int64_t>(nbr_off) + kThreads + i) * 4);
+        hf = *reinterpret_cast<const ushort4 *>(m.fz_h1f + static_cast<int64_t>(h1_off + i) * 4);
     }
     for (int q = i; q < n1 + n2; q += kThreads) {
         float u[C];
         load_prim<D>(prim + static_cast<int64_t>(m.fz_halo[halo_off + q]) * C, u);
         store_vec<C>(Pl + (kThreads + q) * C, u);
     }
+    // this tile cell's face-value positions: its facets' list positions and its sides
+    const ushort4 ofq = make_ushort4(cf.x | ((oinfo & 1) ? kFzSide1 : 0u), cf.y | ((oinfo & 2) ? kFzSide1 : 0u),
+                                     cf.z | ((oinfo & 4) ? kFzSide1 : 0u),
+                                     NL > 3 ? (cf.w | ((oinfo & 8) ? kFzSide1 : 0u)) : kFzNone);
     __syncthreads();
 
     // ---- phase 1: gradients (the first facet's inputs in flight meanwhile) ----
@@ -1194,14 +1213,14 @@ k_fused(FvmMeshDev m, FvmPhys p, FvmWork w, const float *__restrict__ prim, floa
     float geoA[3 * D + 2];
     if (i < nft) fused_facet_load<D>(m, i, fo0, nfo, foff, fA, dA, geoA);
     if (has_own)
-        fused_grad_cell<D, LP>(m, p, w, Pl, FV, GL, m.fz_ngl, i, c0 + i, true, ogv, odisp, oeps, onr,
-                               has_prev, gmean, g0, cf, FL);
+        fused_grad_cell<D, LP>(m, p, w, Pl, GL, FF, m.fz_ngl, i, c0 + i, true, ogv, odisp, oeps, onr, has_prev,
+                               gmean, g0, ofq);
     if (has_h1)
-        fused_grad_cell<D, LP>(m, p, w, Pl, FV, GL, m.fz_ngl, kThreads + i, hgc, false, hgv, hdisp, heps, hnr,
-                               has_prev, gmean, g0, cf, FL);
+        fused_grad_cell<D, LP>(m, p, w, Pl, GL, FF, m.fz_ngl, kThreads + i, hgc, false, hgv, hdisp, heps, hnr,
+                               has_prev, gmean, g0, hf);
     __syncthreads();
 
-    // ---- phase 2: facet fluxes (the second facet's and the epilogue's inputs in flight) ----
+    // ---- phase 2: facet fluxes (the next facet's and the epilogue's inputs in flight) ----
     int64_t fB = 0;
     unsigned dB = 0;
     float geoB[3 * D + 2];
@@ -1221,12 +1240,19 @@ k_fused(FvmMeshDev m, FvmPhys p, FvmWork w, const float *__restrict__ prim, floa
     stage_inputs<D, EPI>(st, c, has_own, U0v, ap, a2p);
     float inv_vol = 0.0f;
     if (has_own) inv_vol = m.cell_inv_vol[c];
-    if (i < nft)
-        fused_facet<D>(m, p, w, Pl, FV, GL, m.fz_ngl, FL, i, fA, dA, geoA, c0, halo_off, has_prev, gmean, g0, meshT,
-                       mt0, physT);
-    if (twoB)
-        fused_facet<D>(m, p, w, Pl, FV, GL, m.fz_ngl, FL, i + kThreads, fB, dB, geoB, c0, halo_off, has_prev, gmean,
-                       g0, meshT, mt0, physT);
+    if (i < nft) fused_facet<D>(m, p, Pl, GL, m.fz_ngl, FF, i, fA, dA, geoA, meshT, mt0, physT);
+    if constexpr (FPT > 2) {
+        // third facet of the thread: its inputs in flight during the second's flux
+        int64_t fC = 0;
+        unsigned dC = 0;
+        float geoC[3 * D + 2];
+        const bool threeC = i + 2 * kThreads < nft;
+        if (threeC) fused_facet_load<D>(m, i + 2 * kThreads, fo0, nfo, foff, fC, dC, geoC);
+        if (twoB) fused_facet<D>(m, p, Pl, GL, m.fz_ngl, FF, i + kThreads, fB, dB, geoB, meshT, mt0, physT);
+        if (threeC) fused_facet<D>(m, p, Pl, GL, m.fz_ngl, FF, i + 2 * kThreads, fC, dC, geoC, meshT, mt0, physT);
+    } else {
+        if (twoB) fused_facet<D>(m, p, Pl, GL, m.fz_ngl, FF, i + kThreads, fB, dB, geoB, meshT, mt0, physT);
+    }
     __syncthreads();
 
     // ---- phase 3: divergence and stage epilogue of the tile's cells ----
@@ -1237,7 +1263,7 @@ k_fused(FvmMeshDev m, FvmPhys p, FvmWork w, const float *__restrict__ prim, floa
         const unsigned tf[4] = {cf.x, cf.y, cf.z, cf.w};
         float Fk[NL][C];
 #pragma unroll
-        for (int k = 0; k < NL; ++k) load_vec<C>(FL + tf[k] * C, Fk[k]);
+        for (int k = 0; k < NL; ++k) load_vec<C>(FF + (2 * tf[k]) * C, Fk[k]);
 #pragma unroll
         for (int k = 0; k < NL; ++k) {
             const float vk = ((oinfo >> k) & 1) ? inv_vol : -inv_vol;
@@ -1311,24 +1337,50 @@ static int check_mesh(const FvmMeshDev *m, const FvmPhys *p, const FvmWork *w, c
     return FVM_OK;
 }
 
-// Dynamic LDS of k_fused in bytes.
+// Dynamic LDS of k_fused in bytes (fused_tiles.py fused_lds_bytes is the same formula).
 template <int D>
 static size_t fused_lds_bytes(const FvmMeshDev &m) {
     // + the 16-bit mesh ids of the tile's cells and the tile's mesh table, for a batch
-    return static_cast<size_t>(fz_fl_offset<D>(m.fz_ncl, m.fz_ngl) + static_cast<int64_t>(m.fz_nfl) * (D + 2)) *
+    return static_cast<size_t>(fz_ff_offset<D>(m.fz_ncl, m.fz_ngl) + static_cast<int64_t>(m.fz_nfl) * 2 * (D + 2)) *
                sizeof(float) +
            (m.mesh_phys ? kThreads * sizeof(unsigned short) + kTileMeshes * 4 * sizeof(float) : 0);
 }
 
-constexpr size_t kFusedLdsMax = 64 * 1024;
+// LDS bytes one workgroup may use on the current device (160 KiB on gfx950), looked up
+// once per device.
+static size_t max_lds_per_workgroup() {
+    constexpr int kMaxDev = 64;
+    static int cached[kMaxDev] = {};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDev) return 64 * 1024;
+    if (cached[dev] <= 0) {
+        int v = 0;
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || v <= 0)
+            v = 64 * 1024;
+        cached[dev] = v;
+    }
+    return static_cast<size_t>(cached[dev]);
+}
+
+template <int D>
+static int check_fused_d(const FvmMeshDev &m, const char *who) {
+    if (!m.fz_tile || !m.fz_halo || !m.fz_hrec || !m.fz_nbr || !m.fz_h1f || !m.fz_fdesc || !m.fz_ffac ||
+        !m.fz_fdesc_x || !m.fz_geo || !m.fz_geo_x || !m.fz_cfac || m.fz_ncl < kThreads || m.fz_ngl < kThreads ||
+        m.fz_ngl > 2 * kThreads || m.fz_ngl > m.fz_ncl || m.fz_ncl > 0xFFF || m.fz_nfl < 1 ||
+        m.fz_nfl > fz_fpt<D>() * kThreads || fused_lds_bytes<D>(m) > max_lds_per_workgroup() ||
+        (m.mesh_phys && m.n_meshes > 0xFFFF)) {
+        set_error("%s: invalid tile tables for the tile-fused stage (or %zu B of LDS beyond the device's %zu)", who,
+                  fused_lds_bytes<D>(m), max_lds_per_workgroup());
+        return FVM_ERR_ARG;
+    }
+    return FVM_OK;
+}
 
 static int check_fused(const FvmMeshDev &m, const FvmWork &w, const char *who) {
-    if (m.dim != 2 || !m.fz_tile || !m.fz_halo || !m.fz_hrec || !m.fz_nbr || !m.fz_fdesc || !m.fz_ffac ||
-        !m.fz_fdesc_x || !m.fz_geo || !m.fz_geo_x || !m.fz_cfac || m.fz_ncl < kThreads || m.fz_ngl < kThreads ||
-        m.fz_ngl > m.fz_ncl || m.fz_ncl > 0xFFF || m.fz_nfl < 1 || fused_lds_bytes<2>(m) > kFusedLdsMax ||
-        (m.mesh_phys && m.n_meshes > 0xFFFF)) {
-        set_error("%s: invalid tile tables for the tile-fused stage", who);
-        return FVM_ERR_ARG;
+    const int e = m.dim == 2 ? check_fused_d<2>(m, who) : m.dim == 3 ? check_fused_d<3>(m, who) : FVM_ERR_ARG;
+    if (e) {
+        if (m.dim != 2 && m.dim != 3) set_error("%s: invalid dimension", who);
+        return e;
     }
     if (!w.gp_prev || w.gp_prev == w.gp) {
         set_error("%s: the tile-fused stage needs work->gp_prev as a buffer separate from work->gp", who);
@@ -1337,38 +1389,46 @@ static int check_fused(const FvmMeshDev &m, const FvmWork &w, const char *who) {
     return FVM_OK;
 }
 
+template <typename K>
+static void launch_fused_kernel(K kernel, dim3 grid, size_t lds, hipStream_t s, const FvmMeshDev &m,
+                                const FvmPhys &p, const FvmWork &w, const float *prim, float *dudt,
+                                const FvmStage &st, int has_prev) {
+    if (lds > 64 * 1024)   // dynamic LDS beyond 64 KiB is requested explicitly
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
+    hipLaunchKernelGGL(kernel, grid, dim3(kThreads), lds, s, m, p, w, prim, dudt, st, has_prev);
+}
+
 template <int D, int EPI>
 static void launch_fused(const FvmMeshDev &m, const FvmPhys &p, const FvmWork &w, const float *prim, float *dudt,
                          const FvmStage &st, hipStream_t s, int has_prev) {
     const dim3 grid(blocks_for(m.n_cells, kThreads));
     const size_t lds = fused_lds_bytes<D>(m);
     switch (p.lim_p) {
-        case 1: hipLaunchKernelGGL((k_fused<D, 1, EPI>), grid, dim3(kThreads), lds, s, m, p, w, prim, dudt, st, has_prev); break;
-        case 2: hipLaunchKernelGGL((k_fused<D, 2, EPI>), grid, dim3(kThreads), lds, s, m, p, w, prim, dudt, st, has_prev); break;
-        case 3: hipLaunchKernelGGL((k_fused<D, 3, EPI>), grid, dim3(kThreads), lds, s, m, p, w, prim, dudt, st, has_prev); break;
-        case 5: hipLaunchKernelGGL((k_fused<D, 5, EPI>), grid, dim3(kThreads), lds, s, m, p, w, prim, dudt, st, has_prev); break;
-        default: hipLaunchKernelGGL((k_fused<D, 4, EPI>), grid, dim3(kThreads), lds, s, m, p, w, prim, dudt, st, has_prev); break;
+        case 1: launch_fused_kernel(k_fused<D, 1, EPI>, grid, lds, s, m, p, w, prim, dudt, st, has_prev); break;
+        case 2: launch_fused_kernel(k_fused<D, 2, EPI>, grid, lds, s, m, p, w, prim, dudt, st, has_prev); break;
+        case 3: launch_fused_kernel(k_fused<D, 3, EPI>, grid, lds, s, m, p, w, prim, dudt, st, has_prev); break;
+        case 5: launch_fused_kernel(k_fused<D, 5, EPI>, grid, lds, s, m, p, w, prim, dudt, st, has_prev); break;
+        default: launch_fused_kernel(k_fused<D, 4, EPI>, grid, lds, s, m, p, w, prim, dudt, st, has_prev); break;
     }
 }
 
 // parts: 1 = boundary-mean pre-pass, 2 = gradient + limiter (+ boundary values),
-// 4 = facet fluxes, 8 = divergence (+ stage update), 16 = tile-fused stage (2-D; the
-// caller has checked the tile tables).  The pre-pass runs only when `need_mean`.
+// 4 = facet fluxes, 8 = divergence (+ stage update), 16 = tile-fused stage (the caller
+// has checked the tile tables).  The pre-pass runs only when `need_mean`.
 template <int D>
 static int launch_rhs(const FvmMeshDev &m, const FvmPhys &p, const FvmWork &w, const float *prim, float *dudt,
                       const FvmStage *st, hipStream_t s, int has_prev, int need_mean, int parts = 15) {
     if ((parts & 1) && need_mean && m.n_groups > 0)
         hipLaunchKernelGGL(k_bc_mean<D>, dim3(m.n_slots), dim3(kThreads), 0, s, m, p, w, prim);
-    if constexpr (D == 2) {
-        if (parts & 16) {
-            if (st) {
-                launch_fused<D, EPI_STAGE>(m, p, w, prim, nullptr, *st, s, has_prev);
-            } else {
-                FvmStage none{};
-                launch_fused<D, EPI_DUDT>(m, p, w, prim, dudt, none, s, has_prev);
-            }
-            return check_launch(st ? "fvm_rk_stage_f32" : "fvm_rhs_f32");
+    if (parts & 16) {
+        if (st) {
+            launch_fused<D, EPI_STAGE>(m, p, w, prim, nullptr, *st, s, has_prev);
+        } else {
+            FvmStage none{};
+            launch_fused<D, EPI_DUDT>(m, p, w, prim, dudt, none, s, has_prev);
         }
+        return check_launch(st ? "fvm_rk_stage_f32" : "fvm_rhs_f32");
     }
     const unsigned int ntiles = blocks_for(m.n_cells, kThreads);
     const dim3 grid(m.persist_blocks > 0 && static_cast<unsigned int>(m.persist_blocks) < ntiles
@@ -1465,7 +1525,9 @@ int fvm_rhs_f32(const FvmMeshDev *m, const FvmPhys *p, const FvmWork *w, const f
     e = check_align(m, w, nullptr, dudt, prim, "fvm_rhs_f32");
     if (!e && m->fz_tile) e = check_fused(*m, *w, "fvm_rhs_f32");
     if (e) return e;
-    if (m->fz_tile) return launch_rhs<2>(*m, *p, *w, prim, dudt, nullptr, as_stream(stream), has_prev, 1, 1 | 16);
+    if (m->fz_tile)
+        return m->dim == 2 ? launch_rhs<2>(*m, *p, *w, prim, dudt, nullptr, as_stream(stream), has_prev, 1, 1 | 16)
+                           : launch_rhs<3>(*m, *p, *w, prim, dudt, nullptr, as_stream(stream), has_prev, 1, 1 | 16);
     return m->dim == 2 ? launch_rhs<2>(*m, *p, *w, prim, dudt, nullptr, as_stream(stream), has_prev, 1)
                        : launch_rhs<3>(*m, *p, *w, prim, dudt, nullptr, as_stream(stream), has_prev, 1);
 }
@@ -1491,7 +1553,11 @@ int fvm_rk_stage_f32(const FvmMeshDev *m, const FvmPhys *p, const FvmWork *w, co
     if (!e && m->fz_tile) e = check_fused(*m, *w, "fvm_rk_stage_f32");
     if (e) return e;
     if (m->fz_tile)
-        return launch_rhs<2>(*m, *p, *w, st->prim, nullptr, st, as_stream(stream), st->has_prev, !st->bc_ready, 1 | 16);
+        return m->dim == 2
+                   ? launch_rhs<2>(*m, *p, *w, st->prim, nullptr, st, as_stream(stream), st->has_prev, !st->bc_ready,
+                                   1 | 16)
+                   : launch_rhs<3>(*m, *p, *w, st->prim, nullptr, st, as_stream(stream), st->has_prev, !st->bc_ready,
+                                   1 | 16);
     return m->dim == 2 ? launch_rhs<2>(*m, *p, *w, st->prim, nullptr, st, as_stream(stream), st->has_prev, !st->bc_ready)
                        : launch_rhs<3>(*m, *p, *w, st->prim, nullptr, st, as_stream(stream), st->has_prev, !st->bc_ready);
 }
@@ -1510,6 +1576,15 @@ int fvm_rk_stage_part_f32(const FvmMeshDev *m, const FvmPhys *p, const FvmWork *
     return m->dim == 2
                ? launch_rhs<2>(*m, *p, *w, st->prim, nullptr, st, as_stream(stream), st->has_prev, !st->bc_ready, parts)
                : launch_rhs<3>(*m, *p, *w, st->prim, nullptr, st, as_stream(stream), st->has_prev, !st->bc_ready, parts);
+}
+
+int fvm_max_lds_per_workgroup(int32_t *out) {
+    if (!out) {
+        set_error("fvm_max_lds_per_workgroup: null out");
+        return FVM_ERR_ARG;
+    }
+    *out = static_cast<int32_t>(max_lds_per_workgroup());
+    return FVM_OK;
 }
 
 int fvm_update_stepsize(const FvmWork *w, int32_t C, int32_t nblk, float atol, float rtol, float mtol, float order,

@@ -216,17 +216,28 @@ class DeviceMesh:
     the boundary-mean slot tables list, per tile, the groups of every mesh the tile
     touches (at most 4)."""
 
-    def __init__(self, mesh, n_comp: int, bc_tags, cfg, device, persist_blocks: int = 0, tiles: bool = True):
-        self._build([host_layout(mesh, n_comp, bc_tags, cfg)], device, persist_blocks, tiles)
+    def __init__(self, mesh, n_comp: int, bc_tags, cfg, device, persist_blocks: int = 0, tiles: bool = True,
+                 lds_limit: int | None = None):
+        self._build([host_layout(mesh, n_comp, bc_tags, cfg)], device, persist_blocks, tiles, lds_limit)
 
     @classmethod
-    def batch(cls, items, device, persist_blocks: int = 0, tiles: bool = True):
+    def batch(cls, items, device, persist_blocks: int = 0, tiles: bool = True, lds_limit: int | None = None):
         """items: iterable of (mesh, n_comp, bc_tags, cfg)."""
         obj = cls.__new__(cls)
-        obj._build([host_layout(*it) for it in items], device, persist_blocks, tiles)
+        obj._build([host_layout(*it) for it in items], device, persist_blocks, tiles, lds_limit)
         return obj
 
-    def _build(self, L, device, persist_blocks, want_tiles=True):
+    @staticmethod
+    def lds_limit_for(device, D: int) -> int:
+        """LDS bytes one workgroup of the fused kernel may use: the device's limit per
+        workgroup (160 KiB on gfx950), or, for a host-only layout, 64 KiB in 2-D (what
+        lets several tiles share a CU) and 160 KiB in 3-D."""
+        dev = torch.device(device)
+        if dev.type == "cuda":
+            return int(nat.max_lds_per_workgroup(dev))
+        return 64 * 1024 if D == 2 else 160 * 1024
+
+    def _build(self, L, device, persist_blocks, want_tiles=True, lds_limit=None):
         D, C = L[0]["D"], L[0]["C"]
         if any(l["D"] != D for l in L):
             raise ValueError("all meshes of a batch must have the same dimension")
@@ -294,17 +305,16 @@ class DeviceMesh:
         fpos[fperm] = np.arange(F)
         h["fac_cell"], h["fperm"], h["fpos"] = fac_cell, fperm, fpos
 
-        # tables of the tile-fused stage kernel (2-D)
+        # tables of the tile-fused stage kernel
         self.tiles, self.tiles_reason = None, "not built" if want_tiles else "disabled"
-        if want_tiles and D == 2:
+        if want_tiles:
+            lim = lds_limit if lds_limit is not None else self.lds_limit_for(device, D)
             try:
                 self.tiles = build_tiles(N, D, nbr_s, fpos[c2f_s], fac_cell[:, fperm], h["gval"], h["disp_s"],
-                                         h["eps"], h["geo"][fperm], bcell, group, cell_mesh, go)
+                                         h["eps"], h["geo"][fperm], bcell, group, cell_mesh, go, lds_limit=lim)
                 self.tiles_reason = None
             except TilesUnavailable as e:
                 self.tiles_reason = str(e)
-        elif want_tiles:
-            self.tiles_reason = "3-D meshes use the three-kernel pipeline"
 
         dev = torch.device(device)
 
@@ -349,7 +359,8 @@ class DeviceMesh:
             fz_nbr=up(tl["nbr"].view(np.int16), torch.int16), fz_fdesc=up(tl["fdesc"].view(np.int32), i32),
             fz_ffac=up_nz(tl["ffac"], i32), fz_fdesc_x=up_nz(tl["fdesc_x"].view(np.int32), i32),
             fz_geo=up(tl["geo"], f32), fz_geo_x=up_nz(tl["geo_x"], f32),
-            fz_cfac=up(tl["cfac"].view(np.int16), torch.int16))
+            fz_cfac=up(tl["cfac"].view(np.int16), torch.int16),
+            fz_h1f=up_nz(tl["h1f"].view(np.int16) if len(tl["h1f"]) else np.zeros((1, 4), np.int16), torch.int16))
         self.struct = nat.FvmMeshDev(**base, fz_ncl=tl["ncl"], fz_ngl=tl["ngl"], fz_nfl=tl["nfl"],
                                      **{k: v.data_ptr() for k, v in self.tz.items()})
 

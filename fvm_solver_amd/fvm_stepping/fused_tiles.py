@@ -24,6 +24,12 @@ index k (cell_facet[k] == f), 14-25 / 26-27 the same for the slot-1 cell, bit 28
 slot-0 cell has the lower global id (A_face_grad's ascending column order), bit 29
 boundary facet, bit 30 the outer (boundary) side is slot 0.  For a boundary facet the
 outer side's local id is 0xFFF.
+
+Face values live in LDS per facet of the list and side, FF[q][side][C]: the gradient
+phase writes them from the cell on that side (a tile cell finds q through its cfac row
+and its side through its slot bits; an H1 cell through the H1 facet table h1f), the
+facet's flux then overwrites FF[q][0].  Tiles hold 256 cells in 2-D and 3-D; a thread
+evaluates up to 2 (2-D) or 3 (3-D) facets of its tile's list.
 """
 from __future__ import annotations
 
@@ -31,8 +37,27 @@ import numpy as np
 
 TILE = 256
 TAB = 12          # ints per tile in the tile table
-NONE16 = 0xFFFF   # local neighbour id of a boundary slot
+NONE16 = 0xFFFF   # local neighbour id of a boundary slot; H1 slot whose facet is not in the list
 LOCAL_MAX = 0xFFF
+SIDE_BIT = 0x8000  # H1 facet table: the H1 cell is the facet's slot-1 side
+TILE_MESHES = 4    # meshes of a batch whose physics a tile keeps in LDS
+
+
+def facets_per_thread(D: int) -> int:
+    """Facets each of the 256 threads of a tile evaluates at most (2-D: 2, 3-D: 3)."""
+    return 2 if D == 2 else 3
+
+
+def fused_lds_bytes(D: int, ncl: int, ngl: int, nfl: int, batch: bool) -> int:
+    """Dynamic LDS of the fused kernel (the same formula as the kernel's launch): primitives
+    Pl [ncl][C], limited V/T gradients GL [D][D+1][ngl], both sides' face values per
+    facet FF [nfl][2][C] (the facet's flux overwrites its side-0 values), each block
+    starting on a 16-byte boundary; for a batch the tile cells' 16-bit mesh ids and a
+    4 x 4-float mesh table."""
+    C = D + 2
+    r4 = lambda x: (x + 3) & ~3
+    floats = r4(ncl * C) + r4(ngl * D * (D + 1)) + nfl * 2 * C
+    return 4 * floats + ((TILE * 2 + TILE_MESHES * 4 * 4) if batch else 0)
 
 
 class TilesUnavailable(ValueError):
@@ -50,7 +75,7 @@ def _lookup(keys: np.ndarray, q: np.ndarray) -> np.ndarray:
 def build_tiles(N: int, D: int, nbr: np.ndarray, cell_facet: np.ndarray, fac_cell: np.ndarray,
                 gval: np.ndarray, disp: np.ndarray, eps: np.ndarray, geo: np.ndarray,
                 bcell: np.ndarray, group: np.ndarray, cell_mesh: np.ndarray, group_off: np.ndarray,
-                lds_limit: int = 64 * 1024) -> dict:
+                lds_limit: int = 64 * 1024, lds_target: int | None = None) -> dict:
     """Tile tables for the fused kernel.
 
     nbr [N, NL] neighbour per slot (>= N: N + boundary index); cell_facet [N, NL] device
@@ -58,7 +83,9 @@ def build_tiles(N: int, D: int, nbr: np.ndarray, cell_facet: np.ndarray, fac_cel
     eps [N]; geo [F, G] device order (first 3D+2 floats: n, n_hat, |n|, 1/(n.d),
     X); bcell/group [Fb]; cell_mesh [N]; group_off = first group of each mesh (+ total).
     Raises TilesUnavailable when a tile's working set does not fit ``lds_limit`` bytes
-    of LDS or the local ids overflow (a badly ordered mesh)."""
+    of LDS or the local ids overflow (a badly ordered mesh).  ``lds_target`` (optional,
+    <= lds_limit) is only reported: whether the tables fit it (e.g. the LDS that lets
+    one more workgroup share a CU)."""
     NL = D + 1
     C = D + 2
     nbr = np.asarray(nbr, np.int64)
@@ -200,6 +227,21 @@ def build_tiles(N: int, D: int, nbr: np.ndarray, cell_facet: np.ndarray, fac_cel
     cfac = np.full((N, 4), 0, np.uint16)
     cfac[:, :NL] = tl.reshape(N, NL).astype(np.uint16)
 
+    # ---- H1 facet table: for each slot of each H1 cell, the tile-local position of the
+    # facet if it is in the tile's list (| SIDE_BIT when the H1 cell is its slot-1 side),
+    # else NONE16: the H1 cell's gradient thread writes its face value there ----
+    h1f = np.full((len(K1), 4), NONE16, np.uint16)
+    if len(K1):
+        hf_t = np.repeat(t1, NL)
+        hf_f = cell_facet[h1c].reshape(-1)
+        pos = np.where(own_t[hf_f] == hf_t, hf_f - fo0[hf_t], -1)
+        rest = pos < 0
+        px = _lookup(xk, hf_t[rest] * F + hf_f[rest])
+        pos[rest] = np.where(px >= 0, nfo[hf_t[rest]] + px - foff[hf_t[rest]], -1)
+        side1 = fac_cell[1, hf_f] == np.repeat(h1c, NL)
+        val = np.where(pos >= 0, pos | np.where(side1, SIDE_BIT, 0), NONE16)
+        h1f[:, :NL] = val.reshape(-1, NL).astype(np.uint16)
+
     # ---- characteristic-group means a tile needs (groups of the meshes it touches) ----
     gcell = np.asarray(bcell, np.int64)[np.asarray(group) >= 0]
     needs = np.zeros(T, bool)
@@ -234,17 +276,19 @@ def build_tiles(N: int, D: int, nbr: np.ndarray, cell_facet: np.ndarray, fac_cel
     halo[halo_off[t1] + (np.arange(len(K1)) - s1[t1])] = h1c
     halo[halo_off[t2] + n1[t2] + (np.arange(len(K2)) - s2[t2])] = K2 % N
 
-    if int(n1.max(initial=0)) > TILE or int((nfo + nfor).max(initial=0)) > 2 * TILE:
-        raise TilesUnavailable("a tile has more than 256 halo cells or 512 facets")
+    fpt = facets_per_thread(D)
+    if int(n1.max(initial=0)) > TILE or int((nfo + nfor).max(initial=0)) > fpt * TILE:
+        raise TilesUnavailable(f"a tile has more than {TILE} halo cells or {fpt * TILE} facets")
     ncl = TILE + int((n1 + n2).max(initial=0))
     ngl = TILE + int(n1.max(initial=0))
     nfl = int((nfo + nfor).max(initial=1))
-    GLN = D * (D + 1)
-    lds = 4 * (ncl * C + ngl * (NL * C + GLN) + nfl * C)
+    batch = int(np.max(cm, initial=0)) > 0
+    lds = fused_lds_bytes(D, ncl, ngl, nfl, batch)
     if lds > lds_limit:
         raise TilesUnavailable(f"tile working set {lds} B exceeds {lds_limit} B of LDS")
-    return dict(tab=tab, halo=halo, hrec=hrec, nbr=rows, fdesc=fdesc, ffac=fx.astype(np.int32), fdesc_x=fdesc_x,
-                geo=geo_c, geo_x=geo_x, cfac=cfac, ncl=ncl, ngl=ngl, nfl=nfl, lds_bytes=lds, n_tiles=T,
+    return dict(tab=tab, halo=halo, hrec=hrec, nbr=rows, h1f=h1f, fdesc=fdesc, ffac=fx.astype(np.int32),
+                fdesc_x=fdesc_x, geo=geo_c, geo_x=geo_x, cfac=cfac, ncl=ncl, ngl=ngl, nfl=nfl, lds_bytes=lds,
+                lds_fits_target=None if lds_target is None else lds <= lds_target, n_tiles=T,
                 halo_cells=(int(n1.sum()), int(n2.sum())), foreign=int(nfor.sum()))
 
 
@@ -288,3 +332,26 @@ def check_tiles(t: dict, N: int, D: int, nbr, cell_facet, fac_cell) -> None:
             c = c0 + i
             for k in range(NL):
                 assert fl[t["cfac"][c, k]] == cell_facet[c, k], (ti, c, k)
+        # H1 facet table: every (facet of the list, side) is written by exactly one cell
+        seen = set()
+        for i in range(nown):
+            c = c0 + i
+            for k in range(NL):
+                q = int(t["cfac"][c, k])
+                seen.add((q, int(fac_cell[1, fl[q]] == c)))
+        for j in range(n1):
+            h = glob[TILE + j]
+            for k in range(NL):
+                v = int(t["h1f"][int(tab[ti, 10]) + j, k])
+                f = cell_facet[h, k]
+                if v == NONE16:
+                    assert f not in set(fl.tolist()), (ti, h, k)
+                    continue
+                q, side = v & ~SIDE_BIT, int(bool(v & SIDE_BIT))
+                assert fl[q] == f and fac_cell[side, f] == h, (ti, h, k)
+                assert (q, side) not in seen
+                seen.add((q, side))
+        for q, f in enumerate(fl):
+            for side in (0, 1):
+                if fac_cell[side, f] < N:
+                    assert (q, side) in seen, (ti, q, side)

@@ -122,7 +122,9 @@ typedef struct FvmMeshDev {
     int32_t persist_blocks;     /* workgroups of the per-cell kernels (0 = one per
                                    256-cell tile); tiles are walked grid-stride       */
     int32_t pad1;
-    /* tile-fused stage (2-D; fz_tile NULL = the three-kernel pipeline).  Tile t holds
+    /* tile-fused stage (2-D and 3-D; fz_tile NULL = the three-kernel pipeline; the
+       tables must fit fvm_max_lds_per_workgroup, fz_ngl <= 512 and fz_nfl <= 512 in 2-D,
+       768 in 3-D).  Tile t holds
        cells 256t..256t+255 (local ids 0..255), its one-ring halo H1 (local ids 256..,
        ascending global id) and H1's other neighbours H2 (after H1).  Its facets are
        the owned run [fo0, fo0+nfo) of the device facet order (lowest adjacent cell in
@@ -134,16 +136,22 @@ typedef struct FvmMeshDev {
     const int32_t *fz_tile;     /* [T][12] halo_off, n1, n2, nbr_off, fo0, nfo, foff,
                                    nfor, g0, ng (group means the tile needs), h1_off  */
     const int32_t *fz_halo;     /* H1 then H2 global ids of each tile, at halo_off    */
-    const float *fz_hrec;       /* [sum n1][16] H1 records at h1_off: LSQ values
-                                   [NL+1][d], displacements [NL][d], eps, info bits   */
+    const float *fz_hrec;       /* [sum n1][16] (2-D) / [sum n1][32] (3-D) H1 records
+                                   at h1_off: LSQ values [NL+1][d], displacements
+                                   [NL][d], eps, zero padding                         */
     const uint16_t *fz_nbr;     /* [rows][4] local neighbour ids (0xFFFF = boundary)
                                    of the tile's 256 + n1 cells, rows at nbr_off      */
     const uint32_t *fz_fdesc;   /* [F] descriptor of each facet in its owner tile     */
     const int32_t *fz_ffac;     /* foreign facet ids of each tile, at foff            */
     const uint32_t *fz_fdesc_x; /* their descriptors                                  */
-    const float *fz_geo;        /* [F][8] n, n_hat, |n|, 1/(n.d), X (device order)    */
-    const float *fz_geo_x;      /* [sum nfor][8] the same for the foreign lists       */
+    const float *fz_geo;        /* [F][8] (2-D) / [F][12] (3-D) n, n_hat, |n|,
+                                   1/(n.d), X, zero padding (device order)            */
+    const float *fz_geo_x;      /* [sum nfor][8 | 12] the same for the foreign lists  */
     const uint16_t *fz_cfac;    /* [N][4] tile-local facet index of each cell slot    */
+    const uint16_t *fz_h1f;     /* [sum n1][4] per H1 cell slot (rows at h1_off): the
+                                   tile-local index of the facet if it is in the
+                                   tile's list, | 0x8000 if the H1 cell is its slot-1
+                                   side; 0xFFFF otherwise                             */
     int32_t fz_ncl;             /* LDS rows: max staged cells (256 + n1 + n2)         */
     int32_t fz_ngl;             /*           max gradient cells (256 + n1)            */
     int32_t fz_nfl;             /*           max facets (nfo + nfor)                  */
@@ -246,6 +254,10 @@ int fvm_rk_stage_part_f32(const FvmMeshDev *m, const FvmPhys *p, const FvmWork *
  * tile-fused stage, which keeps only boundary cells' records. */
 int fvm_grad_f32(const FvmMeshDev *m, const FvmPhys *p, const FvmWork *w, const float *prim, int32_t has_prev,
                  void *stream);
+
+/* LDS bytes one workgroup may use on the current device (the tile tables of the fused
+ * stage must fit it).  No reference counterpart. */
+int fvm_max_lds_per_workgroup(int32_t *out);
 
 /* Adaptive.update_stepsize (integrators.py:42-62) from the partial norms the last
  * adaptive stage left in work->red: dt *= alpha + (1 - alpha) 0.9 E^(-1/order). */

@@ -9,10 +9,11 @@ facet side) must be the very same floats, so both pipelines compute bitwise alik
 import numpy as np
 import pytest
 
-from fvm_solver_amd.config_fvm import ConfigChannel
+from fvm_solver_amd.config_fvm import ConfigChannel, ConfigChannel3D
 from fvm_solver_amd.fvm_stepping.facet_process import DeviceMesh
-from fvm_solver_amd.fvm_stepping.fused_tiles import NONE16, TILE, TilesUnavailable, build_tiles, check_tiles
-from fvm_solver_amd.mesh_gen.channel import channel_mesh_2d
+from fvm_solver_amd.fvm_stepping.fused_tiles import (NONE16, TILE, TilesUnavailable, build_tiles, check_tiles,
+                                                     facets_per_thread, fused_lds_bytes)
+from fvm_solver_amd.mesh_gen.channel import channel_mesh_2d, channel_mesh_3d
 from fvm_solver_amd.mesh_utils.fvm_mesh import mesh_from_tuple
 from fvm_solver_amd.mesh_utils.mesh_store import channel_bc_tags
 from tests.golden_cases import load, mesh_tuple
@@ -20,7 +21,9 @@ from tests.golden_cases import load, mesh_tuple
 
 def _dm(pts, cells, fi, fb, tags, **kw):
     mesh = mesh_from_tuple(pts, cells, fi, fb, device="cpu")
-    return DeviceMesh(mesh, 4, channel_bc_tags(tags, 2), ConfigChannel(), "cpu", **kw)
+    D = mesh.dim
+    cfg = ConfigChannel() if D == 2 else ConfigChannel3D()
+    return DeviceMesh(mesh, D + 2, channel_bc_tags(tags, D), cfg, "cpu", **kw)
 
 
 def _check(dm):
@@ -29,14 +32,18 @@ def _check(dm):
     assert t is not None, dm.tiles_reason
     cf = h["fpos"][h["c2f_s"]]
     fc = h["fac_cell"][:, h["fperm"]]
-    check_tiles(t, dm.N, 2, h["nbr_s"], cf, fc)
+    check_tiles(t, dm.N, dm.dim, h["nbr_s"], cf, fc)
     return t, h, cf, fc
 
 
-@pytest.fixture(scope="module", params=["golden", "channel20k"])
+@pytest.fixture(scope="module", params=["golden", "channel20k", "golden3d", "channel3d_30k"])
 def dm(request):
     if request.param == "golden":
         pts, cells, fi, fb, tags = mesh_tuple(load("ref_2d.npz"))
+    elif request.param == "golden3d":
+        pts, cells, fi, fb, tags = mesh_tuple(load("ref_3d.npz"))
+    elif request.param == "channel3d_30k":
+        pts, cells, (fi, fb), tags = channel_mesh_3d(30000, seed=3)
     else:
         pts, cells, (fi, fb), tags = channel_mesh_2d(20000, seed=3)
     return _dm(pts, cells, fi, fb, tags)
@@ -60,11 +67,13 @@ def test_tables_rebuild_the_mesh(dm):
 
 def test_fused_inputs_are_the_same_floats(dm):
     t, h, cf, fc = _check(dm)
-    NL = 3
+    D = dm.dim
+    NL = D + 1
+    G = 3 * D + 2
     geo = h["geo"][h["fperm"]]
-    # compact facet geometry = the first 3d+2 floats of the facet record
-    assert np.array_equal(t["geo"], geo[:, :8])
-    assert np.array_equal(t["geo_x"], geo[t["ffac"], :8])
+    # compact facet geometry = the first 3d+2 floats of the facet record (zero padded)
+    assert np.array_equal(t["geo"][:, :G], geo[:, :G]) and np.all(t["geo"][:, G:] == 0)
+    assert np.array_equal(t["geo_x"][:, :G], geo[t["ffac"], :G])
     # the three-kernel facet kernel takes each side's displacement from the facet
     # record; the fused kernel from the cell's own displacement list at the side's slot
     F = dm.F
@@ -73,21 +82,25 @@ def test_fused_inputs_are_the_same_floats(dm):
         inner = c < dm.N
         f = np.flatnonzero(inner)
         k = (t["fdesc"][f].astype(np.int64) >> (12 if s == 0 else 26)) & 3
-        assert np.array_equal(h["disp_s"][c[inner], k].astype(np.float32), geo[f, 8 + 2 * s:10 + 2 * s])
+        assert np.array_equal(h["disp_s"][c[inner], k].astype(np.float32), geo[f, G + D * s:G + D * (s + 1)])
     # halo records carry the halo cells' own LSQ values, displacements and eps
     tab = t["tab"]
+    ng, nd = (NL + 1) * D, NL * D
     for ti in range(0, t["n_tiles"], max(1, t["n_tiles"] // 7)):
         ho, n1, h1o = int(tab[ti, 0]), int(tab[ti, 1]), int(tab[ti, 10])
         cells = t["halo"][ho:ho + n1]
         rec = t["hrec"][h1o:h1o + n1]
-        assert np.array_equal(rec[:, :8], h["gval"][cells].astype(np.float32).transpose(0, 2, 1).reshape(n1, -1))
-        assert np.array_equal(rec[:, 8:14], h["disp_s"][cells].astype(np.float32).reshape(n1, -1))
-        assert np.array_equal(rec[:, 14], h["eps"][cells].astype(np.float32))
-        assert np.all(rec[:, 15] == 0)
+        assert np.array_equal(rec[:, :ng], h["gval"][cells].astype(np.float32).transpose(0, 2, 1).reshape(n1, -1))
+        assert np.array_equal(rec[:, ng:ng + nd], h["disp_s"][cells].astype(np.float32).reshape(n1, -1))
+        assert np.array_equal(rec[:, ng + nd], h["eps"][cells].astype(np.float32))
+        assert np.all(rec[:, ng + nd + 1:] == 0)
     # boundary slots are marked, interior ones point at staged cells
     own_rows = t["nbr"][np.repeat(tab[:, 3], TILE) + np.tile(np.arange(TILE), t["n_tiles"])][:dm.N, :NL]
     assert np.array_equal(own_rows == NONE16, h["nbr_s"] >= dm.N)
-    assert t["lds_bytes"] <= 64 * 1024
+    # the LDS the kernel launches with, and the per-thread facet and halo limits
+    assert t["lds_bytes"] == fused_lds_bytes(D, t["ncl"], t["ngl"], t["nfl"], dm.n_meshes > 1)
+    assert t["lds_bytes"] <= (64 if D == 2 else 160) * 1024
+    assert t["nfl"] <= facets_per_thread(D) * TILE and t["ngl"] <= 2 * TILE
 
 
 def test_batch_tables():

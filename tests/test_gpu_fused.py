@@ -12,21 +12,22 @@ import pytest
 import torch
 
 from fvm_solver_amd.fvm_stepping.facet_process import FacetCalc
-from fvm_solver_amd.mesh_gen.channel import channel_mesh_2d
+from fvm_solver_amd.mesh_gen.channel import channel_mesh_2d, channel_mesh_3d
 from tests.golden_cases import load, mesh_tuple
-from tests.gpu_helpers import DEV, initial_state, make_cfg, product_equation
+from tests.gpu_helpers import DEV, announce, initial_state, make_cfg, product_equation
 
 pytestmark = pytest.mark.gpu
 
 
 def _announce(msg):
-    print(f"[gpu-test] {msg}", flush=True)
+    announce(msg)
 
 
-def test_rhs_two_calls_bitwise():
-    d = load("ref_2d.npz")
+@pytest.mark.parametrize("fname", ["ref_2d.npz", "ref_3d.npz"])
+def test_rhs_two_calls_bitwise(fname):
+    d = load(fname)
     pts, cells, fi, fb, tags = mesh_tuple(d)
-    cfg = make_cfg(2)
+    cfg = make_cfg(cells.shape[1] - 1)
     mesh, eq = product_equation(pts, cells, fi, fb, tags, cfg)
     dm = eq.E_props.mesh
     assert dm.tiles is not None, dm.tiles_reason
@@ -47,20 +48,24 @@ def test_rhs_two_calls_bitwise():
         assert torch.equal(a, b), (k, float((a - b).abs().max()))
 
 
-def _channel_eq(n, solver, extra, seed=4):
-    pts, cells, (fi, fb), tags = channel_mesh_2d(n, seed=seed)
-    cfg = make_cfg(2, solver_name=solver, solver_extra=extra)
+def _channel_eq(n, solver, extra, seed=4, dim=2):
+    gen = channel_mesh_2d if dim == 2 else channel_mesh_3d
+    pts, cells, (fi, fb), tags = gen(n, seed=seed)
+    cfg = make_cfg(dim, solver_name=solver, solver_extra=extra)
     c = pts[cells].mean(1)
-    _, U0 = initial_state(c, cfg, 2)
+    _, U0 = initial_state(c, cfg, dim)
     mesh, eq = product_equation(pts, cells, fi, fb, tags, cfg, us_init=U0)
     return eq
 
 
+@pytest.mark.parametrize("dim,n", [(2, 20000), (3, 20000), (3, 300000)])
 @pytest.mark.parametrize("solver,extra", [("Butcher", "RK4"), ("Butcher_adapt", "RK3_SSP4")])
-def test_trajectories_bitwise(solver, extra):
+def test_trajectories_bitwise(solver, extra, dim, n):
     states = []
     for fused in (False, True):
-        eq = _channel_eq(20000, solver, extra)
+        eq = _channel_eq(n, solver, extra, dim=dim)
+        if fused:
+            assert eq.E_props.mesh.tiles is not None, eq.E_props.mesh.tiles_reason
         eq.E_props.set_fused(fused)
         assert eq.E_props.fused == fused
         ts = eq.t_solver
@@ -71,7 +76,7 @@ def test_trajectories_bitwise(solver, extra):
             dts.append(float(ts.dt))
         torch.cuda.synchronize()
         states.append((eq.cells.state.clone(), dts, eq.E_props.cell_grads.clone()))
-    _announce(f"{solver} {extra}: 12 steps fused vs three-kernel, dt {states[1][1][-1]:.4e}")
+    _announce(f"{dim}-D {n} cells {solver} {extra}: 12 steps fused vs three-kernel, dt {states[1][1][-1]:.4e}")
     assert torch.isfinite(states[1][0]).all()
     assert torch.equal(states[0][0], states[1][0])
     assert states[0][1] == states[1][1]

@@ -12,7 +12,7 @@ import torch
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libfvm_hip.so")
-ABI_VERSION = 10
+ABI_VERSION = 11
 
 FVM_ERR_ROW_OVERFLOW = 2
 
@@ -36,7 +36,7 @@ class FvmMeshDev(ctypes.Structure):
                 ("fz_fdesc", c_void_p), ("fz_ffac", c_void_p), ("fz_fdesc_x", c_void_p), ("fz_geo", c_void_p),
                 ("fz_geo_x", c_void_p), ("fz_cfac", c_void_p), ("fz_h1f", c_void_p), ("fz_ncl", i32), ("fz_ngl", i32),
                 ("fz_nfl", i32),
-                ("fz_pad", i32)]
+                ("fz_nh1", i32)]
 
 
 class FvmPhys(ctypes.Structure):

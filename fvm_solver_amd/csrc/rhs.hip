@@ -899,8 +899,10 @@ constexpr unsigned kFzSide1 = 0x8000u;   // H1 facet table: the H1 cell is the s
 
 template <int D>
 __host__ __device__ constexpr int fz_geo_n() { return D == 2 ? 8 : 12; }
+// H1 static record: LSQ values [NL+1][D], displacements [NL][D], eps; stored
+// component-major, [HN][sum n1] (a lane per H1 cell reads one coalesced row per component)
 template <int D>
-__host__ __device__ constexpr int fz_hrec_n() { return D == 2 ? 16 : 32; }
+__host__ __device__ constexpr int fz_hrec_n() { return (D + 2) * D + (D + 1) * D + 1; }
 template <int D>
 __host__ __device__ constexpr int fz_fpt() { return D == 2 ? 2 : 3; }   // facets per thread
 
@@ -1150,10 +1152,12 @@ k_fused(FvmMeshDev m, FvmPhys p, FvmWork w, const float *__restrict__ prim, floa
     int oinfo = 0;
     ushort4 onr = make_ushort4(0, 0, 0, 0), cf = make_ushort4(0, 0, 0, 0);
     if (has_own) {
-        const int64_t c = c0 + i;
         float u[C];
-        load_prim<D>(prim + c * C, u);
+        load_prim<D>(prim + (c0 + i) * C, u);
         store_vec<C>(Pl + i * C, u);
+    }
+    if (has_own) {
+        const int64_t c = c0 + i;
 #pragma unroll
         for (int jj = 0; jj <= NL; ++jj)
 #pragma unroll
@@ -1173,17 +1177,11 @@ k_fused(FvmMeshDev m, FvmPhys p, FvmWork w, const float *__restrict__ prim, floa
     ushort4 hnr = make_ushort4(0, 0, 0, 0), hf = make_ushort4(kFzNone, kFzNone, kFzNone, kFzNone);
     if (has_h1) {
         hgc = m.fz_halo[halo_off + i];
-        const float4 *r = reinterpret_cast<const float4 *>(m.fz_hrec + static_cast<int64_t>(h1_off + i) * HREC);
-        constexpr int HUSE = ((NL + 1) * D + NL * D + 1 + 3) / 4;   // float4s holding data
-        float rv[4 * HUSE];
+        const float *r = m.fz_hrec + h1_off + i;
+        const int64_t hs = m.fz_nh1;             // component stride of the H1 records
+        float rv[HREC];
 #pragma unroll
-        for (int q = 0; q < HUSE; ++q) {
-            const float4 t = r[q];
-            rv[4 * q] = t.x;
-            rv[4 * q + 1] = t.y;
-            rv[4 * q + 2] = t.z;
-            rv[4 * q + 3] = t.w;
-        }
+        for (int q = 0; q < HREC; ++q) rv[q] = r[q * hs];
 #pragma unroll
         for (int jj = 0; jj <= NL; ++jj)
 #pragma unroll

@@ -355,13 +355,14 @@ class DeviceMesh:
             return
         tl = self.tiles
         self.tz = dict(
-            fz_tile=up(tl["tab"], i32), fz_halo=up_nz(tl["halo"], i32), fz_hrec=up_nz(tl["hrec"], f32),
+            fz_tile=up(tl["tab"], i32), fz_halo=up_nz(tl["halo"], i32),
+            fz_hrec=up(tl["hrec"] if tl["nh1"] else np.zeros(1), f32),
             fz_nbr=up(tl["nbr"].view(np.int16), torch.int16), fz_fdesc=up(tl["fdesc"].view(np.int32), i32),
             fz_ffac=up_nz(tl["ffac"], i32), fz_fdesc_x=up_nz(tl["fdesc_x"].view(np.int32), i32),
             fz_geo=up(tl["geo"], f32), fz_geo_x=up_nz(tl["geo_x"], f32),
             fz_cfac=up(tl["cfac"].view(np.int16), torch.int16),
             fz_h1f=up_nz(tl["h1f"].view(np.int16) if len(tl["h1f"]) else np.zeros((1, 4), np.int16), torch.int16))
-        self.struct = nat.FvmMeshDev(**base, fz_ncl=tl["ncl"], fz_ngl=tl["ngl"], fz_nfl=tl["nfl"],
+        self.struct = nat.FvmMeshDev(**base, fz_ncl=tl["ncl"], fz_ngl=tl["ngl"], fz_nfl=tl["nfl"], fz_nh1=tl["nh1"],
                                      **{k: v.data_ptr() for k, v in self.tz.items()})
 
     def set_persist_blocks(self, n: int):

@@ -156,13 +156,14 @@ def build_tiles(N: int, D: int, nbr: np.ndarray, cell_facet: np.ndarray, fac_cel
     rows[h1_row, :NL] = np.where(nbr[h1c] < N, lh, NONE16).astype(np.uint16)
 
     # ---- H1 static records: LSQ values [NL+1][D] (slot order), displacements [NL][D], eps,
-    # zero padding to 16 / 32 floats ----
-    HREC = 16 if D == 2 else 32
-    hrec = np.zeros((len(K1), HREC), np.float32)
-    hrec[:, :(NL + 1) * D] = np.asarray(gval, np.float32)[h1c].transpose(0, 2, 1).reshape(len(K1), -1)
+    # stored component-major [HN][sum n1] (the kernel's lanes read one row per component) ----
+    HN = (NL + 1) * D + NL * D + 1
+    rec = np.zeros((len(K1), HN), np.float32)
+    rec[:, :(NL + 1) * D] = np.asarray(gval, np.float32)[h1c].transpose(0, 2, 1).reshape(len(K1), -1)
     o = (NL + 1) * D
-    hrec[:, o:o + NL * D] = np.asarray(disp, np.float32)[h1c].reshape(len(K1), -1)
-    hrec[:, o + NL * D] = np.asarray(eps, np.float32)[h1c]
+    rec[:, o:o + NL * D] = np.asarray(disp, np.float32)[h1c].reshape(len(K1), -1)
+    rec[:, o + NL * D] = np.asarray(eps, np.float32)[h1c]
+    hrec = np.ascontiguousarray(rec.T)
 
     # ---- facets ----
     lo = fac_cell.min(0)
@@ -287,7 +288,8 @@ def build_tiles(N: int, D: int, nbr: np.ndarray, cell_facet: np.ndarray, fac_cel
     if lds > lds_limit:
         raise TilesUnavailable(f"tile working set {lds} B exceeds {lds_limit} B of LDS")
     return dict(tab=tab, halo=halo, hrec=hrec, nbr=rows, h1f=h1f, fdesc=fdesc, ffac=fx.astype(np.int32),
-                fdesc_x=fdesc_x, geo=geo_c, geo_x=geo_x, cfac=cfac, ncl=ncl, ngl=ngl, nfl=nfl, lds_bytes=lds,
+                fdesc_x=fdesc_x, geo=geo_c, geo_x=geo_x, cfac=cfac, ncl=ncl, ngl=ngl, nfl=nfl, nh1=len(K1),
+                lds_bytes=lds,
                 lds_fits_target=None if lds_target is None else lds <= lds_target, n_tiles=T,
                 halo_cells=(int(n1.sum()), int(n2.sum())), foreign=int(nfor.sum()))
 

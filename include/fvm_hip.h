@@ -136,9 +136,10 @@ typedef struct FvmMeshDev {
     const int32_t *fz_tile;     /* [T][12] halo_off, n1, n2, nbr_off, fo0, nfo, foff,
                                    nfor, g0, ng (group means the tile needs), h1_off  */
     const int32_t *fz_halo;     /* H1 then H2 global ids of each tile, at halo_off    */
-    const float *fz_hrec;       /* [sum n1][16] (2-D) / [sum n1][32] (3-D) H1 records
-                                   at h1_off: LSQ values [NL+1][d], displacements
-                                   [NL][d], eps, zero padding                         */
+    const float *fz_hrec;       /* [HN][fz_nh1] H1 records, component-major (record of
+                                   H1 entry h1_off + i in column h1_off + i): LSQ
+                                   values [NL+1][d], displacements [NL][d], eps;
+                                   HN = 15 (2-D) / 28 (3-D)                            */
     const uint16_t *fz_nbr;     /* [rows][4] local neighbour ids (0xFFFF = boundary)
                                    of the tile's 256 + n1 cells, rows at nbr_off      */
     const uint32_t *fz_fdesc;   /* [F] descriptor of each facet in its owner tile     */
@@ -155,7 +156,7 @@ typedef struct FvmMeshDev {
     int32_t fz_ncl;             /* LDS rows: max staged cells (256 + n1 + n2)         */
     int32_t fz_ngl;             /*           max gradient cells (256 + n1)            */
     int32_t fz_nfl;             /*           max facets (nfo + nfor)                  */
-    int32_t fz_pad;
+    int32_t fz_nh1;             /* sum n1: H1 entries of all tiles (fz_hrec stride)   */
 } FvmMeshDev;
 
 typedef struct FvmPhys {      /* scalars of FluidConstitution / SlopeLimiter / KT      */

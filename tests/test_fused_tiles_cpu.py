@@ -89,11 +89,11 @@ def test_fused_inputs_are_the_same_floats(dm):
     for ti in range(0, t["n_tiles"], max(1, t["n_tiles"] // 7)):
         ho, n1, h1o = int(tab[ti, 0]), int(tab[ti, 1]), int(tab[ti, 10])
         cells = t["halo"][ho:ho + n1]
-        rec = t["hrec"][h1o:h1o + n1]
+        rec = t["hrec"][:, h1o:h1o + n1].T
         assert np.array_equal(rec[:, :ng], h["gval"][cells].astype(np.float32).transpose(0, 2, 1).reshape(n1, -1))
         assert np.array_equal(rec[:, ng:ng + nd], h["disp_s"][cells].astype(np.float32).reshape(n1, -1))
         assert np.array_equal(rec[:, ng + nd], h["eps"][cells].astype(np.float32))
-        assert np.all(rec[:, ng + nd + 1:] == 0)
+        assert rec.shape[1] == ng + nd + 1 and t["hrec"].shape[1] == t["nh1"]
     # boundary slots are marked, interior ones point at staged cells
     own_rows = t["nbr"][np.repeat(tab[:, 3], TILE) + np.tile(np.arange(TILE), t["n_tiles"])][:dm.N, :NL]
     assert np.array_equal(own_rows == NONE16, h["nbr_s"] >= dm.N)

@@ -12,7 +12,7 @@ import torch
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libfvm_hip.so")
-ABI_VERSION = 11
+ABI_VERSION = 12
 
 FVM_ERR_ROW_OVERFLOW = 2
 
@@ -64,6 +64,9 @@ _PROTOS = {
     "fvm_abi_version": (ctypes.c_int, []),
     "fvm_ell_spmm_f32": (ctypes.c_int, [c_void_p, i64, i64, c_void_p, i32, i64, i64, i64, i32, c_void_p, i64, i64,
                                         i32, c_void_p, c_void_p, i64, i64, c_void_p]),
+    "fvm_ell_spmm_f32_shape": (ctypes.c_int, [c_void_p, i64, i64, c_void_p, i32, i64, i64, i64, i32, c_void_p, i64,
+                                              i64, i32, c_void_p, c_void_p, i64, i64, i32, i32, c_void_p]),
+    "fvm_ell_fast_path": (ctypes.c_int, [i64, i64, i32, i64, i64, i32, i64, i64, i32, i64, i64, c_void_p, c_void_p]),
     "fvm_csr_max_row_nnz": (ctypes.c_int, [c_void_p, i64, c_void_p, c_void_p]),
     "fvm_ell_pack_f32": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, i64, i32, c_void_p, c_void_p, i64, c_void_p,
                                         c_void_p]),

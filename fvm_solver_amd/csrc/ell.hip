@@ -51,71 +51,91 @@ k_ell_spmm(const float *__restrict__ vals, int64_t vsm, int64_t vsk, const CT *_
 // are loaded first, then the K gathered rows of x, then the products in slot order, so
 // a row costs two memory round trips instead of one per slot; with B = 4 the x rows and
 // the output row move as 16-byte vectors.  Same products and summation order as
-// k_ell_spmm.
-template <int B, int KT>
-__global__ void __launch_bounds__(256)
+// k_ell_spmm.  Launch shape (the autotuner's choice): RPL rows per lane, the lanes of a
+// block on consecutive rows in each of RPL row groups (so every slot load of a wave stays
+// one contiguous segment), blockDim.x threads per block.
+template <int B, int KT, int RPL>
+__global__ void __launch_bounds__(512)
 k_ell_spmm_fast(const float *__restrict__ vals, const int32_t *__restrict__ cols, int64_t vsk, int64_t M,
                 const float *__restrict__ x, const float *__restrict__ bias, float *__restrict__ y) {
-    const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (r >= M) return;
-    float v[KT];
-    int32_t c[KT];
+    const int64_t r0 = static_cast<int64_t>(blockIdx.x) * blockDim.x * RPL + threadIdx.x;
+    float v[RPL][KT];
+    int32_t c[RPL][KT];
 #pragma unroll
-    for (int k = 0; k < KT; ++k) {
-        v[k] = vals[r + k * vsk];
-        c[k] = cols[r + k * vsk];
-    }
-    float acc[B];
-#pragma unroll
-    for (int j = 0; j < B; ++j) acc[j] = 0.0f;
-    if constexpr (B == 4) {
-        float4 xr[KT];
-#pragma unroll
-        for (int k = 0; k < KT; ++k) xr[k] = reinterpret_cast<const float4 *>(x)[c[k]];
+    for (int q = 0; q < RPL; ++q) {
+        const int64_t r = r0 + static_cast<int64_t>(q) * blockDim.x;
 #pragma unroll
         for (int k = 0; k < KT; ++k) {
-            acc[0] += v[k] * xr[k].x;
-            acc[1] += v[k] * xr[k].y;
-            acc[2] += v[k] * xr[k].z;
-            acc[3] += v[k] * xr[k].w;
+            v[q][k] = r < M ? vals[r + k * vsk] : 0.0f;
+            c[q][k] = r < M ? cols[r + k * vsk] : 0;
         }
-    } else {
-        float xr[KT][B];
-#pragma unroll
-        for (int k = 0; k < KT; ++k)
-#pragma unroll
-            for (int j = 0; j < B; ++j) xr[k][j] = x[static_cast<int64_t>(c[k]) * B + j];
-#pragma unroll
-        for (int k = 0; k < KT; ++k)
-#pragma unroll
-            for (int j = 0; j < B; ++j) acc[j] += v[k] * xr[k][j];
     }
-    if (bias) {
-        const float bb = bias[r];
 #pragma unroll
-        for (int j = 0; j < B; ++j) acc[j] = acc[j] + bb;
+    for (int q = 0; q < RPL; ++q) {
+        const int64_t r = r0 + static_cast<int64_t>(q) * blockDim.x;
+        if (r >= M) break;
+        float acc[B];
+#pragma unroll
+        for (int j = 0; j < B; ++j) acc[j] = 0.0f;
+        if constexpr (B == 4) {
+            float4 xr[KT];
+#pragma unroll
+            for (int k = 0; k < KT; ++k) xr[k] = reinterpret_cast<const float4 *>(x)[c[q][k]];
+#pragma unroll
+            for (int k = 0; k < KT; ++k) {
+                acc[0] += v[q][k] * xr[k].x;
+                acc[1] += v[q][k] * xr[k].y;
+                acc[2] += v[q][k] * xr[k].z;
+                acc[3] += v[q][k] * xr[k].w;
+            }
+        } else {
+            float xr[KT][B];
+#pragma unroll
+            for (int k = 0; k < KT; ++k)
+#pragma unroll
+                for (int j = 0; j < B; ++j) xr[k][j] = x[static_cast<int64_t>(c[q][k]) * B + j];
+#pragma unroll
+            for (int k = 0; k < KT; ++k)
+#pragma unroll
+                for (int j = 0; j < B; ++j) acc[j] += v[q][k] * xr[k][j];
+        }
+        if (bias) {
+            const float bb = bias[r];
+#pragma unroll
+            for (int j = 0; j < B; ++j) acc[j] = acc[j] + bb;
+        }
+        if constexpr (B == 4) {
+            reinterpret_cast<float4 *>(y)[r] = make_float4(acc[0], acc[1], acc[2], acc[3]);
+        } else {
+#pragma unroll
+            for (int j = 0; j < B; ++j) y[r * B + j] = acc[j];
+        }
     }
-    if constexpr (B == 4) {
-        reinterpret_cast<float4 *>(y)[r] = make_float4(acc[0], acc[1], acc[2], acc[3]);
-    } else {
-#pragma unroll
-        for (int j = 0; j < B; ++j) y[r * B + j] = acc[j];
+}
+
+template <int B, int RPL>
+static bool launch_spmm_fast_rpl(const float *vals, const int32_t *cols, int64_t vsk, int64_t M, int K,
+                                 const float *x, const float *bias, float *y, int block, hipStream_t s) {
+    const dim3 grid(blocks_for(M, block * RPL)), tb(block);
+    switch (K) {
+        case 1: hipLaunchKernelGGL((k_ell_spmm_fast<B, 1, RPL>), grid, tb, 0, s, vals, cols, vsk, M, x, bias, y); return true;
+        case 2: hipLaunchKernelGGL((k_ell_spmm_fast<B, 2, RPL>), grid, tb, 0, s, vals, cols, vsk, M, x, bias, y); return true;
+        case 3: hipLaunchKernelGGL((k_ell_spmm_fast<B, 3, RPL>), grid, tb, 0, s, vals, cols, vsk, M, x, bias, y); return true;
+        case 4: hipLaunchKernelGGL((k_ell_spmm_fast<B, 4, RPL>), grid, tb, 0, s, vals, cols, vsk, M, x, bias, y); return true;
+        case 5: hipLaunchKernelGGL((k_ell_spmm_fast<B, 5, RPL>), grid, tb, 0, s, vals, cols, vsk, M, x, bias, y); return true;
+        case 6: hipLaunchKernelGGL((k_ell_spmm_fast<B, 6, RPL>), grid, tb, 0, s, vals, cols, vsk, M, x, bias, y); return true;
+        case 8: hipLaunchKernelGGL((k_ell_spmm_fast<B, 8, RPL>), grid, tb, 0, s, vals, cols, vsk, M, x, bias, y); return true;
+        default: return false;
     }
 }
 
 template <int B>
 static bool launch_spmm_fast(const float *vals, const int32_t *cols, int64_t vsk, int64_t M, int K, const float *x,
-                             const float *bias, float *y, hipStream_t s) {
-    const dim3 grid(blocks_for(M, 256));
-    switch (K) {
-        case 1: hipLaunchKernelGGL((k_ell_spmm_fast<B, 1>), grid, dim3(256), 0, s, vals, cols, vsk, M, x, bias, y); return true;
-        case 2: hipLaunchKernelGGL((k_ell_spmm_fast<B, 2>), grid, dim3(256), 0, s, vals, cols, vsk, M, x, bias, y); return true;
-        case 3: hipLaunchKernelGGL((k_ell_spmm_fast<B, 3>), grid, dim3(256), 0, s, vals, cols, vsk, M, x, bias, y); return true;
-        case 4: hipLaunchKernelGGL((k_ell_spmm_fast<B, 4>), grid, dim3(256), 0, s, vals, cols, vsk, M, x, bias, y); return true;
-        case 5: hipLaunchKernelGGL((k_ell_spmm_fast<B, 5>), grid, dim3(256), 0, s, vals, cols, vsk, M, x, bias, y); return true;
-        case 6: hipLaunchKernelGGL((k_ell_spmm_fast<B, 6>), grid, dim3(256), 0, s, vals, cols, vsk, M, x, bias, y); return true;
-        case 8: hipLaunchKernelGGL((k_ell_spmm_fast<B, 8>), grid, dim3(256), 0, s, vals, cols, vsk, M, x, bias, y); return true;
-        default: return false;
+                             const float *bias, float *y, int rpl, int block, hipStream_t s) {
+    switch (rpl) {
+        case 2: return launch_spmm_fast_rpl<B, 2>(vals, cols, vsk, M, K, x, bias, y, block, s);
+        case 4: return launch_spmm_fast_rpl<B, 4>(vals, cols, vsk, M, K, x, bias, y, block, s);
+        default: return launch_spmm_fast_rpl<B, 1>(vals, cols, vsk, M, K, x, bias, y, block, s);
     }
 }
 
@@ -180,13 +200,19 @@ extern "C" {
 
 const char *fvm_last_error(void) { return g_err; }
 
-int fvm_abi_version(void) { return 11; }
+int fvm_abi_version(void) { return 12; }
 
-int fvm_ell_spmm_f32(const float *vals, int64_t vsm, int64_t vsk, const void *cols, int32_t cols_is64, int64_t csm,
-                     int64_t csk, int64_t M, int32_t K, const float *x, int64_t xsn, int64_t xsb, int32_t B,
-                     const float *bias, float *y, int64_t ysm, int64_t ysb, void *stream) {
+int fvm_ell_spmm_f32_shape(const float *vals, int64_t vsm, int64_t vsk, const void *cols, int32_t cols_is64,
+                           int64_t csm, int64_t csk, int64_t M, int32_t K, const float *x, int64_t xsn, int64_t xsb,
+                           int32_t B, const float *bias, float *y, int64_t ysm, int64_t ysb, int32_t rows_per_lane,
+                           int32_t block_threads, void *stream) {
     if (M < 0 || K < 0 || B < 1 || (M > 0 && (!y || (K > 0 && (!vals || !cols || !x))))) {
         set_error("fvm_ell_spmm_f32: invalid arguments (M=%lld K=%d B=%d)", (long long)M, K, B);
+        return FVM_ERR_ARG;
+    }
+    if ((rows_per_lane != 1 && rows_per_lane != 2 && rows_per_lane != 4) ||
+        (block_threads != 64 && block_threads != 128 && block_threads != 256 && block_threads != 512)) {
+        set_error("fvm_ell_spmm_f32_shape: rows_per_lane must be 1, 2 or 4 and block_threads 64, 128, 256 or 512");
         return FVM_ERR_ARG;
     }
     if (M == 0) return FVM_OK;
@@ -197,8 +223,9 @@ int fvm_ell_spmm_f32(const float *vals, int64_t vsm, int64_t vsk, const void *co
     if (!cols_is64 && K >= 1 && vsm == 1 && csm == 1 && csk == vsk && xsb == 1 && ysb == 1 && xsn == B &&
         ysm == B && (B == 1 || (B == 4 && aligned4))) {
         const int32_t *c32 = static_cast<const int32_t *>(cols);
-        const bool done = B == 4 ? launch_spmm_fast<4>(vals, c32, vsk, M, K, x, bias, y, s)
-                                 : launch_spmm_fast<1>(vals, c32, vsk, M, K, x, bias, y, s);
+        const bool done =
+            B == 4 ? launch_spmm_fast<4>(vals, c32, vsk, M, K, x, bias, y, rows_per_lane, block_threads, s)
+                   : launch_spmm_fast<1>(vals, c32, vsk, M, K, x, bias, y, rows_per_lane, block_threads, s);
         if (done) return check_launch("fvm_ell_spmm_f32");
     }
     if (cols_is64)
@@ -208,6 +235,20 @@ int fvm_ell_spmm_f32(const float *vals, int64_t vsm, int64_t vsk, const void *co
         dispatch_spmm(vals, vsm, vsk, static_cast<const int32_t *>(cols), csm, csk, M, K, x, xsn, xsb, B, bias, y,
                       ysm, ysb, s);
     return check_launch("fvm_ell_spmm_f32");
+}
+
+int fvm_ell_spmm_f32(const float *vals, int64_t vsm, int64_t vsk, const void *cols, int32_t cols_is64, int64_t csm,
+                     int64_t csk, int64_t M, int32_t K, const float *x, int64_t xsn, int64_t xsb, int32_t B,
+                     const float *bias, float *y, int64_t ysm, int64_t ysb, void *stream) {
+    return fvm_ell_spmm_f32_shape(vals, vsm, vsk, cols, cols_is64, csm, csk, M, K, x, xsn, xsb, B, bias, y, ysm, ysb,
+                                  1, 256, stream);
+}
+
+int fvm_ell_fast_path(int64_t vsm, int64_t vsk, int32_t cols_is64, int64_t csm, int64_t csk, int32_t K,
+                      int64_t xsn, int64_t xsb, int32_t B, int64_t ysm, int64_t ysb, const void *x, const void *y) {
+    const bool aligned4 = ((reinterpret_cast<uintptr_t>(x) | reinterpret_cast<uintptr_t>(y)) & 15u) == 0;
+    return !cols_is64 && K >= 1 && K <= 8 && K != 7 && vsm == 1 && csm == 1 && csk == vsk && xsb == 1 && ysb == 1 &&
+           xsn == B && ysm == B && (B == 1 || (B == 4 && aligned4));
 }
 
 int fvm_csr_max_row_nnz(const int64_t *crow, int64_t M, int32_t *out_max, void *stream) {

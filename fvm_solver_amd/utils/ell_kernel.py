@@ -18,6 +18,41 @@ import torch
 from fvm_solver_amd import _native as nat
 
 
+# Launch shapes of the fast ELL kernel the autotuner tries: (rows per lane, threads per
+# block).  All compute bitwise the same products.
+SHAPES = tuple((r, t) for r in (1, 2, 4) for t in (64, 128, 256, 512))
+_TUNED: dict = {}          # (M, B, K, has_bias) -> {"shape": (rows_per_lane, block), "us": {...}}
+_TUNE_REPS = 5
+
+
+def _launch(args, shape, stream):
+    return nat.lib().fvm_ell_spmm_f32_shape(*args, shape[0], shape[1], stream)
+
+
+def _autotune(key, args, stream):
+    """Time every launch shape on the live operands (the product is written each time,
+    identically) and keep the fastest for ``key``, as the reference's Triton autotune does
+    on the first call per key (utils/ell_kernel.py:8-23, 83-98)."""
+    us = {}
+    for shape in SHAPES:
+        nat.check(_launch(args, shape, stream), "fvm_ell_spmm_f32_shape")      # warm-up
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(_TUNE_REPS):
+            _launch(args, shape, stream)
+        e1.record()
+        torch.cuda.synchronize()
+        us[shape] = 1e3 * e0.elapsed_time(e1) / _TUNE_REPS
+    best = min(us, key=us.get)
+    _TUNED[key] = {"shape": best, "us": {f"{r}x{t}": v for (r, t), v in us.items()}}
+    return best
+
+
+def best_config(M: int, B: int, K: int, has_bias: bool = False):
+    """The tuned launch shape for a key, or None before its first product."""
+    return _TUNED.get((M, B, K, bool(has_bias)))
+
+
 def _spmm_into(vals: torch.Tensor, cols: torch.Tensor, x2: torch.Tensor, b, y: torch.Tensor):
     M, K = vals.shape
     N, B = x2.shape
@@ -35,11 +70,18 @@ def _spmm_into(vals: torch.Tensor, cols: torch.Tensor, x2: torch.Tensor, b, y: t
             raise ValueError(f"bias has {b.numel()} entries, expected {M}")
         nat.require_device(b, "b")
         b = b.contiguous()
-    rc = nat.lib().fvm_ell_spmm_f32(
-        nat.ptr(vals), vals.stride(0), vals.stride(1), nat.ptr(cols), int(cols.dtype == torch.int64),
-        cols.stride(0), cols.stride(1), M, K, nat.ptr(x2), x2.stride(0), x2.stride(1), B, nat.ptr(b),
-        nat.ptr(y), y.stride(0), y.stride(1), nat.stream_ptr(x2.device))
-    nat.check(rc, "fvm_ell_spmm_f32")
+    is64 = int(cols.dtype == torch.int64)
+    args = (nat.ptr(vals), vals.stride(0), vals.stride(1), nat.ptr(cols), is64, cols.stride(0), cols.stride(1), M, K,
+            nat.ptr(x2), x2.stride(0), x2.stride(1), B, nat.ptr(b), nat.ptr(y), y.stride(0), y.stride(1))
+    stream = nat.stream_ptr(x2.device)
+    shape = (1, 256)
+    if M > 0 and nat.lib().fvm_ell_fast_path(vals.stride(0), vals.stride(1), is64, cols.stride(0), cols.stride(1), K,
+                                              x2.stride(0), x2.stride(1), B, y.stride(0), y.stride(1),
+                                              nat.ptr(x2), nat.ptr(y)):
+        key = (M, B, K, b is not None)
+        tuned = _TUNED.get(key)
+        shape = tuned["shape"] if tuned is not None else _autotune(key, args, stream)
+    nat.check(_launch(args, shape, stream), "fvm_ell_spmm_f32")
     return y
 
 
@@ -65,9 +107,15 @@ def ell_spmm(vals, cols, x, b=None):
 
 
 def print_best_config():
-    """The reference prints Triton's autotuned block shape (ell_kernel.py:213-222); these
-    kernels have one fixed launch shape: 256-lane blocks, one row per lane."""
-    print("HIP ELL kernel: 256 threads/block, one row per lane, slot-major ELL")
+    """Print the launch shapes the autotuner chose, per (M, B, K, has_bias) key, as the
+    reference prints its Triton best config (ell_kernel.py:213-222)."""
+    if not _TUNED:
+        print("No best config yet — run the kernel first.")
+        return
+    for (M, B, K, hb), t in _TUNED.items():
+        r, th = t["shape"]
+        print(f"HIP ELL config M={M} B={B} K={K} bias={hb}: {r} row(s) per lane, {th} threads per block "
+              f"({t['us'][f'{r}x{th}']:.1f} us)")
 
 
 def csr_to_ell_slot_major(csr: torch.Tensor, K: int | None = None):

@@ -50,6 +50,26 @@ int fvm_ell_spmm_f32(const float *vals, int64_t vals_stride_m, int64_t vals_stri
                      const float *bias,
                      float *y, int64_t y_stride_m, int64_t y_stride_b, void *stream);
 
+/* fvm_ell_spmm_f32 with an explicit launch shape for the fast path (slot-major int32
+ * ELL, contiguous x and y, B in {1, 4}, K <= 8 except 7): rows_per_lane in {1, 2, 4},
+ * block_threads in {64, 128, 256, 512}.  Every shape computes bitwise the same y; the
+ * autotuner times them (the reference autotunes its Triton kernels over block shape,
+ * warps and stages, utils/ell_kernel.py:8-23, 83-98, keyed on M, B, K and the bias).
+ * fvm_ell_spmm_f32 is this call with (1, 256).  Other layouts ignore the shape. */
+int fvm_ell_spmm_f32_shape(const float *vals, int64_t vals_stride_m, int64_t vals_stride_k,
+                           const void *cols, int32_t cols_is64, int64_t cols_stride_m, int64_t cols_stride_k,
+                           int64_t M, int32_t K,
+                           const float *x, int64_t x_stride_n, int64_t x_stride_b, int32_t B,
+                           const float *bias,
+                           float *y, int64_t y_stride_m, int64_t y_stride_b,
+                           int32_t rows_per_lane, int32_t block_threads, void *stream);
+
+/* 1 if a product with these strides, K, B and base pointers takes the fast path (the
+ * launch shape then matters), else 0.  No device access. */
+int fvm_ell_fast_path(int64_t vals_stride_m, int64_t vals_stride_k, int32_t cols_is64, int64_t cols_stride_m,
+                      int64_t cols_stride_k, int32_t K, int64_t x_stride_n, int64_t x_stride_b, int32_t B,
+                      int64_t y_stride_m, int64_t y_stride_b, const void *x, const void *y);
+
 /* ---------------------------------------------------------------------------------
  * CSR -> ELL packing.  Replaces csr_to_ell (utils/ell_kernel.py:225-256): slot k of
  * row r holds the k-th stored entry of the row, padding is col 0 / val 0.

@@ -60,6 +60,37 @@ def test_ell_spmm_random(M, N, K, B, bias):
     assert y1.shape == (M,)
 
 
+def test_ell_launch_shapes_bitwise_and_autotuned(capsys):
+    """Every launch shape the autotuner may pick computes bitwise the same product (same
+    products, same slot order), the first call per (M, B, K, bias) key records its choice,
+    and print_best_config reports it."""
+    from fvm_solver_amd import _native as nat
+    from fvm_solver_amd.utils import ell_kernel as ek
+    rng = np.random.default_rng(5)
+    for M, N, K, B in ((100_003, 90_000, 3, 4), (70_001, 70_001, 2, 1), (4097, 5000, 5, 4)):
+        announce_shape = f"ELL shapes M={M} K={K} B={B}"
+        print(f"[gpu-test] {announce_shape}", flush=True)
+        vals = torch.from_numpy(rng.standard_normal((K, M)).astype(np.float32)).to(DEV)
+        cols = torch.from_numpy(rng.integers(0, N, size=(K, M)).astype(np.int32)).to(DEV)
+        x = torch.from_numpy(rng.standard_normal((N, B)).astype(np.float32)).to(DEV)
+        bias = torch.from_numpy(rng.standard_normal(M).astype(np.float32)).to(DEV)
+        v, c = vals.t(), cols.t()
+        ref = (ell_spmm(v, c, x, bias) if B > 1 else ell_spmv(v, c, x[:, 0], bias)[:, None]).clone()
+        assert ek.best_config(M, B, K, True) is not None
+        xs = x if B > 1 else x[:, :1]
+        for shape in ek.SHAPES:
+            y = torch.empty(M, B, device=DEV)
+            args = (nat.ptr(v), v.stride(0), v.stride(1), nat.ptr(c), 0, c.stride(0), c.stride(1), M, K, nat.ptr(xs),
+                    xs.stride(0), xs.stride(1), B, nat.ptr(bias), nat.ptr(y), y.stride(0), y.stride(1))
+            nat.check(ek._launch(args, shape, nat.stream_ptr(torch.device(DEV))), "shape")
+            assert torch.equal(y, ref), shape
+        dense = (vals.double().cpu().numpy()[:, :, None] * x.double().cpu().numpy()[cols.cpu().numpy()]).sum(0)
+        dense += bias.double().cpu().numpy()[:, None]
+        assert np.all(rel_l2_cols(ref.cpu().numpy(), dense) <= 1e-6)
+    ek.print_best_config()
+    assert "HIP ELL config M=100003 B=4 K=3 bias=True" in capsys.readouterr().out
+
+
 def test_csr_to_ell_bit_exact_and_overflow():
     d = load("ref_csr_to_ell.npz")
     for i in range(3):

@@ -71,3 +71,16 @@ def reorder_cells(points, cells, method: str = "rcm"):
     """(cells_new, perm): the cell array in the new order and the new -> old map."""
     perm = cell_order(points, cells, method)
     return np.asarray(cells)[perm], perm
+
+
+def permuted_mesh(mesh, method: str = "morton", device=None):
+    """(mesh_new, perm): the FVMMesh with its cells renumbered (cells[perm]; vertices,
+    facets and the boundary mask unchanged) and the new -> old map."""
+    import torch
+    pts = mesh.vertices.cpu().numpy()
+    cells = mesh.cells.cpu().numpy()
+    cells_new, perm = reorder_cells(pts, cells, method)
+    cls = type(mesh)
+    new = cls(mesh.vertices, torch.from_numpy(np.ascontiguousarray(cells_new)).to(mesh.cells.dtype), mesh.facets,
+              mesh.bc_facet_mask, device=device if device is not None else mesh.device)
+    return new, perm

@@ -169,7 +169,8 @@ class TSolver(ABC):
 
     def _forward_state(self, U):
         prim, _ = self.cells.convert_state_to_value(U)
-        return self.eq.forward(prim)
+        fwd = getattr(self.eq, "forward_device", None)    # the solver's states are in device order
+        return fwd(prim) if fwd is not None else self.eq.forward(prim)
 
     def _euler_step(self, U):
         return U + self.dt * self._forward_state(U)

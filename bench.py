@@ -45,9 +45,10 @@ def _parse():
     ap.add_argument("--kernel-timing-steps", type=int, default=20)
     ap.add_argument("--mesh-cache", default=None, help="npz file to reuse the generated mesh from (created if absent)")
     ap.add_argument("--no-autotune", action="store_true")
-    ap.add_argument("--traffic-file", default=os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
-                                                           "rk4_channel2d_1M_traffic.json"),
-                    help="measured HBM bytes per launch (rocprofv3 FETCH_SIZE/WRITE_SIZE, calibrated)")
+    ap.add_argument("--traffic-file", default=None,
+                    help="measured HBM bytes per launch (rocprofv3 FETCH_SIZE/WRITE_SIZE, calibrated); default "
+                         "profiles/rk4_channel2d_1M_traffic.json, rk4_channel3d_4M_traffic.json or "
+                         "rk4_batch64_traffic.json by --config (used when its cell count matches the run)")
     ap.add_argument("--no-spmv", action="store_true",
                     help="skip timing the drop-in ELL SpMV/SpMM (ell_spmm, ell_spmv) on the reference's operators of "
                          "the mesh (G_mats interleaved, flux_mat, A_face_grad's pattern), reported as config.spmv")
@@ -82,6 +83,24 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+_TRAFFIC = {"channel2d": "rk4_channel2d_1M_traffic.json", "channel3d": "rk4_channel3d_4M_traffic.json",
+            "batch": "rk4_batch64_traffic.json"}
+
+
+def traffic_of(a, kernel, cells):
+    """Calibrated rocprofv3 HBM bytes per launch of ``kernel`` for this workload (read +
+    write), or None when the file is absent or measured on a different cell count."""
+    path = a.traffic_file or os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                                          _TRAFFIC[a.config])
+    if not os.path.exists(path):
+        return None, None
+    tr = json.load(open(path))
+    k = tr.get("kernels", {}).get(kernel)
+    if not k or tr.get("cells") != cells:
+        return None, None
+    return k["read_bytes"] + k["write_bytes"], os.path.basename(path)
+
+
 def host_cpu():
     """CPU model, physical cores (sockets x cores per socket, from lscpu) and the CPUs
     this process may run on."""
@@ -101,6 +120,17 @@ def host_cpu():
     except AttributeError:
         info["affinity_cpus"] = os.cpu_count()
     return info
+
+
+def tile_summary(dm):
+    """Tile tables of the fused stage: tiles, LDS per workgroup, halo cells (H1, H2)
+    and foreign facets in total."""
+    t = dm.tiles
+    if t is None:
+        return {"reason": dm.tiles_reason}
+    return {"n": int(t["n_tiles"]), "lds_bytes": int(t["lds_bytes"]), "h1_cells": int(t["halo_cells"][0]),
+            "h2_cells": int(t["halo_cells"][1]), "foreign_facets": int(t["foreign"]),
+            "ncl": int(t["ncl"]), "ngl": int(t["ngl"]), "nfl": int(t["nfl"])}
 
 
 def facet_kernel_bytes(N, F, Fb, D):
@@ -183,46 +213,72 @@ def spmv_operators(mesh, C):
             "A_face_grad": (A.coalesce().to_sparse_csr(), (C * N,))}
 
 
-def spmv_bench(mesh, C, dev, reps=50):
+def spmv_bench(mesh, C, dev, reps=50, cold_bytes=768e6, max_copies=16):
     """Per operator: ELL packing (csr_to_ell), the launch-shape autotuning of its first
-    product, then reps products timed with stream events; GB/s from the algorithmic bytes
+    product, then products timed with stream events; GB/s from the algorithmic bytes
     nnz (4 B value + 4 B column) + the right-hand side's used columns + the output (the
-    SpMV figure of the reference's operators).  Working sets under 256 MB can stay in the
-    MALL between the back-to-back products (`mall_caveat`): the GB/s then measure the
-    kernel's access pattern, not proven HBM bytes."""
-    from fvm_solver_amd.utils.ell_kernel import csr_to_ell, ell_spmm, ell_spmv
+    SpMV figure of the reference's operators).
+
+    ``us`` (cold): the products cycle through R copies of the operator, right-hand side
+    and output, R x working set >= 768 MB (three times the 256 MB MALL, at most 16
+    copies), so each product's operands were last touched R - 1 products earlier and
+    have been evicted from the MALL: HBM rates.  ``us_hot``: reps products on one copy
+    back to back (a working set under 256 MB can stay in the MALL between them).
+    Both launch the tuned shape through the C ABI directly."""
+    from fvm_solver_amd import _native as nat
+    from fvm_solver_amd.utils import ell_kernel as ek
     out = {}
     g = torch.Generator().manual_seed(7)
+    stream = nat.stream_ptr(torch.device(dev))
     for name, (A, xshape) in spmv_operators(mesh, C).items():
-        vals, cols = csr_to_ell(A.to(dev))
+        vals, cols = ek.csr_to_ell(A.to(dev))
         x = torch.randn(*xshape, generator=g).to(dev)
-        fn = ell_spmv if len(xshape) == 1 else ell_spmm
-        y = fn(vals, cols, x)
+        fn = ek.ell_spmv if len(xshape) == 1 else ek.ell_spmm
+        y = fn(vals, cols, x)                          # packs, autotunes, checks
         ref = (A.to_dense() @ x.cpu()) if A.shape[0] * A.shape[1] < 2 ** 26 else None
-        torch.cuda.synchronize()
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record()
-        for _ in range(reps):
-            y = fn(vals, cols, x)
-        e1.record()
-        torch.cuda.synchronize()
-        us = e0.elapsed_time(e1) * 1e3 / reps
         B = 1 if len(xshape) == 1 else xshape[1]
-        M = A.shape[0]
+        M, K = vals.shape
         nnz = int(A.values().numel())
         ncols = int(torch.unique(A.col_indices()).numel())
         nbytes = nnz * 8 + ncols * B * 4 + M * B * 4
-        from fvm_solver_amd.utils.ell_kernel import best_config
-        tuned = best_config(M, B, int(vals.shape[1]), False)
-        out[name] = {"rows": M, "K": int(vals.shape[1]), "nnz": nnz, "B": B, "us": us,
-                     "algorithmic_bytes": nbytes, "gbs": nbytes / (us * 1e-6) / 1e9,
+        tuned = ek.best_config(M, B, int(K), False)
+        shape = tuned["shape"] if tuned is not None else (1, 256)
+        R = int(min(max_copies, max(2, -(-cold_bytes // nbytes))))
+        copies = []
+        for r in range(R):
+            v = vals if r == 0 else vals.t().contiguous().t()      # the same slot-major layout
+            c = cols if r == 0 else cols.t().contiguous().t()
+            x2 = (x if r == 0 else x.clone()).reshape(x.shape[0], -1)
+            yr = torch.empty(M, B, device=dev)
+            copies.append(((nat.ptr(v), v.stride(0), v.stride(1), nat.ptr(c), 0, c.stride(0), c.stride(1), M, K,
+                            nat.ptr(x2), x2.stride(0), x2.stride(1), B, 0, nat.ptr(yr), yr.stride(0), yr.stride(1)),
+                           (v, c, x2, yr)))
+        torch.cuda.synchronize()
+
+        def timed(seq):
+            for a, _ in seq[:2]:
+                ek._launch(a, shape, stream)
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for a, _ in seq:
+                ek._launch(a, shape, stream)
+            e1.record()
+            torch.cuda.synchronize()
+            return e0.elapsed_time(e1) * 1e3 / len(seq)
+
+        n_cold = R * max(1, -(-reps // R))
+        us_cold = timed([copies[i % R] for i in range(n_cold)])
+        us_hot = timed([copies[0]] * reps)
+        ok = all(torch.equal(cp[1][3], y.reshape(M, B)) for cp in copies)
+        out[name] = {"rows": M, "K": int(K), "nnz": nnz, "B": B, "us": us_cold, "us_hot": us_hot,
+                     "algorithmic_bytes": nbytes, "gbs": nbytes / (us_cold * 1e-6) / 1e9,
+                     "gbs_hot": nbytes / (us_hot * 1e-6) / 1e9, "copies": R, "copies_equal": ok,
                      "launch_shape": None if tuned is None else
-                     {"rows_per_lane": tuned["shape"][0], "threads_per_block": tuned["shape"][1],
-                      "candidates_us": tuned["us"]},
-                     "working_set_mb": nbytes / 1e6,
-                     "mall_caveat": nbytes < 256e6}
+                     {"rows_per_lane": shape[0], "threads_per_block": shape[1], "candidates_us": tuned["us"]},
+                     "working_set_mb": nbytes / 1e6}
         if ref is not None:
             out[name]["rel_err"] = float((y.cpu() - ref).norm() / ref.norm())
+        del copies
     return out
 
 
@@ -436,15 +492,15 @@ def main():
         out["config"]["cell_size"] = {"min_h": float(hc.min()), "mean_h": float(hc.mean()),
                                       "min_over_mean": float(hc.min() / hc.mean())}
     out["config"]["persist_blocks"] = tuned["persist_blocks"] if tuned else 0
+    out["config"]["tiles"] = tile_summary(E.mesh)
     if tuned:
         out["config"]["autotune_us_per_step"] = tuned["us_per_step"]
     out["config"]["algorithmic_bytes_per_launch"] = dom_bytes
-    if a.traffic_file and os.path.exists(a.traffic_file) and not ab3 and E.fused:
-        tr = json.load(open(a.traffic_file))
-        k = tr.get("kernels", {}).get(dom)
-        if k and tr.get("cells") == N:
-            out["roofline"]["traffic"] = k["read_bytes"] + k["write_bytes"]
-            out["roofline"]["traffic_source"] = os.path.basename(a.traffic_file)
+    if not ab3 and E.fused:
+        t, src = traffic_of(a, dom, N)
+        if t is not None:
+            out["roofline"]["traffic"] = t
+            out["roofline"]["traffic_source"] = src
 
     if not a.no_spmv and D == 2:          # (the 3-D operators take minutes to assemble on the host)
         sp = spmv_bench(mesh, C, dev)
@@ -569,7 +625,7 @@ def batch_main(a):
                                "x ~ U(0.3, 0.8), r ~ U(0.05, 0.15), Mach ~ U(0.1, 0.6)), RK4 at CFL 0.5",
                    "cells": N, "facets": F, "boundary_facets": Fb, "finite": finite, "graph_steps": n_graph,
                    "pipeline": "tile-fused stage" if E.fused else "three kernels",
-                   "kernel_avg_us": avg_us,
+                   "tiles": tile_summary(E.mesh), "kernel_avg_us": avg_us,
                    "kernel_timing": ("timed region / launches (a step is 4 k_fused launches)" if E.fused else
                                      "one event pair around back-to-back launches of each kernel"),
                    "kernel_gbs": kgbs, "stage_us": step_us},
@@ -578,6 +634,11 @@ def batch_main(a):
                      "traffic": None, "timing": "timed region: ms_per_step / stages per step",
                      "bytes_per_launch": kbytes["k_fused"]},
     }
+    if E.fused:
+        t, src = traffic_of(a, "k_fused", N)
+        if t is not None:
+            out["roofline"]["traffic"] = t
+            out["roofline"]["traffic_source"] = src
     print(json.dumps(out), flush=True)
 
 

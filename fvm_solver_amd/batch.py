@@ -170,16 +170,21 @@ def random_channel_batch(n_meshes: int, seed: int = 0, n_min: int = 5000, n_max:
     tuple, for checks against other implementations).  progress(i, n_cells), if given,
     is called after each mesh."""
     from fvm_solver_amd.config_fvm import channel_config
-    from fvm_solver_amd.mesh_gen.channel import ChannelSpec, channel_mesh_2d
+    from fvm_solver_amd.mesh_gen.channel import ChannelSpec, channel_mesh_2d, tile_order
     from fvm_solver_amd.mesh_utils.fvm_mesh import mesh_from_tuple
     from fvm_solver_amd.mesh_utils.mesh_store import channel_bc_tags
     rng = np.random.default_rng(seed)
     problems, params = [], []
+    start = 0                    # the mesh's first cell in the concatenated batch
     for i in range(n_meshes):
         n = int(np.exp(rng.uniform(np.log(n_min), np.log(n_max))))
         cx, r, mach = rng.uniform(0.3, 0.8), rng.uniform(0.05, 0.15), rng.uniform(0.1, 0.6)
         pts, cells, (fi, fb), tags = channel_mesh_2d(n, seed=seed * 1000 + i,
                                                      spec=ChannelSpec(centre=(cx, 0.5), radius=r))
+        # the batch's 256-cell tiles start at multiples of 256 of the concatenated
+        # cells: order this mesh's cells in bisection blocks aligned with them
+        cells = cells[tile_order(pts[cells].mean(axis=1), offset=start % 256)]
+        start += len(cells)
         mesh = mesh_from_tuple(pts, cells, fi, fb, device=device)
         cfg = channel_config(mach=mach, diameter=2 * r, device=str(device))
         c = mesh.geo64["centroids"]

@@ -62,9 +62,10 @@ class ConfigFVM(ABC):
     lim_K: float = 0.1
     exit_cfg: ConfigBC | None = None
     inlet_cfg: ConfigBC | None = None
-    # Cell order on the device (no reference counterpart): "auto" renumbers the cells
-    # along a Morton curve when the mesh's own order is too scattered for the tile-fused
-    # stage (e.g. gmsh output); "morton" / "rcm" always renumber; "never" keeps the order.
+    # Cell order on the device (no reference counterpart): "auto" renumbers the cells in
+    # tile-aligned bisection order when the mesh's own order is too scattered for the
+    # tile-fused stage (e.g. gmsh output); "tiles" / "morton" / "rcm" always renumber;
+    # "never" keeps the order.
     reorder: str = "auto"
 
 

@@ -71,7 +71,8 @@ class FluidConstitution3D(FluidConstitution):
 
 class FVMEquation:
     """``mesh`` is the caller's mesh.  When its cell order is too scattered for the
-    tile-fused stage (cfg.reorder "auto"), or on request ("morton", "rcm"), the device
+    tile-fused stage (cfg.reorder "auto": tile-aligned bisection order), or on request
+    ("tiles", "morton", "rcm"), the device
     works on the same mesh with its cells renumbered (``cell_perm``: device cell i is mesh
     cell cell_perm[i]); ``forward`` then takes and returns mesh-order arrays, while the
     solver's state (``cells.state``, the saved trajectories and ``E_props``) is in device
@@ -86,15 +87,15 @@ class FVMEquation:
         self.cell_perm = None
         self._inv_perm = None
         mode = getattr(cfg, "reorder", "auto") or "never"
-        if mode not in ("auto", "never", "morton", "rcm"):
-            raise ValueError(f"unknown cfg.reorder {mode!r} (auto, never, morton or rcm)")
+        if mode not in ("auto", "never", "tiles", "morton", "rcm"):
+            raise ValueError(f"unknown cfg.reorder {mode!r} (auto, never, tiles, morton or rcm)")
         E = None
         if mode in ("auto", "never"):
             E = FacetCalc(phy_setup, cfg, mesh, n_comp, bc_tag, device=self.device)
-        if mode in ("morton", "rcm") or (mode == "auto" and E.mesh.tiles is None and
-                                          E.mesh.tiles_reason not in (None, "disabled")):
+        if mode in ("tiles", "morton", "rcm") or (mode == "auto" and E.mesh.tiles is None and
+                                                   E.mesh.tiles_reason not in (None, "disabled")):
             from fvm_solver_amd.mesh_utils.reorder import permuted_mesh
-            dev_mesh, perm = permuted_mesh(mesh, "rcm" if mode == "rcm" else "morton", device=self.device)
+            dev_mesh, perm = permuted_mesh(mesh, "tiles" if mode == "auto" else mode, device=self.device)
             E = FacetCalc(phy_setup, cfg, dev_mesh, n_comp, bc_tag, device=self.device)
             self.cell_perm = torch.from_numpy(perm).to(self.device)
             inv = torch.empty_like(self.cell_perm)

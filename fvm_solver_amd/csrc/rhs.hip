@@ -253,7 +253,9 @@ __device__ __forceinline__ void bc_value(const FvmMeshDev &m, const FvmPhys &p, 
 // and the kernel boundary orders the writes before the reads.  A stage epilogue writes
 // into work->bc_part_next, never into the work->bc_part its own launch reads: tiles of
 // one launch run in no fixed order, and a tile that starts after another tile's epilogue
-// must still see the partials of the stage's input.
+// must still see the partials of the stage's input.  red holds one float per thread of
+// the workgroup; the tree sums the first kThreads (a workgroup of 2 kThreads threads has
+// its tile cells on those; the others are never live).
 template <int D>
 __device__ void bc_mean_block(const FvmMeshDev &m, const FvmPhys &p, float *__restrict__ part, int64_t c, bool live,
                               float T, int slot, float *red) {
@@ -291,13 +293,16 @@ __device__ void bc_mean_block(const FvmMeshDev &m, const FvmPhys &p, float *__re
 // Group means (block-uniform call): mean[q] for groups g0 + q, q < ng, each the sum of
 // its mesh's slot partials in a fixed strided-then-tree order (the same bits in every
 // block) divided by the group's facet count (facet_boundary.py:427).
+template <int NT = kThreads>
 __device__ void block_group_means_g(const FvmMeshDev &m, const FvmWork &w, int g0, int ng, float *gmean,
                                     float *red) {
     __syncthreads();                       // the previous tile's readers are done
     for (int q = 0; q < ng; ++q) {
         const int g = g0 + q;
         float v = 0.0f;
-        for (int i = m.group_s0[g] + threadIdx.x; i < m.group_s1[g]; i += kThreads)
+        // the first kThreads threads of a larger workgroup sum, in the same order
+        for (int i = m.group_s0[g] + threadIdx.x; (NT == kThreads || threadIdx.x < kThreads) && i < m.group_s1[g];
+             i += kThreads)
             v += w.bc_part[i * kMaxGroups + (g - m.slot_g0[i])];
         red[threadIdx.x] = v;
         __syncthreads();
@@ -1095,18 +1100,93 @@ __device__ __forceinline__ void fused_facet_load(const FvmMeshDev &m, int q, int
     for (int k = 0; k < 3 * D + 2; ++k) geo[k] = t[k];
 }
 
-// Each phase's global loads are issued before the previous phase's work (they land in
-// registers while it computes), so a tile costs about one memory round trip per phase
-// boundary rather than one per dependent load.  The host guarantees at most 256 H1
-// cells and fz_fpt<D>() x 256 facets per tile.
-template <int D, int LP, int EPI>
-__global__ void __launch_bounds__(kThreads)
+// One gradient cell's register inputs in the fused stage: LSQ row values, displacements,
+// limiter epsilon, local neighbour row and face-value positions (q | side << 15 per slot).
+template <int D>
+struct FzGradIn {
+    float gv[D + 2][D];
+    float disp[D + 1][D];
+    float eps;
+    ushort4 nr, fq;
+};
+
+// Inputs of tile cell c (thread-local row l of the tile's neighbour rows); its cfac row and
+// slot bits give the face-value positions.
+template <int D>
+__device__ __forceinline__ void fz_load_tile_cell(const FvmMeshDev &m, int64_t c, int64_t nbr_row, FzGradIn<D> &g,
+                                                  int &info, ushort4 &cf) {
+    constexpr int NL = D + 1;
+    const int64_t N = m.n_cells;
+#pragma unroll
+    for (int jj = 0; jj <= NL; ++jj)
+#pragma unroll
+        for (int ii = 0; ii < D; ++ii) g.gv[jj][ii] = m.cell_gval[(jj * D + ii) * N + c];
+#pragma unroll
+    for (int k = 0; k < NL; ++k)
+#pragma unroll
+        for (int ii = 0; ii < D; ++ii) g.disp[k][ii] = m.cell_disp[(k * D + ii) * N + c];
+    g.eps = m.cell_eps[c];
+    info = m.cell_info[c];
+    g.nr = *reinterpret_cast<const ushort4 *>(m.fz_nbr + nbr_row * 4);
+    cf = *reinterpret_cast<const ushort4 *>(m.fz_cfac + c * 4);
+}
+
+// Face-value positions of a tile cell: its facets' list positions and its sides.
+template <int D>
+__device__ __forceinline__ ushort4 fz_tile_fq(ushort4 cf, int info) {
+    return make_ushort4(cf.x | ((info & 1) ? kFzSide1 : 0u), cf.y | ((info & 2) ? kFzSide1 : 0u),
+                        cf.z | ((info & 4) ? kFzSide1 : 0u),
+                        D + 1 > 3 ? (cf.w | ((info & 8) ? kFzSide1 : 0u)) : kFzNone);
+}
+
+// Inputs of the tile's H1 entry h (global id returned): its component-major static record,
+// neighbour row and H1 facet table row.
+template <int D>
+__device__ __forceinline__ int64_t fz_load_h1_cell(const FvmMeshDev &m, int h, int halo_off, int h1_off,
+                                                   int nbr_off, FzGradIn<D> &g) {
+    constexpr int NL = D + 1, HREC = fz_hrec_n<D>();
+    const int64_t gc = m.fz_halo[halo_off + h];
+    const float *r = m.fz_hrec + h1_off + h;
+    const int64_t hs = m.fz_nh1;             // component stride of the H1 records
+    float rv[HREC];
+#pragma unroll
+    for (int q = 0; q < HREC; ++q) rv[q] = r[q * hs];
+#pragma unroll
+    for (int jj = 0; jj <= NL; ++jj)
+#pragma unroll
+        for (int ii = 0; ii < D; ++ii) g.gv[jj][ii] = rv[jj * D + ii];
+#pragma unroll
+    for (int k = 0; k < NL; ++k)
+#pragma unroll
+        for (int ii = 0; ii < D; ++ii) g.disp[k][ii] = rv[(NL + 1) * D + k * D + ii];
+    g.eps = rv[(NL + 1) * D + NL * D];
+    g.nr = *reinterpret_cast<const ushort4 *>(m.fz_nbr + (static_cast<int64_t>(nbr_off) + kThreads + h) * 4);
+    g.fq = *reinterpret_cast<const ushort4 *>(m.fz_h1f + static_cast<int64_t>(h1_off + h) * 4);
+    return gc;
+}
+
+// Workgroup shapes (NT threads, one 256-cell tile either way):
+//   NT = kThreads      thread i owns tile cell i, H1 entry i (while i < n1) and facets
+//                      i, i + 256, ... of the tile's list (up to fz_fpt<D>());
+//   NT = 320 .. 512    threads 0..255 own the tile cells, threads 256.. the H1 entries
+//                      (local id = thread id; the host guarantees n1 <= NT - 256), and
+//                      facets i, i + NT, ...: more waves per tile for the same LDS, each
+//                      doing one gradient and fewer facets.
+// Both compute the same numbers (each cell, facet and reduction is evaluated by the same
+// code in the same order).  Each phase's global loads are issued before the previous
+// phase's work (they land in registers while it computes), so a tile costs about one
+// memory round trip per phase boundary rather than one per dependent load.  The host
+// guarantees at most 256 H1 cells and fz_fpt<D>() x 256 facets per tile.
+template <int D, int LP, int EPI, int NT>
+__global__ void __launch_bounds__(NT)
 k_fused(FvmMeshDev m, FvmPhys p, FvmWork w, const float *__restrict__ prim, float *__restrict__ dudt, FvmStage st,
         int has_prev) {
     using Dm = Dims<D>;
     constexpr int C = Dm::C, NL = Dm::NL;
-    constexpr int HREC = fz_hrec_n<D>();
-    constexpr int FPT = fz_fpt<D>();
+    constexpr bool SPLIT = NT > kThreads;
+    constexpr int FS = (fz_fpt<D>() * kThreads + NT - 1) / NT;   // facet slots per thread
+    static_assert(NT % 64 == 0 && NT >= kThreads && NT <= 2 * kThreads, "fused stage: 256 to 512 threads");
+    static_assert(FS >= 1 && FS <= 3, "fused stage: 1 to 3 facets per thread");
     extern __shared__ float4 fz_lds[];     // 16-byte aligned: 2-D records move as ds_*_b128
     float *lds = reinterpret_cast<float *>(fz_lds);
     const int64_t N = m.n_cells;
@@ -1128,12 +1208,13 @@ k_fused(FvmMeshDev m, FvmPhys p, FvmWork w, const float *__restrict__ prim, floa
     float *red = GL;                                   // reductions reuse the gradient rows
     double *sred = reinterpret_cast<double *>(GL);     // (before and after the phases using them)
     const int i = threadIdx.x;
+    const int h = SPLIT ? i - kThreads : i;            // this thread's H1 entry
     const int64_t c0 = tile * kThreads;
     const int nown = static_cast<int>(N - c0 < kThreads ? N - c0 : kThreads);
     const int nft = nfo + nfor;
-    const bool has_own = i < nown, has_h1 = i < n1;
+    const bool has_own = i < nown, has_h1 = h >= 0 && h < n1;
     __shared__ float gmean[kMaxGroups];
-    if (ng > 0) block_group_means_g(m, w, g0, ng, gmean, red);
+    if (ng > 0) block_group_means_g<NT>(m, w, g0, ng, gmean, red);
     int mt0 = 0;
     if (m.mesh_phys) {
         mt0 = m.cell_mesh[c0];
@@ -1148,86 +1229,86 @@ k_fused(FvmMeshDev m, FvmPhys p, FvmWork w, const float *__restrict__ prim, floa
 
     // ---- phase 0: primitives of tile, H1 and H2 to LDS; the gradient inputs of this
     // thread's tile cell and H1 cell to registers ----
-    float ogv[NL + 1][D], odisp[NL][D], oeps = 0.0f;
     int oinfo = 0;
-    ushort4 onr = make_ushort4(0, 0, 0, 0), cf = make_ushort4(0, 0, 0, 0);
-    if (has_own) {
-        float u[C];
-        load_prim<D>(prim + (c0 + i) * C, u);
-        store_vec<C>(Pl + i * C, u);
-    }
-    if (has_own) {
-        const int64_t c = c0 + i;
-#pragma unroll
-        for (int jj = 0; jj <= NL; ++jj)
-#pragma unroll
-            for (int ii = 0; ii < D; ++ii) ogv[jj][ii] = m.cell_gval[(jj * D + ii) * N + c];
-#pragma unroll
-        for (int k = 0; k < NL; ++k)
-#pragma unroll
-            for (int ii = 0; ii < D; ++ii) odisp[k][ii] = m.cell_disp[(k * D + ii) * N + c];
-        oeps = m.cell_eps[c];
-        oinfo = m.cell_info[c];
-        onr = *reinterpret_cast<const ushort4 *>(m.fz_nbr + (static_cast<int64_t>(nbr_off) + i) * 4);
-        cf = *reinterpret_cast<const ushort4 *>(m.fz_cfac + c * 4);
-        if (m.mesh_phys) meshT[i] = static_cast<unsigned short>(m.cell_mesh[c]);
-    }
-    float hgv[NL + 1][D], hdisp[NL][D], heps = 0.0f;
-    int64_t hgc = 0;
-    ushort4 hnr = make_ushort4(0, 0, 0, 0), hf = make_ushort4(kFzNone, kFzNone, kFzNone, kFzNone);
-    if (has_h1) {
-        hgc = m.fz_halo[halo_off + i];
-        const float *r = m.fz_hrec + h1_off + i;
-        const int64_t hs = m.fz_nh1;             // component stride of the H1 records
-        float rv[HREC];
-#pragma unroll
-        for (int q = 0; q < HREC; ++q) rv[q] = r[q * hs];
-#pragma unroll
-        for (int jj = 0; jj <= NL; ++jj)
-#pragma unroll
-            for (int ii = 0; ii < D; ++ii) hgv[jj][ii] = rv[jj * D + ii];
-#pragma unroll
-        for (int k = 0; k < NL; ++k)
-#pragma unroll
-            for (int ii = 0; ii < D; ++ii) hdisp[k][ii] = rv[(NL + 1) * D + k * D + ii];
-        heps = rv[(NL + 1) * D + NL * D];
-        hnr = *reinterpret_cast<const ushort4 *>(m.fz_nbr + (static_cast<int64_t>(nbr_off) + kThreads + i) * 4);
-        hf = *reinterpret_cast<const ushort4 *>(m.fz_h1f + static_cast<int64_t>(h1_off + i) * 4);
-    }
-    for (int q = i; q < n1 + n2; q += kThreads) {
-        float u[C];
-        load_prim<D>(prim + static_cast<int64_t>(m.fz_halo[halo_off + q]) * C, u);
-        store_vec<C>(Pl + (kThreads + q) * C, u);
-    }
-    // this tile cell's face-value positions: its facets' list positions and its sides
-    const ushort4 ofq = make_ushort4(cf.x | ((oinfo & 1) ? kFzSide1 : 0u), cf.y | ((oinfo & 2) ? kFzSide1 : 0u),
-                                     cf.z | ((oinfo & 4) ? kFzSide1 : 0u),
-                                     NL > 3 ? (cf.w | ((oinfo & 8) ? kFzSide1 : 0u)) : kFzNone);
-    __syncthreads();
-
-    // ---- phase 1: gradients (the first facet's inputs in flight during the H1 cell's) ----
-    // The first facet's loads are issued after the tile cell's gradient, not before: its
-    // registers are then not live through that gradient, which keeps the 2-D kernel at 93
-    // VGPRs (5 waves per SIMD, and 5 workgroups per CU with its 29.5 KB of LDS) instead
-    // of 106 (4 waves per SIMD): 206.9 -> 197.2 us per RK4 step at 1M cells.
+    ushort4 cf = make_ushort4(0, 0, 0, 0);
     int64_t fA = 0;
     unsigned dA = 0;
     float geoA[3 * D + 2];
-    if (has_own)
-        fused_grad_cell<D, LP>(m, p, w, Pl, GL, FF, m.fz_ngl, i, c0 + i, true, ogv, odisp, oeps, onr, has_prev,
-                               gmean, g0, ofq);
-    if (i < nft) fused_facet_load<D>(m, i, fo0, nfo, foff, fA, dA, geoA);
-    if (has_h1)
-        fused_grad_cell<D, LP>(m, p, w, Pl, GL, FF, m.fz_ngl, kThreads + i, hgc, false, hgv, hdisp, heps, hnr,
-                               has_prev, gmean, g0, hf);
+    if constexpr (!SPLIT) {
+        FzGradIn<D> og, hg;
+        og.nr = make_ushort4(0, 0, 0, 0);
+        og.eps = 0.0f;
+        if (has_own) {
+            float u[C];
+            load_prim<D>(prim + (c0 + i) * C, u);
+            store_vec<C>(Pl + i * C, u);
+        }
+        if (has_own) {
+            fz_load_tile_cell<D>(m, c0 + i, static_cast<int64_t>(nbr_off) + i, og, oinfo, cf);
+            if (m.mesh_phys) meshT[i] = static_cast<unsigned short>(m.cell_mesh[c0 + i]);
+        }
+        hg.eps = 0.0f;
+        hg.nr = make_ushort4(0, 0, 0, 0);
+        hg.fq = make_ushort4(kFzNone, kFzNone, kFzNone, kFzNone);
+        int64_t hgc = 0;
+        if (has_h1) hgc = fz_load_h1_cell<D>(m, h, halo_off, h1_off, nbr_off, hg);
+        for (int q = i; q < n1 + n2; q += NT) {
+            float u[C];
+            load_prim<D>(prim + static_cast<int64_t>(m.fz_halo[halo_off + q]) * C, u);
+            store_vec<C>(Pl + (kThreads + q) * C, u);
+        }
+        og.fq = fz_tile_fq<D>(cf, oinfo);
+        __syncthreads();
+
+        // ---- phase 1: gradients (the first facet's inputs in flight during the H1
+        // cell's).  The first facet's loads are issued after the tile cell's gradient,
+        // not before: its registers are then not live through that gradient, which keeps
+        // the 2-D kernel at 93 VGPRs (5 waves per SIMD, and 5 workgroups per CU with its
+        // 29.5 KB of LDS) instead of 106 (4 waves per SIMD): 206.9 -> 197.2 us per RK4
+        // step at 1M cells.
+        if (has_own)
+            fused_grad_cell<D, LP>(m, p, w, Pl, GL, FF, m.fz_ngl, i, c0 + i, true, og.gv, og.disp, og.eps, og.nr,
+                                   has_prev, gmean, g0, og.fq);
+        if (i < nft) fused_facet_load<D>(m, i, fo0, nfo, foff, fA, dA, geoA);
+        if (has_h1)
+            fused_grad_cell<D, LP>(m, p, w, Pl, GL, FF, m.fz_ngl, kThreads + i, hgc, false, hg.gv, hg.disp, hg.eps,
+                                   hg.nr, has_prev, gmean, g0, hg.fq);
+    } else {
+        // one gradient cell per thread: tile cell i, or the H1 cell of local id i (its
+        // inputs in the same registers)
+        FzGradIn<D> og;
+        int64_t ogc = c0 + i;
+        if (has_own) {
+            float u[C];
+            load_prim<D>(prim + (c0 + i) * C, u);
+            store_vec<C>(Pl + i * C, u);
+            fz_load_tile_cell<D>(m, c0 + i, static_cast<int64_t>(nbr_off) + i, og, oinfo, cf);
+            if (m.mesh_phys) meshT[i] = static_cast<unsigned short>(m.cell_mesh[c0 + i]);
+        } else if (has_h1) {
+            ogc = fz_load_h1_cell<D>(m, h, halo_off, h1_off, nbr_off, og);
+        }
+        for (int q = i; q < n1 + n2; q += NT) {
+            float u[C];
+            load_prim<D>(prim + static_cast<int64_t>(m.fz_halo[halo_off + q]) * C, u);
+            store_vec<C>(Pl + (kThreads + q) * C, u);
+        }
+        if (has_own) og.fq = fz_tile_fq<D>(cf, oinfo);
+        __syncthreads();
+
+        // ---- phase 1: gradients, then the first facet's loads ----
+        if (has_own || has_h1)
+            fused_grad_cell<D, LP>(m, p, w, Pl, GL, FF, m.fz_ngl, i, ogc, has_own, og.gv, og.disp, og.eps, og.nr,
+                                   has_prev, gmean, g0, og.fq);
+        if (i < nft) fused_facet_load<D>(m, i, fo0, nfo, foff, fA, dA, geoA);
+    }
     __syncthreads();
 
     // ---- phase 2: facet fluxes (the next facet's and the epilogue's inputs in flight) ----
     int64_t fB = 0;
     unsigned dB = 0;
     float geoB[3 * D + 2];
-    const bool twoB = i + kThreads < nft;
-    if (twoB) fused_facet_load<D>(m, i + kThreads, fo0, nfo, foff, fB, dB, geoB);
+    const bool twoB = FS >= 2 && i + NT < nft;
+    if (twoB) fused_facet_load<D>(m, i + NT, fo0, nfo, foff, fB, dB, geoB);
     const int64_t c = c0 + i;
     float dtm = 0.0f;
     if (EPI == EPI_STAGE && has_own) {
@@ -1243,17 +1324,17 @@ k_fused(FvmMeshDev m, FvmPhys p, FvmWork w, const float *__restrict__ prim, floa
     float inv_vol = 0.0f;
     if (has_own) inv_vol = m.cell_inv_vol[c];
     if (i < nft) fused_facet<D>(m, p, Pl, GL, m.fz_ngl, FF, i, fA, dA, geoA, meshT, mt0, physT);
-    if constexpr (FPT > 2) {
+    if constexpr (FS > 2) {
         // third facet of the thread: its inputs in flight during the second's flux
         int64_t fC = 0;
         unsigned dC = 0;
         float geoC[3 * D + 2];
-        const bool threeC = i + 2 * kThreads < nft;
-        if (threeC) fused_facet_load<D>(m, i + 2 * kThreads, fo0, nfo, foff, fC, dC, geoC);
-        if (twoB) fused_facet<D>(m, p, Pl, GL, m.fz_ngl, FF, i + kThreads, fB, dB, geoB, meshT, mt0, physT);
-        if (threeC) fused_facet<D>(m, p, Pl, GL, m.fz_ngl, FF, i + 2 * kThreads, fC, dC, geoC, meshT, mt0, physT);
-    } else {
-        if (twoB) fused_facet<D>(m, p, Pl, GL, m.fz_ngl, FF, i + kThreads, fB, dB, geoB, meshT, mt0, physT);
+        const bool threeC = i + 2 * NT < nft;
+        if (threeC) fused_facet_load<D>(m, i + 2 * NT, fo0, nfo, foff, fC, dC, geoC);
+        if (twoB) fused_facet<D>(m, p, Pl, GL, m.fz_ngl, FF, i + NT, fB, dB, geoB, meshT, mt0, physT);
+        if (threeC) fused_facet<D>(m, p, Pl, GL, m.fz_ngl, FF, i + 2 * NT, fC, dC, geoC, meshT, mt0, physT);
+    } else if constexpr (FS == 2) {
+        if (twoB) fused_facet<D>(m, p, Pl, GL, m.fz_ngl, FF, i + NT, fB, dB, geoB, meshT, mt0, physT);
     }
     __syncthreads();
 
@@ -1370,7 +1451,10 @@ static int check_fused_d(const FvmMeshDev &m, const char *who) {
         !m.fz_fdesc_x || !m.fz_geo || !m.fz_geo_x || !m.fz_cfac || m.fz_ncl < kThreads || m.fz_ngl < kThreads ||
         m.fz_ngl > 2 * kThreads || m.fz_ngl > m.fz_ncl || m.fz_ncl > 0xFFF || m.fz_nfl < 1 ||
         m.fz_nfl > fz_fpt<D>() * kThreads || fused_lds_bytes<D>(m) > max_lds_per_workgroup() ||
-        (m.mesh_phys && m.n_meshes > 0xFFFF)) {
+        (m.mesh_phys && m.n_meshes > 0xFFFF) ||
+        (m.fz_threads != 0 && m.fz_threads != kThreads && m.fz_threads != 320 && m.fz_threads != 384 &&
+         m.fz_threads != 512) ||
+        (m.fz_threads > kThreads && m.fz_ngl > m.fz_threads)) {
         set_error("%s: invalid tile tables for the tile-fused stage (or %zu B of LDS beyond the device's %zu)", who,
                   fused_lds_bytes<D>(m), max_lds_per_workgroup());
         return FVM_ERR_ARG;
@@ -1392,26 +1476,38 @@ static int check_fused(const FvmMeshDev &m, const FvmWork &w, const char *who) {
 }
 
 template <typename K>
-static void launch_fused_kernel(K kernel, dim3 grid, size_t lds, hipStream_t s, const FvmMeshDev &m,
+static void launch_fused_kernel(K kernel, int threads, dim3 grid, size_t lds, hipStream_t s, const FvmMeshDev &m,
                                 const FvmPhys &p, const FvmWork &w, const float *prim, float *dudt,
                                 const FvmStage &st, int has_prev) {
     if (lds > 64 * 1024)   // dynamic LDS beyond 64 KiB is requested explicitly
         (void)hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
-    hipLaunchKernelGGL(kernel, grid, dim3(kThreads), lds, s, m, p, w, prim, dudt, st, has_prev);
+    hipLaunchKernelGGL(kernel, grid, dim3(threads), lds, s, m, p, w, prim, dudt, st, has_prev);
 }
 
-template <int D, int EPI>
-static void launch_fused(const FvmMeshDev &m, const FvmPhys &p, const FvmWork &w, const float *prim, float *dudt,
-                         const FvmStage &st, hipStream_t s, int has_prev) {
+template <int D, int EPI, int NT>
+static void launch_fused_nt(const FvmMeshDev &m, const FvmPhys &p, const FvmWork &w, const float *prim, float *dudt,
+                            const FvmStage &st, hipStream_t s, int has_prev) {
     const dim3 grid(blocks_for(m.n_cells, kThreads));
     const size_t lds = fused_lds_bytes<D>(m);
     switch (p.lim_p) {
-        case 1: launch_fused_kernel(k_fused<D, 1, EPI>, grid, lds, s, m, p, w, prim, dudt, st, has_prev); break;
-        case 2: launch_fused_kernel(k_fused<D, 2, EPI>, grid, lds, s, m, p, w, prim, dudt, st, has_prev); break;
-        case 3: launch_fused_kernel(k_fused<D, 3, EPI>, grid, lds, s, m, p, w, prim, dudt, st, has_prev); break;
-        case 5: launch_fused_kernel(k_fused<D, 5, EPI>, grid, lds, s, m, p, w, prim, dudt, st, has_prev); break;
-        default: launch_fused_kernel(k_fused<D, 4, EPI>, grid, lds, s, m, p, w, prim, dudt, st, has_prev); break;
+        case 1: launch_fused_kernel(k_fused<D, 1, EPI, NT>, NT, grid, lds, s, m, p, w, prim, dudt, st, has_prev); break;
+        case 2: launch_fused_kernel(k_fused<D, 2, EPI, NT>, NT, grid, lds, s, m, p, w, prim, dudt, st, has_prev); break;
+        case 3: launch_fused_kernel(k_fused<D, 3, EPI, NT>, NT, grid, lds, s, m, p, w, prim, dudt, st, has_prev); break;
+        case 5: launch_fused_kernel(k_fused<D, 5, EPI, NT>, NT, grid, lds, s, m, p, w, prim, dudt, st, has_prev); break;
+        default: launch_fused_kernel(k_fused<D, 4, EPI, NT>, NT, grid, lds, s, m, p, w, prim, dudt, st, has_prev); break;
+    }
+}
+
+// One workgroup per 256-cell tile, of m.fz_threads threads (0 = 256; 320, 384, 512).
+template <int D, int EPI>
+static void launch_fused(const FvmMeshDev &m, const FvmPhys &p, const FvmWork &w, const float *prim, float *dudt,
+                         const FvmStage &st, hipStream_t s, int has_prev) {
+    switch (m.fz_threads) {
+        case 320: launch_fused_nt<D, EPI, 320>(m, p, w, prim, dudt, st, s, has_prev); break;
+        case 384: launch_fused_nt<D, EPI, 384>(m, p, w, prim, dudt, st, s, has_prev); break;
+        case 512: launch_fused_nt<D, EPI, 512>(m, p, w, prim, dudt, st, s, has_prev); break;
+        default: launch_fused_nt<D, EPI, kThreads>(m, p, w, prim, dudt, st, s, has_prev); break;
     }
 }
 

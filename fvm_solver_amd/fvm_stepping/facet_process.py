@@ -374,6 +374,24 @@ class DeviceMesh:
         self.struct.persist_blocks = int(n)
         self.struct3.persist_blocks = int(n)
 
+    def fused_thread_options(self) -> list:
+        """Workgroup sizes the tile-fused stage can run this mesh with: 256 (a thread per
+        tile cell, which also takes an H1 cell and up to 2 / 3 facets), and 320, 384, 512
+        where the largest tile's H1 fits the threads beyond the first 256 (tile and H1
+        cells on separate threads, fewer facets each).  All compute bitwise the same."""
+        if self.tiles is None:
+            return []
+        return [256] + [n for n in (320, 384, 512) if self.tiles["ngl"] <= n]
+
+    def set_fused_threads(self, n: int):
+        if n not in self.fused_thread_options():
+            raise ValueError(f"fused-stage threads must be one of {self.fused_thread_options()}")
+        self.struct.fz_threads = int(n)
+
+    @property
+    def fused_threads(self) -> int:
+        return int(self.struct.fz_threads) or 256 if self.tiles is not None else 0
+
 
 def phys_struct(cfg) -> "nat.FvmPhys":
     """Scalars of FluidConstitution / SlopeLimiter / KTDiffusion, rounded to float32 from

@@ -141,7 +141,12 @@ typedef struct FvmMeshDev {
     const float *mesh_phys;     /* [n_meshes][4] mu0, mub0, kappa, 0 per mesh         */
     int32_t persist_blocks;     /* workgroups of the per-cell kernels (0 = one per
                                    256-cell tile); tiles are walked grid-stride       */
-    int32_t pad1;
+    int32_t fz_threads;         /* threads per tile of the tile-fused stage: 0 or 256
+                                   (a thread per tile cell, which also takes an H1
+                                   cell and up to 2 (2-D) / 3 (3-D) facets), or 320,
+                                   384, 512 (tile cells on threads 0..255, H1 cells on
+                                   the others: needs fz_ngl <= fz_threads; fewer facets
+                                   per thread); same results either way              */
     /* tile-fused stage (2-D and 3-D; fz_tile NULL = the three-kernel pipeline; the
        tables must fit fvm_max_lds_per_workgroup, fz_ngl <= 512 and fz_nfl <= 512 in 2-D,
        768 in 3-D).  Tile t holds

@@ -1,6 +1,7 @@
-"""The tile-fused stage (one launch per RHS) against the three-kernel pipeline.
+"""The tile-fused stage (one launch per RHS), with every workgroup size the mesh allows
+(256 to 512 threads per tile), against the three-kernel pipeline.
 
-Both evaluate the same device functions on the same floats in the same order, so every
+All three evaluate the same device functions on the same floats in the same order, so every
 result must be bitwise identical: dU/dt of the reference's stateful forward over two
 calls (the second applies the Neumann correction with the first call's gradients), the
 carried gradients (cell_grads), RK4 and embedded-tableau trajectories, batches of
@@ -33,9 +34,13 @@ def test_rhs_two_calls_bitwise(fname):
     assert dm.tiles is not None, dm.tiles_reason
     _announce(f"fused vs three-kernel rhs, {dm.N} cells, lds {dm.tiles['lds_bytes']} B")
     out = {}
-    for fused in (False, True):
-        E = FacetCalc.from_device_mesh(dm, cfg, DEV, fused=fused)
-        assert E.fused == fused
+    opts = dm.fused_thread_options()
+    assert 256 in opts and 512 in opts, opts
+    for fused in (False, *opts):
+        if fused:
+            dm.set_fused_threads(fused)
+        E = FacetCalc.from_device_mesh(dm, cfg, DEV, fused=bool(fused))
+        assert E.fused == bool(fused)
         res = []
         for call in (0, 1):
             prim = torch.from_numpy(d[f"rhs{call}_prim"]).to(DEV)
@@ -44,8 +49,9 @@ def test_rhs_two_calls_bitwise(fname):
             res.append(E.ubc.clone())
         torch.cuda.synchronize()
         out[fused] = res
-    for k, (a, b) in enumerate(zip(out[False], out[True])):
-        assert torch.equal(a, b), (k, float((a - b).abs().max()))
+    for nt in opts:
+        for k, (a, b) in enumerate(zip(out[False], out[nt])):
+            assert torch.equal(a, b), (nt, k, float((a - b).abs().max()))
 
 
 def _channel_eq(n, solver, extra, seed=4, dim=2):
@@ -62,12 +68,16 @@ def _channel_eq(n, solver, extra, seed=4, dim=2):
 @pytest.mark.parametrize("solver,extra", [("Butcher", "RK4"), ("Butcher_adapt", "RK3_SSP4")])
 def test_trajectories_bitwise(solver, extra, dim, n):
     states = []
-    for fused in (False, True):
+    opts = [256, 320, 384, 512]
+    for fused in (False, *opts):
         eq = _channel_eq(n, solver, extra, dim=dim)
+        if fused and fused not in eq.E_props.mesh.fused_thread_options():
+            continue
         if fused:
             assert eq.E_props.mesh.tiles is not None, eq.E_props.mesh.tiles_reason
-        eq.E_props.set_fused(fused)
-        assert eq.E_props.fused == fused
+            eq.E_props.mesh.set_fused_threads(fused)
+        eq.E_props.set_fused(bool(fused))
+        assert eq.E_props.fused == bool(fused)
         ts = eq.t_solver
         ts.set_dt(float(eq.cfl_dt(eq.cells.get_values()[0], 0.5)))
         dts = []
@@ -76,11 +86,14 @@ def test_trajectories_bitwise(solver, extra, dim, n):
             dts.append(float(ts.dt))
         torch.cuda.synchronize()
         states.append((eq.cells.state.clone(), dts, eq.E_props.cell_grads.clone()))
-    _announce(f"{dim}-D {n} cells {solver} {extra}: 12 steps fused vs three-kernel, dt {states[1][1][-1]:.4e}")
-    assert torch.isfinite(states[1][0]).all()
-    assert torch.equal(states[0][0], states[1][0])
-    assert states[0][1] == states[1][1]
-    assert torch.equal(states[0][2], states[1][2])
+    _announce(f"{dim}-D {n} cells {solver} {extra}: 12 steps fused ({len(states) - 1} workgroup sizes) vs "
+              f"three-kernel, dt {states[1][1][-1]:.4e}")
+    assert len(states) >= 3
+    for st in states[1:]:
+        assert torch.isfinite(st[0]).all()
+        assert torch.equal(states[0][0], st[0])
+        assert states[0][1] == st[1]
+        assert torch.equal(states[0][2], st[2])
 
 
 def test_switching_pipelines_mid_run():
@@ -91,6 +104,7 @@ def test_switching_pipelines_mid_run():
     for eq in (ref, alt):
         eq.t_solver.set_dt(float(eq.cfl_dt(eq.cells.get_values()[0], 0.5)))
     for k in range(6):
+        alt.E_props.mesh.set_fused_threads(512 if k % 4 == 2 else 256)
         alt.E_props.set_fused(k % 2 == 0)
         ref.t_solver._solve_step(None)
         alt.t_solver._solve_step(None)
@@ -102,9 +116,13 @@ def test_batch_and_graph_bitwise():
     from fvm_solver_amd.batch import BatchedSolver, random_channel_batch
     problems, params = random_channel_batch(5, seed=7, n_min=2000, n_max=12000, device=DEV)
     res = []
-    for fused in (False, True):
+    for fused in (False, 256, 320, 384, 512):
         bs = BatchedSolver(problems, device=DEV)
-        bs.E.set_fused(fused)
+        if fused and fused not in bs.E.mesh.fused_thread_options():
+            continue
+        if fused:
+            bs.E.mesh.set_fused_threads(fused)
+        bs.E.set_fused(bool(fused))
         bs.set_dt(0.5 * bs.cfl_dt(0.5))
         for _ in range(3):
             bs.step()
@@ -115,7 +133,9 @@ def test_batch_and_graph_bitwise():
             torch.cuda.synchronize()
             snaps.append(bs.state.clone())
         res.append(snaps)
-    _announce(f"batch of {[p['cells'] for p in params]} cells: fused vs three-kernel, graph replays")
-    for a, b in zip(*res):
-        assert torch.isfinite(b).all()
-        assert torch.equal(a, b)
+    _announce(f"batch of {[p['cells'] for p in params]} cells: fused ({len(res) - 1} workgroup sizes) vs "
+              "three-kernel, graph replays")
+    for other in res[1:]:
+        for a, b in zip(res[0], other):
+            assert torch.isfinite(b).all()
+            assert torch.equal(a, b)

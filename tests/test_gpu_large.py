@@ -1,8 +1,8 @@
 """Full-size GPU runs of the benchmark configurations (run_fvm.py:115-150 at the
 benchmark sizes), and a batch of meshes against the oracle mesh by mesh.
 
-  * 2-D channel, ~1M cells: the tile-fused stage against the three-kernel pipeline,
-    bitwise, over RK4 and AB3 steps (3.9k tiles: far more than can be resident, so every
+  * 2-D channel, ~1M cells: the tile-fused stage (every workgroup size of 256 to 512
+    threads per tile the tables allow) against the three-kernel pipeline, bitwise, over RK4 and AB3 steps (3.9k tiles: far more than can be resident, so every
     ordering between tiles of one launch is exercised); 10 RK4 steps against the fp32
     oracle (<= 1e-4 relative L2 per conserved variable where the fp32 oracle is within
     1e-5 of the fp64 one; rho v, which starts at zero, no further from fp64 than the fp32
@@ -51,18 +51,24 @@ def _eq_1m(mesh_t, solver, extra, cfl):
 @pytest.mark.parametrize("solver,extra,cfl", [("Butcher", "RK4", 0.5), ("AB3", "", 0.2)])
 def test_fused_vs_three_kernels_1m_bitwise(channel_1m, solver, extra, cfl):
     states = []
-    for fused in (False, True):
+    for fused in (False, 256, 320, 384, 512):
         cfg, mesh, eq, U0, dt = _eq_1m(channel_1m, solver, extra, cfl)
         tl = eq.E_props.mesh.tiles
         assert tl is not None, eq.E_props.mesh.tiles_reason
-        eq.E_props.set_fused(fused)
+        if fused and fused not in eq.E_props.mesh.fused_thread_options():
+            continue
+        if fused:
+            eq.E_props.mesh.set_fused_threads(fused)
+        eq.E_props.set_fused(bool(fused))
         for _ in range(3):
             eq.t_solver._solve_step(None)
         torch.cuda.synchronize()
         states.append(eq.cells.state.clone())
-    announce(f"{solver}{extra} 1M cells ({tl['n_tiles']} tiles): 3 steps fused vs three kernels")
-    assert torch.isfinite(states[1]).all()
-    assert torch.equal(states[0], states[1])
+    announce(f"{solver}{extra} 1M cells ({tl['n_tiles']} tiles): 3 steps fused ({len(states) - 1} workgroup sizes) "
+             "vs three kernels")
+    for st in states[1:]:
+        assert torch.isfinite(st).all()
+        assert torch.equal(states[0], st)
 
 
 @pytest.mark.timeout(900)

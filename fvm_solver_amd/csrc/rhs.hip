@@ -914,8 +914,8 @@ __host__ __device__ constexpr int fz_fpt() { return D == 2 ? 2 : 3; }   // facet
 __host__ __device__ inline int64_t fz_r4(int64_t x) { return (x + 3) & ~int64_t(3); }
 
 // LDS layout of k_fused in floats, each block 16-byte aligned: Pl [ncl][C], GL
-// [D][D+1][ngl] (component-major: the gradient phase's writes, one cell per lane, are
-// conflict free), FF [nfl][2][C].
+// [ngl][D (D+1)] (a cell's limited V and T gradients in one row: the flux phase reads a
+// side's gradients with two or three vector loads from one address), FF [nfl][2][C].
 template <int D>
 __host__ __device__ inline int64_t fz_gl_offset(int ncl) {
     return fz_r4(static_cast<int64_t>(ncl) * (D + 2));
@@ -923,6 +923,41 @@ __host__ __device__ inline int64_t fz_gl_offset(int ncl) {
 template <int D>
 __host__ __device__ inline int64_t fz_ff_offset(int ncl, int ngl) {
     return fz_gl_offset<D>(ncl) + fz_r4(static_cast<int64_t>(ngl) * D * (D + 1));
+}
+
+// A cell's row of GL: its limited gradients gl[i][j] (j < D: velocity components, j = D:
+// temperature) at gl_row[i (D+1) + j]; 8-byte aligned in 2-D (6 floats), 16-byte in 3-D.
+template <int D>
+__device__ __forceinline__ void gl_store(float *__restrict__ row, const float (&v)[D * (D + 1)]) {
+    if constexpr (D == 2) {
+        float2 *r = reinterpret_cast<float2 *>(row);
+        r[0] = make_float2(v[0], v[1]);
+        r[1] = make_float2(v[2], v[3]);
+        r[2] = make_float2(v[4], v[5]);
+    } else {
+        float4 *r = reinterpret_cast<float4 *>(row);
+        r[0] = make_float4(v[0], v[1], v[2], v[3]);
+        r[1] = make_float4(v[4], v[5], v[6], v[7]);
+        r[2] = make_float4(v[8], v[9], v[10], v[11]);
+    }
+}
+
+template <int D>
+__device__ __forceinline__ void gl_load(const float *__restrict__ row, float (&g)[D][D + 1]) {
+    if constexpr (D == 2) {
+        const float2 *r = reinterpret_cast<const float2 *>(row);
+        const float2 a = r[0], b = r[1], c = r[2];
+        g[0][0] = a.x; g[0][1] = a.y; g[0][2] = b.x;
+        g[1][0] = b.y; g[1][1] = c.x; g[1][2] = c.y;
+    } else {
+        const float4 *r = reinterpret_cast<const float4 *>(row);
+        const float4 a = r[0], b = r[1], c = r[2];
+        const float t[12] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w, c.x, c.y, c.z, c.w};
+#pragma unroll
+        for (int i = 0; i < D; ++i)
+#pragma unroll
+            for (int j = 0; j <= D; ++j) g[i][j] = t[i * (D + 1) + j];
+    }
 }
 
 // One gradient cell of the fused kernel (tile cell or H1 cell, local id l, global gc):
@@ -988,12 +1023,14 @@ __device__ __forceinline__ void fused_grad_cell(const FvmMeshDev &m, const FvmPh
         const unsigned q = fq[k] & ~kFzSide1, side = (fq[k] & kFzSide1) ? 1u : 0u;
         store_vec<C>(FF + (q * 2 + side) * C, U);
     }
+    float glv[D * (D + 1)];
 #pragma unroll
     for (int ii = 0; ii < D; ++ii) {
 #pragma unroll
-        for (int j = 0; j < D; ++j) GL[(ii * (D + 1) + j) * ngl + l] = g[ii][j] * phi[j];
-        GL[(ii * (D + 1) + D) * ngl + l] = g[ii][D + 1] * phi[D + 1];
+        for (int j = 0; j < D; ++j) glv[ii * (D + 1) + j] = g[ii][j] * phi[j];
+        glv[ii * (D + 1) + D] = g[ii][D + 1] * phi[D + 1];
     }
+    gl_store<D>(GL + l * (D * (D + 1)), glv);
 }
 
 // Facet q of the tile's list (descriptor d, compact geometry geo): both sides' face
@@ -1035,13 +1072,8 @@ __device__ __forceinline__ void fused_facet(const FvmMeshDev &m, const FvmPhys &
         float P0[C], P1[C];
         load_vec<C>(Pl + l0 * C, P0);
         load_vec<C>(Pl + l1 * C, P1);
-#pragma unroll
-        for (int ii = 0; ii < D; ++ii)
-#pragma unroll
-            for (int j = 0; j <= D; ++j) {
-                g0s[ii][j] = GL[(ii * (D + 1) + j) * ngl + l0];
-                g1s[ii][j] = GL[(ii * (D + 1) + j) * ngl + l1];
-            }
+        gl_load<D>(GL + l0 * (D * (D + 1)), g0s);
+        gl_load<D>(GL + l1 * (D * (D + 1)), g1s);
         // A_face_grad row: (-1/d, +1/d) on (slot-0, slot-1) cells, ascending columns
         const float inv_d = geo[2 * D + 1];
 #pragma unroll
@@ -1058,13 +1090,11 @@ __device__ __forceinline__ void fused_facet(const FvmMeshDev &m, const FvmPhys &
         float Pi[C];
         load_vec<C>(Pl + li * C, Pi);
         const int64_t b = static_cast<int64_t>(m.fac_cell[outer0 ? f : NF + f]) - N;
+        gl_load<D>(GL + li * (D * (D + 1)), g0s);
 #pragma unroll
         for (int ii = 0; ii < D; ++ii)
 #pragma unroll
-            for (int j = 0; j <= D; ++j) {
-                g0s[ii][j] = GL[(ii * (D + 1) + j) * ngl + li];
-                g1s[ii][j] = g0s[ii][j];
-            }
+            for (int j = 0; j <= D; ++j) g1s[ii][j] = g0s[ii][j];
 #pragma unroll
         for (int jg = 0; jg <= D; ++jg) {
             const int j = jg < D ? jg : D + 1;

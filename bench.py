@@ -154,16 +154,31 @@ def div_kernel_bytes(N, F, D, stage):
     return N * 4 * (NL + 1 + 1 + C + (C if stage > 0 else 0) + 2 * C) + F * 4 * C
 
 
+def model_stage_bytes(N, F, Fb, D, ab3=False):
+    """Algorithmic bytes of one RHS stage of a single-pass fused kernel, by the per-unit
+    model the roofline is quoted on (every listed array read or written once, fp32
+    values, int32 indices): per cell 99 B (2-D) / 164 B (3-D) of static data (neighbours,
+    LSQ row with its diagonal, facet list, flux_mat values, slots, displacements, eps);
+    per interior facet 28 / 36 B (cells, normal, 1/(n.d), X_orthog); per boundary facet
+    46 / 50 B; the state traffic of classical RK4 averaged over its four stages, 17 C
+    bytes per cell, 68 / 85 B (Adams-Bashforth 3: 20 C bytes, 80 / 100 B)."""
+    C = D + 2
+    static, fi, fb = (99, 28, 46) if D == 2 else (164, 36, 50)
+    dyn = 20 * C if ab3 else 17 * C
+    return N * (static + dyn) + (F - Fb) * fi + Fb * fb
+
+
 def fused_kernel_bytes(N, F, Fb, D, stage, tiles, ab3=False):
-    """k_fused (tile-fused stage), every array it needs read or written once: per cell
-    prim, LSQ values, displacements, eps, info, the local neighbour row and tile-local
-    facet row (4 x u16 each), 1/vol and the stage epilogue's traffic (as k_div); per
-    facet the compact geometry (n, n_hat, |n|, 1/(n.d), X) and the descriptor; per
-    boundary facet its BC data, boundary value written and the interior cell's gradient
-    record read and written; the tile table.  Halo re-reads (tile + H1 + H2 staging)
-    are not algorithmic: they show in the measured traffic."""
+    """k_fused (tile-fused stage), every array this implementation reads or writes once:
+    per cell prim, the LSQ values of the neighbour columns (the own column is derived),
+    displacements, eps, one packed word per slot (local neighbour, list position, side,
+    LSQ order), 1/vol and the stage epilogue's traffic (as k_div); per facet the compact
+    geometry (n, n_hat, |n|, 1/(n.d), X) and the descriptor; per boundary facet its BC
+    data, boundary value written and the interior cell's gradient record read and
+    written; the tile table.  Halo re-reads (tile + H1 + H2 staging) are not counted:
+    they show in the measured traffic."""
     C, NL = D + 2, D + 1
-    per_cell = C + (NL + 1) * D + NL * D + 1 + 1 + 2 + 2 + 1
+    per_cell = C + NL * D + NL * D + 1 + NL + 1
     # RK stage: U0, the accumulator after stage 0, accumulator (or state) and next-stage
     # primitives written; Adams-Bashforth 3: U0 and two past RHS read, state, RHS and
     # primitives written
@@ -447,17 +462,17 @@ def main():
     kbytes = {"k_grad": float(grad_kernel_bytes(N, Fb, D)), "k_facet_flux": float(facet_kernel_bytes(N, F, Fb, D)),
               "k_div": float(np.mean(div_bytes)) if div_bytes else 0.0, "k_fused": float(np.mean(fused_bytes))}
     kgbs = {k: kbytes[k] / (avg_us[k] * 1e-6) / 1e9 for k in avg_us}
-    # single-pass roofline of the step: a stage's compulsory bytes (every array it needs
-    # read or written once, as the tile-fused stage does) over the measured time of one
-    # stage in the timed region; the three-kernel pipeline moves more (the gradient
-    # records and facet fluxes make a round trip through HBM), which this figure charges
-    # to it
-    stage_bytes = kbytes["k_fused"]
+    # single-pass roofline of the step: a stage's algorithmic bytes by the per-unit model
+    # (model_stage_bytes) over the measured time of one stage in the timed region; the
+    # three-kernel pipeline moves more (the gradient records and facet fluxes make a
+    # round trip through HBM), which this figure charges to it
+    stage_bytes = float(model_stage_bytes(N, F, Fb, D, ab3=ab3))
     dom = "k_fused" if E.fused else max(avg_us, key=avg_us.get)
-    dom_bytes = kbytes[dom]
+    dom_bytes = stage_bytes if E.fused else kbytes[dom]
     achieved = stage_bytes / (step_us * 1e-6) / 1e9
-    log(f"stage: {stage_bytes / 1e6:.1f} MB algorithmic ({stage_bytes / N:.0f} B/cell) in {step_us:.1f} us -> "
-        f"{achieved:.0f} GB/s; per kernel GB/s: " + ", ".join(f"{k}={v:.0f}" for k, v in kgbs.items()))
+    log(f"stage: {stage_bytes / 1e6:.1f} MB algorithmic ({stage_bytes / N:.1f} B/cell, model) in {step_us:.1f} us "
+        f"-> {achieved:.0f} GB/s ({achieved / HBM_PEAK_GBS:.3f} of peak); this implementation's own bytes "
+        f"{kbytes['k_fused'] / 1e6:.1f} MB; per kernel GB/s: " + ", ".join(f"{k}={v:.0f}" for k, v in kgbs.items()))
 
     out = {
         "metric": f"cell-updates/s (fused HIP {a.integrator} step, {D}-D NS channel)",
@@ -480,8 +495,12 @@ def main():
                    "kernel_avg_us": avg_us,
                    "kernel_timing": ("timed region / launches (a step is `stages` k_fused launches)" if E.fused else
                                      "one event pair around back-to-back launches of each kernel"),
-                   "kernel_algorithmic_bytes": {k: kbytes[k] for k in avg_us}, "kernel_gbs": kgbs,
-                   "stage_us": step_us, "stage_algorithmic_bytes": stage_bytes},
+                   "kernel_implementation_bytes": {k: kbytes[k] for k in avg_us}, "kernel_gbs": kgbs,
+                   "stage_us": step_us, "stage_algorithmic_bytes": stage_bytes,
+                   "algorithmic_bytes_per_cell": stage_bytes / N,
+                   "algorithmic_model": "per cell static 99 B (2-D) / 164 B (3-D) + per interior facet 28 / 36 B + "
+                                        "per boundary facet 46 / 50 B + RK4 state traffic 17 C B per cell "
+                                        "(AB3: 20 C B)"},
         "roofline": {"bound": "hbm", "kernel": dom if E.fused else "stage (k_grad + k_facet_flux + k_div)",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
@@ -615,8 +634,11 @@ def batch_main(a):
               "k_fused": float(np.mean([fused_kernel_bytes(N, F, Fb, 2, i, (N + 255) // 256) for i in range(4)]))}
     kgbs = {k: kbytes[k] / (avg_us[k] * 1e-6) / 1e9 for k in avg_us}
     dom = max(avg_us, key=avg_us.get)
-    achieved = kbytes["k_fused"] / (step_us * 1e-6) / 1e9
+    stage_bytes = float(model_stage_bytes(N, F, Fb, 2))
+    achieved = stage_bytes / (step_us * 1e-6) / 1e9
     log("kernel averages (us): " + ", ".join(f"{k}={v:.1f}" for k, v in avg_us.items()))
+    log(f"stage: {stage_bytes / 1e6:.1f} MB algorithmic (model) in {step_us:.1f} us -> {achieved:.0f} GB/s "
+        f"({achieved / HBM_PEAK_GBS:.3f} of peak)")
     out = {
         "metric": "cell-updates/s (batched 2-D NS channels, HIP-graph RK4)",
         "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": steps, "warmup": a.warmup,
@@ -629,11 +651,12 @@ def batch_main(a):
                    "tiles": tile_summary(E.mesh), "kernel_avg_us": avg_us,
                    "kernel_timing": ("timed region / launches (a step is 4 k_fused launches)" if E.fused else
                                      "one event pair around back-to-back launches of each kernel"),
-                   "kernel_gbs": kgbs, "stage_us": step_us},
+                   "kernel_gbs": kgbs, "stage_us": step_us, "kernel_implementation_bytes": kbytes,
+                   "algorithmic_bytes_per_cell": stage_bytes / N},
         "roofline": {"bound": "hbm", "kernel": "k_fused" if E.fused else "stage (three kernels)",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": None, "timing": "timed region: ms_per_step / stages per step",
-                     "bytes_per_launch": kbytes["k_fused"]},
+                     "bytes_per_launch": stage_bytes},
     }
     if E.fused:
         t, src = traffic_of(a, "k_fused", N)

@@ -12,7 +12,7 @@ import torch
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libfvm_hip.so")
-ABI_VERSION = 13
+ABI_VERSION = 14
 
 FVM_ERR_ROW_OVERFLOW = 2
 
@@ -32,10 +32,9 @@ class FvmMeshDev(ctypes.Structure):
                 ("slot_g0", c_void_p), ("slot_ng", c_void_p), ("group_s0", c_void_p), ("group_s1", c_void_p),
                 ("group_count", c_void_p), ("cell_mesh", c_void_p), ("mesh_phys", c_void_p),
                 ("persist_blocks", i32), ("fz_threads", i32),
-                ("fz_tile", c_void_p), ("fz_halo", c_void_p), ("fz_hrec", c_void_p), ("fz_nbr", c_void_p),
-                ("fz_fdesc", c_void_p), ("fz_ffac", c_void_p), ("fz_fdesc_x", c_void_p), ("fz_geo", c_void_p),
-                ("fz_geo_x", c_void_p), ("fz_cfac", c_void_p), ("fz_h1f", c_void_p), ("fz_ncl", i32), ("fz_ngl", i32),
-                ("fz_nfl", i32),
+                ("fz_tile", c_void_p), ("fz_halo", c_void_p), ("fz_hrec", c_void_p), ("fz_cw", c_void_p),
+                ("fz_hw", c_void_p), ("fz_fdesc", c_void_p), ("fz_ffac", c_void_p), ("fz_fdesc_x", c_void_p),
+                ("fz_geo", c_void_p), ("fz_geo_x", c_void_p), ("fz_ncl", i32), ("fz_ngl", i32), ("fz_nfl", i32),
                 ("fz_nh1", i32)]
 
 

@@ -341,10 +341,13 @@ __global__ void __launch_bounds__(kThreads) k_bc_mean(FvmMeshDev m, FvmPhys p, F
 // primitives, or boundary values).  The row is summed in slot order with fused
 // multiply-adds: fixed register operands, where the reference's ascending column order
 // would select each column's source per cell.
-template <int D, int LP>
+// With FACE, also the limited face values at the cell's facets, Uf[k][j] = P[j] + phi[j]
+// dU[k][j], from the same reconstruction dU the limiter used (side_values' formula).
+template <int D, int LP, bool FACE = false>
 __device__ __forceinline__ void cell_gradient(const float (&P)[D + 2], const float (&Un)[D + 1][D + 2],
                                               const float (&gv)[D + 2][D], const float (&disp)[D + 1][D], float eps,
-                                              float (&g)[D][D + 2], float (&phi)[D + 2]) {
+                                              float (&g)[D][D + 2], float (&phi)[D + 2],
+                                              float (*Uf)[D + 2] = nullptr) {
     constexpr int C = D + 2, NL = D + 1;
 #pragma unroll
     for (int jj = 0; jj <= NL; ++jj) {
@@ -367,15 +370,21 @@ __device__ __forceinline__ void cell_gradient(const float (&P)[D + 2], const flo
         }
         const float upper = mx - P[j], lower = mn - P[j];
         float ph = 0.0f;
+        float dUk[NL];
 #pragma unroll
         for (int k = 0; k < NL; ++k) {
             float dU = g[0][j] * disp[k][0];
 #pragma unroll
             for (int i = 1; i < D; ++i) dU = fmaf(g[i][j], disp[k][i], dU);
+            dUk[k] = dU;
             const float lk = lim_phi<LP>(dU > 0.0f ? upper : lower, dU, eps);
             ph = (k == 0) ? lk : fminf(ph, lk);
         }
         phi[j] = ph;
+        if constexpr (FACE) {
+#pragma unroll
+            for (int k = 0; k < NL; ++k) Uf[k][j] = fmaf(dUk[k], ph, P[j]);
+        }
     }
 }
 
@@ -737,10 +746,10 @@ __device__ __forceinline__ void stage_epilogue(const FvmMeshDev &m, const FvmPhy
     const bool first = st.stage == 0;
     const bool has2 = st.acc2 != nullptr;
     // Stage flags are uniform; every per-cell load is issued before the first use so
-    // that a cell's memory round trips overlap.
-    double sq[3 * C];
-#pragma unroll
-    for (int j = 0; j < 3 * C; ++j) sq[j] = 0.0;
+    // that a cell's memory round trips overlap.  nrm: the fp32 vectors whose squares the
+    // last stage of an embedded tableau sums in fp64 (acc - acc2, acc, U0); set and read
+    // only on that path.
+    float nrm[3][C];
     float Pn[C];
 #pragma unroll
     for (int j = 0; j < C; ++j) Pn[j] = 0.0f;
@@ -768,10 +777,17 @@ __device__ __forceinline__ void stage_epilogue(const FvmMeshDev &m, const FvmPhy
         const float dt = dtm;
         float kk[C], a[C], a2[C];
 #pragma unroll
-        for (int j = 0; j < C; ++j) {
-            kk[j] = dt * acc[j];
-            a[j] = first ? st.b * kk[j] : ap[j] + st.b * kk[j];
-            a2[j] = first ? st.b2 * kk[j] : a2p[j] + st.b2 * kk[j];
+        for (int j = 0; j < C; ++j) kk[j] = dt * acc[j];
+        if (first) {
+#pragma unroll
+            for (int j = 0; j < C; ++j) a[j] = st.b * kk[j];
+        } else {
+#pragma unroll
+            for (int j = 0; j < C; ++j) a[j] = ap[j] + st.b * kk[j];
+        }
+        if (has2) {
+#pragma unroll
+            for (int j = 0; j < C; ++j) a2[j] = first ? st.b2 * kk[j] : a2p[j] + st.b2 * kk[j];
         }
         if (st.store_k) store_vec<C>(st.kbuf + static_cast<int64_t>(st.stage) * N * C + o, kk);
         float Un[C];
@@ -806,10 +822,9 @@ __device__ __forceinline__ void stage_epilogue(const FvmMeshDev &m, const FvmPhy
             if (has2) {
 #pragma unroll
                 for (int j = 0; j < C; ++j) {
-                    const double dd = static_cast<double>(a[j] - a2[j]);
-                    sq[j] = dd * dd;
-                    sq[C + j] = static_cast<double>(a[j]) * a[j];
-                    sq[2 * C + j] = static_cast<double>(U0v[j]) * U0v[j];
+                    nrm[0][j] = a[j] - a2[j];
+                    nrm[1][j] = a[j];
+                    nrm[2][j] = U0v[j];
                 }
             }
         }
@@ -823,6 +838,13 @@ __device__ __forceinline__ void stage_epilogue(const FvmMeshDev &m, const FvmPhy
     }
     if (st.last && st.acc2) {
         // fp64 block partials of the three column norms (Adaptive.update_stepsize)
+        double sq[3 * C];
+#pragma unroll
+        for (int q = 0; q < 3 * C; ++q) {
+            const double x = live ? static_cast<double>(nrm[q / C][q % C]) : 0.0;
+            sq[q] = x * x;
+        }
+#pragma unroll
         for (int q = 0; q < 3 * C; ++q) {
             sred[threadIdx.x] = sq[q];
             __syncthreads();
@@ -904,10 +926,11 @@ constexpr unsigned kFzSide1 = 0x8000u;   // H1 facet table: the H1 cell is the s
 
 template <int D>
 __host__ __device__ constexpr int fz_geo_n() { return D == 2 ? 8 : 12; }
-// H1 static record: LSQ values [NL+1][D], displacements [NL][D], eps; stored
-// component-major, [HN][sum n1] (a lane per H1 cell reads one coalesced row per component)
+// H1 static record: LSQ values of the neighbour columns [NL][D], displacements [NL][D],
+// eps; stored component-major, [HN][sum n1] (a lane per H1 cell reads one coalesced row
+// per component)
 template <int D>
-__host__ __device__ constexpr int fz_hrec_n() { return (D + 2) * D + (D + 1) * D + 1; }
+__host__ __device__ constexpr int fz_hrec_n() { return 2 * (D + 1) * D + 1; }
 template <int D>
 __host__ __device__ constexpr int fz_fpt() { return D == 2 ? 2 : 3; }   // facets per thread
 
@@ -997,8 +1020,8 @@ __device__ __forceinline__ void fused_grad_cell(const FvmMeshDev &m, const FvmPh
             }
         }
     }
-    float g[D][C], phi[C];
-    cell_gradient<D, LP>(P, Un, gv, disp, eps, g, phi);
+    float g[D][C], phi[C], Uf[NL][C];
+    cell_gradient<D, LP, true>(P, Un, gv, disp, eps, g, phi, Uf);
     if (own && bnd) {
         float *o = w.gp + gc * Dm::REC;
 #pragma unroll
@@ -1012,16 +1035,8 @@ __device__ __forceinline__ void fused_grad_cell(const FvmMeshDev &m, const FvmPh
 #pragma unroll
     for (int k = 0; k < NL; ++k) {
         if (fq[k] == kFzNone) continue;
-        float U[C];
-#pragma unroll
-        for (int j = 0; j < C; ++j) {
-            float dU = g[0][j] * disp[k][0];
-#pragma unroll
-            for (int ii = 1; ii < D; ++ii) dU = fmaf(g[ii][j], disp[k][ii], dU);
-            U[j] = fmaf(dU, phi[j], P[j]);
-        }
         const unsigned q = fq[k] & ~kFzSide1, side = (fq[k] & kFzSide1) ? 1u : 0u;
-        store_vec<C>(FF + (q * 2 + side) * C, U);
+        store_vec<C>(FF + (q * 2 + side) * C, Uf[k]);
     }
     float glv[D * (D + 1)];
 #pragma unroll
@@ -1140,15 +1155,57 @@ struct FzGradIn {
     ushort4 nr, fq;
 };
 
-// Inputs of tile cell c (thread-local row l of the tile's neighbour rows); its cfac row and
-// slot bits give the face-value positions.
+// Slot words of the fused stage (one 32-bit word per cell slot): bits 0-11 the local id
+// of the neighbour behind the slot (0xFFF: boundary facet), 12-21 the facet's position in
+// the tile's list (0x3FF: not in it), 22 the cell is the facet's slot-1 side, 23-24 the
+// slot's position in the reference's local order of the cell's LSQ row.  Decoded into the
+// neighbour row, the face-value positions (q | kFzSide1 on the slot-1 side, kFzNone when
+// not in the list), and the order positions.
 template <int D>
-__device__ __forceinline__ void fz_load_tile_cell(const FvmMeshDev &m, int64_t c, int64_t nbr_row, FzGradIn<D> &g,
-                                                  int &info, ushort4 &cf) {
+__device__ __forceinline__ void fz_decode_words(const unsigned (&wd)[D + 1], FzGradIn<D> &g,
+                                                unsigned (&ref)[D + 1]) {
+    constexpr int NL = D + 1;
+    unsigned nl[4] = {kFzNone, kFzNone, kFzNone, kFzNone}, fq[4] = {kFzNone, kFzNone, kFzNone, kFzNone};
+#pragma unroll
+    for (int k = 0; k < NL; ++k) {
+        const unsigned w = wd[k], l = w & 0xFFFu, q = (w >> 12) & 0x3FFu;
+        nl[k] = l == 0xFFFu ? kFzNone : l;
+        fq[k] = q == 0x3FFu ? kFzNone : (q | (((w >> 22) & 1u) ? kFzSide1 : 0u));
+        ref[k] = (w >> 23) & 3u;
+    }
+    g.nr = make_ushort4(nl[0], nl[1], nl[2], nl[3]);
+    g.fq = make_ushort4(fq[0], fq[1], fq[2], fq[3]);
+}
+
+// The cell's own LSQ column, which is not stored: minus the fp32 sum of the neighbour
+// columns in the reference's local order, as build_sparse_gradient_matrix forms the
+// G_mats diagonal (fvm_mesh.py:5-42) -- bitwise that stored value.
+template <int D>
+__device__ __forceinline__ void fz_own_column(FzGradIn<D> &g, const unsigned (&ref)[D + 1]) {
+    constexpr int NL = D + 1;
+#pragma unroll
+    for (int ii = 0; ii < D; ++ii) {
+        float acc = 0.0f;
+#pragma unroll
+        for (int r = 0; r < NL; ++r) {
+            float t = g.gv[0][ii];
+#pragma unroll
+            for (int k = 1; k < NL; ++k) t = ref[k] == static_cast<unsigned>(r) ? g.gv[k][ii] : t;
+            acc = r == 0 ? t : acc + t;
+        }
+        g.gv[NL][ii] = -acc;
+    }
+}
+
+// Inputs of tile cell c: LSQ values, displacements, eps and slot words; info gets the
+// cell's slot bits (bit k: slot-1 side of facet k) and cf its facets' list positions.
+template <int D>
+__device__ __forceinline__ void fz_load_tile_cell(const FvmMeshDev &m, int64_t c, FzGradIn<D> &g, int &info,
+                                                  ushort4 &cf) {
     constexpr int NL = D + 1;
     const int64_t N = m.n_cells;
 #pragma unroll
-    for (int jj = 0; jj <= NL; ++jj)
+    for (int jj = 0; jj < NL; ++jj)
 #pragma unroll
         for (int ii = 0; ii < D; ++ii) g.gv[jj][ii] = m.cell_gval[(jj * D + ii) * N + c];
 #pragma unroll
@@ -1156,42 +1213,47 @@ __device__ __forceinline__ void fz_load_tile_cell(const FvmMeshDev &m, int64_t c
 #pragma unroll
         for (int ii = 0; ii < D; ++ii) g.disp[k][ii] = m.cell_disp[(k * D + ii) * N + c];
     g.eps = m.cell_eps[c];
-    info = m.cell_info[c];
-    g.nr = *reinterpret_cast<const ushort4 *>(m.fz_nbr + nbr_row * 4);
-    cf = *reinterpret_cast<const ushort4 *>(m.fz_cfac + c * 4);
+    unsigned wd[NL], ref[NL];
+#pragma unroll
+    for (int k = 0; k < NL; ++k) wd[k] = m.fz_cw[k * N + c];
+    fz_decode_words<D>(wd, g, ref);
+    fz_own_column<D>(g, ref);
+    unsigned q[4] = {0u, 0u, 0u, 0u};
+    info = 0;
+#pragma unroll
+    for (int k = 0; k < NL; ++k) {
+        q[k] = (wd[k] >> 12) & 0x3FFu;
+        info |= static_cast<int>((wd[k] >> 22) & 1u) << k;
+    }
+    cf = make_ushort4(q[0], q[1], q[2], q[3]);
 }
 
-// Face-value positions of a tile cell: its facets' list positions and its sides.
-template <int D>
-__device__ __forceinline__ ushort4 fz_tile_fq(ushort4 cf, int info) {
-    return make_ushort4(cf.x | ((info & 1) ? kFzSide1 : 0u), cf.y | ((info & 2) ? kFzSide1 : 0u),
-                        cf.z | ((info & 4) ? kFzSide1 : 0u),
-                        D + 1 > 3 ? (cf.w | ((info & 8) ? kFzSide1 : 0u)) : kFzNone);
-}
-
-// Inputs of the tile's H1 entry h (global id returned): its component-major static record,
-// neighbour row and H1 facet table row.
+// Inputs of the tile's H1 entry h (global id returned): its component-major static record
+// and slot words.
 template <int D>
 __device__ __forceinline__ int64_t fz_load_h1_cell(const FvmMeshDev &m, int h, int halo_off, int h1_off,
-                                                   int nbr_off, FzGradIn<D> &g) {
+                                                   FzGradIn<D> &g) {
     constexpr int NL = D + 1, HREC = fz_hrec_n<D>();
     const int64_t gc = m.fz_halo[halo_off + h];
     const float *r = m.fz_hrec + h1_off + h;
-    const int64_t hs = m.fz_nh1;             // component stride of the H1 records
+    const int64_t hs = m.fz_nh1;             // component stride of the H1 records and words
     float rv[HREC];
 #pragma unroll
     for (int q = 0; q < HREC; ++q) rv[q] = r[q * hs];
+    unsigned wd[NL], ref[NL];
 #pragma unroll
-    for (int jj = 0; jj <= NL; ++jj)
+    for (int k = 0; k < NL; ++k) wd[k] = m.fz_hw[k * hs + h1_off + h];
+#pragma unroll
+    for (int jj = 0; jj < NL; ++jj)
 #pragma unroll
         for (int ii = 0; ii < D; ++ii) g.gv[jj][ii] = rv[jj * D + ii];
 #pragma unroll
     for (int k = 0; k < NL; ++k)
 #pragma unroll
-        for (int ii = 0; ii < D; ++ii) g.disp[k][ii] = rv[(NL + 1) * D + k * D + ii];
-    g.eps = rv[(NL + 1) * D + NL * D];
-    g.nr = *reinterpret_cast<const ushort4 *>(m.fz_nbr + (static_cast<int64_t>(nbr_off) + kThreads + h) * 4);
-    g.fq = *reinterpret_cast<const ushort4 *>(m.fz_h1f + static_cast<int64_t>(h1_off + h) * 4);
+        for (int ii = 0; ii < D; ++ii) g.disp[k][ii] = rv[NL * D + k * D + ii];
+    g.eps = rv[2 * NL * D];
+    fz_decode_words<D>(wd, g, ref);
+    fz_own_column<D>(g, ref);
     return gc;
 }
 
@@ -1226,7 +1288,7 @@ k_fused(FvmMeshDev m, FvmPhys p, FvmWork w, const float *__restrict__ prim, floa
         if (st.tile_active && !st.tile_active[tile]) return;     // a tile the caller does not step
     }
     const int *tb = m.fz_tile + tile * kFzTab;
-    const int halo_off = tb[0], n1 = tb[1], n2 = tb[2], nbr_off = tb[3], fo0 = tb[4], nfo = tb[5];
+    const int halo_off = tb[0], n1 = tb[1], n2 = tb[2], fo0 = tb[4], nfo = tb[5];
     const int foff = tb[6], nfor = tb[7], g0 = tb[8], ng = tb[9], h1_off = tb[10];
     float *Pl = lds;
     float *GL = lds + fz_gl_offset<D>(m.fz_ncl);
@@ -1274,20 +1336,19 @@ k_fused(FvmMeshDev m, FvmPhys p, FvmWork w, const float *__restrict__ prim, floa
             store_vec<C>(Pl + i * C, u);
         }
         if (has_own) {
-            fz_load_tile_cell<D>(m, c0 + i, static_cast<int64_t>(nbr_off) + i, og, oinfo, cf);
+            fz_load_tile_cell<D>(m, c0 + i, og, oinfo, cf);
             if (m.mesh_phys) meshT[i] = static_cast<unsigned short>(m.cell_mesh[c0 + i]);
         }
         hg.eps = 0.0f;
         hg.nr = make_ushort4(0, 0, 0, 0);
         hg.fq = make_ushort4(kFzNone, kFzNone, kFzNone, kFzNone);
         int64_t hgc = 0;
-        if (has_h1) hgc = fz_load_h1_cell<D>(m, h, halo_off, h1_off, nbr_off, hg);
+        if (has_h1) hgc = fz_load_h1_cell<D>(m, h, halo_off, h1_off, hg);
         for (int q = i; q < n1 + n2; q += NT) {
             float u[C];
             load_prim<D>(prim + static_cast<int64_t>(m.fz_halo[halo_off + q]) * C, u);
             store_vec<C>(Pl + (kThreads + q) * C, u);
         }
-        og.fq = fz_tile_fq<D>(cf, oinfo);
         __syncthreads();
 
         // ---- phase 1: gradients (the first facet's inputs in flight during the H1
@@ -1312,17 +1373,16 @@ k_fused(FvmMeshDev m, FvmPhys p, FvmWork w, const float *__restrict__ prim, floa
             float u[C];
             load_prim<D>(prim + (c0 + i) * C, u);
             store_vec<C>(Pl + i * C, u);
-            fz_load_tile_cell<D>(m, c0 + i, static_cast<int64_t>(nbr_off) + i, og, oinfo, cf);
+            fz_load_tile_cell<D>(m, c0 + i, og, oinfo, cf);
             if (m.mesh_phys) meshT[i] = static_cast<unsigned short>(m.cell_mesh[c0 + i]);
         } else if (has_h1) {
-            ogc = fz_load_h1_cell<D>(m, h, halo_off, h1_off, nbr_off, og);
+            ogc = fz_load_h1_cell<D>(m, h, halo_off, h1_off, og);
         }
         for (int q = i; q < n1 + n2; q += NT) {
             float u[C];
             load_prim<D>(prim + static_cast<int64_t>(m.fz_halo[halo_off + q]) * C, u);
             store_vec<C>(Pl + (kThreads + q) * C, u);
         }
-        if (has_own) og.fq = fz_tile_fq<D>(cf, oinfo);
         __syncthreads();
 
         // ---- phase 1: gradients, then the first facet's loads ----
@@ -1477,8 +1537,8 @@ static size_t max_lds_per_workgroup() {
 
 template <int D>
 static int check_fused_d(const FvmMeshDev &m, const char *who) {
-    if (!m.fz_tile || !m.fz_halo || !m.fz_hrec || !m.fz_nbr || !m.fz_h1f || !m.fz_fdesc || !m.fz_ffac ||
-        !m.fz_fdesc_x || !m.fz_geo || !m.fz_geo_x || !m.fz_cfac || m.fz_ncl < kThreads || m.fz_ngl < kThreads ||
+    if (!m.fz_tile || !m.fz_halo || !m.fz_hrec || !m.fz_cw || !m.fz_hw || !m.fz_fdesc || !m.fz_ffac ||
+        !m.fz_fdesc_x || !m.fz_geo || !m.fz_geo_x || m.fz_ncl < kThreads || m.fz_ngl < kThreads ||
         m.fz_ngl > 2 * kThreads || m.fz_ngl > m.fz_ncl || m.fz_ncl > 0xFFF || m.fz_nfl < 1 ||
         m.fz_nfl > fz_fpt<D>() * kThreads || fused_lds_bytes<D>(m) > max_lds_per_workgroup() ||
         (m.mesh_phys && m.n_meshes > 0xFFFF) ||

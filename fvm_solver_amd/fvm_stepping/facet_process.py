@@ -199,7 +199,7 @@ def host_layout(mesh, n_comp: int, bc_tags, cfg) -> dict:
             bc_char[mask, D + 3:] = vinf[None, :] - nn * vn[:, None]
     per = fs64[c2f].sum(1)
     return dict(D=D, C=C, N=N, F=F, Fb=Fb, n_groups=len(modes), c2f_s=c2f_s, nbr_s=nbr_s, slot_s=slot_s, nbk=nbk,
-                gval=gval, info=info, disp_s=disp_s, inv_vol=inv_vol, eps=eps, geo=geo, bcell=bcell,
+                gval=gval, gref=perm.astype(np.int8), info=info, disp_s=disp_s, inv_vol=inv_vol, eps=eps, geo=geo, bcell=bcell,
                 group=group, flags=flags, bc_val=bc_val, bc_l=l_vec.astype(np.float32), bc_ga=bc_ga, bc_gb=bc_gb,
                 bc_char=bc_char, side=side, volumes64=g["volumes"], h64=D * g["volumes"] / per,
                 phys=(float(cfg.viscosity), float(cfg.visc_bulk), float(cfg.thermal_cond)))
@@ -284,7 +284,7 @@ class DeviceMesh:
         self.host = dict(c2f_s=c2f_s, nbr_s=nbr_s, bcell=bcell, group=group, cell_mesh=cell_mesh,
                          blk_slot=blk_slot, slot_g0=slot_g0, slot_ng=slot_ng, group_s0=group_s0,
                          group_s1=group_s1, group_count=group_count)
-        for k in ("slot_s", "nbk", "gval", "info", "disp_s", "inv_vol", "eps", "geo", "flags", "bc_val",
+        for k in ("slot_s", "nbk", "gval", "gref", "info", "disp_s", "inv_vol", "eps", "geo", "flags", "bc_val",
                   "bc_l", "bc_ga", "bc_gb", "bc_char", "side"):
             self.host[k] = cat(k)
         h = self.host
@@ -311,7 +311,8 @@ class DeviceMesh:
             lim = lds_limit if lds_limit is not None else self.lds_limit_for(device, D)
             try:
                 self.tiles = build_tiles(N, D, nbr_s, fpos[c2f_s], fac_cell[:, fperm], h["gval"], h["disp_s"],
-                                         h["eps"], h["geo"][fperm], bcell, group, cell_mesh, go, lds_limit=lim)
+                                         h["eps"], h["geo"][fperm], bcell, group, cell_mesh, go, lds_limit=lim,
+                                         gref=h["gref"])
                 self.tiles_reason = None
             except TilesUnavailable as e:
                 self.tiles_reason = str(e)
@@ -357,11 +358,10 @@ class DeviceMesh:
         self.tz = dict(
             fz_tile=up(tl["tab"], i32), fz_halo=up_nz(tl["halo"], i32),
             fz_hrec=up(tl["hrec"] if tl["nh1"] else np.zeros(1), f32),
-            fz_nbr=up(tl["nbr"].view(np.int16), torch.int16), fz_fdesc=up(tl["fdesc"].view(np.int32), i32),
+            fz_cw=up(tl["cw"].view(np.int32), i32), fz_hw=up_nz(tl["hw"].view(np.int32).reshape(-1), i32),
+            fz_fdesc=up(tl["fdesc"].view(np.int32), i32),
             fz_ffac=up_nz(tl["ffac"], i32), fz_fdesc_x=up_nz(tl["fdesc_x"].view(np.int32), i32),
-            fz_geo=up(tl["geo"], f32), fz_geo_x=up_nz(tl["geo_x"], f32),
-            fz_cfac=up(tl["cfac"].view(np.int16), torch.int16),
-            fz_h1f=up_nz(tl["h1f"].view(np.int16) if len(tl["h1f"]) else np.zeros((1, 4), np.int16), torch.int16))
+            fz_geo=up(tl["geo"], f32), fz_geo_x=up_nz(tl["geo_x"], f32))
         self.struct = nat.FvmMeshDev(**base, fz_ncl=tl["ncl"], fz_ngl=tl["ngl"], fz_nfl=tl["nfl"], fz_nh1=tl["nh1"],
                                      **{k: v.data_ptr() for k, v in self.tz.items()})
 

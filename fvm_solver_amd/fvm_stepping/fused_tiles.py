@@ -60,6 +60,32 @@ def fused_lds_bytes(D: int, ncl: int, ngl: int, nfl: int, batch: bool) -> int:
     return 4 * floats + ((TILE * 2 + TILE_MESHES * 4 * 4) if batch else 0)
 
 
+NONE12 = 0xFFF     # packed slot word: boundary slot (no local neighbour)
+NONE10 = 0x3FF     # packed slot word: facet not in the tile's list (H1 cells)
+
+
+def hrec_n(D: int) -> int:
+    """Floats per H1 record: LSQ values of the NL neighbour columns, displacements, eps."""
+    return 2 * (D + 1) * D + 1
+
+
+def pack_slot_words(nl, q, side, ref):
+    """One 32-bit word per cell slot for the fused kernel: bits 0-11 the local id of the
+    neighbour behind the slot (NONE12 for a boundary facet), 12-21 the facet's position
+    in the tile's list (NONE10: not in it), 22 the cell is the facet's slot-1 side, 23-24
+    the slot's position in the reference's local order of the cell's LSQ row."""
+    nl, q, side, ref = (np.asarray(a, np.int64) for a in (nl, q, side, ref))
+    if np.any((nl < 0) | (nl > NONE12)) or np.any((q < 0) | (q > NONE10)) or np.any((ref < 0) | (ref > 3)):
+        raise TilesUnavailable("slot word field out of range")
+    return (nl | (q << 12) | (side << 22) | (ref << 23)).astype(np.uint32)
+
+
+def unpack_slot_words(w):
+    """Inverse of pack_slot_words: (nl, q, side, ref)."""
+    w = np.asarray(w, np.int64)
+    return w & 0xFFF, (w >> 12) & 0x3FF, (w >> 22) & 1, (w >> 23) & 3
+
+
 class TilesUnavailable(ValueError):
     pass
 
@@ -75,13 +101,16 @@ def _lookup(keys: np.ndarray, q: np.ndarray) -> np.ndarray:
 def build_tiles(N: int, D: int, nbr: np.ndarray, cell_facet: np.ndarray, fac_cell: np.ndarray,
                 gval: np.ndarray, disp: np.ndarray, eps: np.ndarray, geo: np.ndarray,
                 bcell: np.ndarray, group: np.ndarray, cell_mesh: np.ndarray, group_off: np.ndarray,
-                lds_limit: int = 64 * 1024, lds_target: int | None = None) -> dict:
+                lds_limit: int = 64 * 1024, lds_target: int | None = None, gref: np.ndarray | None = None) -> dict:
     """Tile tables for the fused kernel.
 
     nbr [N, NL] neighbour per slot (>= N: N + boundary index); cell_facet [N, NL] device
     facet positions; fac_cell [2, F] device order; gval [N, D, NL+1]; disp [N, NL, D];
     eps [N]; geo [F, G] device order (first 3D+2 floats: n, n_hat, |n|, 1/(n.d),
-    X); bcell/group [Fb]; cell_mesh [N]; group_off = first group of each mesh (+ total).
+    X); bcell/group [Fb]; cell_mesh [N]; group_off = first group of each mesh (+ total);
+    gref [N, NL] the position of each slot's column in the reference's local order of
+    the cell's G_mats row (the order its own column, minus the sum of the others, is
+    summed in; default: slot order).
     Raises TilesUnavailable when a tile's working set does not fit ``lds_limit`` bytes
     of LDS or the local ids overflow (a badly ordered mesh).  ``lds_target`` (optional,
     <= lds_limit) is only reported: whether the tables fit it (e.g. the LDS that lets
@@ -155,12 +184,14 @@ def build_tiles(N: int, D: int, nbr: np.ndarray, cell_facet: np.ndarray, fac_cel
         raise AssertionError("H1 neighbour not staged")
     rows[h1_row, :NL] = np.where(nbr[h1c] < N, lh, NONE16).astype(np.uint16)
 
-    # ---- H1 static records: LSQ values [NL+1][D] (slot order), displacements [NL][D], eps,
-    # stored component-major [HN][sum n1] (the kernel's lanes read one row per component) ----
-    HN = (NL + 1) * D + NL * D + 1
+    # ---- H1 static records: LSQ values of the neighbour columns [NL][D] (slot order; the
+    # kernel derives the cell's own column as the reference does), displacements [NL][D],
+    # eps, stored component-major [HN][sum n1] (the kernel's lanes read one row per
+    # component) ----
+    HN = hrec_n(D)
     rec = np.zeros((len(K1), HN), np.float32)
-    rec[:, :(NL + 1) * D] = np.asarray(gval, np.float32)[h1c].transpose(0, 2, 1).reshape(len(K1), -1)
-    o = (NL + 1) * D
+    rec[:, :NL * D] = np.asarray(gval, np.float32)[h1c][:, :, :NL].transpose(0, 2, 1).reshape(len(K1), -1)
+    o = NL * D
     rec[:, o:o + NL * D] = np.asarray(disp, np.float32)[h1c].reshape(len(K1), -1)
     rec[:, o + NL * D] = np.asarray(eps, np.float32)[h1c]
     hrec = np.ascontiguousarray(rec.T)
@@ -277,6 +308,20 @@ def build_tiles(N: int, D: int, nbr: np.ndarray, cell_facet: np.ndarray, fac_cel
     halo[halo_off[t1] + (np.arange(len(K1)) - s1[t1])] = h1c
     halo[halo_off[t2] + n1[t2] + (np.arange(len(K2)) - s2[t2])] = K2 % N
 
+    # ---- packed slot words of the kernel: tile cells cw [NL][N] (global cell index), H1
+    # entries hw [NL][sum n1] (H1 entry index): see pack_slot_words ----
+    if gref is None:
+        gref = np.broadcast_to(np.arange(NL, dtype=np.int64), (N, NL))
+    gref = np.asarray(gref, np.int64)
+    own_side = (fac_cell[1, cell_facet] == np.arange(N)[:, None]).astype(np.int64)
+    own_nl = rows[own_row, :NL].astype(np.int64)
+    cw = pack_slot_words(np.where(own_nl == NONE16, NONE12, own_nl), cfac[:, :NL].astype(np.int64), own_side,
+                         gref)
+    hv = h1f[:, :NL].astype(np.int64)
+    h_nl = rows[h1_row, :NL].astype(np.int64)
+    hw = pack_slot_words(np.where(h_nl == NONE16, NONE12, h_nl), np.where(hv == NONE16, NONE10, hv & ~SIDE_BIT),
+                         np.where(hv == NONE16, 0, (hv & SIDE_BIT) != 0).astype(np.int64), gref[h1c])
+
     fpt = facets_per_thread(D)
     if int(n1.max(initial=0)) > TILE or int((nfo + nfor).max(initial=0)) > fpt * TILE:
         raise TilesUnavailable(f"a tile has more than {TILE} halo cells or {fpt * TILE} facets")
@@ -287,21 +332,38 @@ def build_tiles(N: int, D: int, nbr: np.ndarray, cell_facet: np.ndarray, fac_cel
     lds = fused_lds_bytes(D, ncl, ngl, nfl, batch)
     if lds > lds_limit:
         raise TilesUnavailable(f"tile working set {lds} B exceeds {lds_limit} B of LDS")
-    return dict(tab=tab, halo=halo, hrec=hrec, nbr=rows, h1f=h1f, fdesc=fdesc, ffac=fx.astype(np.int32),
+    if nfl > NONE10:
+        raise TilesUnavailable(f"a tile lists more than {NONE10} facets")
+    return dict(tab=tab, halo=halo, hrec=hrec, nbr=rows, h1f=h1f, cw=np.ascontiguousarray(cw.T),
+                hw=np.ascontiguousarray(hw.T), fdesc=fdesc, ffac=fx.astype(np.int32),
                 fdesc_x=fdesc_x, geo=geo_c, geo_x=geo_x, cfac=cfac, ncl=ncl, ngl=ngl, nfl=nfl, nh1=len(K1),
                 lds_bytes=lds,
                 lds_fits_target=None if lds_target is None else lds <= lds_target, n_tiles=T,
                 halo_cells=(int(n1.sum()), int(n2.sum())), foreign=int(nfor.sum()))
 
 
-def check_tiles(t: dict, N: int, D: int, nbr, cell_facet, fac_cell) -> None:
+def check_tiles(t: dict, N: int, D: int, nbr, cell_facet, fac_cell, gref=None) -> None:
     """Rebuild, from the tile tables alone, every global relation the fused kernel relies
-    on and compare it with the mesh (raises AssertionError on the first mismatch)."""
+    on and compare it with the mesh (raises AssertionError on the first mismatch).  The
+    packed slot words (what the kernel reads) must decode to the same neighbour rows,
+    list positions and sides as the unpacked tables, and to the slots' reference order."""
     NL = D + 1
     nbr = np.asarray(nbr, np.int64)
     cell_facet = np.asarray(cell_facet, np.int64)
     fac_cell = np.asarray(fac_cell, np.int64)
     tab, halo = t["tab"], t["halo"]
+    # packed words of the tile cells against the unpacked tables
+    nl, q, side, ref = unpack_slot_words(t["cw"].T)
+    orow = t["nbr"][(np.repeat(tab[:, 3], TILE) + np.tile(np.arange(TILE), t["n_tiles"]))[:N], :NL].astype(np.int64)
+    assert np.array_equal(np.where(orow == NONE16, NONE12, orow), nl)
+    assert np.array_equal(t["cfac"][:, :NL].astype(np.int64), q)
+    assert np.array_equal(side, (fac_cell[1, cell_facet] == np.arange(N)[:, None]).astype(np.int64))
+    if gref is not None:
+        assert np.array_equal(np.asarray(gref, np.int64), ref)
+    hn, hq, hs, _ = unpack_slot_words(t["hw"].T)
+    hv = t["h1f"][:, :NL].astype(np.int64)
+    assert np.array_equal(np.where(hv == NONE16, NONE10, hv & ~SIDE_BIT), hq)
+    assert np.array_equal(np.where(hv == NONE16, 0, (hv & SIDE_BIT) != 0), hs)
     for ti in range(t["n_tiles"]):
         ho, n1, n2, no, fo0, nfo, foff, nfor = (int(v) for v in tab[ti, :8])
         c0 = ti * TILE

@@ -148,8 +148,8 @@ typedef struct FvmMeshDev {
                                    the others: needs fz_ngl <= fz_threads; fewer facets
                                    per thread); same results either way              */
     /* tile-fused stage (2-D and 3-D; fz_tile NULL = the three-kernel pipeline; the
-       tables must fit fvm_max_lds_per_workgroup, fz_ngl <= 512 and fz_nfl <= 512 in 2-D,
-       768 in 3-D).  Tile t holds
+       tables must fit fvm_max_lds_per_workgroup, fz_ncl <= 4095, fz_ngl <= 512 and
+       fz_nfl <= 512 in 2-D, 768 in 3-D).  Tile t holds
        cells 256t..256t+255 (local ids 0..255), its one-ring halo H1 (local ids 256..,
        ascending global id) and H1's other neighbours H2 (after H1).  Its facets are
        the owned run [fo0, fo0+nfo) of the device facet order (lowest adjacent cell in
@@ -158,26 +158,31 @@ typedef struct FvmMeshDev {
        cell, 12-13 its slot index k, 14-25 / 26-27 the same for the slot-1 cell, 28
        slot-0 cell has the lower global id, 29 boundary facet, 30 the outer side is
        slot 0 (outer local id 0xFFF).                                                 */
-    const int32_t *fz_tile;     /* [T][12] halo_off, n1, n2, nbr_off, fo0, nfo, foff,
+    const int32_t *fz_tile;     /* [T][12] halo_off, n1, n2, (unused), fo0, nfo, foff,
                                    nfor, g0, ng (group means the tile needs), h1_off  */
     const int32_t *fz_halo;     /* H1 then H2 global ids of each tile, at halo_off    */
     const float *fz_hrec;       /* [HN][fz_nh1] H1 records, component-major (record of
                                    H1 entry h1_off + i in column h1_off + i): LSQ
-                                   values [NL+1][d], displacements [NL][d], eps;
-                                   HN = 15 (2-D) / 28 (3-D)                            */
-    const uint16_t *fz_nbr;     /* [rows][4] local neighbour ids (0xFFFF = boundary)
-                                   of the tile's 256 + n1 cells, rows at nbr_off      */
+                                   values of the NL neighbour columns [NL][d] (the
+                                   own column is derived, below), displacements
+                                   [NL][d], eps; HN = 13 (2-D) / 25 (3-D)             */
+    const uint32_t *fz_cw;      /* [NL][N] slot words of every cell (slot-major):
+                                   bits 0-11 local id of the neighbour behind the
+                                   slot in the cell's tile (0xFFF: boundary facet),
+                                   12-21 the facet's position in the tile's list,
+                                   22 the cell is the facet's slot-1 side, 23-24 the
+                                   slot's position in the reference's local order of
+                                   the LSQ row: the kernel sums the neighbour columns
+                                   in that order and negates for the cell's own
+                                   column, bitwise the stored G_mats diagonal        */
+    const uint32_t *fz_hw;      /* [NL][fz_nh1] the same for H1 entries (local ids in
+                                   the tile; position 0x3FF: facet not in the list)  */
     const uint32_t *fz_fdesc;   /* [F] descriptor of each facet in its owner tile     */
     const int32_t *fz_ffac;     /* foreign facet ids of each tile, at foff            */
     const uint32_t *fz_fdesc_x; /* their descriptors                                  */
     const float *fz_geo;        /* [F][8] (2-D) / [F][12] (3-D) n, n_hat, |n|,
                                    1/(n.d), X, zero padding (device order)            */
     const float *fz_geo_x;      /* [sum nfor][8 | 12] the same for the foreign lists  */
-    const uint16_t *fz_cfac;    /* [N][4] tile-local facet index of each cell slot    */
-    const uint16_t *fz_h1f;     /* [sum n1][4] per H1 cell slot (rows at h1_off): the
-                                   tile-local index of the facet if it is in the
-                                   tile's list, | 0x8000 if the H1 cell is its slot-1
-                                   side; 0xFFFF otherwise                             */
     int32_t fz_ncl;             /* LDS rows: max staged cells (256 + n1 + n2)         */
     int32_t fz_ngl;             /*           max gradient cells (256 + n1)            */
     int32_t fz_nfl;             /*           max facets (nfo + nfor)                  */

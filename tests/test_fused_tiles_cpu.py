@@ -1,8 +1,8 @@
 """Tile tables of the tile-fused stage kernel, checked on the host (no GPU).
 
 Every relation the kernel reads from the tables (local neighbour ids, facet sides and
-slots, tile-local facet positions) is rebuilt from the tables alone and compared with
-the mesh's global arrays; the values the fused kernel takes from other arrays than the
+slots, tile-local facet positions, packed into one 32-bit word per cell slot) is rebuilt
+from the tables alone and compared with the mesh's global arrays; the values the fused kernel takes from other arrays than the
 three-kernel pipeline (compact facet geometry, halo records, the displacements of each
 facet side) must be the very same floats, so both pipelines compute bitwise alike.
 """
@@ -11,8 +11,9 @@ import pytest
 
 from fvm_solver_amd.config_fvm import ConfigChannel, ConfigChannel3D
 from fvm_solver_amd.fvm_stepping.facet_process import DeviceMesh
-from fvm_solver_amd.fvm_stepping.fused_tiles import (NONE16, TILE, TilesUnavailable, build_tiles, check_tiles,
-                                                     facets_per_thread, fused_lds_bytes)
+from fvm_solver_amd.fvm_stepping.fused_tiles import (NONE10, NONE12, NONE16, TILE, TilesUnavailable, build_tiles,
+                                                     check_tiles, facets_per_thread, fused_lds_bytes, pack_slot_words,
+                                                     unpack_slot_words)
 from fvm_solver_amd.mesh_gen.channel import channel_mesh_2d, channel_mesh_3d
 from fvm_solver_amd.mesh_utils.fvm_mesh import mesh_from_tuple
 from fvm_solver_amd.mesh_utils.mesh_store import channel_bc_tags
@@ -32,7 +33,7 @@ def _check(dm):
     assert t is not None, dm.tiles_reason
     cf = h["fpos"][h["c2f_s"]]
     fc = h["fac_cell"][:, h["fperm"]]
-    check_tiles(t, dm.N, dm.dim, h["nbr_s"], cf, fc)
+    check_tiles(t, dm.N, dm.dim, h["nbr_s"], cf, fc, gref=h["gref"])
     return t, h, cf, fc
 
 
@@ -83,14 +84,16 @@ def test_fused_inputs_are_the_same_floats(dm):
         f = np.flatnonzero(inner)
         k = (t["fdesc"][f].astype(np.int64) >> (12 if s == 0 else 26)) & 3
         assert np.array_equal(h["disp_s"][c[inner], k].astype(np.float32), geo[f, G + D * s:G + D * (s + 1)])
-    # halo records carry the halo cells' own LSQ values, displacements and eps
+    # halo records carry the halo cells' own LSQ values (neighbour columns), displacements
+    # and eps
     tab = t["tab"]
-    ng, nd = (NL + 1) * D, NL * D
+    ng, nd = NL * D, NL * D
     for ti in range(0, t["n_tiles"], max(1, t["n_tiles"] // 7)):
         ho, n1, h1o = int(tab[ti, 0]), int(tab[ti, 1]), int(tab[ti, 10])
         cells = t["halo"][ho:ho + n1]
         rec = t["hrec"][:, h1o:h1o + n1].T
-        assert np.array_equal(rec[:, :ng], h["gval"][cells].astype(np.float32).transpose(0, 2, 1).reshape(n1, -1))
+        assert np.array_equal(rec[:, :ng],
+                              h["gval"][cells][:, :, :NL].astype(np.float32).transpose(0, 2, 1).reshape(n1, -1))
         assert np.array_equal(rec[:, ng:ng + nd], h["disp_s"][cells].astype(np.float32).reshape(n1, -1))
         assert np.array_equal(rec[:, ng + nd], h["eps"][cells].astype(np.float32))
         assert rec.shape[1] == ng + nd + 1 and t["hrec"].shape[1] == t["nh1"]
@@ -101,6 +104,38 @@ def test_fused_inputs_are_the_same_floats(dm):
     assert t["lds_bytes"] == fused_lds_bytes(D, t["ncl"], t["ngl"], t["nfl"], dm.n_meshes > 1)
     assert t["lds_bytes"] <= (64 if D == 2 else 160) * 1024
     assert t["nfl"] <= facets_per_thread(D) * TILE and t["ngl"] <= 2 * TILE
+
+
+def test_own_lsq_column_from_slot_words(dm):
+    """The kernel does not load a cell's own G_mats column: it sums the neighbour columns
+    in the reference's local order (the slot words' order field) in fp32 and negates,
+    as build_sparse_gradient_matrix does (fvm_mesh.py:5-42); bitwise the stored column."""
+    t, h, cf, fc = _check(dm)
+    D = dm.dim
+    NL = D + 1
+    gv = h["gval"].astype(np.float32)                      # [N, D, NL+1], slot order
+    _, _, _, ref = unpack_slot_words(t["cw"].T)              # [N, NL]
+    for i in range(D):
+        acc = None
+        for r in range(NL):
+            term = np.zeros(dm.N, np.float32)
+            for k in range(NL):
+                term = np.where(ref[:, k] == r, gv[:, i, k], term)
+            acc = term if acc is None else (acc + term).astype(np.float32)
+        assert np.array_equal(-acc, gv[:, i, NL]), i
+
+
+def test_slot_words_round_trip():
+    rng = np.random.default_rng(3)
+    nl = rng.integers(0, NONE12 + 1, 1000)
+    q = rng.integers(0, NONE10 + 1, 1000)
+    side = rng.integers(0, 2, 1000)
+    ref = rng.integers(0, 4, 1000)
+    out = unpack_slot_words(pack_slot_words(nl, q, side, ref))
+    for a, b in zip(out, (nl, q, side, ref)):
+        assert np.array_equal(a, b)
+    with pytest.raises(TilesUnavailable):
+        pack_slot_words([NONE12 + 1], [0], [0], [0])
 
 
 def test_batch_tables():

@@ -45,6 +45,9 @@ def _parse():
     ap.add_argument("--kernel-timing-steps", type=int, default=20)
     ap.add_argument("--mesh-cache", default=None, help="npz file to reuse the generated mesh from (created if absent)")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--fused-threads", type=int, default=None,
+                    help="threads per tile of the tile-fused stage (256, 320, 384, 512) instead of the autotuner's "
+                         "choice (with --no-autotune)")
     ap.add_argument("--traffic-file", default=None,
                     help="measured HBM bytes per launch (rocprofv3 FETCH_SIZE/WRITE_SIZE, calibrated); default "
                          "profiles/rk4_channel2d_1M_traffic.json, rk4_channel3d_4M_traffic.json or "
@@ -396,6 +399,8 @@ def main():
         log(f"autotune: {tuned['choice']} " + ", ".join(f"{k}:{v:.0f}us" for k, v in tuned["us_per_step"].items()))
     elif eq.E_props.mesh.tiles is None:
         log(f"tile-fused stage unavailable: {eq.E_props.mesh.tiles_reason}")
+    elif a.fused_threads:
+        eq.E_props.mesh.set_fused_threads(a.fused_threads)
     torch.cuda.synchronize()
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     wall0 = time.perf_counter()
@@ -588,6 +593,8 @@ def batch_main(a):
         from fvm_solver_amd import autotune
         tuned = autotune.tune(s.step, s.E, N, 2, reps=6)
         log(f"autotune: {tuned['choice']} " + ", ".join(f"{k}:{v:.0f}us" for k, v in tuned["us_per_step"].items()))
+    elif a.fused_threads and s.E.fused:
+        s.E.mesh.set_fused_threads(a.fused_threads)
     n_graph = 10
     g = s.capture_graph(n_graph)
     steps = max(n_graph, (a.steps // n_graph) * n_graph)

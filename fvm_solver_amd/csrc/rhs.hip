@@ -102,6 +102,29 @@ __device__ __forceinline__ float lim_phi(float delta, float dU, float eps) {
     return 1.0f;
 }
 
+// A facet's geometry record as the flux takes it, [n (D), n_hat (D), |n|, 1/(n_hat.d),
+// X_orthog (D)], from n, 1/(n_hat.d) and X_orthog: |n| and n_hat are derived from the fp32
+// normal as the reference's FacetCalc derives them (facet_size = |n|, normals_hat =
+// n / facet_size, facet_process.py:84-86), through the hardware square root and
+// reciprocal (1 ulp each).  Both pipelines call this, so they stay bitwise alike.
+template <int D>
+__device__ __forceinline__ void facet_geo_full(const float (&n)[D], float inv_d, const float (&X)[D],
+                                               float (&geo)[3 * D + 2]) {
+    float ss = n[0] * n[0];
+#pragma unroll
+    for (int i = 1; i < D; ++i) ss = fmaf(n[i], n[i], ss);
+    const float fs = sqrt_hw(ss);
+    const float ir = __builtin_amdgcn_rcpf(fs);
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+        geo[i] = n[i];
+        geo[D + i] = n[i] * ir;
+        geo[2 * D + 2 + i] = X[i];
+    }
+    geo[2 * D] = fs;
+    geo[2 * D + 1] = inv_d;
+}
+
 // Workgroups are dealt round-robin to the 8 XCDs (workgroup w runs on XCD w % 8), each
 // with its own L2.  Give each XCD a contiguous range of tiles instead, so that the
 // neighbour gathers of Morton-adjacent tiles hit the L2 that already holds them.
@@ -689,9 +712,13 @@ k_facet_flux(FvmMeshDev m, FvmPhys p, FvmWork w, const float *__restrict__ prim)
                 gdn[jg] = m.bc_ga[b * C + j] * P0[j] + m.bc_gb[b * C + j];
             }
         }
-        float gr[3 * D + 2];
+        float gr[3 * D + 2], gn[D], gx[D];
 #pragma unroll
-        for (int k = 0; k < 3 * D + 2; ++k) gr[k] = geo[k];
+        for (int i = 0; i < D; ++i) {
+            gn[i] = geo[i];
+            gx[i] = geo[2 * D + 2 + i];
+        }
+        facet_geo_full<D>(gn, geo[2 * D + 1], gx, gr);
         float Fx[C];
         facet_flux<D>(p, mu0, mub0, kappa, U0s, U1s, g0s, g1s, gdn, gr, Fx);
         float *out = w.fflux + f * C;
@@ -924,8 +951,9 @@ constexpr int kFzTab = 12;
 constexpr unsigned kFzNone = 0xFFFFu;
 constexpr unsigned kFzSide1 = 0x8000u;   // H1 facet table: the H1 cell is the slot-1 side
 
+// compact facet record of the fused stage: n [D], 1/(n_hat.d), X_orthog [D]
 template <int D>
-__host__ __device__ constexpr int fz_geo_n() { return D == 2 ? 8 : 12; }
+__host__ __device__ constexpr int fz_geo_n() { return 2 * D + 1; }
 // H1 static record: LSQ values of the neighbour columns [NL][D], displacements [NL][D],
 // eps; stored component-major, [HN][sum n1] (a lane per H1 cell reads one coalesced row
 // per component)
@@ -1054,7 +1082,7 @@ __device__ __forceinline__ void fused_grad_cell(const FvmMeshDev &m, const FvmPh
 template <int D>
 __device__ __forceinline__ void fused_facet(const FvmMeshDev &m, const FvmPhys &p, const float *Pl, const float *GL,
                                             int ngl, float *FF, int q, int64_t f, unsigned d,
-                                            const float (&geo)[3 * D + 2], const unsigned short *meshT, int mt0,
+                                            const float (&gc)[2 * D + 1], const unsigned short *meshT, int mt0,
                                             const float *physT) {
     using Dm = Dims<D>;
     constexpr int C = Dm::C;
@@ -1079,6 +1107,16 @@ __device__ __forceinline__ void fused_facet(const FvmMeshDev &m, const FvmPhys &
             mub0 = m.mesh_phys[4 * mesh + 1];
             kappa = m.mesh_phys[4 * mesh + 2];
         }
+    }
+    float geo[3 * D + 2];
+    {
+        float n[D], X[D];
+#pragma unroll
+        for (int i = 0; i < D; ++i) {
+            n[i] = gc[i];
+            X[i] = gc[D + 1 + i];
+        }
+        facet_geo_full<D>(n, gc[D], X, geo);
     }
     float U0s[C], U1s[C], g0s[D][D + 1], g1s[D][D + 1], gdn[D + 1];
     load_vec<C>(FF + (2 * q) * C, U0s);
@@ -1125,24 +1163,21 @@ __device__ __forceinline__ void fused_facet(const FvmMeshDev &m, const FvmPhys &
 // foreign list).
 template <int D>
 __device__ __forceinline__ void fused_facet_load(const FvmMeshDev &m, int q, int fo0, int nfo, int foff, int64_t &f,
-                                                 unsigned &d, float (&geo)[3 * D + 2]) {
+                                                 unsigned &d, float (&geo)[2 * D + 1]) {
     constexpr int FG = fz_geo_n<D>();
+    // rows of 2D+1 floats are only 4-byte aligned: dword-aligned vector loads
+    typedef float f4u __attribute__((ext_vector_type(4), aligned(4)));
     const bool owned = q < nfo;
     f = owned ? static_cast<int64_t>(fo0) + q : static_cast<int64_t>(m.fz_ffac[foff + q - nfo]);
     d = owned ? m.fz_fdesc[f] : m.fz_fdesc_x[foff + q - nfo];
-    const float4 *src = reinterpret_cast<const float4 *>(
-        owned ? m.fz_geo + f * FG : m.fz_geo_x + static_cast<int64_t>(foff + q - nfo) * FG);
-    float t[FG];
+    const float *src = owned ? m.fz_geo + f * FG : m.fz_geo_x + static_cast<int64_t>(foff + q - nfo) * FG;
+    const f4u a = *reinterpret_cast<const f4u *>(src);
+    geo[0] = a.x;
+    geo[1] = a.y;
+    geo[2] = a.z;
+    geo[3] = a.w;
 #pragma unroll
-    for (int r = 0; r < FG / 4; ++r) {
-        const float4 a = src[r];
-        t[4 * r] = a.x;
-        t[4 * r + 1] = a.y;
-        t[4 * r + 2] = a.z;
-        t[4 * r + 3] = a.w;
-    }
-#pragma unroll
-    for (int k = 0; k < 3 * D + 2; ++k) geo[k] = t[k];
+    for (int k = 4; k < FG; ++k) geo[k] = src[k];
 }
 
 // One gradient cell's register inputs in the fused stage: LSQ row values, displacements,
@@ -1325,7 +1360,7 @@ k_fused(FvmMeshDev m, FvmPhys p, FvmWork w, const float *__restrict__ prim, floa
     ushort4 cf = make_ushort4(0, 0, 0, 0);
     int64_t fA = 0;
     unsigned dA = 0;
-    float geoA[3 * D + 2];
+    float geoA[2 * D + 1];
     if constexpr (!SPLIT) {
         FzGradIn<D> og, hg;
         og.nr = make_ushort4(0, 0, 0, 0);
@@ -1396,7 +1431,7 @@ k_fused(FvmMeshDev m, FvmPhys p, FvmWork w, const float *__restrict__ prim, floa
     // ---- phase 2: facet fluxes (the next facet's and the epilogue's inputs in flight) ----
     int64_t fB = 0;
     unsigned dB = 0;
-    float geoB[3 * D + 2];
+    float geoB[2 * D + 1];
     const bool twoB = FS >= 2 && i + NT < nft;
     if (twoB) fused_facet_load<D>(m, i + NT, fo0, nfo, foff, fB, dB, geoB);
     const int64_t c = c0 + i;
@@ -1418,7 +1453,7 @@ k_fused(FvmMeshDev m, FvmPhys p, FvmWork w, const float *__restrict__ prim, floa
         // third facet of the thread: its inputs in flight during the second's flux
         int64_t fC = 0;
         unsigned dC = 0;
-        float geoC[3 * D + 2];
+        float geoC[2 * D + 1];
         const bool threeC = i + 2 * NT < nft;
         if (threeC) fused_facet_load<D>(m, i + 2 * NT, fo0, nfo, foff, fC, dC, geoC);
         if (twoB) fused_facet<D>(m, p, Pl, GL, m.fz_ngl, FF, i + NT, fB, dB, geoB, meshT, mt0, physT);

@@ -60,6 +60,13 @@ def fused_lds_bytes(D: int, ncl: int, ngl: int, nfl: int, batch: bool) -> int:
     return 4 * floats + ((TILE * 2 + TILE_MESHES * 4 * 4) if batch else 0)
 
 
+def compact_geometry(geo, D: int) -> np.ndarray:
+    """The fused stage's facet record from the full one (n [D], n_hat [D], |n|, 1/(n_hat.d),
+    X_orthog [D], ...): n, 1/(n_hat.d), X_orthog -- 2 D + 1 floats."""
+    geo = np.asarray(geo, np.float32)
+    return np.ascontiguousarray(np.concatenate([geo[:, :D], geo[:, 2 * D + 1:3 * D + 2]], 1))
+
+
 NONE12 = 0xFFF     # packed slot word: boundary slot (no local neighbour)
 NONE10 = 0x3FF     # packed slot word: facet not in the tile's list (H1 cells)
 
@@ -240,10 +247,9 @@ def build_tiles(N: int, D: int, nbr: np.ndarray, cell_facet: np.ndarray, fac_cel
 
     fdesc = descriptors(own_t, np.arange(F))
     fdesc_x = descriptors(ft, fx)
-    GF = 3 * D + 2 if D == 2 else 12       # n, n_hat, |n|, 1/(n.d), X  (3-D padded to 12)
-    geo = np.asarray(geo, np.float32)
-    geo_c = np.zeros((F, 8 if D == 2 else 12), np.float32)
-    geo_c[:, :3 * D + 2] = geo[:, :3 * D + 2]
+    # compact facet geometry: n [D], 1/(n_hat.d), X_orthog [D] (2 D + 1 floats; the kernel
+    # derives |n| and n_hat from n as the reference's FacetCalc does, facet_process.py:84-86)
+    geo_c = compact_geometry(geo, D)
     geo_x = geo_c[fx]
 
     # ---- tile-local facet index of each cell slot ----

@@ -180,9 +180,10 @@ typedef struct FvmMeshDev {
     const uint32_t *fz_fdesc;   /* [F] descriptor of each facet in its owner tile     */
     const int32_t *fz_ffac;     /* foreign facet ids of each tile, at foff            */
     const uint32_t *fz_fdesc_x; /* their descriptors                                  */
-    const float *fz_geo;        /* [F][8] (2-D) / [F][12] (3-D) n, n_hat, |n|,
-                                   1/(n.d), X, zero padding (device order)            */
-    const float *fz_geo_x;      /* [sum nfor][8 | 12] the same for the foreign lists  */
+    const float *fz_geo;        /* [F][2d+1] n, 1/(n_hat.d), X_orthog (device order);
+                                   |n| and n_hat are derived from n as the
+                                   reference derives them (facet_process.py:84-86)   */
+    const float *fz_geo_x;      /* [sum nfor][2d+1] the same for the foreign lists    */
     int32_t fz_ncl;             /* LDS rows: max staged cells (256 + n1 + n2)         */
     int32_t fz_ngl;             /*           max gradient cells (256 + n1)            */
     int32_t fz_nfl;             /*           max facets (nfo + nfor)                  */

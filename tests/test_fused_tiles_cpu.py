@@ -72,9 +72,12 @@ def test_fused_inputs_are_the_same_floats(dm):
     NL = D + 1
     G = 3 * D + 2
     geo = h["geo"][h["fperm"]]
-    # compact facet geometry = the first 3d+2 floats of the facet record (zero padded)
-    assert np.array_equal(t["geo"][:, :G], geo[:, :G]) and np.all(t["geo"][:, G:] == 0)
-    assert np.array_equal(t["geo_x"][:, :G], geo[t["ffac"], :G])
+    # compact facet geometry = n, 1/(n_hat.d), X_orthog of the facet record (the kernels
+    # derive |n| and n_hat from n)
+    cols = list(range(D)) + list(range(2 * D + 1, 3 * D + 2))
+    assert t["geo"].shape[1] == 2 * D + 1
+    assert np.array_equal(t["geo"], geo[:, cols])
+    assert np.array_equal(t["geo_x"], geo[t["ffac"]][:, cols])
     # the three-kernel facet kernel takes each side's displacement from the facet
     # record; the fused kernel from the cell's own displacement list at the side's slot
     F = dm.F

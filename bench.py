@@ -174,14 +174,15 @@ def model_stage_bytes(N, F, Fb, D, ab3=False):
 def fused_kernel_bytes(N, F, Fb, D, stage, tiles, ab3=False):
     """k_fused (tile-fused stage), every array this implementation reads or writes once:
     per cell prim, the LSQ values of the neighbour columns (the own column is derived),
-    displacements, eps, one packed word per slot (local neighbour, list position, side,
+    the displacements of all slots but the last (which is minus their sum), eps, one
+    packed word per slot (local neighbour, list position, side,
     LSQ order), 1/vol and the stage epilogue's traffic (as k_div); per facet the compact
     geometry (n, 1/(n.d), X) and the descriptor; per boundary facet its BC
     data, boundary value written and the interior cell's gradient record read and
     written; the tile table.  Halo re-reads (tile + H1 + H2 staging) are not counted:
     they show in the measured traffic."""
     C, NL = D + 2, D + 1
-    per_cell = C + NL * D + NL * D + 1 + NL + 1
+    per_cell = C + NL * D + (NL - 1) * D + 1 + NL + 1
     # RK stage: U0, the accumulator after stage 0, accumulator (or state) and next-stage
     # primitives written; Adams-Bashforth 3: U0 and two past RHS read, state, RHS and
     # primitives written

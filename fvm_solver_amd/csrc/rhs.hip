@@ -954,11 +954,25 @@ constexpr unsigned kFzSide1 = 0x8000u;   // H1 facet table: the H1 cell is the s
 // compact facet record of the fused stage: n [D], 1/(n_hat.d), X_orthog [D]
 template <int D>
 __host__ __device__ constexpr int fz_geo_n() { return 2 * D + 1; }
-// H1 static record: LSQ values of the neighbour columns [NL][D], displacements [NL][D],
-// eps; stored component-major, [HN][sum n1] (a lane per H1 cell reads one coalesced row
-// per component)
+// H1 static record: LSQ values of the neighbour columns [NL][D], displacements of the
+// first NL - 1 slots [NL-1][D], eps; stored component-major, [HN][sum n1] (a lane per H1
+// cell reads one coalesced row per component)
 template <int D>
-__host__ __device__ constexpr int fz_hrec_n() { return 2 * (D + 1) * D + 1; }
+__host__ __device__ constexpr int fz_hrec_n() { return (D + 1) * D + D * D + 1; }
+
+// A cell's last slot displacement: minus the fp32 sum of the others (the facet midpoints
+// average to the centroid; the mesh layout stores exactly this value, so the pipelines
+// that load it see the same floats).
+template <int D>
+__device__ __forceinline__ void last_disp(float (&disp)[D + 1][D]) {
+#pragma unroll
+    for (int ii = 0; ii < D; ++ii) {
+        float acc = disp[0][ii];
+#pragma unroll
+        for (int k = 1; k < D; ++k) acc = acc + disp[k][ii];
+        disp[D][ii] = -acc;
+    }
+}
 template <int D>
 __host__ __device__ constexpr int fz_fpt() { return D == 2 ? 2 : 3; }   // facets per thread
 
@@ -1244,9 +1258,10 @@ __device__ __forceinline__ void fz_load_tile_cell(const FvmMeshDev &m, int64_t c
 #pragma unroll
         for (int ii = 0; ii < D; ++ii) g.gv[jj][ii] = m.cell_gval[(jj * D + ii) * N + c];
 #pragma unroll
-    for (int k = 0; k < NL; ++k)
+    for (int k = 0; k < NL - 1; ++k)
 #pragma unroll
         for (int ii = 0; ii < D; ++ii) g.disp[k][ii] = m.cell_disp[(k * D + ii) * N + c];
+    last_disp<D>(g.disp);
     g.eps = m.cell_eps[c];
     unsigned wd[NL], ref[NL];
 #pragma unroll
@@ -1283,10 +1298,11 @@ __device__ __forceinline__ int64_t fz_load_h1_cell(const FvmMeshDev &m, int h, i
 #pragma unroll
         for (int ii = 0; ii < D; ++ii) g.gv[jj][ii] = rv[jj * D + ii];
 #pragma unroll
-    for (int k = 0; k < NL; ++k)
+    for (int k = 0; k < NL - 1; ++k)
 #pragma unroll
         for (int ii = 0; ii < D; ++ii) g.disp[k][ii] = rv[NL * D + k * D + ii];
-    g.eps = rv[2 * NL * D];
+    last_disp<D>(g.disp);
+    g.eps = rv[NL * D + (NL - 1) * D];
     fz_decode_words<D>(wd, g, ref);
     fz_own_column<D>(g, ref);
     return gc;

@@ -76,7 +76,15 @@ def host_layout(mesh, n_comp: int, bc_tags, cfg) -> dict:
     c2f_s = np.take_along_axis(c2f, perm, 1)
     nbr_s = np.take_along_axis(comb, perm, 1)
     slot_s = np.take_along_axis(slots, perm, 1)
-    disp_s = np.take_along_axis(g["cent_to_facet_disp"], perm[..., None], 1)
+    # centroid -> facet-midpoint displacements in slot order, fp32; a cell's displacements
+    # sum to zero (the facet midpoints average to the centroid), so the last slot's is
+    # stored as minus the fp32 sum of the others: the tile-fused stage derives it instead
+    # of loading it, and every kernel then sees the same floats
+    disp_s = np.take_along_axis(g["cent_to_facet_disp"], perm[..., None], 1).astype(np.float32)
+    dsum = disp_s[:, 0, :].copy()
+    for k in range(1, NL - 1):
+        dsum = (dsum + disp_s[:, k, :]).astype(np.float32)
+    disp_s[:, NL - 1, :] = -dsum
     flat = c2f_s.reshape(-1)
     o = np.argsort(flat, kind="stable")
     counts = np.bincount(flat, minlength=F)
@@ -120,14 +128,14 @@ def host_layout(mesh, n_comp: int, bc_tags, cfg) -> dict:
     dproj = np.einsum("nd,nd->n", nh64[inter], cd)
     geo[inter, 2 * D + 1] = 1.0 / dproj
     geo[inter, 2 * D + 2:3 * D + 2] = cd / dproj[:, None] - nh64[inter]
-    # centroid -> facet-midpoint displacement of the slot-0 and slot-1 cells
-    disp64 = g["cent_to_facet_disp"]
+    # centroid -> facet-midpoint displacement of the slot-0 and slot-1 cells (the cells'
+    # own fp32 displacement lists, above)
     for k in range(NL):
-        f_k = c2f[:, k]
-        sl = slots[:, k]
+        f_k = c2f_s[:, k]
+        sl = slot_s[:, k]
         for sv in (0, 1):
             sel = sl == sv
-            geo[f_k[sel], 3 * D + 2 + sv * D:3 * D + 2 + (sv + 1) * D] = disp64[sel, k]
+            geo[f_k[sel], 3 * D + 2 + sv * D:3 * D + 2 + (sv + 1) * D] = disp_s[sel, k]
 
     # ---- boundary facets ----
     dm, nm, dv, nv, ffm, ilm = _bc_tables(bc_tags, C)
@@ -199,7 +207,8 @@ def host_layout(mesh, n_comp: int, bc_tags, cfg) -> dict:
             bc_char[mask, D + 3:] = vinf[None, :] - nn * vn[:, None]
     per = fs64[c2f].sum(1)
     return dict(D=D, C=C, N=N, F=F, Fb=Fb, n_groups=len(modes), c2f_s=c2f_s, nbr_s=nbr_s, slot_s=slot_s, nbk=nbk,
-                gval=gval, gref=perm.astype(np.int8), info=info, disp_s=disp_s, inv_vol=inv_vol, eps=eps, geo=geo, bcell=bcell,
+                gval=gval, gref=perm.astype(np.int8), info=info, disp_s=disp_s, inv_vol=inv_vol, eps=eps, geo=geo,
+                bcell=bcell,
                 group=group, flags=flags, bc_val=bc_val, bc_l=l_vec.astype(np.float32), bc_ga=bc_ga, bc_gb=bc_gb,
                 bc_char=bc_char, side=side, volumes64=g["volumes"], h64=D * g["volumes"] / per,
                 phys=(float(cfg.viscosity), float(cfg.visc_bulk), float(cfg.thermal_cond)))

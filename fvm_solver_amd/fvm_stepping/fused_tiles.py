@@ -72,8 +72,9 @@ NONE10 = 0x3FF     # packed slot word: facet not in the tile's list (H1 cells)
 
 
 def hrec_n(D: int) -> int:
-    """Floats per H1 record: LSQ values of the NL neighbour columns, displacements, eps."""
-    return 2 * (D + 1) * D + 1
+    """Floats per H1 record: LSQ values of the NL neighbour columns, the displacements of
+    the first NL - 1 slots (the last is minus their sum), eps."""
+    return (D + 1) * D + D * D + 1
 
 
 def pack_slot_words(nl, q, side, ref):
@@ -192,15 +193,15 @@ def build_tiles(N: int, D: int, nbr: np.ndarray, cell_facet: np.ndarray, fac_cel
     rows[h1_row, :NL] = np.where(nbr[h1c] < N, lh, NONE16).astype(np.uint16)
 
     # ---- H1 static records: LSQ values of the neighbour columns [NL][D] (slot order; the
-    # kernel derives the cell's own column as the reference does), displacements [NL][D],
-    # eps, stored component-major [HN][sum n1] (the kernel's lanes read one row per
-    # component) ----
+    # kernel derives the cell's own column as the reference does), displacements of the
+    # first NL - 1 slots [NL-1][D] (the last is minus their sum), eps, stored
+    # component-major [HN][sum n1] (the kernel's lanes read one row per component) ----
     HN = hrec_n(D)
     rec = np.zeros((len(K1), HN), np.float32)
     rec[:, :NL * D] = np.asarray(gval, np.float32)[h1c][:, :, :NL].transpose(0, 2, 1).reshape(len(K1), -1)
     o = NL * D
-    rec[:, o:o + NL * D] = np.asarray(disp, np.float32)[h1c].reshape(len(K1), -1)
-    rec[:, o + NL * D] = np.asarray(eps, np.float32)[h1c]
+    rec[:, o:o + (NL - 1) * D] = np.asarray(disp, np.float32)[h1c][:, :NL - 1].reshape(len(K1), -1)
+    rec[:, o + (NL - 1) * D] = np.asarray(eps, np.float32)[h1c]
     hrec = np.ascontiguousarray(rec.T)
 
     # ---- facets ----

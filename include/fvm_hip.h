@@ -101,7 +101,9 @@ typedef struct FvmMeshDev {
     const int32_t *cell_info;   /* [N] bit k: this cell's slot in facet k;
                                    bits 8+2k..9+2k: facet k's position in the
                                    neighbour's list                                   */
-    const float *cell_disp;     /* [NL][d][N] centroid -> facet midpoint              */
+    const float *cell_disp;     /* [NL][d][N] centroid -> facet midpoint; slot NL-1
+                                   holds -(fp32 sum of slots 0..NL-2 in slot order),
+                                   which the fused stage derives instead of reading  */
     const float *cell_gval;     /* [NL+1][d][N] LSQ gradient row values (G_mats rows
                                    d*c+i) in slot order: the column of neighbour k
                                    (or boundary value) for k < NL, then the cell's    */
@@ -165,7 +167,8 @@ typedef struct FvmMeshDev {
                                    H1 entry h1_off + i in column h1_off + i): LSQ
                                    values of the NL neighbour columns [NL][d] (the
                                    own column is derived, below), displacements
-                                   [NL][d], eps; HN = 13 (2-D) / 25 (3-D)             */
+                                   of slots 0..NL-2 [NL-1][d] (the last is minus
+                                   their sum), eps; HN = 11 (2-D) / 22 (3-D)          */
     const uint32_t *fz_cw;      /* [NL][N] slot words of every cell (slot-major):
                                    bits 0-11 local id of the neighbour behind the
                                    slot in the cell's tile (0xFFF: boundary facet),

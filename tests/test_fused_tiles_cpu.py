@@ -90,14 +90,14 @@ def test_fused_inputs_are_the_same_floats(dm):
     # halo records carry the halo cells' own LSQ values (neighbour columns), displacements
     # and eps
     tab = t["tab"]
-    ng, nd = NL * D, NL * D
+    ng, nd = NL * D, (NL - 1) * D
     for ti in range(0, t["n_tiles"], max(1, t["n_tiles"] // 7)):
         ho, n1, h1o = int(tab[ti, 0]), int(tab[ti, 1]), int(tab[ti, 10])
         cells = t["halo"][ho:ho + n1]
         rec = t["hrec"][:, h1o:h1o + n1].T
         assert np.array_equal(rec[:, :ng],
                               h["gval"][cells][:, :, :NL].astype(np.float32).transpose(0, 2, 1).reshape(n1, -1))
-        assert np.array_equal(rec[:, ng:ng + nd], h["disp_s"][cells].astype(np.float32).reshape(n1, -1))
+        assert np.array_equal(rec[:, ng:ng + nd], h["disp_s"][cells][:, :NL - 1].astype(np.float32).reshape(n1, -1))
         assert np.array_equal(rec[:, ng + nd], h["eps"][cells].astype(np.float32))
         assert rec.shape[1] == ng + nd + 1 and t["hrec"].shape[1] == t["nh1"]
     # boundary slots are marked, interior ones point at staged cells
@@ -126,6 +126,20 @@ def test_own_lsq_column_from_slot_words(dm):
                 term = np.where(ref[:, k] == r, gv[:, i, k], term)
             acc = term if acc is None else (acc + term).astype(np.float32)
         assert np.array_equal(-acc, gv[:, i, NL]), i
+
+
+def test_last_displacement_is_minus_the_sum(dm):
+    """The stage derives a cell's last slot displacement as minus the fp32 sum of the
+    others; the stored list (which the three-kernel pipeline and the facet records use)
+    holds exactly that value (its distance to the reference's displacement is checked in
+    test_layout_cpu.py::test_facet_record_side_displacements)."""
+    t, h, cf, fc = _check(dm)
+    NL = dm.dim + 1
+    d = h["disp_s"]
+    acc = d[:, 0, :].copy()
+    for k in range(1, NL - 1):
+        acc = (acc + d[:, k, :]).astype(np.float32)
+    assert d.dtype == np.float32 and np.array_equal(d[:, NL - 1, :], -acc)
 
 
 def test_slot_words_round_trip():

@@ -175,9 +175,9 @@ def fused_kernel_bytes(N, F, Fb, D, stage, tiles, ab3=False):
     """k_fused (tile-fused stage), every array this implementation reads or writes once:
     per cell prim, the LSQ values of the neighbour columns (the own column is derived),
     the displacements of all slots but the last (which is minus their sum), eps, one
-    packed word per slot (local neighbour, list position, side,
-    LSQ order), 1/vol and the stage epilogue's traffic (as k_div); per facet the compact
-    geometry (n, 1/(n.d), X) and the descriptor; per boundary facet its BC
+    packed word per slot (local neighbour, list position, side, LSQ order), 1/vol and the
+    stage epilogue's traffic (as k_div); per facet the compact geometry (n, 1/(n.d),
+    X_orthog's D - 1 tangent coordinates) and the descriptor; per boundary facet its BC
     data, boundary value written and the interior cell's gradient record read and
     written; the tile table.  Halo re-reads (tile + H1 + H2 staging) are not counted:
     they show in the measured traffic."""
@@ -187,7 +187,7 @@ def fused_kernel_bytes(N, F, Fb, D, stage, tiles, ab3=False):
     # primitives written; Adams-Bashforth 3: U0 and two past RHS read, state, RHS and
     # primitives written
     dyn = 6 * C if ab3 else C + (C if stage > 0 else 0) + 2 * C
-    per_facet = (2 * D + 1) + 1
+    per_facet = 2 * D + 1
     per_bc = 1 + C + D + 2 * C + (2 * D + 3) + 2 + C + 2 * (D + 1) * C
     return 4 * (N * (per_cell + dyn) + F * per_facet + Fb * per_bc + tiles * 12)
 

@@ -12,7 +12,7 @@ import torch
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libfvm_hip.so")
-ABI_VERSION = 16
+ABI_VERSION = 17
 
 FVM_ERR_ROW_OVERFLOW = 2
 

@@ -102,23 +102,48 @@ __device__ __forceinline__ float lim_phi(float delta, float dU, float eps) {
     return 1.0f;
 }
 
-// A facet's geometry record as the flux takes it, [n (D), n_hat (D), |n|, 1/(n_hat.d),
-// X_orthog (D)], from n, 1/(n_hat.d) and X_orthog: |n| and n_hat are derived from the fp32
-// normal as the reference's FacetCalc derives them (facet_size = |n|, normals_hat =
-// n / facet_size, facet_process.py:84-86), through the hardware square root and
-// reciprocal (1 ulp each).  Both pipelines call this, so they stay bitwise alike.
+// X_orthog = d / (n_hat.d) - n_hat (facet_process.py:88-95) is orthogonal to n_hat
+// (X.n_hat = 1 - |n_hat|^2 = 0), so the records hold its D - 1 coordinates in a tangent
+// basis of n_hat: in 2-D t = (-n_hat_y, n_hat_x); in 3-D the basis of Duff et al.,
+// "Building an Orthonormal Basis, Revisited" (JCGT 6(1), 2017), with sign(+-0) = +1.
 template <int D>
-__device__ __forceinline__ void facet_geo_full(const float (&n)[D], float inv_d, const float (&X)[D],
+__device__ __forceinline__ void tangent_X(const float (&nh)[D], const float (&xt)[D - 1], float (&X)[D]) {
+    if constexpr (D == 2) {
+        X[0] = -nh[1] * xt[0];
+        X[1] = nh[0] * xt[0];
+    } else {
+        const float sg = nh[2] >= 0.0f ? 1.0f : -1.0f;
+        const float a = -__builtin_amdgcn_rcpf(sg + nh[2]);
+        const float b = nh[0] * nh[1] * a;
+        const float t1[3] = {fmaf(sg * nh[0] * nh[0], a, 1.0f), sg * b, -sg * nh[0]};
+        const float t2[3] = {b, fmaf(nh[1] * nh[1], a, sg), -nh[1]};
+#pragma unroll
+        for (int i = 0; i < 3; ++i) X[i] = fmaf(xt[0], t1[i], xt[1] * t2[i]);
+    }
+}
+
+// A facet's geometry record as the flux takes it, [n (D), n_hat (D), |n|, 1/(n_hat.d),
+// X_orthog (D)], from n, 1/(n_hat.d) and X_orthog's tangent coordinates: |n| and n_hat
+// are derived from the fp32 normal as the reference's FacetCalc derives them (facet_size
+// = |n|, normals_hat = n / facet_size, facet_process.py:84-86), through the hardware
+// square root and reciprocal (1 ulp each).  Both pipelines call this, so they stay
+// bitwise alike.
+template <int D>
+__device__ __forceinline__ void facet_geo_full(const float (&n)[D], float inv_d, const float (&xt)[D - 1],
                                                float (&geo)[3 * D + 2]) {
     float ss = n[0] * n[0];
 #pragma unroll
     for (int i = 1; i < D; ++i) ss = fmaf(n[i], n[i], ss);
     const float fs = sqrt_hw(ss);
     const float ir = __builtin_amdgcn_rcpf(fs);
+    float nh[D], X[D];
+#pragma unroll
+    for (int i = 0; i < D; ++i) nh[i] = n[i] * ir;
+    tangent_X<D>(nh, xt, X);
 #pragma unroll
     for (int i = 0; i < D; ++i) {
         geo[i] = n[i];
-        geo[D + i] = n[i] * ir;
+        geo[D + i] = nh[i];
         geo[2 * D + 2 + i] = X[i];
     }
     geo[2 * D] = fs;
@@ -712,12 +737,11 @@ k_facet_flux(FvmMeshDev m, FvmPhys p, FvmWork w, const float *__restrict__ prim)
                 gdn[jg] = m.bc_ga[b * C + j] * P0[j] + m.bc_gb[b * C + j];
             }
         }
-        float gr[3 * D + 2], gn[D], gx[D];
+        float gr[3 * D + 2], gn[D], gx[D - 1];
 #pragma unroll
-        for (int i = 0; i < D; ++i) {
-            gn[i] = geo[i];
-            gx[i] = geo[2 * D + 2 + i];
-        }
+        for (int i = 0; i < D; ++i) gn[i] = geo[i];
+#pragma unroll
+        for (int i = 0; i < D - 1; ++i) gx[i] = geo[2 * D + 2 + i];
         facet_geo_full<D>(gn, geo[2 * D + 1], gx, gr);
         float Fx[C];
         facet_flux<D>(p, mu0, mub0, kappa, U0s, U1s, g0s, g1s, gdn, gr, Fx);
@@ -951,9 +975,10 @@ constexpr int kFzTab = 12;
 constexpr unsigned kFzNone = 0xFFFFu;
 constexpr unsigned kFzSide1 = 0x8000u;   // H1 facet table: the H1 cell is the slot-1 side
 
-// compact facet record of the fused stage: n [D], 1/(n_hat.d), X_orthog [D]
+// compact facet record of the fused stage: n [D], 1/(n_hat.d), X_orthog's tangent
+// coordinates [D - 1]
 template <int D>
-__host__ __device__ constexpr int fz_geo_n() { return 2 * D + 1; }
+__host__ __device__ constexpr int fz_geo_n() { return 2 * D; }
 // H1 static record: LSQ values of the neighbour columns [NL][D], displacements of the
 // first NL - 1 slots [NL-1][D], eps; stored component-major, [HN][sum n1] (a lane per H1
 // cell reads one coalesced row per component)
@@ -1096,7 +1121,7 @@ __device__ __forceinline__ void fused_grad_cell(const FvmMeshDev &m, const FvmPh
 template <int D>
 __device__ __forceinline__ void fused_facet(const FvmMeshDev &m, const FvmPhys &p, const float *Pl, const float *GL,
                                             int ngl, float *FF, int q, int64_t f, unsigned d,
-                                            const float (&gc)[2 * D + 1], const unsigned short *meshT, int mt0,
+                                            const float (&gc)[2 * D], const unsigned short *meshT, int mt0,
                                             const float *physT) {
     using Dm = Dims<D>;
     constexpr int C = Dm::C;
@@ -1124,13 +1149,12 @@ __device__ __forceinline__ void fused_facet(const FvmMeshDev &m, const FvmPhys &
     }
     float geo[3 * D + 2];
     {
-        float n[D], X[D];
+        float n[D], xt[D - 1];
 #pragma unroll
-        for (int i = 0; i < D; ++i) {
-            n[i] = gc[i];
-            X[i] = gc[D + 1 + i];
-        }
-        facet_geo_full<D>(n, gc[D], X, geo);
+        for (int i = 0; i < D; ++i) n[i] = gc[i];
+#pragma unroll
+        for (int i = 0; i < D - 1; ++i) xt[i] = gc[D + 1 + i];
+        facet_geo_full<D>(n, gc[D], xt, geo);
     }
     float U0s[C], U1s[C], g0s[D][D + 1], g1s[D][D + 1], gdn[D + 1];
     load_vec<C>(FF + (2 * q) * C, U0s);
@@ -1177,21 +1201,27 @@ __device__ __forceinline__ void fused_facet(const FvmMeshDev &m, const FvmPhys &
 // foreign list).
 template <int D>
 __device__ __forceinline__ void fused_facet_load(const FvmMeshDev &m, int q, int fo0, int nfo, int foff, int64_t &f,
-                                                 unsigned &d, float (&geo)[2 * D + 1]) {
+                                                 unsigned &d, float (&geo)[2 * D]) {
     constexpr int FG = fz_geo_n<D>();
-    // rows of 2D+1 floats are only 4-byte aligned: dword-aligned vector loads
-    typedef float f4u __attribute__((ext_vector_type(4), aligned(4)));
     const bool owned = q < nfo;
     f = owned ? static_cast<int64_t>(fo0) + q : static_cast<int64_t>(m.fz_ffac[foff + q - nfo]);
     d = owned ? m.fz_fdesc[f] : m.fz_fdesc_x[foff + q - nfo];
     const float *src = owned ? m.fz_geo + f * FG : m.fz_geo_x + static_cast<int64_t>(foff + q - nfo) * FG;
-    const f4u a = *reinterpret_cast<const f4u *>(src);
-    geo[0] = a.x;
-    geo[1] = a.y;
-    geo[2] = a.z;
-    geo[3] = a.w;
+    // rows of 2D floats: 16-byte aligned in 2-D (one load), 8-byte aligned in 3-D
+    if constexpr (D == 2) {
+        const float4 v = *reinterpret_cast<const float4 *>(src);
+        geo[0] = v.x;
+        geo[1] = v.y;
+        geo[2] = v.z;
+        geo[3] = v.w;
+    } else {
 #pragma unroll
-    for (int k = 4; k < FG; ++k) geo[k] = src[k];
+        for (int k = 0; k < FG; k += 2) {
+            const float2 v = *reinterpret_cast<const float2 *>(src + k);
+            geo[k] = v.x;
+            geo[k + 1] = v.y;
+        }
+    }
 }
 
 // One gradient cell's register inputs in the fused stage: LSQ row values, displacements,
@@ -1376,7 +1406,7 @@ k_fused(FvmMeshDev m, FvmPhys p, FvmWork w, const float *__restrict__ prim, floa
     ushort4 cf = make_ushort4(0, 0, 0, 0);
     int64_t fA = 0;
     unsigned dA = 0;
-    float geoA[2 * D + 1];
+    float geoA[2 * D];
     if constexpr (!SPLIT) {
         FzGradIn<D> og, hg;
         og.nr = make_ushort4(0, 0, 0, 0);
@@ -1447,7 +1477,7 @@ k_fused(FvmMeshDev m, FvmPhys p, FvmWork w, const float *__restrict__ prim, floa
     // ---- phase 2: facet fluxes (the next facet's and the epilogue's inputs in flight) ----
     int64_t fB = 0;
     unsigned dB = 0;
-    float geoB[2 * D + 1];
+    float geoB[2 * D];
     const bool twoB = FS >= 2 && i + NT < nft;
     if (twoB) fused_facet_load<D>(m, i + NT, fo0, nfo, foff, fB, dB, geoB);
     const int64_t c = c0 + i;
@@ -1469,7 +1499,7 @@ k_fused(FvmMeshDev m, FvmPhys p, FvmWork w, const float *__restrict__ prim, floa
         // third facet of the thread: its inputs in flight during the second's flux
         int64_t fC = 0;
         unsigned dC = 0;
-        float geoC[2 * D + 1];
+        float geoC[2 * D];
         const bool threeC = i + 2 * NT < nft;
         if (threeC) fused_facet_load<D>(m, i + 2 * NT, fo0, nfo, foff, fC, dC, geoC);
         if (twoB) fused_facet<D>(m, p, Pl, GL, m.fz_ngl, FF, i + NT, fB, dB, geoB, meshT, mt0, physT);

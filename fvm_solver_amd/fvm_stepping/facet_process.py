@@ -24,7 +24,7 @@ import torch
 
 from fvm_solver_amd import _native as nat
 from fvm_solver_amd.config_fvm import BCMode
-from fvm_solver_amd.fvm_stepping.fused_tiles import TilesUnavailable, build_tiles
+from fvm_solver_amd.fvm_stepping.fused_tiles import TilesUnavailable, build_tiles, device_geometry
 
 _K_BC_PART = 64
 
@@ -338,7 +338,7 @@ class DeviceMesh:
         self.t = dict(
             cell_facet=up(fpos[c2f_s].T, i32), cell_nbr=up(nbr_s.T, i32), cell_info=up(h["info"], i32),
             cell_disp=up(h["disp_s"].transpose(1, 2, 0), f32), cell_gval=up(h["gval"].transpose(2, 1, 0), f32),
-            cell_inv_vol=up(h["inv_vol"], f32), cell_eps=up(h["eps"], f32), fac_geo=up(h["geo"][fperm], f32),
+            cell_inv_vol=up(h["inv_vol"], f32), cell_eps=up(h["eps"], f32), fac_geo=up(device_geometry(h["geo"], D)[fperm], f32),
             fac_cell=up(fac_cell[:, fperm], i32),
             bc_cell=up_nz(bcell, i32), bc_group=up_nz(group, i32), bc_flags=up_nz(h["flags"], i32),
             bc_val=up_nz(h["bc_val"], f32), bc_l=up_nz(h["bc_l"], f32), bc_ga=up_nz(h["bc_ga"], f32),

@@ -60,11 +60,45 @@ def fused_lds_bytes(D: int, ncl: int, ngl: int, nfl: int, batch: bool) -> int:
     return 4 * floats + ((TILE * 2 + TILE_MESHES * 4 * 4) if batch else 0)
 
 
+def tangent_basis(nh) -> np.ndarray:
+    """Unit tangent vectors [F][D-1][D] of the unit normals nh [F][D] (fp64): in 2-D
+    (-n_y, n_x); in 3-D the basis of Duff et al., "Building an Orthonormal Basis,
+    Revisited" (JCGT 6(1), 2017), with sign(+-0) = +1 -- the basis the kernels rebuild
+    from their fp32 n_hat."""
+    nh = np.asarray(nh, np.float64)
+    if nh.shape[1] == 2:
+        return np.stack([-nh[:, 1], nh[:, 0]], 1)[:, None, :]
+    sg = np.where(nh[:, 2] >= 0, 1.0, -1.0)
+    a = -1.0 / (sg + nh[:, 2])
+    b = nh[:, 0] * nh[:, 1] * a
+    t1 = np.stack([1.0 + sg * nh[:, 0] ** 2 * a, sg * b, -sg * nh[:, 0]], 1)
+    t2 = np.stack([b, sg + nh[:, 1] ** 2 * a, -nh[:, 1]], 1)
+    return np.stack([t1, t2], 1)
+
+
+def tangent_coords(geo, D: int) -> np.ndarray:
+    """X_orthog's coordinates [F][D-1] in the tangent basis of n_hat, from the full facet
+    record (n_hat at columns D..2D-1, X_orthog at 2D+2..3D+1).  X_orthog = d / (n_hat.d) -
+    n_hat is orthogonal to n_hat, so these coordinates carry all of it."""
+    geo = np.asarray(geo, np.float64)
+    return np.einsum("fjd,fd->fj", tangent_basis(geo[:, D:2 * D]), geo[:, 2 * D + 2:3 * D + 2])
+
+
 def compact_geometry(geo, D: int) -> np.ndarray:
     """The fused stage's facet record from the full one (n [D], n_hat [D], |n|, 1/(n_hat.d),
-    X_orthog [D], ...): n, 1/(n_hat.d), X_orthog -- 2 D + 1 floats."""
-    geo = np.asarray(geo, np.float32)
-    return np.ascontiguousarray(np.concatenate([geo[:, :D], geo[:, 2 * D + 1:3 * D + 2]], 1))
+    X_orthog [D], ...): n, 1/(n_hat.d), X_orthog's tangent coordinates -- 2 D floats."""
+    geo = np.asarray(geo, np.float64)
+    out = np.concatenate([geo[:, :D], geo[:, 2 * D + 1:2 * D + 2], tangent_coords(geo, D)], 1)
+    return np.ascontiguousarray(out.astype(np.float32))
+
+
+def device_geometry(geo, D: int) -> np.ndarray:
+    """The full facet record as the kernels read it: X_orthog's D columns hold its D - 1
+    tangent coordinates, then 0 (the kernels rebuild X_orthog from them)."""
+    g = np.array(geo, np.float64, copy=True)
+    g[:, 2 * D + 2:3 * D + 2] = 0.0
+    g[:, 2 * D + 2:3 * D + 1] = tangent_coords(geo, D)
+    return g
 
 
 NONE12 = 0xFFF     # packed slot word: boundary slot (no local neighbour)
@@ -248,8 +282,9 @@ def build_tiles(N: int, D: int, nbr: np.ndarray, cell_facet: np.ndarray, fac_cel
 
     fdesc = descriptors(own_t, np.arange(F))
     fdesc_x = descriptors(ft, fx)
-    # compact facet geometry: n [D], 1/(n_hat.d), X_orthog [D] (2 D + 1 floats; the kernel
-    # derives |n| and n_hat from n as the reference's FacetCalc does, facet_process.py:84-86)
+    # compact facet geometry: n [D], 1/(n_hat.d), X_orthog's tangent coordinates (2 D
+    # floats; the kernel derives |n| and n_hat from n as the reference's FacetCalc does,
+    # facet_process.py:84-86, and X_orthog from n_hat)
     geo_c = compact_geometry(geo, D)
     geo_x = geo_c[fx]
 

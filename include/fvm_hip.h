@@ -113,7 +113,9 @@ typedef struct FvmMeshDev {
     /* Facets are stored in a device order of their own (by lowest adjacent cell), and
        cell_facet holds positions in that order.                                        */
     const float *fac_geo;       /* [F][G] G = 12 (2-D) / 20 (3-D): n[d], n_hat[d],
-                                   |n|, 1/(n_hat.d), X_orthog[d] (0 on the boundary),
+                                   |n|, 1/(n_hat.d), X_orthog's d-1 coordinates in
+                                   the tangent basis of n_hat and a 0 (X_orthog is
+                                   orthogonal to n_hat; 0 on the boundary),
                                    centroid->midpoint of the slot-0, slot-1 cells  */
     const int32_t *fac_cell;    /* [2][F] slot-0 and slot-1 cell; N + boundary index
                                    for the outer side of a boundary facet             */
@@ -183,10 +185,13 @@ typedef struct FvmMeshDev {
     const uint32_t *fz_fdesc;   /* [F] descriptor of each facet in its owner tile     */
     const int32_t *fz_ffac;     /* foreign facet ids of each tile, at foff            */
     const uint32_t *fz_fdesc_x; /* their descriptors                                  */
-    const float *fz_geo;        /* [F][2d+1] n, 1/(n_hat.d), X_orthog (device order);
-                                   |n| and n_hat are derived from n as the
-                                   reference derives them (facet_process.py:84-86)   */
-    const float *fz_geo_x;      /* [sum nfor][2d+1] the same for the foreign lists    */
+    const float *fz_geo;        /* [F][2d] n, 1/(n_hat.d), X_orthog's tangent
+                                   coordinates [d-1] (device order); |n| and n_hat
+                                   are derived from n as the reference derives them
+                                   (facet_process.py:84-86), X_orthog from n_hat and
+                                   the coordinates (2-D tangent (-n_hat_y, n_hat_x);
+                                   3-D the basis of Duff et al., JCGT 6(1), 2017)    */
+    const float *fz_geo_x;      /* [sum nfor][2d] the same for the foreign lists      */
     int32_t fz_ncl;             /* LDS rows: max staged cells (256 + n1 + n2)         */
     int32_t fz_ngl;             /*           max gradient cells (256 + n1)            */
     int32_t fz_nfl;             /*           max facets (nfo + nfor)                  */

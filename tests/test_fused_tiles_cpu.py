@@ -12,8 +12,8 @@ import pytest
 from fvm_solver_amd.config_fvm import ConfigChannel, ConfigChannel3D
 from fvm_solver_amd.fvm_stepping.facet_process import DeviceMesh
 from fvm_solver_amd.fvm_stepping.fused_tiles import (NONE10, NONE12, NONE16, TILE, TilesUnavailable, build_tiles,
-                                                     check_tiles, facets_per_thread, fused_lds_bytes, pack_slot_words,
-                                                     unpack_slot_words)
+                                                     check_tiles, device_geometry, facets_per_thread, fused_lds_bytes,
+                                                     pack_slot_words, tangent_basis, tangent_coords, unpack_slot_words)
 from fvm_solver_amd.mesh_gen.channel import channel_mesh_2d, channel_mesh_3d
 from fvm_solver_amd.mesh_utils.fvm_mesh import mesh_from_tuple
 from fvm_solver_amd.mesh_utils.mesh_store import channel_bc_tags
@@ -72,12 +72,16 @@ def test_fused_inputs_are_the_same_floats(dm):
     NL = D + 1
     G = 3 * D + 2
     geo = h["geo"][h["fperm"]]
-    # compact facet geometry = n, 1/(n_hat.d), X_orthog of the facet record (the kernels
-    # derive |n| and n_hat from n)
-    cols = list(range(D)) + list(range(2 * D + 1, 3 * D + 2))
-    assert t["geo"].shape[1] == 2 * D + 1
-    assert np.array_equal(t["geo"], geo[:, cols])
-    assert np.array_equal(t["geo_x"], geo[t["ffac"]][:, cols])
+    # compact facet geometry = n, 1/(n_hat.d) of the facet record and X_orthog's tangent
+    # coordinates (the kernels derive |n| and n_hat from n, X_orthog from n_hat); the
+    # three-kernel pipeline's record holds the same coordinates in X_orthog's columns
+    assert t["geo"].shape[1] == 2 * D
+    assert np.array_equal(t["geo"][:, :D], geo[:, :D].astype(np.float32))
+    assert np.array_equal(t["geo"][:, D], geo[:, 2 * D + 1].astype(np.float32))
+    assert np.array_equal(t["geo_x"], t["geo"][t["ffac"]])
+    dg = device_geometry(h["geo"], D)[h["fperm"]].astype(np.float32)
+    assert np.array_equal(dg[:, 2 * D + 2:3 * D + 1], t["geo"][:, D + 1:])
+    assert not dg[:, 3 * D + 1].any()
     # the three-kernel facet kernel takes each side's displacement from the facet
     # record; the fused kernel from the cell's own displacement list at the side's slot
     F = dm.F
@@ -86,7 +90,7 @@ def test_fused_inputs_are_the_same_floats(dm):
         inner = c < dm.N
         f = np.flatnonzero(inner)
         k = (t["fdesc"][f].astype(np.int64) >> (12 if s == 0 else 26)) & 3
-        assert np.array_equal(h["disp_s"][c[inner], k].astype(np.float32), geo[f, G + D * s:G + D * (s + 1)])
+        assert np.array_equal(h["disp_s"][c[inner], k].astype(np.float32), dg[f, G + D * s:G + D * (s + 1)])
     # halo records carry the halo cells' own LSQ values (neighbour columns), displacements
     # and eps
     tab = t["tab"]
@@ -185,3 +189,43 @@ def test_lds_limit_is_enforced():
         build_tiles(dm.N, 2, h["nbr_s"], h["fpos"][h["c2f_s"]], h["fac_cell"][:, h["fperm"]], h["gval"], h["disp_s"],
                     h["eps"], h["geo"][h["fperm"]], h["bcell"], h["group"], h["cell_mesh"],
                     dm.group_offsets, lds_limit=1024)
+
+
+@pytest.mark.parametrize("D", [2, 3])
+def test_tangent_basis_orthonormal(D):
+    """The tangent vectors are unit, orthogonal to each other and to n_hat, for random
+    normals and the axis-aligned ones (both signs of a zero component)."""
+    rng = np.random.default_rng(D)
+    nh = rng.normal(size=(4000, D))
+    ax = np.concatenate([np.eye(D), -np.eye(D)])
+    nh = np.concatenate([nh, ax, ax * np.where(ax == 0, -0.0, 1.0)])
+    nh /= np.linalg.norm(nh, axis=1, keepdims=True)
+    T = tangent_basis(nh)
+    assert T.shape == (len(nh), D - 1, D)
+    B = np.concatenate([T, nh[:, None, :]], 1)         # [F][D][D]: tangents, then n_hat
+    assert np.max(np.abs(np.einsum("fid,fjd->fij", B, B) - np.eye(D))) < 1e-12
+
+
+def test_tangent_coords_rebuild_x_orthog(dm):
+    """X_orthog of the reference (d / (n_hat.d) - n_hat) is orthogonal to n_hat, so its
+    tangent coordinates rebuild it to fp32 round-off of the fp32 record (whose X_orthog
+    keeps a normal part of that size), also through the fp32 coordinates the kernels
+    read; boundary facets stay 0."""
+    h, D = dm.host, dm.dim
+    geo = h["geo"].astype(np.float64)
+    X = geo[:, 2 * D + 2:3 * D + 2]
+    nh = geo[:, D:2 * D]
+    scale = max(1.0, np.abs(X).max())
+    assert np.max(np.abs((X * nh).sum(1))) < 1e-6 * scale
+    xt = tangent_coords(geo, D)
+    T = tangent_basis(nh)
+    rebuilt = np.einsum("fj,fjd->fd", xt, T)
+    assert np.max(np.abs(rebuilt - X)) < 1e-6 * scale
+    rebuilt32 = np.einsum("fj,fjd->fd", xt.astype(np.float32).astype(np.float64), T)
+    assert np.max(np.abs(rebuilt32 - X)) < 2e-6 * scale
+    # a dropped or transposed tangent vector fails the same bound
+    assert D == 2 or np.max(np.abs(np.einsum("fj,fjd->fd", xt[:, ::-1], T) - X)) > 1e-3 * scale
+    zero = ~np.any(X != 0, axis=1)
+    assert zero.any() and not xt[zero].any()
+    dg = device_geometry(h["geo"], D)
+    assert np.array_equal(dg[:, :2 * D + 2], geo[:, :2 * D + 2]) and np.array_equal(dg[:, 3 * D + 2:], geo[:, 3 * D + 2:])

@@ -301,7 +301,10 @@ class DeviceMesh:
 
         # the two sides of every facet, and the device facet order: by lowest adjacent
         # cell, so a run of facets touches a run of cells (cells are space-filling-curve
-        # ordered) and each cell finds its facets' fluxes close together
+        # ordered) and each cell finds its facets' fluxes close together; within the run
+        # of a 256-cell tile the boundary facets go last, so the waves that evaluate a
+        # tile's facets meet the boundary-condition branch in one place (every facet's
+        # flux and every cell's sum over its slots are the same in any facet order)
         slot_s = h["slot_s"]
         fac_cell = np.full((2, F), -1, np.int64)
         fac_cell[slot_s, c2f_s] = np.broadcast_to(np.arange(N)[:, None], c2f_s.shape)
@@ -309,7 +312,8 @@ class DeviceMesh:
         fac_cell[1 - slot_s[outer], c2f_s[outer]] = nbr_s[outer]
         if (fac_cell < 0).any():
             raise ValueError("a facet is missing one of its two sides")
-        fperm = np.argsort(fac_cell.min(0), kind="stable")
+        lo = fac_cell.min(0)
+        fperm = np.lexsort((lo, fac_cell.max(0) >= N, lo // 256))
         fpos = np.empty(F, np.int64)
         fpos[fperm] = np.arange(F)
         h["fac_cell"], h["fperm"], h["fpos"] = fac_cell, fperm, fpos

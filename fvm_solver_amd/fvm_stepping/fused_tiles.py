@@ -15,8 +15,8 @@ Local cell numbering of tile t (cells c0 = 256 t ...):
     256+n1 .. +n2-1    H2, ascending global id
 
 Facets of tile t: its *owned* facets -- those whose lowest adjacent cell is in the tile,
-a contiguous run [fo0, fo0 + nfo) of the device facet order (sorted by lowest adjacent
-cell) -- then its *foreign* facets, interior facets of a tile cell owned by an earlier
+a contiguous run [fo0, fo0 + nfo) of the device facet order (grouped by the tile of the
+lowest adjacent cell) -- then its *foreign* facets, interior facets of a tile cell owned by an earlier
 tile (evaluated by both tiles, each keeping its own cell's share).
 
 Facet descriptor (uint32): bits 0-11 local id of the slot-0 cell, 12-13 that cell's slot
@@ -241,8 +241,8 @@ def build_tiles(N: int, D: int, nbr: np.ndarray, cell_facet: np.ndarray, fac_cel
     # ---- facets ----
     lo = fac_cell.min(0)
     hi = fac_cell.max(0)
-    if np.any(np.diff(lo) < 0):
-        raise ValueError("facets are not in device order (ascending lowest cell)")
+    if np.any(np.diff(lo // TILE) < 0):
+        raise ValueError("facets are not in device order (ascending tile of the lowest cell)")
     own_t = lo // TILE
     fo0 = np.searchsorted(own_t, np.arange(T), side="left")
     fo1 = np.searchsorted(own_t, np.arange(T), side="right")

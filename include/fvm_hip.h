@@ -110,8 +110,9 @@ typedef struct FvmMeshDev {
     const float *cell_inv_vol;  /* [N] fp32(1/vol): |flux_mat| entries                */
     const float *cell_eps;      /* [N] limiter epsilon (K sqrt(vol))^(p+1)            */
     /* per facet ------------------------------------------------------------------- */
-    /* Facets are stored in a device order of their own (by lowest adjacent cell), and
-       cell_facet holds positions in that order.                                        */
+    /* Facets are stored in a device order of their own (by lowest adjacent cell, the
+       boundary facets of each 256-cell run of lowest cells last), and cell_facet holds
+       positions in that order.                                                         */
     const float *fac_geo;       /* [F][G] G = 12 (2-D) / 20 (3-D): n[d], n_hat[d],
                                    |n|, 1/(n_hat.d), X_orthog's d-1 coordinates in
                                    the tangent basis of n_hat and a 0 (X_orthog is

@@ -197,7 +197,10 @@ def test_facet_sides_and_device_order(case):
     assert (fc[:, inter] < N).all() and (fc[0, inter] != fc[1, inter]).all()
     perm, pos = h["fperm"], h["fpos"]
     assert np.array_equal(np.sort(perm), np.arange(F)) and np.array_equal(pos[perm], np.arange(F))
-    key = fc.min(0)[perm]
+    # device order: by the 256-cell run of the lowest adjacent cell, boundary facets of a
+    # run last, then by lowest cell
+    lo, bnd = fc.min(0)[perm], (fc.max(0) >= N)[perm]
+    key = (lo // 256) * 2 * (N + 1) + bnd * (N + 1) + lo
     assert (np.diff(key) >= 0).all()
 
 

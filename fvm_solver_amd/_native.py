@@ -12,7 +12,7 @@ import torch
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libfvm_hip.so")
-ABI_VERSION = 17
+ABI_VERSION = 18
 
 FVM_ERR_ROW_OVERFLOW = 2
 
@@ -55,7 +55,7 @@ class FvmStage(ctypes.Structure):
                 ("stage", i32), ("last", i32), ("store_k", i32), ("has_prev", i32), ("bc_ready", i32),
                 ("b", f32), ("b2", f32), ("n_inc", i32), ("inc_idx", i32 * 10), ("inc_coef", f32 * 10),
                 ("mode", i32), ("n_hist", i32), ("f_m1", c_void_p), ("f_m2", c_void_p), ("f_out", c_void_p),
-                ("tile_active", c_void_p)]
+                ("tile_list", c_void_p), ("n_tile_list", i32)]
 
 
 _PROTOS = {

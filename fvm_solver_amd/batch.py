@@ -111,17 +111,19 @@ class BatchedSolver:
     def set_dt(self, dts):
         self.dt.copy_(torch.as_tensor(np.asarray(dts, np.float32)))
 
-    def tile_mask(self, meshes: np.ndarray) -> torch.Tensor:
-        """uint8 flags of the 256-cell tiles holding a cell of any mesh with meshes[i]
-        true (the tiles a masked step() computes)."""
+    def tile_mask(self, meshes: np.ndarray) -> torch.Tensor | None:
+        """int32 ids, ascending, of the 256-cell tiles holding a cell of any mesh with
+        meshes[i] true (the tiles a masked step() computes, one workgroup each); None
+        when no mesh is selected."""
         if not self.E.fused:
             raise RuntimeError("masked steps need the tile-fused stage")
         cs = np.concatenate([[0], np.cumsum(np.asarray(meshes, bool))])
         lo, hi = self._tile_mesh
-        return torch.from_numpy((cs[hi + 1] - cs[lo] > 0).astype(np.uint8)).to(self.device)
+        ids = np.flatnonzero(cs[hi + 1] - cs[lo] > 0).astype(np.int32)
+        return torch.from_numpy(ids).to(self.device) if len(ids) else None
 
     def step(self, tiles: torch.Tensor | None = None):
-        """One fused RK step of every mesh of the batch, or, with a tile mask from
+        """One fused RK step of every mesh of the batch, or, with a tile list from
         tile_mask(), of the meshes it was made for (the other meshes' cells are left
         stale or computed from stale inputs: overwrite them afterwards)."""
         self._ensure_prim()
@@ -134,7 +136,8 @@ class BatchedSolver:
                               kbuf=None, prim_next=nxt.data_ptr(), U_out=out.data_ptr(), dt=self.dt.data_ptr(),
                               stage=i, last=int(i == self.stages - 1), store_k=0,
                               b=float(self.b[i]), b2=float(self.b2[i]),
-                              tile_active=None if tiles is None else tiles.data_ptr())
+                              tile_list=None if tiles is None else tiles.data_ptr(),
+                              n_tile_list=0 if tiles is None else int(tiles.numel()))
             if i < self.stages - 1 and self.A[i + 1][i] != 0:
                 st.n_inc = 1
                 st.inc_idx[0] = i

@@ -200,7 +200,7 @@ extern "C" {
 
 const char *fvm_last_error(void) { return g_err; }
 
-int fvm_abi_version(void) { return 17; }
+int fvm_abi_version(void) { return 18; }
 
 int fvm_ell_spmm_f32_shape(const float *vals, int64_t vsm, int64_t vsk, const void *cols, int32_t cols_is64,
                            int64_t csm, int64_t csk, int64_t M, int32_t K, const float *x, int64_t xsn, int64_t xsb,

@@ -1364,9 +1364,12 @@ k_fused(FvmMeshDev m, FvmPhys p, FvmWork w, const float *__restrict__ prim, floa
     float *lds = reinterpret_cast<float *>(fz_lds);
     const int64_t N = m.n_cells;
     const int64_t ntiles = (N + kThreads - 1) / kThreads;
-    const int64_t tile = xcd_tile(blockIdx.x, ntiles);
+    int64_t tile = xcd_tile(blockIdx.x, ntiles);
     if constexpr (EPI == EPI_STAGE) {
-        if (st.tile_active && !st.tile_active[tile]) return;     // a tile the caller does not step
+        if (st.tile_list) {                      // grid = the caller's tile list
+            tile = st.tile_list[xcd_tile(blockIdx.x, st.n_tile_list)];
+            if (tile < 0 || tile >= ntiles) return;
+        }
     }
     const int *tb = m.fz_tile + tile * kFzTab;
     const int halo_off = tb[0], n1 = tb[1], n2 = tb[2], fo0 = tb[4], nfo = tb[5];
@@ -1659,7 +1662,8 @@ static void launch_fused_kernel(K kernel, int threads, dim3 grid, size_t lds, hi
 template <int D, int EPI, int NT>
 static void launch_fused_nt(const FvmMeshDev &m, const FvmPhys &p, const FvmWork &w, const float *prim, float *dudt,
                             const FvmStage &st, hipStream_t s, int has_prev) {
-    const dim3 grid(blocks_for(m.n_cells, kThreads));
+    const dim3 grid(EPI == EPI_STAGE && st.tile_list ? static_cast<unsigned int>(st.n_tile_list)
+                                                     : blocks_for(m.n_cells, kThreads));
     const size_t lds = fused_lds_bytes<D>(m);
     switch (p.lim_p) {
         case 1: launch_fused_kernel(k_fused<D, 1, EPI, NT>, NT, grid, lds, s, m, p, w, prim, dudt, st, has_prev); break;
@@ -1752,8 +1756,13 @@ static int check_stage(const FvmMeshDev *m, const FvmWork *w, const FvmStage *st
         set_error("%s: a stage needs work->bc_part_next as a buffer separate from work->bc_part", who);
         return FVM_ERR_ARG;
     }
-    if (st && st->tile_active && !m_has_tiles) {
-        set_error("%s: a tile mask needs the tile-fused stage (2-D mesh with tile tables)", who);
+    if (st && st->tile_list && !m_has_tiles) {
+        set_error("%s: a tile list needs the tile-fused stage (mesh with tile tables)", who);
+        return FVM_ERR_ARG;
+    }
+    if (st && st->tile_list && (st->n_tile_list < 1 || st->n_tile_list > (m->n_cells + kThreads - 1) / kThreads)) {
+        set_error("%s: n_tile_list %d outside [1, %lld]", who, st->n_tile_list,
+                  static_cast<long long>((m->n_cells + kThreads - 1) / kThreads));
         return FVM_ERR_ARG;
     }
     if (st && st->mode == 1) {

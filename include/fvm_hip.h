@@ -269,12 +269,15 @@ typedef struct FvmStage {
     const float *f_m1;     /* [N][C]                                                  */
     const float *f_m2;     /* [N][C]                                                  */
     float *f_out;          /* [N][C]                                                  */
-    /* [ceil(N / 256)] or NULL: with the tile-fused stage, 256-cell tiles whose flag is 0
-       are skipped (nothing of them is read or written); a batch re-steps only the meshes
-       whose cells lie in flagged tiles.  Cells of other meshes that share a flagged tile
-       are computed from stale inputs and must be overwritten by the caller.  No
-       reference counterpart. */
-    const uint8_t *tile_active;
+    /* [n_tile_list] distinct ids of 256-cell tiles, or NULL: with the tile-fused stage
+       only these tiles are stepped, one workgroup each (the other tiles are neither read
+       nor written, and the listed tiles spread over every XCD whatever their ids); a
+       batch re-steps only the meshes whose cells lie in listed tiles.  Cells of other
+       meshes that share a listed tile are computed from stale inputs and must be
+       overwritten by the caller.  1 <= n_tile_list <= ceil(N / 256); ids outside
+       [0, ceil(N / 256)) are skipped.  No reference counterpart. */
+    const int32_t *tile_list;
+    int32_t n_tile_list;
 } FvmStage;
 
 int fvm_rk_stage_f32(const FvmMeshDev *m, const FvmPhys *p, const FvmWork *w, const FvmStage *st,

@@ -374,26 +374,37 @@ k_mp_window_predict(const float *__restrict__ U, const float *__restrict__ f0, c
     U1[t] = v1;
 }
 
+// The per-mesh reductions below take the variable count as a template argument CT (4 or
+// 5; 0 = runtime C, up to 8) so that a thread's fp64 partial sums are registers: indexed by
+// a runtime count they live in scratch memory, a load and a store per update.
 // grid (64, n_meshes): per mesh q, fp64 partial sums of (w a_j)^2 and (w (a_j - b_j))^2
+template <int CT>
 __global__ void __launch_bounds__(256)
 k_mesh_sumsq2_partial(const float *__restrict__ A, const float *__restrict__ B, const float *__restrict__ win,
-                      const int64_t *__restrict__ off, int C, double *__restrict__ part) {
+                      const int64_t *__restrict__ off, int C_rt, double *__restrict__ part) {
     __shared__ double red[256];
+    constexpr int CM = CT ? CT : 8;
+    const int C = CT ? CT : C_rt;
     const int q = blockIdx.y;
     const int64_t a = off[q], b = off[q + 1];
-    double s[16];
-    for (int j = 0; j < 2 * C; ++j) s[j] = 0.0;
+    double s[2 * CM];
+#pragma unroll
+    for (int j = 0; j < 2 * CM; ++j) s[j] = 0.0;
     for (int64_t c = a + static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; c < b;
          c += static_cast<int64_t>(gridDim.x) * blockDim.x) {
         const float w = win[c];
-        for (int j = 0; j < C; ++j) {
+#pragma unroll
+        for (int j = 0; j < CM; ++j) {
+            if (j >= C) continue;
             const float x = A[c * C + j];
             const float v = w * x, d = w * (x - B[c * C + j]);
             s[j] += static_cast<double>(v) * v;
             s[C + j] += static_cast<double>(d) * d;
         }
     }
-    for (int j = 0; j < 2 * C; ++j) {
+#pragma unroll
+    for (int j = 0; j < 2 * CM; ++j) {
+        if (j >= 2 * C) continue;
         red[threadIdx.x] = s[j];
         __syncthreads();
         for (int h = 128; h > 0; h >>= 1) {
@@ -420,23 +431,31 @@ __global__ void k_mesh_rms2_final(const double *__restrict__ part, int nblk, int
 }
 
 // grid (64, n_meshes): per mesh q, fp64 partial sums of (w x_j)^2 over its cells
+template <int CT>
 __global__ void __launch_bounds__(256)
 k_mesh_sumsq_partial(const float *__restrict__ X, const float *__restrict__ win, const int64_t *__restrict__ off,
-                     int C, double *__restrict__ part) {
+                     int C_rt, double *__restrict__ part) {
     __shared__ double red[256];
+    constexpr int CM = CT ? CT : 8;
+    const int C = CT ? CT : C_rt;
     const int q = blockIdx.y;
     const int64_t a = off[q], b = off[q + 1];
-    double s[8];
-    for (int j = 0; j < C; ++j) s[j] = 0.0;
+    double s[CM];
+#pragma unroll
+    for (int j = 0; j < CM; ++j) s[j] = 0.0;
     for (int64_t c = a + static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; c < b;
          c += static_cast<int64_t>(gridDim.x) * blockDim.x) {
         const float w = win ? win[c] : 1.0f;
-        for (int j = 0; j < C; ++j) {
+#pragma unroll
+        for (int j = 0; j < CM; ++j) {
+            if (j >= C) continue;
             const float v = w * X[c * C + j];
             s[j] += static_cast<double>(v) * v;
         }
     }
-    for (int j = 0; j < C; ++j) {
+#pragma unroll
+    for (int j = 0; j < CM; ++j) {
+        if (j >= C) continue;
         red[threadIdx.x] = s[j];
         __syncthreads();
         for (int h = 128; h > 0; h >>= 1) {
@@ -461,26 +480,34 @@ __global__ void k_mesh_rms_final(const double *__restrict__ part, int nblk, int 
 }
 
 // grid (64, n_meshes): block b of mesh q sums cells off[q] + b, + 64 * 256 ... strided
+template <int CT>
 __global__ void __launch_bounds__(256)
 k_mp_defect_partial(const float *__restrict__ U0, const float *__restrict__ U1, const float *__restrict__ f0,
                     const float *__restrict__ f1, const float *__restrict__ win, const int64_t *__restrict__ off,
-                    int C, double *__restrict__ part) {
+                    int C_rt, double *__restrict__ part) {
     __shared__ double red[256];
+    constexpr int CM = CT ? CT : 8;
+    const int C = CT ? CT : C_rt;
     const int q = blockIdx.y;
     const int64_t a = off[q], b = off[q + 1];
-    double s[16];
-    for (int j = 0; j < 2 * C; ++j) s[j] = 0.0;
+    double s[2 * CM];
+#pragma unroll
+    for (int j = 0; j < 2 * CM; ++j) s[j] = 0.0;
     for (int64_t c = a + static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; c < b;
          c += static_cast<int64_t>(gridDim.x) * blockDim.x) {
         const float hw = 0.5f * win[c];
-        for (int j = 0; j < C; ++j) {
+#pragma unroll
+        for (int j = 0; j < CM; ++j) {
+            if (j >= C) continue;
             const int64_t o = c * C + j;
             const float d = (U1[o] - U0[o]) - hw * (f0[o] + f1[o]);
             s[j] += static_cast<double>(d) * d;
             s[C + j] += static_cast<double>(U1[o]) * U1[o];
         }
     }
-    for (int j = 0; j < 2 * C; ++j) {
+#pragma unroll
+    for (int j = 0; j < 2 * CM; ++j) {
+        if (j >= 2 * C) continue;
         red[threadIdx.x] = s[j];
         __syncthreads();
         for (int h = 128; h > 0; h >>= 1) {
@@ -853,20 +880,26 @@ k_transpose_bf16(const float *__restrict__ W, int64_t nrow, int64_t ncol, __bf16
 // grid (64, n_meshes): per mesh, fp64 partial sums of the Simpson defect of the window
 // d = U1 - U0 - w/6 (f0 + 4 fm + f1), of the Hermite midpoint defect
 // e = Um - (U0 + U1)/2 - w/8 (f0 - f1), and of U1^2, per variable
+template <int CT>
 __global__ void __launch_bounds__(256)
 k_mp_defect3_partial(const float *__restrict__ U0, const float *__restrict__ Um, const float *__restrict__ U1,
                      const float *__restrict__ f0, const float *__restrict__ fm, const float *__restrict__ f1,
-                     const float *__restrict__ win, const int64_t *__restrict__ off, int C,
+                     const float *__restrict__ win, const int64_t *__restrict__ off, int C_rt,
                      double *__restrict__ part) {
     __shared__ double red[256];
+    constexpr int CM = CT ? CT : 8;
+    const int C = CT ? CT : C_rt;
     const int q = blockIdx.y;
     const int64_t a = off[q], b = off[q + 1];
-    double s[24];
-    for (int j = 0; j < 3 * C; ++j) s[j] = 0.0;
+    double s[3 * CM];
+#pragma unroll
+    for (int j = 0; j < 3 * CM; ++j) s[j] = 0.0;
     for (int64_t c = a + static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; c < b;
          c += static_cast<int64_t>(gridDim.x) * blockDim.x) {
         const float w = win[c];
-        for (int j = 0; j < C; ++j) {
+#pragma unroll
+        for (int j = 0; j < CM; ++j) {
+            if (j >= C) continue;
             const int64_t o = c * C + j;
             const float d = (U1[o] - U0[o]) - (w / 6.0f) * ((f0[o] + 4.0f * fm[o]) + f1[o]);
             const float e = (Um[o] - 0.5f * (U0[o] + U1[o])) - (w * 0.125f) * (f0[o] - f1[o]);
@@ -875,7 +908,9 @@ k_mp_defect3_partial(const float *__restrict__ U0, const float *__restrict__ Um,
             s[2 * C + j] += static_cast<double>(U1[o]) * U1[o];
         }
     }
-    for (int j = 0; j < 3 * C; ++j) {
+#pragma unroll
+    for (int j = 0; j < 3 * CM; ++j) {
+        if (j >= 3 * C) continue;
         red[threadIdx.x] = s[j];
         __syncthreads();
         for (int h = 128; h > 0; h >>= 1) {
@@ -1074,7 +1109,12 @@ int fvm_mesh_rms2(const float *A, const float *B, const float *win, const int64_
         return FVM_ERR_ARG;
     }
     hipStream_t s = as_stream(stream);
-    hipLaunchKernelGGL(k_mesh_sumsq2_partial, dim3(64, n_meshes), dim3(256), 0, s, A, B, win, off, C, part);
+    if (C == 4)
+        hipLaunchKernelGGL(k_mesh_sumsq2_partial<4>, dim3(64, n_meshes), dim3(256), 0, s, A, B, win, off, C, part);
+    else if (C == 5)
+        hipLaunchKernelGGL(k_mesh_sumsq2_partial<5>, dim3(64, n_meshes), dim3(256), 0, s, A, B, win, off, C, part);
+    else
+        hipLaunchKernelGGL(k_mesh_sumsq2_partial<0>, dim3(64, n_meshes), dim3(256), 0, s, A, B, win, off, C, part);
     hipLaunchKernelGGL(k_mesh_rms2_final, dim3(blocks_for(static_cast<int64_t>(n_meshes) * 2 * C, 256)), dim3(256), 0,
                        s, part, 64, n_meshes, C, off, floor_abs, out1, out2);
     return check_launch("fvm_mesh_rms2");
@@ -1087,7 +1127,12 @@ int fvm_mesh_rms(const float *X, const float *win, const int64_t *off, int32_t n
         return FVM_ERR_ARG;
     }
     hipStream_t s = as_stream(stream);
-    hipLaunchKernelGGL(k_mesh_sumsq_partial, dim3(64, n_meshes), dim3(256), 0, s, X, win, off, C, part);
+    if (C == 4)
+        hipLaunchKernelGGL(k_mesh_sumsq_partial<4>, dim3(64, n_meshes), dim3(256), 0, s, X, win, off, C, part);
+    else if (C == 5)
+        hipLaunchKernelGGL(k_mesh_sumsq_partial<5>, dim3(64, n_meshes), dim3(256), 0, s, X, win, off, C, part);
+    else
+        hipLaunchKernelGGL(k_mesh_sumsq_partial<0>, dim3(64, n_meshes), dim3(256), 0, s, X, win, off, C, part);
     hipLaunchKernelGGL(k_mesh_rms_final, dim3(blocks_for(static_cast<int64_t>(n_meshes) * C, 256)), dim3(256), 0, s,
                        part, 64, n_meshes, C, off, floor_abs, out);
     return check_launch("fvm_mesh_rms");
@@ -1101,7 +1146,12 @@ int fvm_mp_defect(const float *U0, const float *U1, const float *f0, const float
         return FVM_ERR_ARG;
     }
     hipStream_t s = as_stream(stream);
-    hipLaunchKernelGGL(k_mp_defect_partial, dim3(64, n_meshes), dim3(256), 0, s, U0, U1, f0, f1, win, off, C, part);
+    if (C == 4)
+        hipLaunchKernelGGL(k_mp_defect_partial<4>, dim3(64, n_meshes), dim3(256), 0, s, U0, U1, f0, f1, win, off, C, part);
+    else if (C == 5)
+        hipLaunchKernelGGL(k_mp_defect_partial<5>, dim3(64, n_meshes), dim3(256), 0, s, U0, U1, f0, f1, win, off, C, part);
+    else
+        hipLaunchKernelGGL(k_mp_defect_partial<0>, dim3(64, n_meshes), dim3(256), 0, s, U0, U1, f0, f1, win, off, C, part);
     hipLaunchKernelGGL(k_mp_defect_final, dim3(blocks_for(static_cast<int64_t>(n_meshes) * 2 * C, 256)), dim3(256), 0,
                        s, part, 64, n_meshes, 2 * C, out);
     return check_launch("fvm_mp_defect");
@@ -1230,8 +1280,15 @@ int fvm_mp_defect3(const float *U0, const float *Um, const float *U1, const floa
         return FVM_ERR_ARG;
     }
     hipStream_t s = as_stream(stream);
-    hipLaunchKernelGGL(k_mp_defect3_partial, dim3(64, n_meshes), dim3(256), 0, s, U0, Um, U1, f0, fm, f1, win, off, C,
-                       part);
+    if (C == 4)
+        hipLaunchKernelGGL(k_mp_defect3_partial<4>, dim3(64, n_meshes), dim3(256), 0, s, U0, Um, U1, f0, fm, f1, win, off,
+                           C, part);
+    else if (C == 5)
+        hipLaunchKernelGGL(k_mp_defect3_partial<5>, dim3(64, n_meshes), dim3(256), 0, s, U0, Um, U1, f0, fm, f1, win, off,
+                           C, part);
+    else
+        hipLaunchKernelGGL(k_mp_defect3_partial<0>, dim3(64, n_meshes), dim3(256), 0, s, U0, Um, U1, f0, fm, f1, win, off,
+                           C, part);
     hipLaunchKernelGGL(k_mp_defect_final, dim3(blocks_for(static_cast<int64_t>(n_meshes) * 3 * C, 256)), dim3(256), 0,
                        s, part, 64, n_meshes, 3 * C, out);
     return check_launch("fvm_mp_defect3");

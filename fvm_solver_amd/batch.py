@@ -111,16 +111,19 @@ class BatchedSolver:
     def set_dt(self, dts):
         self.dt.copy_(torch.as_tensor(np.asarray(dts, np.float32)))
 
-    def tile_mask(self, meshes: np.ndarray) -> torch.Tensor | None:
+    def tile_mask(self, meshes: np.ndarray) -> torch.Tensor:
         """int32 ids, ascending, of the 256-cell tiles holding a cell of any mesh with
-        meshes[i] true (the tiles a masked step() computes, one workgroup each); None
-        when no mesh is selected."""
+        meshes[i] true (the tiles a masked step() computes, one workgroup each).  At
+        least one mesh must be selected (a step without a list steps every mesh)."""
         if not self.E.fused:
             raise RuntimeError("masked steps need the tile-fused stage")
-        cs = np.concatenate([[0], np.cumsum(np.asarray(meshes, bool))])
+        sel = np.asarray(meshes, bool)
+        if sel.shape != (self.n_meshes,) or not sel.any():
+            raise ValueError(f"tile_mask: expected {self.n_meshes} flags with at least one set")
+        cs = np.concatenate([[0], np.cumsum(sel)])
         lo, hi = self._tile_mesh
         ids = np.flatnonzero(cs[hi + 1] - cs[lo] > 0).astype(np.int32)
-        return torch.from_numpy(ids).to(self.device) if len(ids) else None
+        return torch.from_numpy(ids).to(self.device)
 
     def step(self, tiles: torch.Tensor | None = None):
         """One fused RK step of every mesh of the batch, or, with a tile list from

@@ -120,6 +120,9 @@ def test_masked_step_equals_full_step_on_selected_meshes():
     ref.set_dt(s.dt.cpu().numpy())
     sel = np.array([True, False, True, False, True])
     tiles = s.tile_mask(sel)
+    assert tiles.dtype == torch.int32 and bool((tiles[1:] > tiles[:-1]).all())
+    with pytest.raises(ValueError):
+        s.tile_mask(np.zeros(5, bool))          # an empty list would step every mesh
     for _ in range(6):
         s.step(tiles)
         ref.step()

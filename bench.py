@@ -62,15 +62,18 @@ def _parse():
                    help="measure the learned window corrector against the plain RK4 solver instead")
     g.add_argument("--train-meshes", type=int, default=64)
     g.add_argument("--train-windows", type=int, default=157, help="windows per training mesh (64 x 157 = 10k pairs)")
-    g.add_argument("--train-iters", type=int, default=6000,
-                   help="Adam iterations (2000 gave 420/1024 accepted predictions, 0.87x plain; 6000 639/1024, "
-                        "1.03x; 12000 on 300 windows per mesh 636/1024)")
+    g.add_argument("--train-iters", type=int, default=12000,
+                   help="Adam iterations (three layers: 2000 gave 420/1024 accepted predictions, 6000 639/1024; "
+                        "one layer: 6000 610/1024, 12000 645/1024)")
     g.add_argument("--eval-meshes", type=int, default=64)
     g.add_argument("--eval-windows", type=int, default=16)
     g.add_argument("--eval-spinup", type=int, default=20,
                    help="windows of plain RK4 run on the evaluation meshes before the comparison starts")
     g.add_argument("--window", type=int, default=8, help="solver steps per predicted window")
     g.add_argument("--settle", type=int, default=1, help="plain RK4 steps after each window")
+    g.add_argument("--layers", type=int, default=1,
+                   help="message-passing layers of the corrector network (3: 1.03x plain, 2: 1.12x, 1: 1.26x at "
+                        "6000 iterations; the input already carries one ring of least-squares gradients)")
     g.add_argument("--n-min", type=int, default=5000)
     g.add_argument("--n-max", type=int, default=200_000)
     g.add_argument("--oracle-meshes", type=int, default=2,
@@ -702,7 +705,7 @@ def learned_main(a):
     ts.set_dt(ts.cfl_dt(0.5))
     for _ in range(a.eval_spinup * a.window):         # past the impulsive start, as the evaluation
         ts.step()
-    lc = LearnedCorrector(ts, k=a.window, seed=a.seed, tol=a.tol, settle=a.settle)
+    lc = LearnedCorrector(ts, k=a.window, layers=a.layers, seed=a.seed, tol=a.tol, settle=a.settle)
     data = lc.generate(a.train_windows, log=log)
     gen_s = time.time() - t0
     log(f"training data: {a.train_meshes} meshes x {a.train_windows} windows ({ts.dm.N} cells) in {gen_s:.1f}s")
@@ -723,7 +726,7 @@ def learned_main(a):
     plain.set_dt(dts)
     learn = BatchedSolver(ep, device=dev)
     learn.set_dt(dts)
-    lc = LearnedCorrector(learn, k=a.window, tol=a.tol, settle=a.settle)
+    lc = LearnedCorrector(learn, k=a.window, layers=a.layers, tol=a.tol, settle=a.settle)
     lc.stats = stats
     lc._apply_stats()
     lc.net.load_state_dict(weights)
@@ -855,7 +858,7 @@ def learned_main(a):
         "tol": a.tol,
         "dtype": "bf16 network (fp32 accumulation), f32 solver",
         "data": "synthetic: trajectories of the batched solver on seeded random channel meshes",
-        "config": {"workload": "learned corrector, 3 message-passing layers, hidden 128, window of "
+        "config": {"workload": f"learned corrector, {a.layers} message-passing layers, hidden 128, window of "
                                f"{a.window} RK4 steps at CFL 0.5 + {a.settle} plain settle step(s)",
                    "train_meshes": a.train_meshes,
                    "train_pairs": a.train_meshes * a.train_windows, "train_iters": a.train_iters,
@@ -880,7 +883,7 @@ def corrector_timing_main(a):
     ep, _ = random_channel_batch(a.eval_meshes, seed=a.seed + 2000, n_min=a.n_min, n_max=a.n_max, device=dev)
     s = BatchedSolver(ep, device=dev)
     s.set_dt(s.cfl_dt(0.5))
-    lc = LearnedCorrector(s, k=a.window, tol=a.tol, settle=a.settle)
+    lc = LearnedCorrector(s, k=a.window, layers=a.layers, tol=a.tol, settle=a.settle)
     lc.set_stats(s.state)
     lc.plain_window()
     torch.cuda.synchronize()

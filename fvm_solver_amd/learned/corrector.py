@@ -87,7 +87,7 @@ class LearnedCorrector:
 
     KX = 32
 
-    def __init__(self, solver, k: int = 8, hidden: int = 128, layers: int = 3, seed: int = 0, tol: float = 3.5e-4,
+    def __init__(self, solver, k: int = 8, hidden: int = 128, layers: int = 1, seed: int = 0, tol: float = 3.5e-4,
                  settle: int = 1):
         self.s = solver
         self.k = k

@@ -27,7 +27,7 @@ window is accepted f(Um) and f(U1) are the next window's fm and f0.  A window is
 accepted for a mesh only if ||d|| / ||U1|| and ||e|| / ||U1|| over the mesh's whole
 conserved state are at most tol.  tol bounds a window's defect, not the state error, which
 grows over consecutive accepted windows: on 64 random channel meshes the default 3.5e-4
-kept every mesh's state within 6e-4 of plain RK4 after 17 windows, where 1e-3 let the
+kept every mesh's state within 6.1e-4 of plain RK4 after 17 windows, where 1e-3 let the
 largest meshes drift to 2e-3.  The meshes that fail are re-run with the plain fp32 RK4
 solver (a masked step computes only the 256-cell tiles holding their cells), the others
 keep their predictions, and f is evaluated at the merged midpoint and end to renew the

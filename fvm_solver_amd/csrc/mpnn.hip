@@ -657,19 +657,24 @@ k_mp_layer(const __bf16 *__restrict__ h, const int32_t *__restrict__ nbr, int64_
 // One wave per 16-cell tile (grid-stride over tiles, four tiles per pass): the X rows are the B operand (lane
 // g|col holds inputs 8g..8g+7 of cell col), W_in^T the A operand, kept in registers with
 // the lane's bias values for the whole kernel; eight v_mfma_f32_16x16x32_bf16 give
-// features x cells, stored as 8-byte bf16 chunks.
+// features x cells, stored as 16-byte bf16 chunks (the rows of W_in^T permuted so that a
+// lane's outputs of two blocks are 8 consecutive features).
 __global__ void __launch_bounds__(256)
 k_mp_encode(const __bf16 *__restrict__ X, int64_t N, const __bf16 *__restrict__ WT, const float *__restrict__ bias,
             __bf16 *__restrict__ h0) {
     constexpr int H = 128, KX = 32;
     const int lane = threadIdx.x & 63, g = lane >> 4, col = lane & 15;
+    // row m = 4g' + r' of MFMA block cb computes feature 32(cb/2) + 8g' + 4(cb%2) + r', so
+    // the lane holding rows 4g .. 4g+3 of blocks 2q and 2q+1 holds features 32q + 8g .. +7
+    // of its cell and stores them as one 16-byte chunk
+    auto feat = [](int cb, int m) { return 32 * (cb >> 1) + 8 * (m >> 2) + 4 * (cb & 1) + (m & 3); };
     bf16x8_t wf[8];
     float bb[8][4];
 #pragma unroll
     for (int cb = 0; cb < 8; ++cb) {
-        wf[cb] = *reinterpret_cast<const bf16x8_t *>(WT + (16 * cb + col) * KX + 8 * g);
+        wf[cb] = *reinterpret_cast<const bf16x8_t *>(WT + feat(cb, col) * KX + 8 * g);
 #pragma unroll
-        for (int r = 0; r < 4; ++r) bb[cb][r] = bias[16 * cb + 4 * g + r];
+        for (int r = 0; r < 4; ++r) bb[cb][r] = bias[feat(cb, 4 * g + r)];
     }
     const int64_t ntiles = (N + 15) / 16;
     const int64_t nw = static_cast<int64_t>(gridDim.x) * (blockDim.x >> 6);
@@ -693,15 +698,19 @@ k_mp_encode(const __bf16 *__restrict__ X, int64_t N, const __bf16 *__restrict__ 
             const int64_t row = (t0 + u * nw) * 16 + col;
             const bool live = t0 + u * nw < ntiles && row < N;
 #pragma unroll
-            for (int cb = 0; cb < 8; ++cb) {
-                const f32x4_t acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf[cb], xv[u],
-                                                                              f32x4_t{0.0f, 0.0f, 0.0f, 0.0f}, 0, 0, 0);
+            for (int q = 0; q < 4; ++q) {
+                const f32x4_t a0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf[2 * q], xv[u],
+                                                                             f32x4_t{0.0f, 0.0f, 0.0f, 0.0f}, 0, 0, 0);
+                const f32x4_t a1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf[2 * q + 1], xv[u],
+                                                                             f32x4_t{0.0f, 0.0f, 0.0f, 0.0f}, 0, 0, 0);
                 if (live) {
-                    typedef __bf16 bf16x4_t __attribute__((ext_vector_type(4)));
-                    bf16x4_t o;
+                    bf16x8_t o;
 #pragma unroll
-                    for (int r = 0; r < 4; ++r) o[r] = __bf16(fmaxf(acc[r] + bb[cb][r], 0.0f));
-                    *reinterpret_cast<bf16x4_t *>(h0 + row * H + 16 * cb + 4 * g) = o;
+                    for (int r = 0; r < 4; ++r) {
+                        o[r] = __bf16(fmaxf(a0[r] + bb[2 * q][r], 0.0f));
+                        o[4 + r] = __bf16(fmaxf(a1[r] + bb[2 * q + 1][r], 0.0f));
+                    }
+                    *reinterpret_cast<bf16x8_t *>(h0 + row * H + 32 * q + 8 * g) = o;
                 }
             }
         }
